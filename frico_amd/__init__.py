@@ -1,0 +1,262 @@
+"""frico_amd -- MI355X (gfx950) EFIE Method-of-Moments hot path behind FRICO's entry points.
+
+Python front-end of libfrico_hip.so.  The names follow the reference: `Context.fill` is
+compute_matrix / compute_matrix_approx, `factor` + `solve` are solve_general's zgetrf/zgetrs,
+`fields` is eval_fields, and `antenna_sweep` / `bistatic_sweep` are the two do_sweep entry
+points (src/emw_solver.h:85-88, src/lapack_wrappers.h:28-35, src/emw_postpro_common.h:33,
+src/emw_ana_antenna.h:52, src/emw_ana_bistatic_rcs.h:45).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib, meshes
+from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C128)
+
+__all__ = ["Context", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
+           "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR"]
+
+
+class FricoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class SingularMatrixError(FricoError):
+    """zgetrf info > 0: exactly zero pivot at column `code` (1-based)."""
+
+
+class NotConvergedError(FricoError):
+    """Iterative refinement stopped above the tolerance (the iterate is still available)."""
+
+
+@dataclass
+class SolveInfo:
+    iterations: int
+    converged: bool
+    max_rel_residual: float
+
+
+def _p(a, t=C.c_double):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _modes(mode, prec=None):
+    m = {"quad": FILL_QUAD, "approx": FILL_APPROX}.get(mode, mode)
+    if prec is None:
+        return m
+    return m, {"c128": C128, "c64": C64}.get(prec, prec)
+
+
+def _form(form):
+    return {"c128": SOLVE_C128, "c64_ir": SOLVE_C64_IR}.get(form, form)
+
+
+class Context:
+    """One device, one stream; owns the mesh, Z, its factors, the RHS block and the solution."""
+
+    def __init__(self, device: int = 0):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        self._check(self._L.frico_ctx_create(device, C.byref(h)))
+        self._h = h
+        self.mesh = None
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.frico_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, ok=(0,)):
+        if rc in ok:
+            return rc
+        msg = self._L.frico_last_error().decode()
+        if rc > 0:
+            raise SingularMatrixError(rc, msg)
+        if rc == _lib.FRICO_ERR_NOCONV:
+            raise NotConvergedError(rc, msg)
+        raise FricoError(rc, msg)
+
+    # ---- configuration / profiling --------------------------------------------------------
+    def set_block(self, nb: int):
+        self._check(self._L.frico_ctx_set_block(self._h, nb))
+
+    def set_profiling(self, on: bool = True):
+        self._check(self._L.frico_ctx_set_profiling(self._h, int(on)))
+
+    def profile(self) -> dict:
+        p = _lib.frico_profile()
+        self._check(self._L.frico_ctx_get_profile(self._h, C.byref(p)))
+        return {k: getattr(p, k) for k, _ in p._fields_}
+
+    def synchronize(self):
+        self._check(self._L.frico_ctx_synchronize(self._h))
+
+    # ---- mesh -------------------------------------------------------------------------------
+    def load_mesh(self, sm: meshes.SurfaceMesh, degree: int = 5):
+        self._check(self._L.frico_mesh_load(
+            self._h, _p(sm.vertices), sm.nv, _p(sm.triangles, C.c_int64), _p(sm.tags, C.c_int32), sm.nt,
+            _p(sm.bedges, C.c_int64), _p(sm.btags, C.c_int32), sm.bedges.shape[0], degree))
+        self.mesh = sm
+        self.degree = degree
+        return self
+
+    @property
+    def N(self) -> int:
+        return int(self._L.frico_num_unknowns(self._h))
+
+    @property
+    def T(self) -> int:
+        return int(self._L.frico_num_triangles(self._h))
+
+    def rwg_table(self):
+        arr = (_lib.frico_rwg * self.N)()
+        self._check(self._L.frico_rwg_table(self._h, arr))
+        return arr
+
+    def reorient_deltagap(self, ifaces, force=False):
+        ifa = np.ascontiguousarray(ifaces, np.int32)
+        signs = np.ones(self.N)
+        applied = C.c_int(0)
+        self._check(self._L.frico_reorient_deltagap(self._h, _p(ifa, C.c_int32), len(ifa), int(force),
+                                                    _p(signs), C.byref(applied)))
+        return signs if applied.value else None
+
+    # ---- matrix -------------------------------------------------------------------------------
+    def fill(self, freq: float, mode="quad", prec="c128"):
+        m, p = _modes(mode, prec)
+        self._check(self._L.frico_fill(self._h, float(freq), m, p))
+
+    def symmetrize(self):
+        self._check(self._L.frico_symmetrize(self._h))
+
+    def matrix(self) -> np.ndarray:
+        N = self.N
+        Z = np.empty((N, N), np.complex128, order="F")
+        self._check(self._L.frico_matrix_get(self._h, _p(Z), N))
+        return Z
+
+    def set_matrix(self, Z):
+        Z = np.asfortranarray(Z, np.complex128)
+        self._check(self._L.frico_matrix_set(self._h, _p(Z), Z.shape[0], Z.shape[0]))
+
+    # ---- right-hand sides ---------------------------------------------------------------------
+    def rhs_deltagap(self, freq, ifaces, voltage=1.0, signs=None):
+        ifa = np.ascontiguousarray(ifaces, np.int32)
+        v = complex(voltage)
+        self._check(self._L.frico_rhs_deltagap(self._h, float(freq), _p(ifa, C.c_int32), len(ifa), v.real,
+                                               v.imag, None if signs is None else _p(np.ascontiguousarray(signs, np.float64))))
+
+    def rhs_planewave(self, freq, theta, phi, eth=1.0, eph=0.0):
+        th = np.atleast_1d(np.asarray(theta, np.float64))
+        ph = np.broadcast_to(np.asarray(phi, np.float64), th.shape).copy()
+        et = np.ascontiguousarray(np.broadcast_to(np.asarray(eth, np.complex128), th.shape))
+        ep = np.ascontiguousarray(np.broadcast_to(np.asarray(eph, np.complex128), th.shape))
+        self._check(self._L.frico_rhs_planewave(self._h, float(freq), th.size, _p(th), _p(ph), _p(et), _p(ep)))
+
+    def set_rhs(self, B):
+        B = np.asfortranarray(np.atleast_2d(np.asarray(B, np.complex128).T).T)
+        self._check(self._L.frico_rhs_set(self._h, _p(B), B.shape[0], B.shape[1]))
+
+    def rhs(self) -> np.ndarray:
+        n = int(self._L.frico_num_rhs(self._h))
+        B = np.empty((self.N, n), np.complex128, order="F")
+        self._check(self._L.frico_rhs_get(self._h, _p(B), self.N))
+        return B
+
+    # ---- factor / solve -----------------------------------------------------------------------
+    def factor(self, form="c128"):
+        return self._check(self._L.frico_factor(self._h, _form(form)))
+
+    def solve(self, tol=1e-12, maxit=10) -> SolveInfo:
+        info = _lib.frico_solve_info()
+        self._check(self._L.frico_solve(self._h, float(tol), int(maxit), C.byref(info)))
+        return SolveInfo(info.iterations, bool(info.converged), info.max_rel_residual)
+
+    def solution(self) -> np.ndarray:
+        n = int(self._L.frico_num_rhs(self._h))
+        X = np.empty((self.N, n), np.complex128, order="F")
+        self._check(self._L.frico_solution_get(self._h, _p(X), self.N))
+        return X
+
+    def set_solution(self, X):
+        X = np.asfortranarray(np.atleast_2d(np.asarray(X, np.complex128).T).T)
+        self._check(self._L.frico_solution_set(self._h, _p(X), X.shape[0], X.shape[1]))
+
+    def residual(self) -> np.ndarray:
+        n = int(self._L.frico_num_rhs(self._h))
+        r = np.empty(n)
+        self._check(self._L.frico_residual(self._h, _p(r)))
+        return r
+
+    # ---- post-processing ----------------------------------------------------------------------
+    def triangle_currents(self, col=0):
+        T = self.T
+        AJ = np.empty((T, 3), np.complex128)
+        AdivJ = np.empty(T, np.complex128)
+        self._check(self._L.frico_triangle_currents(self._h, int(col), _p(AJ), _p(AdivJ)))
+        return AJ, AdivJ
+
+    def fields(self, freq, pts, cols=None, h_sign=1.0):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        E = np.empty((n, 3), np.complex128)
+        H = np.empty((n, 3), np.complex128)
+        cc = None if cols is None else np.ascontiguousarray(cols, np.int64)
+        self._check(self._L.frico_fields(self._h, float(freq), _p(pts), n,
+                                         None if cc is None else _p(cc, C.c_int64), float(h_sign), _p(E), _p(H)))
+        return E, H
+
+    # ---- whole analyses (do_sweep mirrors) ----------------------------------------------------
+    def antenna_sweep(self, freqs, ifaces, voltage=1.0, mode="quad", R=10.0, Z0=50.0,
+                      force_symmetry=False, force_reorient=False, form="c128", tol=1e-12, maxit=10):
+        f = np.atleast_1d(np.asarray(freqs, np.float64))
+        ifa = np.ascontiguousarray(ifaces, np.int32)
+        port = np.zeros((f.size, 10))
+        gain = np.zeros((f.size, 3, 360))
+        v = complex(voltage)
+        self._check(self._L.frico_antenna_sweep(self._h, _p(f), f.size, _modes(mode), 0, _p(ifa, C.c_int32),
+                                                len(ifa), v.real, v.imag, R, Z0, int(force_symmetry),
+                                                int(force_reorient), _form(form), tol, maxit, _p(port), _p(gain)))
+        out = []
+        for i in range(f.size):
+            p = port[i]
+            out.append({"current": complex(p[0], p[1]), "power": complex(p[2], p[3]),
+                        "impedance": complex(p[4], p[5]), "gamma": complex(p[6], p[7]), "swr": p[8],
+                        "residual": p[9], "gain": gain[i]})
+        return out
+
+    def bistatic_sweep(self, freqs, theta_r, phi_r, eth=1.0, eph=1.0, mode="quad", R=10.0, form="c128",
+                       tol=1e-12, maxit=10):
+        f = np.atleast_1d(np.asarray(freqs, np.float64))
+        tab = np.zeros((f.size, 359, 4))
+        self._check(self._L.frico_bistatic_sweep(self._h, _p(f), f.size, _modes(mode), theta_r, phi_r, eth, eph,
+                                                 R, _form(form), tol, maxit, _p(tab)))
+        return tab
+
+    def monostatic_sweep(self, freqs, theta, phi, eth=1.0, eph=0.0, mode="quad", R=10.0, form="c128",
+                         tol=1e-10, maxit=10):
+        f = np.atleast_1d(np.asarray(freqs, np.float64))
+        th = np.ascontiguousarray(theta, np.float64)
+        ph = np.ascontiguousarray(np.broadcast_to(np.asarray(phi, np.float64), th.shape))
+        sig = np.zeros((f.size, th.size, 2))
+        infos = (_lib.frico_solve_info * f.size)()
+        self._check(self._L.frico_monostatic_sweep(self._h, _p(f), f.size, _modes(mode), th.size, _p(th), _p(ph),
+                                                   eth, eph, R, _form(form), tol, maxit, _p(sig), infos))
+        return sig, [SolveInfo(i.iterations, bool(i.converged), i.max_rel_residual) for i in infos]

@@ -1,0 +1,1237 @@
+// C ABI of the MI355X EFIE library: context, mesh ingestion, fill, RHS, factor/solve, fields.
+#include "../../include/frico_gpu.h"
+
+#include "common.hpp"
+#include "lu.hpp"
+#include "mesh_dev.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <list>
+#include <string>
+#include <vector>
+
+#include "dunavant_rules.inc"
+
+namespace {
+thread_local std::string g_err;
+}
+
+namespace frico {
+
+void set_error(const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int hip_fail(hipError_t e, const char* what, const char* file, int line)
+{
+    set_error("HIP error %d (%s) in %s at %s:%d", (int)e, hipGetErrorString(e), what, file, line);
+    return e == hipErrorOutOfMemory ? FRICO_ERR_NOMEM : FRICO_ERR_HIP;
+}
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t count)
+    {
+        if (count <= n && p)
+            return 0;
+        release();
+        if (count == 0)
+            return 0;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+            return FRICO_ERR_NOMEM;
+        }
+        n = count;
+        return 0;
+    }
+    int upload(const T* h, size_t count, hipStream_t st)
+    {
+        int rc = ensure(count);
+        if (rc)
+            return rc;
+        if (count)
+            FRICO_HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+        return 0;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+namespace {
+
+struct vec3 {
+    double x, y, z;
+};
+inline vec3 operator+(vec3 a, vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline vec3 operator-(vec3 a, vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline vec3 operator*(vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline vec3 operator/(vec3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline vec3 cross(vec3 a, vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double norm(vec3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+struct QP {
+    vec3 p;
+    double w;
+};
+
+// Heron's formula in Kahan's stable ordering (src/quadratures.cpp:41-57).
+double kahan_area(vec3 p0, vec3 p1, vec3 p2)
+{
+    std::array<double, 3> l{norm(p1 - p0), norm(p2 - p0), norm(p2 - p1)};
+    std::sort(l.begin(), l.end());
+    const double a = l[2], b = l[1], c = l[0];
+    return 0.25 * std::sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)));
+}
+
+// Symmetric Dunavant rule on a triangle: each orbit expands to 1, 3 or 6 barycentric
+// permutations in a fixed order; weights are scaled by the triangle area
+// (src/quadratures.cpp:321-372).
+void dunavant(int degree, vec3 p0, vec3 p1, vec3 p2, std::vector<QP>& out)
+{
+    const int rule = degree == 0 ? 0 : degree - 1;
+    const double area = kahan_area(p0, p1, p2);
+    auto at = [&](double a, double b, double c) { return p0 * a + p1 * b + p2 * c; };
+    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++) {
+        const auto& d = frico_dunavant_orbits[o];
+        const double a = d.a, b = d.b, c = d.c, w = d.w * area;
+        if (d.orbit == 1) {
+            out.push_back({at(a, b, c), w});
+        } else if (d.orbit == 3) {
+            out.push_back({at(a, b, c), w});
+            out.push_back({at(b, c, a), w});
+            out.push_back({at(b, a, c), w});
+        } else {
+            out.push_back({at(a, b, c), w});
+            out.push_back({at(a, c, b), w});
+            out.push_back({at(b, a, c), w});
+            out.push_back({at(b, c, a), w});
+            out.push_back({at(c, a, b), w});
+            out.push_back({at(c, b, a), w});
+        }
+    }
+}
+
+struct Neighbours {
+    int64_t tm = -1, tp = -1;
+    int lm = -1, lp = -1;
+    bool has_iface = false;
+    int iface = 0;
+};
+
+} // namespace
+
+struct HostMesh {
+    int64_t nv = 0, nt = 0;
+    std::vector<vec3> V;
+    std::vector<std::array<int64_t, 3>> T;
+    std::vector<int> tag;
+    std::vector<std::pair<int64_t, int64_t>> edges;
+    std::vector<frico_rwg> bf;
+    int degree = -1, q = 0;
+    // device-side arrays prepared on the host
+    std::vector<double> cen, area, sqrtA, alpha, tri_sgn, rwg_len, rwg_Am, rwg_Ap;
+    std::vector<double4> qreg, qsub;
+    std::vector<int> tri_rwg, rwg_t, rwg_le;
+    std::vector<int2> near;
+};
+
+} // namespace frico
+
+using namespace frico;
+
+struct frico_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int nb = 64;
+    int max_groups = 0;
+    bool profiling = false;
+    bool have_mesh = false;
+    HostMesh hm;
+    int64_t N = 0;
+    // device mesh
+    DevBuf<double> d_cen, d_area, d_sqrtA, d_alpha, d_tri_sgn, d_rwg_len, d_rwg_Am, d_rwg_Ap;
+    DevBuf<double4> d_qreg, d_qsub;
+    DevBuf<int> d_tri_rwg, d_rwg_t, d_rwg_le;
+    DevBuf<int2> d_near;
+    DevMesh dm{};
+    // matrix state
+    DevBuf<c128> Z64;
+    DevBuf<c64> Z32, LU32;
+    DevBuf<unsigned char> Cbuf;
+    int zprec = -1;        // precision of the current Z (-1: none)
+    bool z64_factored = false;
+    int factored = -1;     // form of the current factorisation (-1: none)
+    int nb_factored = 64;  // block width the factors were computed with
+    DevBuf<int> ipiv;
+    // lu workspace
+    DevBuf<unsigned> counters;
+    DevBuf<int> flags;
+    DevBuf<double> cand_val;
+    DevBuf<int> cand_idx;
+    DevBuf<unsigned char> cand_row, diag_row;
+    // rhs / solution
+    int64_t nrhs = 0;
+    bool have_x = false;
+    DevBuf<c128> B, X, R;
+    DevBuf<c64> D32;
+    DevBuf<double> norms;
+    // fields
+    DevBuf<c128> TS, EH;
+    DevBuf<double> pts;
+    DevBuf<int2> req;
+    DevBuf<PlaneWaveDev> waves;
+    // profiling
+    KernelTimer t_fill, t_rhs, t_factor, t_solve, t_refine, t_gemm, t_tripair;
+};
+
+namespace {
+
+int check_ctx(frico_ctx* c)
+{
+    if (!c) {
+        set_error("null context");
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int set_dev(frico_ctx* c)
+{
+    FRICO_HIP_CHECK(hipSetDevice(c->device));
+    return 0;
+}
+
+void phase_begin(frico_ctx* c, KernelTimer& t)
+{
+    if (c->profiling)
+        t.begin(c->st);
+}
+void phase_end(frico_ctx* c, KernelTimer& t)
+{
+    if (c->profiling)
+        t.end(c->st, 0.0);
+}
+
+int ensure_lu_work(frico_ctx* c)
+{
+    const int64_t npan = (c->N + 31) / 32 + 1;
+    int rc;
+    if ((rc = c->counters.ensure((size_t)npan)))
+        return rc;
+    if ((rc = c->flags.ensure(2)))
+        return rc;
+    if ((rc = c->cand_val.ensure(2 * LU_MAX_GROUPS)))
+        return rc;
+    if ((rc = c->cand_idx.ensure(2 * LU_MAX_GROUPS)))
+        return rc;
+    if ((rc = c->cand_row.ensure((size_t)2 * LU_MAX_GROUPS * LU_NB_MAX * sizeof(c128))))
+        return rc;
+    if ((rc = c->diag_row.ensure((size_t)2 * LU_NB_MAX * sizeof(c128))))
+        return rc;
+    return 0;
+}
+
+LuWork lu_work(frico_ctx* c)
+{
+    LuWork w;
+    w.counters = c->counters.p;
+    w.ncounters = (int64_t)c->counters.n;
+    w.flags = c->flags.p;
+    w.cand_val = c->cand_val.p;
+    w.cand_idx = c->cand_idx.p;
+    w.cand_row = c->cand_row.p;
+    w.diag_row = c->diag_row.p;
+    w.max_groups = std::min(c->max_groups, LU_MAX_GROUPS);
+    return w;
+}
+
+double kwave(double freq, double* inv_ksq, double* omega_out)
+{
+    const double omega = 2.0 * PI * freq;
+    if (inv_ksq)
+        *inv_ksq = 1. / (omega * omega * MU0 * EPS0);
+    if (omega_out)
+        *omega_out = omega;
+    return omega * std::sqrt(MU0 * EPS0);
+}
+
+} // namespace
+
+extern "C" {
+
+const char* frico_last_error(void) { return g_err.c_str(); }
+
+int frico_ctx_create(int device, frico_ctx** out)
+{
+    g_err.clear();
+    if (!out)
+        return set_error("null output pointer"), -2;
+    int ndev = 0;
+    FRICO_HIP_CHECK(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return set_error("device %d not present (%d devices)", device, ndev), -1;
+    auto* c = new frico_ctx;
+    c->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        return hip_fail(e, "stream create", __FILE__, __LINE__);
+    }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
+    }
+    *out = c;
+    return 0;
+}
+
+int frico_ctx_destroy(frico_ctx* c)
+{
+    if (!c)
+        return 0;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->st);
+    (void)hipStreamDestroy(c->st);
+    delete c;
+    return 0;
+}
+
+int frico_ctx_synchronize(frico_ctx* c)
+{
+    if (check_ctx(c))
+        return -1;
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int frico_ctx_set_profiling(frico_ctx* c, int enable)
+{
+    if (check_ctx(c))
+        return -1;
+    c->profiling = enable != 0;
+    for (KernelTimer* t : {&c->t_fill, &c->t_rhs, &c->t_factor, &c->t_solve, &c->t_refine, &c->t_gemm, &c->t_tripair}) {
+        t->reset();
+        t->enabled = c->profiling;
+    }
+    return 0;
+}
+
+int frico_ctx_get_profile(frico_ctx* c, frico_profile* out)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!out)
+        return set_error("null output"), -2;
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    for (KernelTimer* t : {&c->t_fill, &c->t_rhs, &c->t_factor, &c->t_solve, &c->t_refine, &c->t_gemm, &c->t_tripair})
+        t->collect();
+    out->fill_ms = c->t_fill.total_ms;
+    out->rhs_ms = c->t_rhs.total_ms;
+    out->factor_ms = c->t_factor.total_ms;
+    out->solve_ms = c->t_solve.total_ms;
+    out->refine_ms = c->t_refine.total_ms;
+    out->gemm_ms = c->t_gemm.total_ms;
+    out->gemm_flops = c->t_gemm.total_work;
+    out->gemm_launches = c->t_gemm.launches;
+    out->tripair_ms = c->t_tripair.total_ms;
+    out->tripair_pairs = c->t_tripair.total_work;
+    out->tripair_launches = c->t_tripair.launches;
+    return 0;
+}
+
+int frico_ctx_set_block(frico_ctx* c, int nb)
+{
+    if (check_ctx(c))
+        return -1;
+    if (nb != 32 && nb != 64 && nb != 128)
+        return set_error("block size must be 32, 64 or 128"), -2;
+    c->nb = nb;
+    return 0;
+}
+
+int frico_mesh_load(frico_ctx* c, const double* vertices, int64_t nv, const int64_t* triangles, const int32_t* tri_tags,
+                    int64_t nt, const int64_t* iface_edges, const int32_t* iface_tags, int64_t nif, int degree)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!vertices || nv < 3)
+        return set_error("need at least 3 vertices"), -2;
+    if (!triangles || nt < 1)
+        return set_error("need at least one triangle"), -4;
+    if (nif < 0 || (nif > 0 && !iface_edges))
+        return set_error("bad interface edge list"), -7;
+    if (degree < 0 || degree > FRICO_DUNAVANT_MAX_DEGREE)
+        return set_error("Dunavant quadrature: degree %d out of range 0..%d", degree, FRICO_DUNAVANT_MAX_DEGREE), -10;
+    if (nt > (int64_t)1 << 30)
+        return set_error("too many triangles"), FRICO_ERR_LIMIT;
+    int rc = set_dev(c);
+    if (rc)
+        return rc;
+    HostMesh hm;
+    hm.nv = nv;
+    hm.nt = nt;
+    hm.V.resize(nv);
+    for (int64_t i = 0; i < nv; i++)
+        hm.V[i] = {vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2]};
+    hm.T.resize(nt);
+    hm.tag.resize(nt);
+    std::vector<std::pair<int64_t, int64_t>> all;
+    all.reserve(3 * nt);
+    auto mk = [](int64_t a, int64_t b) { return std::make_pair(std::min(a, b), std::max(a, b)); };
+    for (int64_t t = 0; t < nt; t++) {
+        for (int k = 0; k < 3; k++) {
+            const int64_t v = triangles[3 * t + k];
+            if (v < 0 || v >= nv)
+                return set_error("triangle %lld references vertex %lld", (long long)t, (long long)v), -3;
+            hm.T[t][k] = v;
+        }
+        hm.tag[t] = tri_tags ? tri_tags[t] : 0;
+        all.push_back(mk(hm.T[t][0], hm.T[t][1]));
+        all.push_back(mk(hm.T[t][1], hm.T[t][2]));
+        all.push_back(mk(hm.T[t][2], hm.T[t][0]));
+    }
+    // sorted unique edge list (src/input_gmsh.cpp:136-140)
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    hm.edges = std::move(all);
+    auto offset = [&](std::pair<int64_t, int64_t> e) -> int64_t {
+        auto it = std::lower_bound(hm.edges.begin(), hm.edges.end(), e);
+        return (it != hm.edges.end() && *it == e) ? (int64_t)(it - hm.edges.begin()) : -1;
+    };
+    std::vector<Neighbours> nb(hm.edges.size());
+    std::vector<std::pair<int64_t, int>> ifl;
+    for (int64_t b = 0; b < nif; b++) {
+        const int64_t o = offset(mk(iface_edges[2 * b], iface_edges[2 * b + 1]));
+        if (o < 0)
+            return set_error("lonely edge: 1D entity %d edge (%lld, %lld) is not on any triangle",
+                             iface_tags ? iface_tags[b] : 0, (long long)iface_edges[2 * b], (long long)iface_edges[2 * b + 1]),
+                   FRICO_ERR_MESH;
+        ifl.push_back({o, iface_tags ? iface_tags[b] : 0});
+    }
+    // T- = first (lower) triangle, T+ = second; a third one is an error (src/input_gmsh.cpp:216-262)
+    for (int64_t t = 0; t < nt; t++) {
+        const auto& tr = hm.T[t];
+        const std::pair<int64_t, int64_t> es[3] = {mk(tr[0], tr[1]), mk(tr[1], tr[2]), mk(tr[2], tr[0])};
+        for (int ie = 0; ie < 3; ie++) {
+            Neighbours& en = nb[offset(es[ie])];
+            if (en.tp >= 0)
+                return set_error("edge shared by more than two triangles (tags %d, %d and %d): no RWG basis",
+                                 hm.tag[en.tm], hm.tag[en.tp], hm.tag[t]),
+                       FRICO_ERR_MESH;
+            if (en.tm >= 0) {
+                en.tp = t;
+                en.lp = ie;
+            } else {
+                en.tm = t;
+                en.lm = ie;
+            }
+        }
+    }
+    for (auto& [o, tg] : ifl)
+        if (nb[o].tp >= 0) {
+            nb[o].has_iface = true;
+            nb[o].iface = tg;
+        }
+    auto tri_area = [&](int64_t t) {
+        const auto& tr = hm.T[t];
+        return 0.5 * norm(cross(hm.V[tr[1]] - hm.V[tr[0]], hm.V[tr[2]] - hm.V[tr[1]]));
+    };
+    const int evmap[3] = {2, 0, 1};
+    for (size_t ie = 0; ie < nb.size(); ie++) {
+        const Neighbours& en = nb[ie];
+        if (en.tp < 0)
+            continue;
+        frico_rwg b{};
+        b.itminus = en.tm;
+        b.itplus = en.tp;
+        b.loc_eminus = en.lm;
+        b.loc_eplus = en.lp;
+        b.edge_index = (int64_t)ie;
+        b.has_interface = en.has_iface;
+        b.interface_tag = en.iface;
+        const vec3 d = hm.V[hm.edges[ie].second] - hm.V[hm.edges[ie].first];
+        b.length = std::hypot(d.x, d.y);
+        b.Aminus = tri_area(en.tm);
+        b.Aplus = tri_area(en.tp);
+        const vec3 pm = hm.V[hm.T[en.tm][evmap[en.lm]]], pp = hm.V[hm.T[en.tp][evmap[en.lp]]];
+        b.pminus[0] = pm.x;
+        b.pminus[1] = pm.y;
+        b.pminus[2] = pm.z;
+        b.pplus[0] = pp.x;
+        b.pplus[1] = pp.y;
+        b.pplus[2] = pp.z;
+        hm.bf.push_back(b);
+    }
+    const int64_t N = (int64_t)hm.bf.size();
+    if (N < 1)
+        return set_error("mesh has no internal edge: no unknowns"), FRICO_ERR_MESH;
+    if (N > 65535)
+        return set_error("N = %lld exceeds the 65535-unknown launch limit", (long long)N), FRICO_ERR_LIMIT;
+
+    // per-triangle data
+    hm.degree = degree;
+    hm.q = frico_dunavant_npoints[degree == 0 ? 0 : degree - 1];
+    hm.cen.resize(3 * nt);
+    hm.area.resize(nt);
+    hm.sqrtA.resize(nt);
+    hm.alpha.resize(9 * nt);
+    hm.tri_rwg.assign(3 * nt, -1);
+    hm.tri_sgn.assign(3 * nt, 0.0);
+    hm.qreg.clear();
+    hm.qsub.clear();
+    hm.qreg.reserve(nt * hm.q);
+    hm.qsub.reserve(nt * 3 * hm.q);
+    std::vector<QP> qp;
+    for (int64_t t = 0; t < nt; t++) {
+        const auto& tr = hm.T[t];
+        const vec3 p0 = hm.V[tr[0]], p1 = hm.V[tr[1]], p2 = hm.V[tr[2]];
+        const vec3 c = (p0 + p1 + p2) / 3.0; // barycenter (src/geom_mesh.cpp:64-69)
+        hm.cen[3 * t] = c.x;
+        hm.cen[3 * t + 1] = c.y;
+        hm.cen[3 * t + 2] = c.z;
+        hm.area[t] = tri_area(t);
+        hm.sqrtA[t] = std::sqrt(hm.area[t]);
+        for (int m = 0; m < 3; m++) {
+            const vec3 a = c - hm.V[tr[evmap[m]]];
+            hm.alpha[9 * t + 3 * m] = a.x;
+            hm.alpha[9 * t + 3 * m + 1] = a.y;
+            hm.alpha[9 * t + 3 * m + 2] = a.z;
+        }
+        qp.clear();
+        dunavant(degree, p0, p1, p2, qp);
+        for (auto& x : qp)
+            hm.qreg.push_back(double4{x.p.x - c.x, x.p.y - c.y, x.p.z - c.z, x.w});
+        qp.clear();
+        // sub-triangle rule (src/quadratures.cpp:2685-2702)
+        dunavant(degree, p0, p1, c, qp);
+        dunavant(degree, p1, p2, c, qp);
+        dunavant(degree, p2, p0, c, qp);
+        for (auto& x : qp)
+            hm.qsub.push_back(double4{x.p.x - c.x, x.p.y - c.y, x.p.z - c.z, x.w});
+    }
+    hm.rwg_t.resize(2 * N);
+    hm.rwg_le.resize(2 * N);
+    hm.rwg_len.resize(N);
+    hm.rwg_Am.resize(N);
+    hm.rwg_Ap.resize(N);
+    for (int64_t n = 0; n < N; n++) {
+        const frico_rwg& b = hm.bf[n];
+        hm.rwg_t[2 * n] = (int)b.itminus;
+        hm.rwg_t[2 * n + 1] = (int)b.itplus;
+        hm.rwg_le[2 * n] = b.loc_eminus;
+        hm.rwg_le[2 * n + 1] = b.loc_eplus;
+        hm.rwg_len[n] = b.length;
+        hm.rwg_Am[n] = b.Aminus;
+        hm.rwg_Ap[n] = b.Aplus;
+        hm.tri_rwg[3 * b.itminus + b.loc_eminus] = (int)n;
+        hm.tri_sgn[3 * b.itminus + b.loc_eminus] = -1.0;
+        hm.tri_rwg[3 * b.itplus + b.loc_eplus] = (int)n;
+        hm.tri_sgn[3 * b.itplus + b.loc_eplus] = 1.0;
+    }
+    // pairs (i, j) whose row function j has a triangle of the column function i
+    hm.near.clear();
+    for (int64_t i = 0; i < N; i++) {
+        int js[6], nj = 0;
+        for (int s = 0; s < 2; s++) {
+            const int t = hm.rwg_t[2 * i + s];
+            for (int e = 0; e < 3; e++) {
+                const int j = hm.tri_rwg[3 * t + e];
+                if (j < 0)
+                    continue;
+                bool dup = false;
+                for (int x = 0; x < nj; x++)
+                    dup |= js[x] == j;
+                if (!dup)
+                    js[nj++] = j;
+            }
+        }
+        std::sort(js, js + nj);
+        for (int x = 0; x < nj; x++)
+            hm.near.push_back(int2{(int)i, js[x]});
+    }
+
+    // upload
+    hipStream_t st = c->st;
+    if ((rc = c->d_cen.upload(hm.cen.data(), hm.cen.size(), st)) || (rc = c->d_area.upload(hm.area.data(), hm.area.size(), st)) ||
+        (rc = c->d_sqrtA.upload(hm.sqrtA.data(), hm.sqrtA.size(), st)) ||
+        (rc = c->d_alpha.upload(hm.alpha.data(), hm.alpha.size(), st)) ||
+        (rc = c->d_tri_sgn.upload(hm.tri_sgn.data(), hm.tri_sgn.size(), st)) ||
+        (rc = c->d_rwg_len.upload(hm.rwg_len.data(), hm.rwg_len.size(), st)) ||
+        (rc = c->d_rwg_Am.upload(hm.rwg_Am.data(), hm.rwg_Am.size(), st)) ||
+        (rc = c->d_rwg_Ap.upload(hm.rwg_Ap.data(), hm.rwg_Ap.size(), st)) ||
+        (rc = c->d_qreg.upload(hm.qreg.data(), hm.qreg.size(), st)) ||
+        (rc = c->d_qsub.upload(hm.qsub.data(), hm.qsub.size(), st)) ||
+        (rc = c->d_tri_rwg.upload(hm.tri_rwg.data(), hm.tri_rwg.size(), st)) ||
+        (rc = c->d_rwg_t.upload(hm.rwg_t.data(), hm.rwg_t.size(), st)) ||
+        (rc = c->d_rwg_le.upload(hm.rwg_le.data(), hm.rwg_le.size(), st)) ||
+        (rc = c->d_near.upload(hm.near.data(), hm.near.size(), st)))
+        return rc;
+    FRICO_HIP_CHECK(hipStreamSynchronize(st));
+    c->hm = std::move(hm);
+    c->N = N;
+    c->have_mesh = true;
+    DevMesh& m = c->dm;
+    m.T = (int)nt;
+    m.N = (int)N;
+    m.q = c->hm.q;
+    m.cen = c->d_cen.p;
+    m.area = c->d_area.p;
+    m.sqrtA = c->d_sqrtA.p;
+    m.alpha = c->d_alpha.p;
+    m.qreg = c->d_qreg.p;
+    m.qsub = c->d_qsub.p;
+    m.tri_rwg = c->d_tri_rwg.p;
+    m.tri_sgn = c->d_tri_sgn.p;
+    m.rwg_t = c->d_rwg_t.p;
+    m.rwg_le = c->d_rwg_le.p;
+    m.rwg_len = c->d_rwg_len.p;
+    m.rwg_Am = c->d_rwg_Am.p;
+    m.rwg_Ap = c->d_rwg_Ap.p;
+    c->zprec = -1;
+    c->factored = -1;
+    c->nrhs = 0;
+    c->have_x = false;
+    return 0;
+}
+
+int64_t frico_num_unknowns(const frico_ctx* c) { return c ? c->N : -1; }
+int64_t frico_num_triangles(const frico_ctx* c) { return c && c->have_mesh ? c->hm.nt : -1; }
+int64_t frico_num_rhs(const frico_ctx* c) { return c ? c->nrhs : -1; }
+
+int frico_rwg_table(const frico_ctx* c, frico_rwg* out)
+{
+    if (!c || !c->have_mesh)
+        return set_error("no mesh loaded"), FRICO_ERR_STATE;
+    if (!out)
+        return set_error("null output"), -2;
+    std::memcpy(out, c->hm.bf.data(), c->hm.bf.size() * sizeof(frico_rwg));
+    return 0;
+}
+
+int frico_reorient_deltagap(frico_ctx* c, const int32_t* ifaces, int nif, int force, double* signs, int* applied)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_mesh)
+        return set_error("no mesh loaded"), FRICO_ERR_STATE;
+    if (!signs || !applied)
+        return set_error("null output"), -5;
+    const HostMesh& hm = c->hm;
+    for (int64_t n = 0; n < c->N; n++)
+        signs[n] = 1.0;
+    *applied = 0;
+    // gap functions and whether T+ does not come after T- in surface-tag order
+    bool fix = false;
+    std::list<std::pair<std::pair<int64_t, int64_t>, int64_t>> edges;
+    for (int64_t n = 0; n < c->N; n++) {
+        const frico_rwg& b = hm.bf[n];
+        if (!b.has_interface)
+            continue;
+        for (int f = 0; f < nif; f++)
+            if (b.interface_tag == ifaces[f]) {
+                if (hm.tag[b.itplus] <= hm.tag[b.itminus])
+                    fix = true;
+                edges.push_back({hm.edges[b.edge_index], n});
+            }
+    }
+    if (!(fix || force))
+        return 0;
+    if (edges.empty())
+        return set_error("no edges identified for the delta-gap"), -2;
+    // chain the gap edges end to end (src/emw_solver.cpp:271-315)
+    struct Seg {
+        int64_t a, b, n;
+    };
+    std::list<Seg> chain;
+    chain.push_back({edges.front().first.first, edges.front().first.second, edges.front().second});
+    edges.pop_front();
+    while (!edges.empty()) {
+        const size_t before = edges.size();
+        for (auto it = edges.begin(); it != edges.end();) {
+            const Seg f = chain.front(), bk = chain.back();
+            const int64_t e0 = it->first.first, e1 = it->first.second, n = it->second;
+            if (e0 == f.a)
+                chain.push_front({e1, e0, n});
+            else if (e1 == f.a)
+                chain.push_front({e0, e1, n});
+            else if (e0 == bk.b)
+                chain.push_back({e0, e1, n});
+            else if (e1 == bk.b)
+                chain.push_back({e1, e0, n});
+            else {
+                ++it;
+                continue;
+            }
+            it = edges.erase(it);
+        }
+        if (edges.size() == before)
+            return set_error("delta-gap edges are not topologically connected"), -2;
+    }
+    auto tri_bar = [&](int64_t t) {
+        const auto& tr = hm.T[t];
+        return (hm.V[tr[0]] + hm.V[tr[1]] + hm.V[tr[2]]) / 3.0;
+    };
+    auto edge_bar = [&](int64_t e) { return (hm.V[hm.edges[e].first] + hm.V[hm.edges[e].second]) / 2.0; };
+    const frico_rwg& b0 = hm.bf[chain.front().n];
+    vec3 dir = tri_bar(b0.itplus) - edge_bar(b0.edge_index);
+    double sign = 1;
+    for (const Seg& s : chain) {
+        const frico_rwg& b = hm.bf[s.n];
+        const vec3 cur = tri_bar(b.itplus) - edge_bar(b.edge_index);
+        if (dot(dir, cur) < 0)
+            sign *= -1;
+        signs[s.n] = sign;
+        dir = cur;
+    }
+    *applied = 1;
+    return 0;
+}
+
+int frico_fill(frico_ctx* c, double freq, int mode, int prec)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_mesh)
+        return set_error("no mesh loaded"), FRICO_ERR_STATE;
+    if (!(freq > 0))
+        return set_error("frequency must be positive"), -2;
+    if (mode != FRICO_FILL_QUAD && mode != FRICO_FILL_APPROX)
+        return set_error("unknown fill mode %d", mode), -3;
+    if (prec != FRICO_C128 && prec != FRICO_C64)
+        return set_error("unknown precision %d", prec), -4;
+    int rc = set_dev(c);
+    if (rc)
+        return rc;
+    const int64_t N = c->N, T = c->hm.nt;
+    void* Z = nullptr;
+    if (prec == FRICO_C128) {
+        if ((rc = c->Z64.ensure((size_t)(N * N))))
+            return rc;
+        Z = c->Z64.p;
+    } else {
+        if ((rc = c->Z32.ensure((size_t)(N * N))))
+            return rc;
+        Z = c->Z32.p;
+    }
+    if (mode == FRICO_FILL_QUAD) {
+        const size_t csz = (size_t)9 * T * T * (prec == FRICO_C128 ? sizeof(c128) : sizeof(c64));
+        if ((rc = c->Cbuf.ensure(csz)))
+            return rc;
+    }
+    double inv_ksq;
+    const double k = kwave(freq, &inv_ksq, nullptr);
+    phase_begin(c, c->t_fill);
+    if (mode == FRICO_FILL_QUAD)
+        rc = launch_fill_quad(c->dm, k, inv_ksq, prec, c->Cbuf.p, Z, N, c->d_near.p, (int64_t)c->hm.near.size(), c->st,
+                              c->profiling ? &c->t_tripair : nullptr);
+    else
+        rc = launch_fill_approx(c->dm, k, inv_ksq, prec, Z, N, c->st);
+    phase_end(c, c->t_fill);
+    if (rc)
+        return rc;
+    c->zprec = prec;
+    c->z64_factored = false;
+    c->factored = -1;
+    return 0;
+}
+
+int frico_symmetrize(frico_ctx* c)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->zprec != FRICO_C128 || c->z64_factored)
+        return set_error("symmetrize needs an unfactored complex128 Z"), FRICO_ERR_STATE;
+    return launch_symmetrize(c->Z64.p, c->N, c->N, c->st);
+}
+
+int frico_matrix_get(frico_ctx* c, double* Zh, int64_t ldz)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->zprec < 0)
+        return set_error("no matrix"), FRICO_ERR_STATE;
+    if (!Zh || ldz < c->N)
+        return set_error("bad output / leading dimension"), -3;
+    const int64_t N = c->N;
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    if (c->zprec == FRICO_C128) {
+        FRICO_HIP_CHECK(hipMemcpy2D(Zh, ldz * sizeof(c128), c->Z64.p, N * sizeof(c128), N * sizeof(c128), N,
+                                    hipMemcpyDeviceToHost));
+    } else {
+        std::vector<c64> tmp((size_t)(N * N));
+        FRICO_HIP_CHECK(hipMemcpy(tmp.data(), c->Z32.p, tmp.size() * sizeof(c64), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; i++)
+            for (int64_t j = 0; j < N; j++) {
+                Zh[2 * (j + i * ldz)] = tmp[j + i * N].re;
+                Zh[2 * (j + i * ldz) + 1] = tmp[j + i * N].im;
+            }
+    }
+    return 0;
+}
+
+int frico_matrix_set(frico_ctx* c, const double* Zh, int64_t n, int64_t ldz)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->have_mesh && n != c->N)
+        return set_error("n must equal the number of unknowns of the loaded mesh"), -3;
+    if (!Zh || n < 1 || ldz < n)
+        return set_error("bad matrix arguments"), -2;
+    int rc = set_dev(c);
+    if (rc)
+        return rc;
+    if (!c->have_mesh)
+        c->N = n;
+    if ((rc = c->Z64.ensure((size_t)(n * n))))
+        return rc;
+    FRICO_HIP_CHECK(hipMemcpy2DAsync(c->Z64.p, n * sizeof(c128), Zh, ldz * sizeof(c128), n * sizeof(c128), n,
+                                     hipMemcpyHostToDevice, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    c->zprec = FRICO_C128;
+    c->z64_factored = false;
+    c->factored = -1;
+    return 0;
+}
+
+static int rhs_alloc(frico_ctx* c, int64_t nrhs)
+{
+    int rc;
+    const size_t sz = (size_t)(c->N * nrhs);
+    if ((rc = c->B.ensure(sz)) || (rc = c->X.ensure(sz)))
+        return rc;
+    c->nrhs = nrhs;
+    c->have_x = false;
+    return 0;
+}
+
+int frico_rhs_set(frico_ctx* c, const double* Bh, int64_t ldb, int64_t nrhs)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->N < 1)
+        return set_error("no system size"), FRICO_ERR_STATE;
+    if (!Bh || ldb < c->N || nrhs < 1)
+        return set_error("bad rhs arguments"), -2;
+    int rc = set_dev(c);
+    if (rc || (rc = rhs_alloc(c, nrhs)))
+        return rc;
+    FRICO_HIP_CHECK(hipMemcpy2DAsync(c->B.p, c->N * sizeof(c128), Bh, ldb * sizeof(c128), c->N * sizeof(c128), nrhs,
+                                     hipMemcpyHostToDevice, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int frico_rhs_get(frico_ctx* c, double* Bh, int64_t ldb)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->nrhs < 1)
+        return set_error("no rhs"), FRICO_ERR_STATE;
+    if (!Bh || ldb < c->N)
+        return set_error("bad output"), -2;
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    FRICO_HIP_CHECK(hipMemcpy2D(Bh, ldb * sizeof(c128), c->B.p, c->N * sizeof(c128), c->N * sizeof(c128), c->nrhs,
+                                hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int frico_rhs_deltagap(frico_ctx* c, double freq, const int32_t* ifaces, int nif, double v_re, double v_im,
+                       const double* signs)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_mesh)
+        return set_error("no mesh loaded"), FRICO_ERR_STATE;
+    if (!(freq > 0))
+        return set_error("frequency must be positive"), -2;
+    if (nif < 0 || (nif > 0 && !ifaces))
+        return set_error("bad interface list"), -3;
+    int rc = set_dev(c);
+    if (rc || (rc = rhs_alloc(c, 1)))
+        return rc;
+    const double omega = 2.0 * PI * freq;
+    std::vector<c128> V((size_t)c->N, c128{0, 0});
+    for (int64_t n = 0; n < c->N; n++) {
+        const frico_rwg& b = c->hm.bf[n];
+        if (!b.has_interface)
+            continue;
+        for (int f = 0; f < nif; f++)
+            if (b.interface_tag == ifaces[f]) {
+                // v = (0, -l/(omega mu0)); V += v * voltage * sign
+                const double vi = -b.length / (omega * MU0);
+                const double s = signs ? signs[n] : 1.0;
+                V[n].re += (-vi * v_im) * s;
+                V[n].im += (vi * v_re) * s;
+            }
+    }
+    phase_begin(c, c->t_rhs);
+    FRICO_HIP_CHECK(hipMemcpyAsync(c->B.p, V.data(), V.size() * sizeof(c128), hipMemcpyHostToDevice, c->st));
+    phase_end(c, c->t_rhs);
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int frico_rhs_planewave(frico_ctx* c, double freq, int64_t nang, const double* theta, const double* phi,
+                        const double* eth, const double* eph)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_mesh)
+        return set_error("no mesh loaded"), FRICO_ERR_STATE;
+    if (!(freq > 0))
+        return set_error("frequency must be positive"), -2;
+    if (nang < 1 || nang > 65535 || !theta || !phi || !eth || !eph)
+        return set_error("bad incidence arrays"), -3;
+    int rc = set_dev(c);
+    if (rc || (rc = rhs_alloc(c, nang)))
+        return rc;
+    std::vector<PlaneWaveDev> w((size_t)nang);
+    for (int64_t a = 0; a < nang; a++) {
+        const double th = theta[a], ph = phi[a];
+        const double dir[3] = {std::sin(th) * std::cos(ph), std::sin(th) * std::sin(ph), std::cos(th)};
+        const double tht[3] = {std::cos(th) * std::cos(ph), std::cos(th) * std::sin(ph), -std::sin(th)};
+        const double pht[3] = {-std::sin(ph), std::cos(ph), 0.0};
+        for (int d = 0; d < 3; d++) {
+            w[a].dir[d] = dir[d];
+            w[a].e0r[d] = eth[2 * a] * tht[d] + eph[2 * a] * pht[d];
+            w[a].e0i[d] = eth[2 * a + 1] * tht[d] + eph[2 * a + 1] * pht[d];
+        }
+    }
+    if ((rc = c->waves.upload(w.data(), w.size(), c->st)))
+        return rc;
+    double omega;
+    const double k = kwave(freq, nullptr, &omega);
+    phase_begin(c, c->t_rhs);
+    rc = launch_rhs_planewave(c->dm, k, omega * MU0, c->waves.p, (int)nang, c->B.p, c->N, c->st);
+    phase_end(c, c->t_rhs);
+    return rc;
+}
+
+static int read_lu_flags(frico_ctx* c)
+{
+    int h[2] = {0, 0};
+    FRICO_HIP_CHECK(hipMemcpyAsync(h, c->flags.p, sizeof h, hipMemcpyDeviceToHost, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    if (h[0])
+        return set_error("LU panel barrier exceeded its wait limit"), FRICO_ERR_SYNC;
+    if (h[1] > 0)
+        set_error("singular matrix: zero pivot at column %d", h[1]);
+    return h[1];
+}
+
+int frico_factor(frico_ctx* c, int form)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->zprec < 0)
+        return set_error("no matrix to factor"), FRICO_ERR_STATE;
+    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR)
+        return set_error("unknown factorisation form %d", form), -2;
+    int rc = set_dev(c);
+    if (rc || (rc = ensure_lu_work(c)) || (rc = c->ipiv.ensure((size_t)c->N)))
+        return rc;
+    const int64_t N = c->N;
+    LuWork wk = lu_work(c);
+    if (form == FRICO_SOLVE_C128) {
+        if (c->zprec != FRICO_C128 || c->z64_factored)
+            return set_error("complex128 factorisation needs an unfactored complex128 Z"), FRICO_ERR_STATE;
+        phase_begin(c, c->t_factor);
+        rc = lu_factor<double>(wk, c->Z64.p, N, N, c->ipiv.p, c->nb, c->st, c->profiling ? &c->t_gemm : nullptr);
+        phase_end(c, c->t_factor);
+        c->z64_factored = true;
+    } else {
+        if ((rc = c->LU32.ensure((size_t)(N * N))))
+            return rc;
+        if (c->zprec == FRICO_C128 && c->z64_factored)
+            return set_error("Z was overwritten by a complex128 factorisation"), FRICO_ERR_STATE;
+        phase_begin(c, c->t_factor);
+        if (c->zprec == FRICO_C128)
+            rc = launch_convert_c128_c64(c->Z64.p, N, c->LU32.p, N, N, N, c->st);
+        else {
+            const hipError_t e =
+                hipMemcpyAsync(c->LU32.p, c->Z32.p, (size_t)(N * N) * sizeof(c64), hipMemcpyDeviceToDevice, c->st);
+            rc = e == hipSuccess ? 0 : hip_fail(e, "copy Z32", __FILE__, __LINE__);
+        }
+        if (!rc)
+            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb, c->st, c->profiling ? &c->t_gemm : nullptr);
+        phase_end(c, c->t_factor);
+    }
+    if (rc)
+        return rc;
+    c->factored = form;
+    c->nb_factored = c->nb;
+    return read_lu_flags(c);
+}
+
+static int col_norms(frico_ctx* c, const c128* X, int64_t cols, std::vector<double>& out)
+{
+    int rc;
+    if ((rc = c->norms.ensure((size_t)cols)))
+        return rc;
+    if ((rc = launch_col_norm2(X, c->N, c->N, cols, c->norms.p, c->st)))
+        return rc;
+    out.resize((size_t)cols);
+    FRICO_HIP_CHECK(hipMemcpyAsync(out.data(), c->norms.p, (size_t)cols * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    for (auto& v : out)
+        v = std::sqrt(v);
+    return 0;
+}
+
+// R = B - Z X with the kept (unfactored) Z.
+static int residual(frico_ctx* c, KernelTimer* kt)
+{
+    const int64_t N = c->N, nr = c->nrhs;
+    int rc;
+    if ((rc = c->R.ensure((size_t)(N * nr))))
+        return rc;
+    FRICO_HIP_CHECK(hipMemcpyAsync(c->R.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
+    if (c->zprec == FRICO_C128) {
+        if (c->z64_factored)
+            return set_error("residual needs the unfactored Z"), FRICO_ERR_STATE;
+        return gemm_sub<double>(N, nr, N, c->Z64.p, N, c->X.p, N, c->R.p, N, c->st, kt);
+    }
+    return gemm_sub_mixed(N, nr, N, c->Z32.p, N, c->X.p, N, c->R.p, N, c->st, kt);
+}
+
+int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
+{
+    if (check_ctx(c))
+        return -1;
+    if (c->factored < 0)
+        return set_error("solve before factor"), FRICO_ERR_STATE;
+    if (c->nrhs < 1)
+        return set_error("no right-hand side"), FRICO_ERR_STATE;
+    int rc = set_dev(c);
+    if (rc)
+        return rc;
+    const int64_t N = c->N, nr = c->nrhs;
+    frico_solve_info inf{0, 1, -1.0};
+    if (c->factored == FRICO_SOLVE_C128) {
+        phase_begin(c, c->t_solve);
+        FRICO_HIP_CHECK(hipMemcpyAsync(c->X.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
+        rc = lu_solve<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->X.p, N, nr, c->st);
+        phase_end(c, c->t_solve);
+        if (rc)
+            return rc;
+        FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+        c->have_x = true;
+        if (info)
+            *info = inf;
+        return 0;
+    }
+    if (maxit < 1)
+        return set_error("maxit must be >= 1"), -3;
+    if ((rc = c->R.ensure((size_t)(N * nr))) || (rc = c->D32.ensure((size_t)(N * nr))))
+        return rc;
+    std::vector<double> bn, rn;
+    if ((rc = col_norms(c, c->B.p, nr, bn)))
+        return rc;
+    // X = 0, R = B
+    FRICO_HIP_CHECK(hipMemsetAsync(c->X.p, 0, (size_t)(N * nr) * sizeof(c128), c->st));
+    FRICO_HIP_CHECK(hipMemcpyAsync(c->R.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
+    inf.converged = 0;
+    for (int it = 1; it <= maxit; it++) {
+        KernelTimer& tt = it == 1 ? c->t_solve : c->t_refine;
+        phase_begin(c, tt);
+        if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
+            return rc;
+        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->D32.p, N, nr, c->st)))
+            return rc;
+        if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
+            return rc;
+        phase_end(c, tt);
+        phase_begin(c, c->t_refine);
+        if ((rc = residual(c, nullptr)))
+            return rc;
+        phase_end(c, c->t_refine);
+        if ((rc = col_norms(c, c->R.p, nr, rn)))
+            return rc;
+        double worst = 0;
+        for (int64_t j = 0; j < nr; j++) {
+            const double rel = bn[j] > 0 ? rn[j] / bn[j] : rn[j];
+            worst = std::max(worst, rel);
+        }
+        inf.iterations = it;
+        inf.max_rel_residual = worst;
+        if (worst <= tol) {
+            inf.converged = 1;
+            break;
+        }
+    }
+    c->have_x = true;
+    if (info)
+        *info = inf;
+    if (!inf.converged)
+        return set_error("refinement stopped at relative residual %.3e > %.3e after %d sweeps", inf.max_rel_residual, tol,
+                         inf.iterations),
+               FRICO_ERR_NOCONV;
+    return 0;
+}
+
+int frico_residual(frico_ctx* c, double* rel)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_x)
+        return set_error("no solution"), FRICO_ERR_STATE;
+    if (!rel)
+        return set_error("null output"), -2;
+    int rc = set_dev(c);
+    if (rc || (rc = residual(c, nullptr)))
+        return rc;
+    std::vector<double> bn, rn;
+    if ((rc = col_norms(c, c->B.p, c->nrhs, bn)) || (rc = col_norms(c, c->R.p, c->nrhs, rn)))
+        return rc;
+    for (int64_t j = 0; j < c->nrhs; j++)
+        rel[j] = bn[j] > 0 ? rn[j] / bn[j] : rn[j];
+    return 0;
+}
+
+int frico_solution_get(frico_ctx* c, double* Xh, int64_t ldx)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_x)
+        return set_error("no solution"), FRICO_ERR_STATE;
+    if (!Xh || ldx < c->N)
+        return set_error("bad output"), -2;
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    FRICO_HIP_CHECK(hipMemcpy2D(Xh, ldx * sizeof(c128), c->X.p, c->N * sizeof(c128), c->N * sizeof(c128), c->nrhs,
+                                hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int frico_solution_set(frico_ctx* c, const double* Xh, int64_t ldx, int64_t nrhs)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!Xh || ldx < c->N || nrhs < 1 || (c->nrhs > 0 && nrhs != c->nrhs))
+        return set_error("bad solution arguments"), -2;
+    int rc = set_dev(c);
+    if (rc)
+        return rc;
+    if (c->nrhs < 1 && (rc = rhs_alloc(c, nrhs)))
+        return rc;
+    FRICO_HIP_CHECK(hipMemcpy2DAsync(c->X.p, c->N * sizeof(c128), Xh, ldx * sizeof(c128), c->N * sizeof(c128), nrhs,
+                                     hipMemcpyHostToDevice, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    c->have_x = true;
+    return 0;
+}
+
+static int scatter_all(frico_ctx* c)
+{
+    int rc;
+    const int64_t T = c->hm.nt;
+    if (c->nrhs > 65535)
+        return set_error("too many columns"), FRICO_ERR_LIMIT;
+    if ((rc = c->TS.ensure((size_t)(4 * T * c->nrhs))))
+        return rc;
+    return launch_scatter(c->dm, c->X.p, c->N, (int)c->nrhs, c->TS.p, c->st);
+}
+
+int frico_triangle_currents(frico_ctx* c, int64_t col, double* AJ, double* AdivJ)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_mesh || !c->have_x)
+        return set_error("no solution"), FRICO_ERR_STATE;
+    if (col < 0 || col >= c->nrhs)
+        return set_error("column out of range"), -2;
+    int rc = set_dev(c);
+    if (rc || (rc = scatter_all(c)))
+        return rc;
+    const int64_t T = c->hm.nt;
+    std::vector<c128> h((size_t)(4 * T));
+    FRICO_HIP_CHECK(hipMemcpyAsync(h.data(), c->TS.p + 4 * T * col, h.size() * sizeof(c128), hipMemcpyDeviceToHost, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    for (int64_t t = 0; t < T; t++) {
+        for (int d = 0; d < 3; d++) {
+            if (AJ) {
+                AJ[2 * (3 * t + d)] = h[4 * t + d].re;
+                AJ[2 * (3 * t + d) + 1] = h[4 * t + d].im;
+            }
+        }
+        if (AdivJ) {
+            AdivJ[2 * t] = h[4 * t + 3].re;
+            AdivJ[2 * t + 1] = h[4 * t + 3].im;
+        }
+    }
+    return 0;
+}
+
+int frico_fields(frico_ctx* c, double freq, const double* pts, int64_t npts, const int64_t* cols, double h_sign,
+                 double* E, double* H)
+{
+    if (check_ctx(c))
+        return -1;
+    if (!c->have_mesh || !c->have_x)
+        return set_error("no solution"), FRICO_ERR_STATE;
+    if (!(freq > 0))
+        return set_error("frequency must be positive"), -2;
+    if (!pts || npts < 1 || npts > (1 << 30))
+        return set_error("bad points"), -3;
+    if (!E && !H)
+        return set_error("null outputs"), -7;
+    int rc = set_dev(c);
+    if (rc || (rc = scatter_all(c)))
+        return rc;
+    std::vector<int2> rq((size_t)npts);
+    for (int64_t p = 0; p < npts; p++) {
+        const int64_t col = cols ? cols[p] : 0;
+        if (col < 0 || col >= c->nrhs)
+            return set_error("point %lld uses column %lld out of range", (long long)p, (long long)col), -5;
+        rq[p] = int2{(int)p, (int)col};
+    }
+    if ((rc = c->pts.upload(pts, (size_t)(3 * npts), c->st)) || (rc = c->req.upload(rq.data(), rq.size(), c->st)) ||
+        (rc = c->EH.ensure((size_t)(6 * npts))))
+        return rc;
+    double omega;
+    const double k = kwave(freq, nullptr, &omega);
+    if ((rc = launch_fields(c->dm, k, omega * MU0, c->TS.p, c->pts.p, c->req.p, (int)npts, h_sign >= 0 ? 1.0 : -1.0,
+                            c->EH.p, c->st)))
+        return rc;
+    std::vector<c128> eh((size_t)(6 * npts));
+    FRICO_HIP_CHECK(hipMemcpyAsync(eh.data(), c->EH.p, eh.size() * sizeof(c128), hipMemcpyDeviceToHost, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    for (int64_t p = 0; p < npts; p++)
+        for (int d = 0; d < 3; d++) {
+            if (E) {
+                E[2 * (3 * p + d)] = eh[6 * p + d].re;
+                E[2 * (3 * p + d) + 1] = eh[6 * p + d].im;
+            }
+            if (H) {
+                H[2 * (3 * p + d)] = eh[6 * p + 3 + d].re;
+                H[2 * (3 * p + d) + 1] = eh[6 * p + 3 + d].im;
+            }
+        }
+    return 0;
+}
+
+} // extern "C"

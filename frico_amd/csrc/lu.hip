@@ -1,0 +1,717 @@
+// Dense complex LU with partial pivoting, triangular solves and iterative-refinement helpers
+// for gfx950.  Replaces zgetrf/zgetrs (src/lapack_wrappers.cpp:53-133).
+//
+// Right-looking blocked LU, block width nb:
+//   panel_factor   one launch per panel; G workgroups each keep a slab of R rows x nb columns
+//                  in LDS and factor the panel column by column (pivot = first max of |re|+|im|,
+//                  as izamax).  One grid barrier per column: every workgroup publishes its local
+//                  pivot candidate together with that candidate row, so after the barrier every
+//                  workgroup can pick the winner and perform the row interchange locally.
+//   laswp          applies the panel's interchanges to the columns right of the panel.  Columns
+//                  left of the panel are not swapped; the forward solve applies each panel's
+//                  interchanges just before using that panel's L, which is equivalent.
+//   trsm_block     U12 = L11^{-1} A12 (unit lower, in LDS).
+//   cgemm_sub      A22 -= L21 U12, complex GEMM on MFMA (v_mfma_f64_16x16x4f64 or
+//                  v_mfma_f32_16x16x4f32), 64x64 complex output tile per workgroup.
+#include "common.hpp"
+#include "lu.hpp"
+
+namespace frico {
+
+// ----------------------------------------------------------------------------------------------
+// MFMA wrappers.  Operand lane layout for the 16x16x4 shapes: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the 4 accumulator values of lane l are D[4*(l/16)+r][l%16], r = 0..3.
+// ----------------------------------------------------------------------------------------------
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+    typedef double acc_t __attribute__((ext_vector_type(4)));
+    __device__ static inline acc_t op(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct Mfma<float> {
+    typedef float acc_t __attribute__((ext_vector_type(4)));
+    __device__ static inline acc_t op(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+// ----------------------------------------------------------------------------------------------
+// C[M x N] -= A[M x K] * B[K x N], complex, column-major.
+// Workgroup = 4 waves (2 x 2), tile BM x BN complex, wave tile (WM*16) x (WN*16).
+// The MFMA computes the transposed product (B^T A^T) so that the 16 lanes of an accumulator row
+// hold 16 consecutive rows of C: stores are contiguous down a column.
+// ----------------------------------------------------------------------------------------------
+constexpr int GBM = 64, GBN = 64, GBK = 16, GWM = 2, GWN = 2;
+
+template <typename T, typename TA>
+__global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
+                                                 const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+                                                 int64_t ldc)
+{
+    using MF = Mfma<T>;
+    using acc_t = typename MF::acc_t;
+    __shared__ cx<T> sA[GBK][GBM + 1];
+    __shared__ cx<T> sB[GBK][GBN + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv & 1, wn = wv >> 1;
+    const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
+
+    acc_t accr[GWM][GWN], acci[GWM][GWN];
+#pragma unroll
+    for (int a = 0; a < GWM; a++)
+#pragma unroll
+        for (int b = 0; b < GWN; b++) {
+            accr[a][b] = acc_t{0, 0, 0, 0};
+            acci[a][b] = acc_t{0, 0, 0, 0};
+        }
+
+    constexpr int LA = GBM * GBK / 256, LB = GBK * GBN / 256;
+    cx<T> ra[LA], rb[LB];
+    auto gload = [&](int64_t k0) {
+#pragma unroll
+        for (int s = 0; s < LA; s++) {
+            const int e = tid + 256 * s;
+            const int r = e % GBM, kk = e / GBM;
+            const int64_t gr = m0 + r, gk = k0 + kk;
+            if (gr < M && gk < K) {
+                const cx<TA> v = A[gr + gk * lda];
+                ra[s] = cx<T>{(T)v.re, (T)v.im};
+            } else {
+                ra[s] = cx<T>{0, 0};
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < LB; s++) {
+            const int e = tid + 256 * s;
+            const int kk = e % GBK, c = e / GBK;
+            const int64_t gk = k0 + kk, gc = n0 + c;
+            rb[s] = (gk < K && gc < N) ? B[gk + gc * ldb] : cx<T>{0, 0};
+        }
+    };
+    auto sstore = [&]() {
+#pragma unroll
+        for (int s = 0; s < LA; s++) {
+            const int e = tid + 256 * s;
+            sA[e / GBM][e % GBM] = ra[s];
+        }
+#pragma unroll
+        for (int s = 0; s < LB; s++) {
+            const int e = tid + 256 * s;
+            sB[e % GBK][e / GBK] = rb[s];
+        }
+    };
+
+    gload(0);
+    for (int64_t k0 = 0; k0 < K; k0 += GBK) {
+        __syncthreads();
+        sstore();
+        __syncthreads();
+        if (k0 + GBK < K)
+            gload(k0 + GBK);
+#pragma unroll
+        for (int kk = 0; kk < GBK; kk += 4) {
+            cx<T> af[GWM], bf[GWN];
+#pragma unroll
+            for (int a = 0; a < GWM; a++)
+                af[a] = sA[kk + (lane >> 4)][wm * GWM * 16 + a * 16 + (lane & 15)];
+#pragma unroll
+            for (int b = 0; b < GWN; b++)
+                bf[b] = sB[kk + (lane >> 4)][wn * GWN * 16 + b * 16 + (lane & 15)];
+#pragma unroll
+            for (int a = 0; a < GWM; a++)
+#pragma unroll
+                for (int b = 0; b < GWN; b++) {
+                    // D^T = B^T A^T:  re += br*ar - bi*ai ; im += br*ai + bi*ar
+                    accr[a][b] = MF::op(bf[b].re, af[a].re, accr[a][b]);
+                    accr[a][b] = MF::op(-bf[b].im, af[a].im, accr[a][b]);
+                    acci[a][b] = MF::op(bf[b].re, af[a].im, acci[a][b]);
+                    acci[a][b] = MF::op(bf[b].im, af[a].re, acci[a][b]);
+                }
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < GWM; a++)
+#pragma unroll
+        for (int b = 0; b < GWN; b++) {
+            const int64_t row = m0 + wm * GWM * 16 + a * 16 + (lane & 15);
+            if (row >= M)
+                continue;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t col = n0 + wn * GWN * 16 + b * 16 + 4 * (lane >> 4) + r;
+                if (col < N) {
+                    cx<T>* p = C + row + col * ldc;
+                    cx<T> v = *p;
+                    v.re -= accr[a][b][r];
+                    v.im -= acci[a][b][r];
+                    *p = v;
+                }
+            }
+        }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Panel factorisation with a grid barrier per column.
+// ----------------------------------------------------------------------------------------------
+struct PanelSync {
+    unsigned* counter; // one counter per panel launch, zeroed before the factorisation
+    int* abort_flag;   // set if a barrier wait exceeds its spin limit
+    double* cand_val;  // [2][G]
+    int* cand_idx;     // [2][G]
+    void* cand_row;    // [2][G][wmax] complex
+    void* diag_row;    // [2][wmax] complex
+    int* info;         // first zero pivot (1-based global column), 0 if none
+};
+
+constexpr long SPIN_LIMIT = 40000000; // ~seconds; a wait this long means a missing workgroup
+
+__device__ inline bool grid_barrier(unsigned* counter, unsigned target, int* abort_flag)
+{
+    __syncthreads();
+    __shared__ int aborted;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        long spins = 0;
+        int ab = 0;
+        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > SPIN_LIMIT) {
+                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ab = 1;
+                break;
+            }
+            if ((spins & 1023) == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ab = 1;
+                break;
+            }
+        }
+        __threadfence();
+        aborted = ab;
+    }
+    __syncthreads();
+    return aborted == 0;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
+                                                    int R, int G, int* __restrict__ ipiv, PanelSync ps,
+                                                    unsigned* counter)
+{
+    extern __shared__ unsigned char smem_raw[];
+    cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw); // [w][R] column-major
+    cx<T>* prow = slab + (size_t)w * R;                // [w]
+    __shared__ double rv[4];
+    __shared__ int ri[4];
+    __shared__ int s_piv;
+    __shared__ double s_pval;
+
+    const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int64_t r0 = (int64_t)g * R;                 // first local row of this workgroup (panel-relative)
+    const int nr = (int)((m - r0) < R ? (m - r0) : R); // rows owned
+    cx<T>* Ap = A + k + k * lda;                       // panel origin
+
+    for (int e = tid; e < w * R; e += 256) {
+        const int c = e / R, r = e % R;
+        slab[e] = r < nr ? Ap[(r0 + r) + (int64_t)c * lda] : cx<T>{0, 0};
+    }
+    __syncthreads();
+
+    double* cval = ps.cand_val;
+    int* cidx = ps.cand_idx;
+    cx<T>* crow = reinterpret_cast<cx<T>*>(ps.cand_row);
+    cx<T>* drow = reinterpret_cast<cx<T>*>(ps.diag_row);
+
+    for (int c = 0; c < w; c++) {
+        const int par = c & 1;
+        // ---- local pivot candidate among rows >= c (panel-relative)
+        double best = -1.0;
+        int bidx = 0x7fffffff;
+        for (int r = tid; r < nr; r += 256) {
+            const int64_t gr = r0 + r;
+            if (gr < c)
+                continue;
+            const double v = (double)cabs1(slab[(size_t)c * R + r]);
+            if (v > best) { // rows visited in increasing order per thread: first max kept
+                best = v;
+                bidx = (int)gr;
+            }
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double ov = __shfl_xor(best, o, 64);
+            const int oi = __shfl_xor(bidx, o, 64);
+            if (ov > best || (ov == best && oi < bidx)) {
+                best = ov;
+                bidx = oi;
+            }
+        }
+        if (lane == 0) {
+            rv[wv] = best;
+            ri[wv] = bidx;
+        }
+        __syncthreads();
+        if (tid == 0) {
+            for (int x = 1; x < 4; x++)
+                if (rv[x] > rv[0] || (rv[x] == rv[0] && ri[x] < ri[0])) {
+                    rv[0] = rv[x];
+                    ri[0] = ri[x];
+                }
+        }
+        __syncthreads();
+        const double lbest = rv[0];
+        const int lidx = ri[0];
+
+        if (G > 1) {
+            // ---- publish candidate (value, index, row) and, if owned, the diagonal row
+            if (tid == 0) {
+                cval[par * G + g] = lbest;
+                cidx[par * G + g] = lidx;
+            }
+            if (lbest >= 0.0) {
+                const int lr = lidx - (int)r0;
+                for (int j = tid; j < w; j += 256)
+                    crow[((size_t)par * G + g) * w + j] = slab[(size_t)j * R + lr];
+            }
+            const bool own_diag = (c >= r0) && (c < r0 + nr);
+            if (own_diag)
+                for (int j = tid; j < w; j += 256)
+                    drow[(size_t)par * w + j] = slab[(size_t)j * R + (c - r0)];
+            if (!grid_barrier(counter, (unsigned)(c + 1) * (unsigned)G, ps.abort_flag))
+                return;
+            // ---- global winner (first max), identical in every workgroup
+            if (wv == 0) {
+                double b = -1.0;
+                int bi = 0x7fffffff, bg = 0;
+                for (int x = lane; x < G; x += 64) {
+                    const double v = cval[par * G + x];
+                    const int ii = cidx[par * G + x];
+                    if (v > b || (v == b && ii < bi)) {
+                        b = v;
+                        bi = ii;
+                        bg = x;
+                    }
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    const double ov = __shfl_xor(b, o, 64);
+                    const int oi = __shfl_xor(bi, o, 64);
+                    const int og = __shfl_xor(bg, o, 64);
+                    if (ov > b || (ov == b && oi < bi)) {
+                        b = ov;
+                        bi = oi;
+                        bg = og;
+                    }
+                }
+                if (lane == 0) {
+                    s_pval = b;
+                    s_piv = bi;
+                    ri[0] = bg;
+                }
+            }
+            __syncthreads();
+            const int gw = ri[0];
+            for (int j = tid; j < w; j += 256)
+                prow[j] = crow[((size_t)par * G + gw) * w + j];
+            __syncthreads();
+            const int p = s_piv;
+            if (s_pval > 0.0) {
+                if (c >= r0 && c < r0 + nr)
+                    for (int j = tid; j < w; j += 256)
+                        slab[(size_t)j * R + (c - r0)] = prow[j];
+                if (p != c && p >= r0 && p < r0 + nr)
+                    for (int j = tid; j < w; j += 256)
+                        slab[(size_t)j * R + (p - r0)] = drow[(size_t)par * w + j];
+            }
+        } else {
+            // single workgroup: swap in LDS directly
+            if (tid == 0) {
+                s_piv = lidx;
+                s_pval = lbest;
+            }
+            __syncthreads();
+            const int p = s_piv;
+            if (s_pval > 0.0 && p != c)
+                for (int j = tid; j < w; j += 256) {
+                    cx<T> t = slab[(size_t)j * R + c];
+                    slab[(size_t)j * R + c] = slab[(size_t)j * R + p];
+                    slab[(size_t)j * R + p] = t;
+                }
+            __syncthreads();
+            for (int j = tid; j < w; j += 256)
+                prow[j] = slab[(size_t)j * R + c];
+        }
+        __syncthreads();
+        const int p = s_piv;
+        if (g == 0 && tid == 0) {
+            ipiv[k + c] = (int)(k + (s_pval > 0.0 ? p : c));
+            if (!(s_pval > 0.0) && *ps.info == 0)
+                *ps.info = (int)(k + c + 1);
+        }
+        if (s_pval > 0.0) {
+            // ---- scale the column below the diagonal and update the rest of the panel
+            const cx<T> piv = prow[c];
+            const cx<T> rp = cdiv(cx<T>{1, 0}, piv);
+            const int rstart = (int)((c + 1 - r0) > 0 ? (c + 1 - r0) : 0);
+            for (int r = rstart + tid; r < nr; r += 256)
+                slab[(size_t)c * R + r] = slab[(size_t)c * R + r] * rp;
+            __syncthreads();
+            const int rows = nr - rstart, cols = w - c - 1;
+            for (int e = tid; e < rows * cols; e += 256) {
+                const int r = rstart + e % rows, j = c + 1 + e / rows;
+                const cx<T> l = slab[(size_t)c * R + r];
+                const cx<T> u = prow[j];
+                cx<T>& a = slab[(size_t)j * R + r];
+                a.re -= l.re * u.re - l.im * u.im;
+                a.im -= l.re * u.im + l.im * u.re;
+            }
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < w * R; e += 256) {
+        const int c = e / R, r = e % R;
+        if (r < nr)
+            Ap[(r0 + r) + (int64_t)c * lda] = slab[e];
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// Row interchanges of one panel applied to columns [col0, col0+ncols).
+// ----------------------------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols, int64_t k,
+                                             int w, const int* __restrict__ ipiv)
+{
+    const int64_t j = col0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= col0 + ncols)
+        return;
+    cx<T>* col = A + j * lda;
+    for (int c = 0; c < w; c++) {
+        const int64_t p = ipiv[k + c];
+        if (p != k + c) {
+            cx<T> t = col[k + c];
+            col[k + c] = col[p];
+            col[p] = t;
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------------
+// In-place triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block.
+// UPPER=false: unit lower (forward).  UPPER=true: non-unit upper (backward).
+// ----------------------------------------------------------------------------------------------
+template <typename T, bool UPPER>
+__global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict__ D, int64_t ldd, cx<T>* __restrict__ B,
+                                                  int64_t ldb, int64_t ncols)
+{
+    __shared__ cx<T> sD[64][65];
+    __shared__ cx<T> sX[64][65];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e % 64, c = e / 64;
+        sD[r][c] = (r < w && c < w) ? D[r + (int64_t)c * ldd] : cx<T>{0, 0};
+        sX[r][c] = (r < w && c < nc) ? B[r + (c0 + c) * ldb] : cx<T>{0, 0};
+    }
+    __syncthreads();
+    const int col = tid & 63, rg = tid >> 6;
+    if (!UPPER) {
+        for (int r = 0; r < w; r++) {
+            const cx<T> x = sX[r][col];
+            for (int rr = r + 1 + rg; rr < w; rr += 4) {
+                const cx<T> l = sD[rr][r];
+                sX[rr][col].re -= l.re * x.re - l.im * x.im;
+                sX[rr][col].im -= l.re * x.im + l.im * x.re;
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int r = w - 1; r >= 0; r--) {
+            if (rg == 0)
+                sX[r][col] = cdiv(sX[r][col], sD[r][r]);
+            __syncthreads();
+            const cx<T> x = sX[r][col];
+            for (int rr = rg; rr < r; rr += 4) {
+                const cx<T> u = sD[rr][r];
+                sX[rr][col].re -= u.re * x.re - u.im * x.im;
+                sX[rr][col].im -= u.re * x.im + u.im * x.re;
+            }
+            __syncthreads();
+        }
+    }
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e % 64, c = e / 64;
+        if (r < w && c < nc)
+            B[r + (c0 + c) * ldb] = sX[r][c];
+    }
+}
+
+__global__ void zero_u32(unsigned* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        p[i] = 0;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Elementwise helpers for mixed precision and refinement.
+// ----------------------------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void convert_mat(const cx<TI>* __restrict__ X, int64_t ldx, cx<TO>* __restrict__ Y,
+                                                   int64_t ldy, int64_t rows)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (r < rows) {
+        const cx<TI> v = X[r + c * ldx];
+        Y[r + c * ldy] = cx<TO>{(TO)v.re, (TO)v.im};
+    }
+}
+
+// X64 += D32
+__global__ __launch_bounds__(256) void add_correction(c128* __restrict__ X, int64_t ldx, const c64* __restrict__ D,
+                                                      int64_t ldd, int64_t rows)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (r < rows) {
+        const c64 d = D[r + c * ldd];
+        X[r + c * ldx].re += (double)d.re;
+        X[r + c * ldx].im += (double)d.im;
+    }
+}
+
+// out[c] = sum_r |X[r,c]|^2, one workgroup per column.
+__global__ __launch_bounds__(256) void col_norm2(const c128* __restrict__ X, int64_t ldx, int64_t rows,
+                                                 double* __restrict__ out)
+{
+    const int64_t c = blockIdx.x;
+    double s = 0;
+    for (int64_t r = threadIdx.x; r < rows; r += 256) {
+        const c128 v = X[r + c * ldx];
+        s += v.re * v.re + v.im * v.im;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        s += __shfl_xor(s, o, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[c] = red[0] + red[1] + red[2] + red[3];
+}
+
+// ----------------------------------------------------------------------------------------------
+// Host drivers
+// ----------------------------------------------------------------------------------------------
+template <typename T>
+int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb, cx<T>* C,
+             int64_t ldc, hipStream_t st, KernelTimer* kt)
+{
+    if (M <= 0 || N <= 0 || K <= 0)
+        return 0;
+    const int64_t gx = (M + GBM - 1) / GBM, gy = (N + GBN - 1) / GBN;
+    if (gy > 65535)
+        return set_error("gemm: too many column tiles"), -1;
+    if (kt)
+        kt->begin(st);
+    hipLaunchKernelGGL((cgemm_sub<T, T>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+    if (kt)
+        kt->end(st, 8.0 * (double)M * (double)N * (double)K);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const c128* B, int64_t ldb, c128* C,
+                   int64_t ldc, hipStream_t st, KernelTimer* kt)
+{
+    if (M <= 0 || N <= 0 || K <= 0)
+        return 0;
+    const int64_t gx = (M + GBM - 1) / GBM, gy = (N + GBN - 1) / GBN;
+    if (gy > 65535)
+        return set_error("gemm: too many column tiles"), -1;
+    if (kt)
+        kt->begin(st);
+    hipLaunchKernelGGL((cgemm_sub<double, float>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B,
+                       ldb, C, ldc);
+    if (kt)
+        kt->end(st, 8.0 * (double)M * (double)N * (double)K);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Z = (Z + Z^T) / 2 in place (src/emw_solver.h:118-120): one thread per pair j > i.
+__global__ __launch_bounds__(256) void symmetrize_k(c128* __restrict__ Z, int64_t n, int64_t ldz)
+{
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (j >= n || j <= i)
+        return;
+    const c128 a = Z[j + i * ldz], b = Z[i + j * ldz];
+    const c128 s{(a.re + b.re) / 2.0, (a.im + b.im) / 2.0};
+    Z[j + i * ldz] = s;
+    Z[i + j * ldz] = s;
+}
+
+int launch_symmetrize(c128* Z, int64_t n, int64_t ldz, hipStream_t st)
+{
+    if (n > 65535)
+        return set_error("symmetrize: n too large"), -1;
+    hipLaunchKernelGGL(symmetrize_k, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, st, Z, n, ldz);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T> int panel_rows(int w)
+{
+    // slab of R rows x w complex plus the pivot row must fit in 160 KB of LDS with headroom
+    const size_t budget = 144 * 1024;
+    int R = (int)(budget / ((size_t)w * sizeof(cx<T>))) - 1;
+    R = R >= 256 ? 256 : (R >= 128 ? 128 : (R >= 64 ? 64 : 32));
+    return R;
+}
+
+template <typename T>
+int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer)
+{
+    if (nb > LU_NB_MAX || nb < 1)
+        return set_error("lu: block size out of range"), -1;
+    const int R = panel_rows<T>(nb);
+    const int64_t npan = (n + nb - 1) / nb;
+    FRICO_HIP_CHECK(allow_full_lds(reinterpret_cast<const void*>(&panel_factor<T>)));
+    if ((int64_t)wk.ncounters < npan)
+        return set_error("lu: sync workspace too small"), -1;
+    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((npan + 255) / 256)), dim3(256), 0, st, wk.counters, npan);
+    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
+    FRICO_HIP_CHECK(hipGetLastError());
+    PanelSync ps{};
+    ps.abort_flag = wk.flags;
+    ps.info = wk.flags + 1;
+    ps.cand_val = wk.cand_val;
+    ps.cand_idx = wk.cand_idx;
+    ps.cand_row = wk.cand_row;
+    ps.diag_row = wk.diag_row;
+    for (int64_t p = 0; p < npan; p++) {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        const int64_t m = n - k;
+        const int G = (int)((m + R - 1) / R);
+        if (G > wk.max_groups)
+            return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, wk.max_groups), -1;
+        const size_t sh = ((size_t)w * R + w) * sizeof(cx<T>);
+        hipLaunchKernelGGL((panel_factor<T>), dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
+                           wk.counters + p);
+        FRICO_HIP_CHECK(hipGetLastError());
+        const int64_t nright = n - k - w;
+        if (nright > 0) {
+            hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((nright + 255) / 256)), dim3(256), 0, st, A, lda, k + w,
+                               nright, k, w, ipiv);
+            // U12 = L11^{-1} A12, in chunks of 64 rows of the block when nb > 64
+            for (int s = 0; s < w; s += 64) {
+                const int ws = (w - s) < 64 ? (w - s) : 64;
+                cx<T>* D = A + (k + s) + (k + s) * lda;
+                cx<T>* Bb = A + (k + s) + (k + w) * lda;
+                hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nright + 63) / 64)), dim3(256), 0, st, ws, D, lda,
+                                   Bb, lda, nright);
+                if (s + ws < w) {
+                    int rc = gemm_sub<T>(w - s - ws, nright, ws, A + (k + s + ws) + (k + s) * lda, lda, Bb, lda,
+                                         A + (k + s + ws) + (k + w) * lda, lda, st, nullptr);
+                    if (rc)
+                        return rc;
+                }
+            }
+            FRICO_HIP_CHECK(hipGetLastError());
+            int rc = gemm_sub<T>(nright, nright, w, A + (k + w) + k * lda, lda, A + k + (k + w) * lda, lda,
+                                 A + (k + w) + (k + w) * lda, lda, st, gemm_timer);
+            if (rc)
+                return rc;
+        }
+    }
+    return 0;
+}
+
+template <typename T>
+int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
+             hipStream_t st)
+{
+    const int64_t npan = (n + nb - 1) / nb;
+    for (int64_t p = 0; p < npan; p++) {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0, nrhs, k,
+                           w, ipiv);
+        for (int s = 0; s < w; s += 64) {
+            const int ws = (w - s) < 64 ? (w - s) : 64;
+            hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
+                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+            const int64_t below = n - (k + s + ws);
+            if (below > 0) {
+                int rc = gemm_sub<T>(below, nrhs, ws, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s), ldb,
+                                     B + (k + s + ws), ldb, st, nullptr);
+                if (rc)
+                    return rc;
+            }
+        }
+    }
+    for (int64_t p = npan - 1; p >= 0; p--) {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        for (int s = ((w - 1) / 64) * 64; s >= 0; s -= 64) {
+            const int ws = (w - s) < 64 ? (w - s) : 64;
+            hipLaunchKernelGGL((trsm_block<T, true>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
+                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+            const int64_t above = k + s;
+            if (above > 0) {
+                int rc = gemm_sub<T>(above, nrhs, ws, A + (k + s) * lda, lda, B + (k + s), ldb, B, ldb, st, nullptr);
+                if (rc)
+                    return rc;
+            }
+        }
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_convert_c128_c64(const c128* X, int64_t ldx, c64* Y, int64_t ldy, int64_t rows, int64_t cols, hipStream_t st)
+{
+    for (int64_t c0 = 0; c0 < cols; c0 += 65535) {
+        const int64_t nc = (cols - c0) < 65535 ? (cols - c0) : 65535;
+        hipLaunchKernelGGL((convert_mat<double, float>), dim3((unsigned)((rows + 255) / 256), (unsigned)nc), dim3(256), 0, st,
+                           X + c0 * ldx, ldx, Y + c0 * ldy, ldy, rows);
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_add_correction(c128* X, int64_t ldx, const c64* D, int64_t ldd, int64_t rows, int64_t cols, hipStream_t st)
+{
+    hipLaunchKernelGGL(add_correction, dim3((unsigned)((rows + 255) / 256), (unsigned)cols), dim3(256), 0, st, X, ldx, D,
+                       ldd, rows);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_col_norm2(const c128* X, int64_t ldx, int64_t rows, int64_t cols, double* out, hipStream_t st)
+{
+    hipLaunchKernelGGL(col_norm2, dim3((unsigned)cols), dim3(256), 0, st, X, ldx, rows, out);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template int gemm_sub<double>(int64_t, int64_t, int64_t, const c128*, int64_t, const c128*, int64_t, c128*, int64_t,
+                              hipStream_t, KernelTimer*);
+template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, const c64*, int64_t, c64*, int64_t,
+                             hipStream_t, KernelTimer*);
+template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
+template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
+template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t);
+template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t);
+template int panel_rows<double>(int);
+template int panel_rows<float>(int);
+
+} // namespace frico

@@ -1,0 +1,42 @@
+// Dense complex LU / solve / refinement drivers (device side in lu.hip).
+#pragma once
+
+#include "common.hpp"
+
+#include <vector>
+
+namespace frico {
+
+constexpr int LU_NB_MAX = 128;
+constexpr int LU_MAX_GROUPS = 512;
+
+// Device workspace of the panel factorisation.
+struct LuWork {
+    unsigned* counters = nullptr; // one barrier counter per panel
+    int64_t ncounters = 0;
+    int* flags = nullptr; // [0] abort, [1] info
+    double* cand_val = nullptr;
+    int* cand_idx = nullptr;
+    void* cand_row = nullptr;
+    void* diag_row = nullptr;
+    int max_groups = 0; // co-resident workgroup limit of the panel kernel
+};
+
+template <typename T>
+int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb, cx<T>* C,
+             int64_t ldc, hipStream_t st, KernelTimer* kt);
+// Residual GEMM with a complex64-stored A and FP64 arithmetic: C -= double(A) * B.
+int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const c128* B, int64_t ldb, c128* C,
+                   int64_t ldc, hipStream_t st, KernelTimer* kt);
+template <typename T> int panel_rows(int w);
+template <typename T>
+int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
+template <typename T>
+int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
+             hipStream_t st);
+int launch_convert_c128_c64(const c128* X, int64_t ldx, c64* Y, int64_t ldy, int64_t rows, int64_t cols, hipStream_t st);
+int launch_add_correction(c128* X, int64_t ldx, const c64* D, int64_t ldd, int64_t rows, int64_t cols, hipStream_t st);
+int launch_col_norm2(const c128* X, int64_t ldx, int64_t rows, int64_t cols, double* out, hipStream_t st);
+int launch_symmetrize(c128* Z, int64_t n, int64_t ldz, hipStream_t st);
+
+} // namespace frico

@@ -1,0 +1,188 @@
+/*
+ * frico_gpu.h -- C ABI of the MI355X (gfx950) EFIE Method-of-Moments library.
+ *
+ * Plain C: pointers, sizes and status codes; no exceptions cross it and no C++ or HIP types
+ * appear in the signatures.  Complex numbers are interleaved (re, im) doubles, the layout of
+ * std::complex<double>; matrices are column-major with an explicit leading dimension, like the
+ * reference's Eigen matrices (src/eigen.h:36-40).  All device memory is owned by the context.
+ *
+ * Each entry point names the reference interface it replaces (paths relative to the reference
+ * source root).  Status convention: 0 = OK; -k = argument k invalid (LAPACK style); > 0 =
+ * singular pivot at that 1-based column (zgetrf's info); FRICO_ERR_* for the rest.  The text
+ * of the last error of the calling thread is returned by frico_last_error().
+ */
+#ifndef FRICO_GPU_H
+#define FRICO_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRICO_OK 0
+#define FRICO_ERR_HIP (-1000)    /* a HIP runtime call failed */
+#define FRICO_ERR_NOMEM (-1001)  /* device allocation failed */
+#define FRICO_ERR_STATE (-1002)  /* call order (e.g. solve before factor, no mesh) */
+#define FRICO_ERR_MESH (-1003)   /* connectivity error: lonely edge or edge in > 2 triangles */
+#define FRICO_ERR_NOCONV (-1004) /* iterative refinement did not reach the tolerance */
+#define FRICO_ERR_SYNC (-1005)   /* a device-wide barrier exceeded its wait limit */
+#define FRICO_ERR_LIMIT (-1006)  /* problem exceeds a documented size limit */
+
+/* Matrix fill modes: compute_matrix (src/emw_solver.cpp:45-142) and compute_matrix_approx
+ * (src/emw_solver.cpp:144-220). */
+#define FRICO_FILL_QUAD 0
+#define FRICO_FILL_APPROX 1
+
+/* Precisions.  FRICO_C128: complex128 everywhere.  FRICO_C64: complex64 storage. */
+#define FRICO_C128 0
+#define FRICO_C64 1
+
+/* Factorisation / solve forms (frico_factor):
+ *   FRICO_SOLVE_C128      complex128 LU of Z, in place (zgetrf semantics)
+ *   FRICO_SOLVE_C64_IR    complex64 LU of a copy of Z, FP64 residual b - Z x against the kept Z,
+ *                         iterative refinement to the requested tolerance */
+#define FRICO_SOLVE_C128 0
+#define FRICO_SOLVE_C64_IR 1
+
+typedef struct frico_ctx frico_ctx;
+
+/* One RWG basis function (src/rwg_basis.h:41-62), in matrix order. */
+typedef struct frico_rwg {
+    int64_t itminus, itplus;     /* T- < T+ (src/input_gmsh.cpp:243-262) */
+    int32_t loc_eminus, loc_eplus;
+    int64_t edge_index;          /* position in the sorted unique edge list */
+    int32_t has_interface, interface_tag;
+    double length;               /* hypot(dx, dy) of the edge (src/geom_mesh.cpp:46-52) */
+    double Aminus, Aplus;        /* 0.5 |e0 x e1| (src/geom_mesh.cpp:71-78) */
+    double pminus[3], pplus[3];  /* free vertices, evmap {2,0,1} (src/rwg_basis.cpp:47-50) */
+} frico_rwg;
+
+typedef struct frico_solve_info {
+    int32_t iterations;          /* refinement sweeps performed (0 for FRICO_SOLVE_C128) */
+    int32_t converged;           /* 1 if every column met the tolerance */
+    double max_rel_residual;     /* max over columns of ||b - Z x|| / ||b|| (if computed) */
+} frico_solve_info;
+
+typedef struct frico_profile {
+    double fill_ms, rhs_ms, factor_ms, solve_ms, refine_ms; /* per phase, HIP events */
+    double gemm_ms, gemm_flops;  /* trailing-update GEMM launches of the last factorisations */
+    int64_t gemm_launches;
+    double tripair_ms, tripair_pairs; /* triangle-pair kernel of the quadrature fill */
+    int64_t tripair_launches;
+} frico_profile;
+
+/* Text of the last error on this thread ("" if none). */
+const char* frico_last_error(void);
+
+/* Context: one device, one HIP stream (default priority), owned device buffers. */
+int frico_ctx_create(int device, frico_ctx** out);
+int frico_ctx_destroy(frico_ctx* ctx);
+int frico_ctx_synchronize(frico_ctx* ctx);
+/* Enables HIP-event timing of phases and of every GEMM / fill launch (off by default). */
+int frico_ctx_set_profiling(frico_ctx* ctx, int enable);
+int frico_ctx_get_profile(frico_ctx* ctx, frico_profile* out);
+/* Block width of the LU (32, 64 or 128). */
+int frico_ctx_set_block(frico_ctx* ctx, int nb);
+
+/*
+ * Mesh ingestion -> connectivity -> RWG function space, and the per-triangle quadrature data of
+ * Dunavant degree `degree` (0..20).  Replaces the connectivity part of load_from_gmsh
+ * (src/input_gmsh.cpp:120-140, 143-199, 216-322) and make_function_space
+ * (src/rwg_basis.cpp:27-68); vertices must already be compacted.  iface_edges are the edges of
+ * 1D entities (pairs of vertex indices) with their entity tags.
+ */
+int frico_mesh_load(frico_ctx* ctx, const double* vertices, int64_t nv, const int64_t* triangles,
+                    const int32_t* tri_tags, int64_t nt, const int64_t* iface_edges,
+                    const int32_t* iface_tags, int64_t nif, int degree);
+int64_t frico_num_unknowns(const frico_ctx* ctx);
+int64_t frico_num_triangles(const frico_ctx* ctx);
+int frico_rwg_table(const frico_ctx* ctx, frico_rwg* out);
+
+/*
+ * Delta-gap edge orientation signs: reorient_deltagap_edges (src/emw_solver.cpp:222-350).
+ * signs[N] receives +-1 for the gap functions when the reference would reorient (then
+ * *applied = 1), otherwise all ones and *applied = 0.
+ */
+int frico_reorient_deltagap(frico_ctx* ctx, const int32_t* ifaces, int nif, int force,
+                            double* signs, int* applied);
+
+/* Impedance matrix Z (N x N) of the loaded mesh at `freq`, kept on the device:
+ * compute_matrix / compute_matrix_approx (src/emw_solver.h:85-86).  prec = FRICO_C128 or
+ * FRICO_C64 (complex64 bulk, FP64 near/self entries). */
+int frico_fill(frico_ctx* ctx, double freq, int mode, int prec);
+/* Z = (Z + Z^T)/2 (src/emw_solver.h:118-120), complex128 matrix only. */
+int frico_symmetrize(frico_ctx* ctx);
+/* Copies Z (complex128 after a C128 fill; complex64 values widened after a C64 fill). */
+int frico_matrix_get(frico_ctx* ctx, double* Z, int64_t ldz);
+/* Loads an arbitrary complex128 N x N matrix as Z (N from the mesh, or n if no mesh). */
+int frico_matrix_set(frico_ctx* ctx, const double* Z, int64_t n, int64_t ldz);
+
+/* Right-hand sides, one column per excitation, kept on the device.
+ * update_rhs(delta_gap) (src/emw_solver.cpp:352-378): column 0 = sum over the gap functions of
+ * -j l/(omega mu0) * V * sign; signs may be NULL.  update_rhs(plane_wave)
+ * (src/emw_solver.cpp:380-416, src/sources.h:34-75): column a for incidence (theta[a], phi[a])
+ * with complex amplitudes eth[2a..2a+1], eph[2a..2a+1]. */
+int frico_rhs_deltagap(frico_ctx* ctx, double freq, const int32_t* ifaces, int nif, double v_re,
+                       double v_im, const double* signs);
+int frico_rhs_planewave(frico_ctx* ctx, double freq, int64_t nang, const double* theta,
+                        const double* phi, const double* eth, const double* eph);
+int frico_rhs_set(frico_ctx* ctx, const double* B, int64_t ldb, int64_t nrhs);
+int frico_rhs_get(frico_ctx* ctx, double* B, int64_t ldb);
+
+/* LU of Z: zgetrf (src/lapack_wrappers.cpp:69) in the given form.  Returns > 0 for a zero
+ * pivot (the 1-based column), like zgetrf's info. */
+int frico_factor(frico_ctx* ctx, int form);
+/* X = Z^{-1} B for every right-hand side column: zgetrs (src/lapack_wrappers.cpp:83,124),
+ * followed by refinement for FRICO_SOLVE_C64_IR until max_c ||b_c - Z x_c||/||b_c|| <= tol or
+ * maxit sweeps.  info may be NULL. */
+int frico_solve(frico_ctx* ctx, double tol, int maxit, frico_solve_info* info);
+/* Relative residual ||b_c - Z x_c|| / ||b_c|| per column against a freshly supplied Z (the
+ * current Z must be unfactored, e.g. after a new frico_fill). */
+int frico_residual(frico_ctx* ctx, double* rel /* nrhs */);
+int frico_solution_get(frico_ctx* ctx, double* X, int64_t ldx);
+int frico_solution_set(frico_ctx* ctx, const double* X, int64_t ldx, int64_t nrhs);
+int64_t frico_num_rhs(const frico_ctx* ctx);
+
+/* RWG -> triangle scatter of solution column `col` (src/emw_solver.h:137-164):
+ * tri_AJ (T x 3 complex, row-major [t][xyz]) and tri_AdivJ (T complex). */
+int frico_triangle_currents(frico_ctx* ctx, int64_t col, double* tri_AJ, double* tri_AdivJ);
+
+/* E and H at points from the solution columns (eval_fields, src/emw_postpro_common.cpp:31-66).
+ * Point p uses solution column cols[p] (cols may be NULL: column 0).  E, H: npts x 3 complex,
+ * row-major.  h_sign = +1 keeps the reference's exp(+jkR) in H (:60-61), -1 uses exp(-jkR). */
+int frico_fields(frico_ctx* ctx, double freq, const double* pts, int64_t npts, const int64_t* cols,
+                 double h_sign, double* E, double* H);
+
+/*
+ * Whole analyses, mirroring the reference entry points for one frequency each.
+ *
+ * frico_antenna_sweep: do_sweep(simulation&, antenna_analysis&) (src/emw_ana_antenna.cpp:236-295)
+ *   per frequency: fill (mode), optional symmetrisation, delta-gap RHS, solve, port values
+ *   (src/emw_ana_antenna.cpp:76-105) and gain on the xy/yz/xz cuts (:33-65).
+ *   port_out[10*f ..]: I(re,im) P(re,im) Zin(re,im) Gamma(re,im) SWR, residual
+ *   gain_out[1080*f ..]: 3 x 360 gains, planes xy, yz, xz.
+ * frico_bistatic_sweep: do_sweep(simulation&, const bistatic_rcs_analysis&)
+ *   (src/emw_ana_bistatic_rcs.cpp:120-151); table_out[359*4*f ..] = VV_theta HH_theta VV_phi HH_phi.
+ * frico_monostatic_sweep: nang plane waves at (theta[a], phi[a]) solved as one block; the
+ *   back-scattered sigma_VV, sigma_HH at (R, theta[a], phi[a]) (formula of
+ *   src/emw_ana_bistatic_rcs.cpp:62-82); sigma_out[2*nang*f ..].
+ * `form` selects FRICO_SOLVE_C128 or FRICO_SOLVE_C64_IR (then tol/maxit apply).
+ */
+int frico_antenna_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, int degree_unused,
+                        const int32_t* ifaces, int nif, double v_re, double v_im, double R,
+                        double Z0, int force_symmetry, int force_reorient, int form, double tol,
+                        int maxit, double* port_out, double* gain_out);
+int frico_bistatic_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, double theta_r,
+                         double phi_r, double eth, double eph, double R, int form, double tol,
+                         int maxit, double* table_out);
+int frico_monostatic_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, int64_t nang,
+                           const double* theta, const double* phi, double eth, double eph,
+                           double R, int form, double tol, int maxit, double* sigma_out,
+                           frico_solve_info* info_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FRICO_GPU_H */

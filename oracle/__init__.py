@@ -1,0 +1,254 @@
+"""ctypes front-end of the FP64 CPU oracle (oracle/cpu_ref.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the checker and by
+bench.py's cpu_baseline leg.  The product package (frico_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfrico_oracle.so")
+_lib = None
+
+_d = C.POINTER(C.c_double)
+_i64 = C.POINTER(C.c_int64)
+_i32 = C.POINTER(C.c_int32)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++ -fopenmp)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE] + (["-B"] if force else []), check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_mesh_build.argtypes = [_d, C.c_int64, _i64, _i32, C.c_int64, _i64, _i32, C.c_int64,
+                                     C.POINTER(C.c_void_p)]
+        L.orc_mesh_free.argtypes = [C.c_void_p]
+        L.orc_mesh_num_rwg.argtypes = [C.c_void_p]
+        L.orc_mesh_num_rwg.restype = C.c_int64
+        L.orc_mesh_num_edges.argtypes = [C.c_void_p]
+        L.orc_mesh_num_edges.restype = C.c_int64
+        L.orc_mesh_rwg.argtypes = [C.c_void_p, _i64, _i64, _i32, _i32, _i64, _i32, _i32, _d, _d, _d,
+                                   _d, _d]
+        L.orc_mesh_edges.argtypes = [C.c_void_p, _i64]
+        L.orc_dunavant.argtypes = [C.c_int, _d, _d, _d, _d, _d]
+        L.orc_subtri.argtypes = [C.c_void_p, C.c_int64, C.c_int, _d, _d]
+        L.orc_fill_quad.argtypes = [C.c_void_p, C.c_double, C.c_int, _d]
+        L.orc_fill_approx.argtypes = [C.c_void_p, C.c_double, _d]
+        L.orc_reorient_deltagap.argtypes = [C.c_void_p, _i32, C.c_int, C.c_int, _d,
+                                            C.POINTER(C.c_int)]
+        L.orc_rhs_deltagap.argtypes = [C.c_void_p, C.c_double, _i32, C.c_int, C.c_double,
+                                       C.c_double, _d, _d]
+        L.orc_rhs_planewave.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_double, _d]
+        L.orc_lu_solve.argtypes = [_d, C.c_int64, _d, C.c_int64]
+        L.orc_scatter.argtypes = [C.c_void_p, _d, _d, _d, _d]
+        L.orc_eval_fields.argtypes = [C.c_void_p, C.c_double, _d, _d, _d, C.c_int64, _d, _d]
+        L.orc_port_values.argtypes = [C.c_void_p, _d, _i32, C.c_int, C.c_double, C.c_double, _d,
+                                      C.c_double, _d]
+        L.orc_gain.argtypes = [C.c_void_p, C.c_double, _d, _d, C.c_double, C.c_double, _d]
+        L.orc_rcs_unit_Ei.argtypes = [_d, C.c_double, C.c_double, C.c_double, _d]
+        L.orc_bistatic.argtypes = [C.c_void_p, C.c_double, _d, _d, C.c_double, _d]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_d):
+    return a.ctypes.data_as(t)
+
+
+class OracleMesh:
+    """Mesh + RWG function space built by the oracle's restatement of the reference."""
+
+    def __init__(self, sm):
+        L = lib()
+        self.sm = sm
+        h = C.c_void_p()
+        rc = L.orc_mesh_build(_p(sm.vertices), sm.nv, _p(sm.triangles, _i64), _p(sm.tags, _i32),
+                              sm.nt, _p(sm.bedges, _i64), _p(sm.btags, _i32), sm.bedges.shape[0],
+                              C.byref(h))
+        if rc != 0:
+            raise ValueError(f"oracle mesh build failed ({rc})")
+        self.h = h
+        self.N = L.orc_mesh_num_rwg(h)
+        N = self.N
+        self.itm = np.zeros(N, np.int64)
+        self.itp = np.zeros(N, np.int64)
+        self.lem = np.zeros(N, np.int32)
+        self.lep = np.zeros(N, np.int32)
+        self.edge = np.zeros(N, np.int64)
+        self.has_iface = np.zeros(N, np.int32)
+        self.iface = np.zeros(N, np.int32)
+        self.length = np.zeros(N)
+        self.Am = np.zeros(N)
+        self.Ap = np.zeros(N)
+        self.pm = np.zeros((N, 3))
+        self.pp = np.zeros((N, 3))
+        L.orc_mesh_rwg(h, _p(self.itm, _i64), _p(self.itp, _i64), _p(self.lem, _i32),
+                       _p(self.lep, _i32), _p(self.edge, _i64), _p(self.has_iface, _i32),
+                       _p(self.iface, _i32), _p(self.length), _p(self.Am), _p(self.Ap), _p(self.pm),
+                       _p(self.pp))
+        ne = L.orc_mesh_num_edges(h)
+        self.edges = np.zeros((ne, 2), np.int64)
+        L.orc_mesh_edges(h, _p(self.edges, _i64))
+
+    def __del__(self):
+        try:
+            lib().orc_mesh_free(self.h)
+        except Exception:
+            pass
+
+    def fill(self, freq, degree=5, approx=False):
+        Z = np.zeros((self.N, self.N), np.complex128, order="F")
+        if approx:
+            rc = lib().orc_fill_approx(self.h, freq, _p(Z))
+        else:
+            rc = lib().orc_fill_quad(self.h, freq, degree, _p(Z))
+        if rc:
+            raise ValueError(f"fill failed ({rc})")
+        return Z
+
+    def reorient(self, ifaces, force=False):
+        signs = np.ones(self.N)
+        applied = C.c_int(0)
+        ifa = np.asarray(ifaces, np.int32)
+        rc = lib().orc_reorient_deltagap(self.h, _p(ifa, _i32), len(ifa), int(force), _p(signs),
+                                         C.byref(applied))
+        if rc:
+            raise ValueError("delta-gap edges are not connected")
+        return signs if applied.value else None
+
+    def rhs_deltagap(self, freq, ifaces, voltage=1.0, signs=None):
+        V = np.zeros(self.N, np.complex128)
+        ifa = np.asarray(ifaces, np.int32)
+        lib().orc_rhs_deltagap(self.h, freq, _p(ifa, _i32), len(ifa), complex(voltage).real,
+                               complex(voltage).imag, None if signs is None else _p(signs),
+                               _p(V))
+        return V
+
+    def rhs_planewave(self, freq, theta, phi, eth=1.0, eph=0.0, degree=5):
+        V = np.zeros(self.N, np.complex128)
+        rc = lib().orc_rhs_planewave(self.h, freq, degree, theta, phi, complex(eth).real,
+                                     complex(eth).imag, complex(eph).real, complex(eph).imag,
+                                     _p(V))
+        if rc:
+            raise ValueError("bad degree")
+        return V
+
+    def scatter(self, I):
+        I = np.ascontiguousarray(I, np.complex128)
+        T = self.sm.nt
+        AJ = np.zeros((T, 3), np.complex128)
+        AdivJ = np.zeros(T, np.complex128)
+        J = np.zeros((T, 3), np.complex128)
+        lib().orc_scatter(self.h, _p(I), _p(AJ),
+                          _p(AdivJ), _p(J))
+        return AJ, AdivJ, J
+
+    def fields(self, freq, AJ, AdivJ, pts):
+        pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+        E = np.zeros((pts.shape[0], 3), np.complex128)
+        H = np.zeros((pts.shape[0], 3), np.complex128)
+        AJ = np.ascontiguousarray(AJ, np.complex128)
+        AdivJ = np.ascontiguousarray(AdivJ, np.complex128)
+        lib().orc_eval_fields(self.h, freq, _p(AJ),
+                              _p(AdivJ), _p(pts), pts.shape[0],
+                              _p(E), _p(H))
+        return E, H
+
+    def port_values(self, I, ifaces, voltage=1.0, signs=None, Z0=50.0):
+        I = np.ascontiguousarray(I, np.complex128)
+        ifa = np.asarray(ifaces, np.int32)
+        out = np.zeros(10)
+        lib().orc_port_values(self.h, _p(I), _p(ifa, _i32), len(ifa),
+                              complex(voltage).real, complex(voltage).imag,
+                              None if signs is None else _p(signs), Z0, _p(out))
+        return {"current": complex(out[0], out[1]), "power": complex(out[2], out[3]),
+                "impedance": complex(out[4], out[5]), "gamma": complex(out[6], out[7]),
+                "swr": out[8]}
+
+    def gain(self, freq, AJ, AdivJ, pwr, R=10.0):
+        g = np.zeros(1080)
+        AJ = np.ascontiguousarray(AJ, np.complex128)
+        AdivJ = np.ascontiguousarray(AdivJ, np.complex128)
+        lib().orc_gain(self.h, freq, _p(AJ), _p(AdivJ),
+                       pwr, R, _p(g))
+        return g.reshape(3, 360)
+
+    def bistatic(self, freq, AJ, AdivJ, R=10.0):
+        tab = np.zeros((359, 4))
+        AJ = np.ascontiguousarray(AJ, np.complex128)
+        AdivJ = np.ascontiguousarray(AdivJ, np.complex128)
+        lib().orc_bistatic(self.h, freq, _p(AJ),
+                           _p(AdivJ), R, _p(tab.reshape(-1)))
+        return tab
+
+
+def dunavant(degree, p0, p1, p2):
+    pts = np.zeros((100, 3))
+    w = np.zeros(100)
+    n = lib().orc_dunavant(degree, _p(np.asarray(p0, np.float64)), _p(np.asarray(p1, np.float64)),
+                           _p(np.asarray(p2, np.float64)), _p(pts), _p(w))
+    if n < 0:
+        raise ValueError("degree out of range")
+    return pts[:n], w[:n]
+
+
+def lu_solve(A, B):
+    """zgetrf/zgetrs restatement; returns (X, info)."""
+    A = np.array(A, np.complex128, order="F", copy=True)
+    B2 = np.array(B, np.complex128, order="F", copy=True)
+    if B2.ndim == 1:
+        B2 = B2.reshape(-1, 1, order="F")
+    info = lib().orc_lu_solve(_p(A), A.shape[0],
+                              _p(B2), B2.shape[1])
+    return (B2[:, 0] if np.ndim(B) == 1 else B2), info
+
+
+def rcs_unit_Ei(Es, theta_s, phi_s, R):
+    out = np.zeros(2)
+    Es = np.ascontiguousarray(Es, np.complex128)
+    lib().orc_rcs_unit_Ei(_p(Es), theta_s, phi_s, R, _p(out))
+    return out
+
+
+# ---- end-to-end reference flows (restating do_sweep for one frequency) ------------------------
+
+def antenna_analysis(sm, freq, degree=5, approx=False, voltage=1.0, R=10.0, Z0=50.0,
+                     force_reorient=False, force_symmetry=False):
+    """One frequency of do_sweep(sim, antenna_analysis&): src/emw_ana_antenna.cpp:236-272,
+    run_context src/emw_solver.h:92-167, postpro_context src/emw_ana_antenna.cpp:107-149."""
+    m = OracleMesh(sm)
+    signs = m.reorient(sm.feed, force_reorient)
+    Z = m.fill(freq, degree, approx)
+    V = m.rhs_deltagap(freq, sm.feed, voltage, signs)
+    if force_symmetry:
+        Z = (Z + Z.T) / 2.0
+    I, info = lu_solve(Z, V)
+    AJ, AdivJ, _ = m.scatter(I)
+    port = m.port_values(I, sm.feed, voltage, signs, Z0)
+    gains = m.gain(freq, AJ, AdivJ, port["power"].real, R)
+    return {"mesh": m, "I": I, "V": V, "port": port, "gain": gains, "signs": signs, "info": info}
+
+
+def bistatic_analysis(sm, freq, theta, phi, eth=1.0, eph=1.0, degree=5, approx=False, R=10.0):
+    """One frequency of do_sweep(sim, bistatic_rcs_analysis&): src/emw_ana_bistatic_rcs.cpp:120-151."""
+    m = OracleMesh(sm)
+    Z = m.fill(freq, degree, approx)
+    V = m.rhs_planewave(freq, theta, phi, eth, eph, degree)
+    I, info = lu_solve(Z, V)
+    AJ, AdivJ, _ = m.scatter(I)
+    tab = m.bistatic(freq, AJ, AdivJ, R)
+    return {"mesh": m, "I": I, "V": V, "table": tab, "info": info}

@@ -1,0 +1,944 @@
+/*
+ * cpu_ref — FP64 CPU restatement of frico's EFIE Method-of-Moments hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle that the HIP library is checked
+ * against.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it, and only as the checker or the CPU baseline; the product library never links or
+ * calls it.
+ *
+ * Every function restates the reference's algorithm from its source read as text, keeping
+ * its loop structure, accumulation order and arithmetic expression order, with the
+ * citation (path:line, relative to the reference root) next to it.  The Dunavant table
+ * is the published data (D. A. Dunavant 1985) at the 15 digits the reference uses.
+ *
+ * Parity pinning (see tests/test_oracle_*.py): quadrature exactness and weight sums,
+ * RWG invariants (T- < T+, divergence, normal continuity), numpy second restatement of
+ * the fill on tiny meshes, Z symmetry properties, LU backward error and scipy zgetrf,
+ * Hertzian-dipole closed form for the fields, loose physics pins (dipole Zin, gain,
+ * plate physical-optics RCS) and the reference-held dipole impedance table.
+ *
+ * Build: oracle/Makefile (g++ -O2 -fopenmp -shared).  ABI: plain extern "C".
+ */
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstdint>
+#include <cstring>
+#include <list>
+#include <stdexcept>
+#include <vector>
+
+#include "../frico_amd/csrc/dunavant_rules.inc"
+
+namespace {
+
+using cd = std::complex<double>;
+
+/* constants: src/constants.h:23-24 */
+constexpr double MU0 = 1.256637061435917e-06;
+constexpr double EPS0 = 8.8541878188e-12;
+
+/* vec3 arithmetic: src/geom_point.h:30-138 */
+struct vec3 {
+    double x, y, z;
+};
+inline vec3 operator+(const vec3& a, const vec3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline vec3 operator-(const vec3& a, const vec3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline vec3 operator*(const vec3& a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline vec3 operator/(const vec3& a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(const vec3& a, const vec3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline vec3 cross(const vec3& a, const vec3& b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm(const vec3& v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+struct cvec3 {
+    cd x, y, z;
+};
+
+struct qpoint {
+    vec3 p;
+    double w;
+};
+
+struct edge_t {
+    int64_t iv0, iv1; /* iv0 < iv1: src/geom_mesh.h:35-57 */
+    bool operator<(const edge_t& o) const { return iv0 < o.iv0 || (iv0 == o.iv0 && iv1 < o.iv1); }
+    bool operator==(const edge_t& o) const { return iv0 == o.iv0 && iv1 == o.iv1; }
+};
+inline edge_t mk_edge(int64_t a, int64_t b) { return {std::min(a, b), std::max(a, b)}; }
+
+struct neighbours {
+    int64_t itminus = -1;
+    int loc_eminus = -1;
+    int64_t itplus = -1;
+    int loc_eplus = -1;
+    bool has_iface = false;
+    int iface = 0;
+};
+
+struct rwg {
+    double Aminus, Aplus;
+    int64_t itminus, itplus;
+    vec3 pminus, pplus;
+    double length;
+    int64_t edge_index, matrix_index;
+    bool has_iface;
+    int iface;
+    int loc_eminus, loc_eplus;
+};
+
+} // namespace
+
+struct orc_mesh {
+    std::vector<vec3> V;
+    std::vector<std::array<int64_t, 3>> T;
+    std::vector<int> tag;
+    std::vector<edge_t> edges;
+    std::vector<neighbours> nb;
+    std::vector<rwg> bf;
+    /* per-triangle quadrature caches (same values the reference regenerates per pair) */
+    int cached_degree = -1;
+    std::vector<std::vector<qpoint>> q_reg, q_sub;
+};
+
+namespace {
+
+/* measure(triangle) = 0.5 |e0 x e1|: src/geom_mesh.cpp:71-78 */
+double tri_measure(const orc_mesh& m, int64_t t)
+{
+    const auto& tr = m.T[t];
+    vec3 e0 = m.V[tr[1]] - m.V[tr[0]];
+    vec3 e1 = m.V[tr[2]] - m.V[tr[1]];
+    return 0.5 * norm(cross(e0, e1));
+}
+
+/* barycenter(triangle): src/geom_mesh.cpp:64-69 */
+vec3 tri_bar(const orc_mesh& m, int64_t t)
+{
+    const auto& tr = m.T[t];
+    return (m.V[tr[0]] + m.V[tr[1]] + m.V[tr[2]]) / 3.0;
+}
+
+/* measure(edge) = hypot(dx, dy) -- z ignored, as the reference does: src/geom_mesh.cpp:46-52 */
+double edge_measure(const orc_mesh& m, const edge_t& e)
+{
+    vec3 d = m.V[e.iv1] - m.V[e.iv0];
+    return std::hypot(d.x, d.y);
+}
+
+vec3 edge_bar(const orc_mesh& m, const edge_t& e) { return (m.V[e.iv0] + m.V[e.iv1]) / 2.0; }
+
+/* area_triangle_kahan: src/quadratures.cpp:41-57 */
+double area_kahan(const vec3& p0, const vec3& p1, const vec3& p2)
+{
+    const double l10 = norm(p1 - p0);
+    const double l20 = norm(p2 - p0);
+    const double l21 = norm(p2 - p1);
+    std::array<double, 3> len{l10, l20, l21};
+    std::sort(len.begin(), len.end());
+    const double a = len[2], b = len[1], c = len[0];
+    return 0.25 * std::sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)));
+}
+
+inline vec3 bary(const vec3& p0, const vec3& p1, const vec3& p2, double l0, double l1, double l2)
+{
+    return p0 * l0 + p1 * l1 + p2 * l2;
+}
+
+/* dunavant(degree, p0, p1, p2): src/quadratures.cpp:321-372 (orbit expansion order :348-368) */
+std::vector<qpoint> dunavant(int degree, const vec3& p0, const vec3& p1, const vec3& p2)
+{
+    if (degree > FRICO_DUNAVANT_MAX_DEGREE || degree < 0)
+        throw std::invalid_argument("Dunavant quadrature: degree out of range");
+    int rule = (degree == 0) ? 0 : degree - 1;
+    double area = area_kahan(p0, p1, p2);
+    std::vector<qpoint> ret;
+    ret.reserve(frico_dunavant_npoints[rule]);
+    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++) {
+        const auto& d = frico_dunavant_orbits[o];
+        double l0 = d.a, l1 = d.b, l2 = d.c, w = d.w * area;
+        switch (d.orbit) {
+        case 1:
+            ret.push_back({bary(p0, p1, p2, l0, l1, l2), w});
+            break;
+        case 3:
+            ret.push_back({bary(p0, p1, p2, l0, l1, l2), w});
+            ret.push_back({bary(p0, p1, p2, l1, l2, l0), w});
+            ret.push_back({bary(p0, p1, p2, l1, l0, l2), w});
+            break;
+        case 6:
+            ret.push_back({bary(p0, p1, p2, l0, l1, l2), w});
+            ret.push_back({bary(p0, p1, p2, l0, l2, l1), w});
+            ret.push_back({bary(p0, p1, p2, l1, l0, l2), w});
+            ret.push_back({bary(p0, p1, p2, l1, l2, l0), w});
+            ret.push_back({bary(p0, p1, p2, l2, l0, l1), w});
+            ret.push_back({bary(p0, p1, p2, l2, l1, l0), w});
+            break;
+        }
+    }
+    return ret;
+}
+
+/* integrate: src/quadratures.cpp:2678-2683 */
+std::vector<qpoint> integrate(const orc_mesh& m, int64_t t, int degree)
+{
+    const auto& tr = m.T[t];
+    return dunavant(degree, m.V[tr[0]], m.V[tr[1]], m.V[tr[2]]);
+}
+
+/* integrate_subtri: src/quadratures.cpp:2685-2702 */
+std::vector<qpoint> integrate_subtri(const orc_mesh& m, int64_t t, int degree)
+{
+    const auto& tr = m.T[t];
+    vec3 bar = tri_bar(m, t);
+    auto ret = dunavant(degree, m.V[tr[0]], m.V[tr[1]], bar);
+    auto q = dunavant(degree, m.V[tr[1]], m.V[tr[2]], bar);
+    ret.insert(ret.end(), q.begin(), q.end());
+    q = dunavant(degree, m.V[tr[2]], m.V[tr[0]], bar);
+    ret.insert(ret.end(), q.begin(), q.end());
+    return ret;
+}
+
+void cache_rules(orc_mesh& m, int degree)
+{
+    if (m.cached_degree == degree)
+        return;
+    m.q_reg.resize(m.T.size());
+    m.q_sub.resize(m.T.size());
+    for (size_t t = 0; t < m.T.size(); t++) {
+        m.q_reg[t] = integrate(m, (int64_t)t, degree);
+        m.q_sub[t] = integrate_subtri(m, (int64_t)t, degree);
+    }
+    m.cached_degree = degree;
+}
+
+inline double kwave(double freq)
+{
+    double omega = 2.0 * M_PI * freq;
+    return omega * std::sqrt(MU0 * EPS0);
+}
+
+} // namespace
+
+extern "C" {
+
+/*
+ * Build mesh connectivity and the RWG function space.
+ *   edges: every triangle pushes (v0,v1),(v1,v2),(v2,v0); sort + unique   src/input_gmsh.cpp:120-140
+ *   boundary (1D entity) edges must exist among them, else lonely_edge    src/input_gmsh.cpp:143-199
+ *   T-/T+ by first/second visit, T- < T+; >2 triangles -> error           src/input_gmsh.cpp:216-262
+ *   interface tag of internal edges that are 1D entity edges              src/input_gmsh.cpp:264-269
+ *   RWG per internal edge in edge order, free vertex evmap {2,0,1}         src/rwg_basis.cpp:27-68
+ * Vertices must already be compacted (every vertex used), as load_from_gmsh does
+ * (src/input_gmsh.cpp:335-394).
+ * Returns 0, -1 (lonely edge), -2 (edge shared by more than two triangles), -3 (bad index).
+ */
+int orc_mesh_build(const double* verts, int64_t nv, const int64_t* tris, const int32_t* tags,
+                   int64_t nt, const int64_t* bedges, const int32_t* btags, int64_t nbe,
+                   orc_mesh** out)
+{
+    auto* m = new orc_mesh;
+    m->V.resize(nv);
+    for (int64_t i = 0; i < nv; i++)
+        m->V[i] = {verts[3 * i], verts[3 * i + 1], verts[3 * i + 2]};
+    m->T.resize(nt);
+    m->tag.resize(nt);
+    for (int64_t t = 0; t < nt; t++) {
+        for (int k = 0; k < 3; k++) {
+            int64_t v = tris[3 * t + k];
+            if (v < 0 || v >= nv) {
+                delete m;
+                return -3;
+            }
+            m->T[t][k] = v;
+        }
+        m->tag[t] = tags ? tags[t] : 0;
+        const auto& tr = m->T[t];
+        m->edges.push_back(mk_edge(tr[0], tr[1]));
+        m->edges.push_back(mk_edge(tr[1], tr[2]));
+        m->edges.push_back(mk_edge(tr[2], tr[0]));
+    }
+    std::sort(m->edges.begin(), m->edges.end());
+    m->edges.erase(std::unique(m->edges.begin(), m->edges.end()), m->edges.end());
+
+    auto offset = [&](const edge_t& e) -> int64_t {
+        auto it = std::lower_bound(m->edges.begin(), m->edges.end(), e);
+        if (it != m->edges.end() && *it == e)
+            return it - m->edges.begin();
+        return -1;
+    };
+
+    std::vector<std::pair<int64_t, int>> bptr;
+    for (int64_t b = 0; b < nbe; b++) {
+        int64_t o = offset(mk_edge(bedges[2 * b], bedges[2 * b + 1]));
+        if (o < 0) {
+            delete m;
+            return -1;
+        }
+        bptr.push_back({o, btags ? btags[b] : 0});
+    }
+
+    m->nb.assign(m->edges.size(), neighbours{});
+    std::vector<int> flags(m->edges.size(), 0);
+    for (int64_t t = 0; t < nt; t++) {
+        const auto& tr = m->T[t];
+        std::array<edge_t, 3> es{mk_edge(tr[0], tr[1]), mk_edge(tr[1], tr[2]), mk_edge(tr[2], tr[0])};
+        for (int ie = 0; ie < 3; ie++) {
+            int64_t o = offset(es[ie]);
+            auto& en = m->nb[o];
+            if (en.itplus >= 0) {
+                delete m;
+                return -2;
+            }
+            if (flags[o]) {
+                if (en.itminus < t) {
+                    en.itplus = t;
+                    en.loc_eplus = ie;
+                } else {
+                    en.itplus = en.itminus;
+                    en.itminus = t;
+                    en.loc_eplus = en.loc_eminus;
+                    en.loc_eminus = ie;
+                }
+            } else {
+                en.itminus = t;
+                en.loc_eminus = ie;
+                flags[o] = 1;
+            }
+        }
+    }
+    for (auto& [o, tg] : bptr) {
+        if (m->nb[o].itplus >= 0) {
+            m->nb[o].has_iface = true;
+            m->nb[o].iface = tg;
+        }
+    }
+
+    const int evmap[3] = {2, 0, 1};
+    int64_t midx = 0;
+    for (size_t ie = 0; ie < m->nb.size(); ie++) {
+        const auto& en = m->nb[ie];
+        if (en.itplus < 0)
+            continue;
+        const auto& Tm = m->T[en.itminus];
+        const auto& Tp = m->T[en.itplus];
+        rwg b;
+        b.Aminus = tri_measure(*m, en.itminus);
+        b.Aplus = tri_measure(*m, en.itplus);
+        b.itminus = en.itminus;
+        b.itplus = en.itplus;
+        b.pminus = m->V[Tm[evmap[en.loc_eminus]]];
+        b.pplus = m->V[Tp[evmap[en.loc_eplus]]];
+        b.length = edge_measure(*m, m->edges[ie]);
+        b.edge_index = (int64_t)ie;
+        b.matrix_index = midx++;
+        b.has_iface = en.has_iface;
+        b.iface = en.iface;
+        b.loc_eminus = en.loc_eminus;
+        b.loc_eplus = en.loc_eplus;
+        m->bf.push_back(b);
+    }
+    *out = m;
+    return 0;
+}
+
+void orc_mesh_free(orc_mesh* m) { delete m; }
+int64_t orc_mesh_num_rwg(const orc_mesh* m) { return (int64_t)m->bf.size(); }
+int64_t orc_mesh_num_edges(const orc_mesh* m) { return (int64_t)m->edges.size(); }
+
+/* Export the RWG table (one row per basis function, in matrix order). */
+void orc_mesh_rwg(const orc_mesh* m, int64_t* itm, int64_t* itp, int32_t* lem, int32_t* lep,
+                  int64_t* edge, int32_t* has_iface, int32_t* iface, double* len, double* Am,
+                  double* Ap, double* pm, double* pp)
+{
+    for (size_t i = 0; i < m->bf.size(); i++) {
+        const auto& b = m->bf[i];
+        itm[i] = b.itminus;
+        itp[i] = b.itplus;
+        lem[i] = b.loc_eminus;
+        lep[i] = b.loc_eplus;
+        edge[i] = b.edge_index;
+        has_iface[i] = b.has_iface;
+        iface[i] = b.iface;
+        len[i] = b.length;
+        Am[i] = b.Aminus;
+        Ap[i] = b.Aplus;
+        pm[3 * i] = b.pminus.x;
+        pm[3 * i + 1] = b.pminus.y;
+        pm[3 * i + 2] = b.pminus.z;
+        pp[3 * i] = b.pplus.x;
+        pp[3 * i + 1] = b.pplus.y;
+        pp[3 * i + 2] = b.pplus.z;
+    }
+}
+
+void orc_mesh_edges(const orc_mesh* m, int64_t* e2)
+{
+    for (size_t i = 0; i < m->edges.size(); i++) {
+        e2[2 * i] = m->edges[i].iv0;
+        e2[2 * i + 1] = m->edges[i].iv1;
+    }
+}
+
+/* Quadrature rule on three points (for tests): returns the number of points written. */
+int orc_dunavant(int degree, const double* p0, const double* p1, const double* p2, double* pts,
+                 double* w)
+{
+    try {
+        auto q = dunavant(degree, {p0[0], p0[1], p0[2]}, {p1[0], p1[1], p1[2]}, {p2[0], p2[1], p2[2]});
+        for (size_t i = 0; i < q.size(); i++) {
+            pts[3 * i] = q[i].p.x;
+            pts[3 * i + 1] = q[i].p.y;
+            pts[3 * i + 2] = q[i].p.z;
+            w[i] = q[i].w;
+        }
+        return (int)q.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+int orc_subtri(const orc_mesh* m, int64_t t, int degree, double* pts, double* w)
+{
+    try {
+        auto q = integrate_subtri(*m, t, degree);
+        for (size_t i = 0; i < q.size(); i++) {
+            pts[3 * i] = q[i].p.x;
+            pts[3 * i + 1] = q[i].p.y;
+            pts[3 * i + 2] = q[i].p.z;
+            w[i] = q[i].w;
+        }
+        return (int)q.size();
+    } catch (...) {
+        return -1;
+    }
+}
+
+/*
+ * Quadrature-mode impedance matrix: src/emw_solver.cpp:45-142.
+ * Z is N x N complex128 column-major; entry (j, i) = Z[j + i*N] (src/emw_solver.cpp:139).
+ * The reference regenerates both rules per (i, j); the rules depend only on the triangle and
+ * the split flag, so they are computed once per triangle here (identical values).
+ */
+int orc_fill_quad(orc_mesh* m, double freq, int degree, double* Zout)
+{
+    if (degree < 0 || degree > FRICO_DUNAVANT_MAX_DEGREE)
+        return -2;
+    cache_rules(*m, degree);
+    const int64_t N = (int64_t)m->bf.size();
+    cd* Z = reinterpret_cast<cd*>(Zout);
+    double omega = 2.0 * M_PI * freq;
+    double k = omega * std::sqrt(MU0 * EPS0);
+    double inv_ksq = 1. / (omega * omega * MU0 * EPS0);
+
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t ii = 0; ii < N; ii++) {
+        const rwg& ibf = m->bf[ii];
+        const auto& iqps_minus = m->q_reg[ibf.itminus];
+        const auto& iqps_plus = m->q_reg[ibf.itplus];
+        for (int64_t jj = 0; jj < N; jj++) {
+            const rwg& jbf = m->bf[jj];
+            bool split_minus = (jbf.itminus == ibf.itminus) || (jbf.itminus == ibf.itplus);
+            bool split_plus = (jbf.itplus == ibf.itminus) || (jbf.itplus == ibf.itplus);
+            const auto& jqps_minus = split_minus ? m->q_sub[jbf.itminus] : m->q_reg[jbf.itminus];
+            const auto& jqps_plus = split_plus ? m->q_sub[jbf.itplus] : m->q_reg[jbf.itplus];
+
+            cd entry = 0.0;
+            double prodl = 0.25 * ibf.length * jbf.length * M_1_PI;
+
+            auto block = [&](const std::vector<qpoint>& iq, const std::vector<qpoint>& jq,
+                             const vec3& pi, const vec3& pj, double invAA, double sgn) {
+                for (const auto& iqp : iq) {
+                    for (const auto& jqp : jq) {
+                        double v = 0.25 * dot(iqp.p - pi, jqp.p - pj);
+                        double s = inv_ksq;
+                        double w = iqp.w * jqp.w * invAA;
+                        double Rij = norm(iqp.p - jqp.p);
+                        cd g = std::exp(cd{0, -k * Rij}) / Rij;
+                        if (sgn > 0)
+                            entry += prodl * w * (v - s) * g;
+                        else
+                            entry -= prodl * w * (v - s) * g;
+                    }
+                }
+            };
+            block(iqps_minus, jqps_minus, ibf.pminus, jbf.pminus, 1. / (ibf.Aminus * jbf.Aminus), +1);
+            block(iqps_minus, jqps_plus, ibf.pminus, jbf.pplus, 1. / (ibf.Aminus * jbf.Aplus), -1);
+            block(iqps_plus, jqps_minus, ibf.pplus, jbf.pminus, 1. / (ibf.Aplus * jbf.Aminus), -1);
+            block(iqps_plus, jqps_plus, ibf.pplus, jbf.pplus, 1. / (ibf.Aplus * jbf.Aplus), +1);
+            Z[jbf.matrix_index + ibf.matrix_index * N] = entry;
+        }
+    }
+    return 0;
+}
+
+/*
+ * Approximate (MOM3D centroid) impedance matrix: src/emw_solver.cpp:144-220.
+ * Self pairs use the equivalent-disk term 0.25*(sqrt(4 pi)/sqrt(A) - jk)/pi (:185-214).
+ */
+int orc_fill_approx(orc_mesh* m, double freq, double* Zout)
+{
+    const int64_t N = (int64_t)m->bf.size();
+    cd* Z = reinterpret_cast<cd*>(Zout);
+    double omega = 2.0 * M_PI * freq;
+    double k = omega * std::sqrt(MU0 * EPS0);
+    double inv_ksq = 1. / (omega * omega * MU0 * EPS0);
+    double sqrt_4pi = std::sqrt(4 * M_PI);
+
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int64_t ii = 0; ii < N; ii++) {
+        const rwg& ibf = m->bf[ii];
+        vec3 iTmbar = tri_bar(*m, ibf.itminus);
+        vec3 iTpbar = tri_bar(*m, ibf.itplus);
+        for (int64_t jj = 0; jj < N; jj++) {
+            const rwg& jbf = m->bf[jj];
+            vec3 jTmbar = tri_bar(*m, jbf.itminus);
+            vec3 jTpbar = tri_bar(*m, jbf.itplus);
+            bool selfmm = ibf.itminus == jbf.itminus;
+            bool selfmp = ibf.itminus == jbf.itplus;
+            bool selfpm = ibf.itplus == jbf.itminus;
+            bool selfpp = ibf.itplus == jbf.itplus;
+            double sqrt_Am = std::sqrt(tri_measure(*m, ibf.itminus));
+            double sqrt_Ap = std::sqrt(tri_measure(*m, ibf.itplus));
+            cd jk{0., k};
+            cd entry = 0.0;
+            auto term = [&](const vec3& ci, const vec3& pi, const vec3& cj, const vec3& pj,
+                            bool self, double sqrtA) -> cd {
+                double rho_c = dot(ci - pi, cj - pj);
+                double R = norm(ci - cj);
+                cd G = std::exp(cd{0, -k * R}) / (4.0 * M_PI * R);
+                if (self)
+                    G = 0.25 * (sqrt_4pi / sqrtA - jk) * M_1_PI;
+                return ibf.length * jbf.length * (0.25 * rho_c * G - inv_ksq * G);
+            };
+            entry += term(iTmbar, ibf.pminus, jTmbar, jbf.pminus, selfmm, sqrt_Am);
+            entry -= term(iTmbar, ibf.pminus, jTpbar, jbf.pplus, selfmp, sqrt_Am);
+            entry -= term(iTpbar, ibf.pplus, jTmbar, jbf.pminus, selfpm, sqrt_Ap);
+            entry += term(iTpbar, ibf.pplus, jTpbar, jbf.pplus, selfpp, sqrt_Ap);
+            Z[jbf.matrix_index + ibf.matrix_index * N] = entry;
+        }
+    }
+    return 0;
+}
+
+/*
+ * Delta-gap edge reorientation: src/emw_solver.cpp:222-350.  Writes signs[N] (only the gap
+ * entries are meaningful) and sets *applied = 1 when the reference would have reoriented.
+ * Returns 0 on success or when no fix is needed, -1 if no edges / not connected.
+ */
+int orc_reorient_deltagap(const orc_mesh* m, const int32_t* ifaces, int nif, int force,
+                          double* signs, int* applied)
+{
+    *applied = 0;
+    bool fix_needed = false;
+    std::list<std::pair<edge_t, int64_t>> edges;
+    for (const auto& b : m->bf) {
+        if (!b.has_iface)
+            continue;
+        for (int f = 0; f < nif; f++) {
+            if (b.iface == ifaces[f]) {
+                if (m->tag[b.itplus] <= m->tag[b.itminus])
+                    fix_needed = true;
+                edges.push_back({m->edges[b.edge_index], b.matrix_index});
+            }
+        }
+    }
+    if (!(fix_needed || force))
+        return 0;
+    if (edges.empty())
+        return -1;
+    struct nodepair {
+        int64_t first, second, bf_index;
+    };
+    std::list<nodepair> seglist;
+    auto eit = edges.begin();
+    seglist.push_back({eit->first.iv0, eit->first.iv1, eit->second});
+    edges.erase(eit);
+    while (!edges.empty()) {
+        size_t before = edges.size();
+        for (eit = edges.begin(); eit != edges.end();) {
+            auto f = seglist.front();
+            auto b = seglist.back();
+            auto cur = eit++;
+            /* the reference erases `cur` and may test it again in the next `if`; a removed
+             * element is never revisited here, which matches it whenever one edge matches once */
+            bool erased = false;
+            if (!erased && !edges.empty() && cur->first.iv0 == f.first) {
+                seglist.push_front({cur->first.iv1, cur->first.iv0, cur->second});
+                edges.erase(cur);
+                erased = true;
+            }
+            if (!erased && !edges.empty() && cur->first.iv1 == f.first) {
+                seglist.push_front({cur->first.iv0, cur->first.iv1, cur->second});
+                edges.erase(cur);
+                erased = true;
+            }
+            if (!erased && !edges.empty() && cur->first.iv0 == b.second) {
+                seglist.push_back({cur->first.iv0, cur->first.iv1, cur->second});
+                edges.erase(cur);
+                erased = true;
+            }
+            if (!erased && !edges.empty() && cur->first.iv1 == b.second) {
+                seglist.push_back({cur->first.iv1, cur->first.iv0, cur->second});
+                edges.erase(cur);
+                erased = true;
+            }
+        }
+        if (edges.size() == before)
+            return -1;
+    }
+    const auto& seg0 = seglist.front();
+    const auto& bf0 = m->bf[seg0.bf_index];
+    vec3 dir = tri_bar(*m, bf0.itplus) - edge_bar(*m, m->edges[bf0.edge_index]);
+    double sign = +1;
+    for (auto& seg : seglist) {
+        const auto& b = m->bf[seg.bf_index];
+        vec3 curdir = tri_bar(*m, b.itplus) - edge_bar(*m, m->edges[b.edge_index]);
+        if (dot(dir, curdir) < 0)
+            sign *= -1;
+        signs[b.matrix_index] = sign;
+        dir = curdir;
+    }
+    *applied = 1;
+    return 0;
+}
+
+/* update_rhs(delta_gap): src/emw_solver.cpp:352-378.  V accumulates (+=).  signs may be NULL. */
+int orc_rhs_deltagap(const orc_mesh* m, double freq, const int32_t* ifaces, int nif, double vre,
+                     double vim, const double* signs, double* Vout)
+{
+    cd* V = reinterpret_cast<cd*>(Vout);
+    double omega = 2.0 * M_PI * freq;
+    cd volt{vre, vim};
+    for (const auto& b : m->bf) {
+        if (!b.has_iface)
+            continue;
+        for (int f = 0; f < nif; f++) {
+            if (b.iface == ifaces[f]) {
+                cd v{0.0, -b.length / (omega * MU0)};
+                double sign = signs ? signs[b.matrix_index] : 1.0;
+                V[b.matrix_index] += v * volt * sign;
+            }
+        }
+    }
+    return 0;
+}
+
+/*
+ * update_rhs(plane_wave): src/emw_solver.cpp:380-416 with plane_wave src/sources.h:34-75.
+ * E(r) = (Eth theta_hat + Eph phi_hat) exp(+j k dir.r), dir pointing from the origin to the
+ * source.  V accumulates (+=).
+ */
+int orc_rhs_planewave(orc_mesh* m, double freq, int degree, double theta, double phi, double eth_re,
+                      double eth_im, double eph_re, double eph_im, double* Vout)
+{
+    if (degree < 0 || degree > FRICO_DUNAVANT_MAX_DEGREE)
+        return -2;
+    cache_rules(*m, degree);
+    cd* V = reinterpret_cast<cd*>(Vout);
+    double omega = 2.0 * M_PI * freq;
+    double k = omega * std::sqrt(MU0 * EPS0);
+    vec3 dir{std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+    vec3 th{std::cos(theta) * std::cos(phi), std::cos(theta) * std::sin(phi), -std::sin(theta)};
+    vec3 ph{-std::sin(phi), std::cos(phi), 0.0};
+    cd Eth{eth_re, eth_im}, Eph{eph_re, eph_im};
+    cvec3 E0{Eth * th.x + Eph * ph.x, Eth * th.y + Eph * ph.y, Eth * th.z + Eph * ph.z};
+    cd jk{0.0, k};
+    auto field = [&](const vec3& p) -> cvec3 {
+        cd ph_ = std::exp(jk * dot(dir, p));
+        return {E0.x * ph_, E0.y * ph_, E0.z * ph_};
+    };
+    for (const auto& b : m->bf) {
+        cd v{0.0, 0.0};
+        double wminus = 0.5 * b.length / b.Aminus;
+        double wplus = 0.5 * b.length / b.Aplus;
+        for (const auto& qp : m->q_reg[b.itminus]) {
+            cvec3 E = field(qp.p);
+            vec3 r = qp.p - b.pminus;
+            /* Eigen's real.dot(complex): sum of r_i * E_i (no conjugation of a real vector) */
+            v -= wminus * qp.w * (r.x * E.x + r.y * E.y + r.z * E.z);
+        }
+        for (const auto& qp : m->q_reg[b.itplus]) {
+            cvec3 E = field(qp.p);
+            vec3 r = qp.p - b.pplus;
+            v += wplus * qp.w * (r.x * E.x + r.y * E.y + r.z * E.z);
+        }
+        V[b.matrix_index] += v / cd{0, -omega * MU0};
+    }
+    return 0;
+}
+
+/*
+ * Dense complex LU with partial pivoting and solve -- the LAPACK zgetrf/zgetrs semantics that
+ * solve_general uses (src/lapack_wrappers.cpp:53-133): pivot = first max of |re|+|im| (izamax),
+ * row interchanges over the whole row, unit-lower L, B overwritten by X.  A (n x n, col-major)
+ * is overwritten by the factors.  Returns 0, or k>0 if U(k,k) is exactly zero (1-based).
+ */
+int orc_lu_solve(double* Aout, int64_t n, double* Bout, int64_t nrhs)
+{
+    cd* A = reinterpret_cast<cd*>(Aout);
+    cd* B = reinterpret_cast<cd*>(Bout);
+    std::vector<int64_t> ipiv(n);
+    int info = 0;
+    auto a = [&](int64_t i, int64_t j) -> cd& { return A[i + j * n]; };
+    for (int64_t j = 0; j < n; j++) {
+        int64_t p = j;
+        double best = -1;
+        for (int64_t i = j; i < n; i++) {
+            double v = std::abs(a(i, j).real()) + std::abs(a(i, j).imag());
+            if (v > best) {
+                best = v;
+                p = i;
+            }
+        }
+        ipiv[j] = p;
+        if (a(p, j) != cd(0)) {
+            if (p != j)
+                for (int64_t c = 0; c < n; c++)
+                    std::swap(a(j, c), a(p, c));
+            cd rp = 1.0 / a(j, j);
+            for (int64_t i = j + 1; i < n; i++)
+                a(i, j) *= rp;
+        } else if (info == 0) {
+            info = (int)(j + 1);
+        }
+#pragma omp parallel for schedule(static)
+        for (int64_t c = j + 1; c < n; c++) {
+            cd u = a(j, c);
+            if (u != cd(0))
+                for (int64_t i = j + 1; i < n; i++)
+                    a(i, c) -= a(i, j) * u;
+        }
+    }
+    if (info)
+        return info;
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < nrhs; r++) {
+        cd* b = B + r * n;
+        for (int64_t j = 0; j < n; j++)
+            if (ipiv[j] != j)
+                std::swap(b[j], b[ipiv[j]]);
+        for (int64_t j = 0; j < n; j++) {
+            cd x = b[j];
+            for (int64_t i = j + 1; i < n; i++)
+                b[i] -= a(i, j) * x;
+        }
+        for (int64_t j = n - 1; j >= 0; j--) {
+            b[j] /= a(j, j);
+            cd x = b[j];
+            for (int64_t i = 0; i < j; i++)
+                b[i] -= a(i, j) * x;
+        }
+    }
+    return 0;
+}
+
+/*
+ * RWG -> triangle scatter of one current vector: src/emw_solver.h:137-164.
+ * tri_AJ and tri_J are T x 3 complex (row-major: [t][xyz]); tri_AdivJ is T complex.
+ */
+int orc_scatter(const orc_mesh* m, const double* Iin, double* AJout, double* AdivJout, double* Jout)
+{
+    const cd* I = reinterpret_cast<const cd*>(Iin);
+    size_t T = m->T.size();
+    cd* AJ = reinterpret_cast<cd*>(AJout);
+    cd* AdivJ = reinterpret_cast<cd*>(AdivJout);
+    cd* J = reinterpret_cast<cd*>(Jout);
+    std::fill(AJ, AJ + 3 * T, cd(0));
+    std::fill(AdivJ, AdivJ + T, cd(0));
+    if (J)
+        std::fill(J, J + 3 * T, cd(0));
+    for (const auto& b : m->bf) {
+        vec3 bm = tri_bar(*m, b.itminus), bp = tri_bar(*m, b.itplus);
+        double Am = tri_measure(*m, b.itminus), Ap = tri_measure(*m, b.itplus);
+        cd Ie = I[b.matrix_index];
+        vec3 fm = (bm - b.pminus) * (-b.length) / (2.0 * b.Aminus); /* eval_minus rwg_basis.h:76-81 */
+        vec3 fp = (bp - b.pplus) * b.length / (2.0 * b.Aplus);      /* eval_plus  rwg_basis.h:83-88 */
+        cd Jm[3] = {fm.x * Ie, fm.y * Ie, fm.z * Ie};
+        cd Jp[3] = {fp.x * Ie, fp.y * Ie, fp.z * Ie};
+        for (int c = 0; c < 3; c++) {
+            AJ[3 * b.itminus + c] += Am * Jm[c];
+            AJ[3 * b.itplus + c] += Ap * Jp[c];
+            if (J) {
+                J[3 * b.itminus + c] += Jm[c];
+                J[3 * b.itplus + c] += Jp[c];
+            }
+        }
+        AdivJ[b.itminus] += Am * (-b.length / b.Aminus) * Ie; /* div_minus rwg_basis.h:90-94 */
+        AdivJ[b.itplus] += Ap * (b.length / b.Aplus) * Ie;    /* div_plus  rwg_basis.h:96-100 */
+    }
+    return 0;
+}
+
+/*
+ * eval_fields at points: src/emw_postpro_common.cpp:31-66.  E uses exp(-jkR), H uses the
+ * reference's exp(+jkR) (:60-61).  E, H: npts x 3 complex (row-major).
+ */
+int orc_eval_fields(const orc_mesh* m, double freq, const double* AJin, const double* AdivJin,
+                    const double* pts, int64_t npts, double* Eout, double* Hout)
+{
+    const cd* AJ = reinterpret_cast<const cd*>(AJin);
+    const cd* AdivJ = reinterpret_cast<const cd*>(AdivJin);
+    cd* Eo = reinterpret_cast<cd*>(Eout);
+    cd* Ho = reinterpret_cast<cd*>(Hout);
+    const double omega = 2.0 * M_PI * freq;
+    const double k = omega * std::sqrt(MU0 * EPS0);
+    const cd jomega{0.0, omega};
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int64_t ip = 0; ip < npts; ip++) {
+        vec3 pt{pts[3 * ip], pts[3 * ip + 1], pts[3 * ip + 2]};
+        cd E[3] = {0, 0, 0}, H[3] = {0, 0, 0};
+        for (size_t t = 0; t < m->T.size(); t++) {
+            vec3 bar = tri_bar(*m, (int64_t)t);
+            vec3 vR = pt - bar;
+            double R = norm(vR);
+            double _4piR = 4 * M_PI * R;
+            double _4piR3 = _4piR * R * R;
+            cd jkR{0, k * R};
+            cd g = std::exp(-jkR) / _4piR;
+            cd J[3] = {AJ[3 * t], AJ[3 * t + 1], AJ[3 * t + 2]};
+            cd divJ = AdivJ[t];
+            double kR2 = std::pow(k * R, 2);
+            double vr[3] = {vR.x, vR.y, vR.z};
+            for (int c = 0; c < 3; c++)
+                E[c] += -jomega * MU0 * (J[c] - divJ * (1.0 + jkR) * vr[c] / kR2) * g;
+            cd RxJ[3] = {vR.y * J[2] - vR.z * J[1], vR.z * J[0] - vR.x * J[2], vR.x * J[1] - vR.y * J[0]};
+            cd gradg = -((1.0 + jkR) / _4piR3) * std::exp(jkR);
+            for (int c = 0; c < 3; c++)
+                H[c] += RxJ[c] * gradg;
+        }
+        for (int c = 0; c < 3; c++) {
+            Eo[3 * ip + c] = E[c];
+            Ho[3 * ip + c] = H[c];
+        }
+    }
+    return 0;
+}
+
+/*
+ * Port values of a delta-gap: src/emw_ana_antenna.cpp:76-105; gamma/swr
+ * src/emw_postpro_common.cpp:99-123.  out[0..9] = I(re,im), P(re,im), Z(re,im), Gamma(re,im),
+ * swr, unused.
+ */
+int orc_port_values(const orc_mesh* m, const double* Iin, const int32_t* ifaces, int nif,
+                    double vre, double vim, const double* signs, double Z0, double* out)
+{
+    const cd* Ivec = reinterpret_cast<const cd*>(Iin);
+    cd volt{vre, vim}, cur = 0.0, pwr = 0.0;
+    for (const auto& b : m->bf) {
+        if (!b.has_iface)
+            continue;
+        for (int f = 0; f < nif; f++) {
+            if (b.iface == ifaces[f]) {
+                double sign = signs ? signs[b.matrix_index] : 1.0;
+                cd I = b.length * Ivec[b.matrix_index] * sign;
+                cur += I;
+                pwr += 0.5 * volt * std::conj(I);
+            }
+        }
+    }
+    cd Zin = volt / cur;
+    cd gam = (Zin - Z0) / (Zin + Z0);
+    double s = (1.0 + std::abs(gam)) / (1.0 - std::abs(gam));
+    out[0] = cur.real();
+    out[1] = cur.imag();
+    out[2] = pwr.real();
+    out[3] = pwr.imag();
+    out[4] = Zin.real();
+    out[5] = Zin.imag();
+    out[6] = gam.real();
+    out[7] = gam.imag();
+    out[8] = s;
+    out[9] = 0;
+    return 0;
+}
+
+/*
+ * Gain on the three reference cuts (xy, yz, xz), 360 x 1 degree at distance R:
+ * src/emw_ana_antenna.cpp:33-65.  G = 4 pi |P|^2 |S| / pwr with S = E x conj(H) / 2 and
+ * |S| = sqrt of Eigen's conjugating S.dot(S).  gains: 3 x 360 doubles.
+ */
+int orc_gain(const orc_mesh* m, double freq, const double* AJ, const double* AdivJ, double pwr,
+             double R, double* gains)
+{
+    std::vector<double> pts(3 * 1080);
+    for (int plane = 0; plane < 3; plane++)
+        for (int deg = 0; deg < 360; deg++) {
+            double th = deg * M_PI / 180;
+            double c = std::cos(th), s = std::sin(th);
+            double* P = &pts[3 * (plane * 360 + deg)];
+            if (plane == 0) {
+                P[0] = R * c; P[1] = R * s; P[2] = 0.0;
+            } else if (plane == 1) {
+                P[0] = 0.0; P[1] = R * c; P[2] = R * s;
+            } else {
+                P[0] = -R * c; P[1] = 0.0; P[2] = R * s;
+            }
+        }
+    std::vector<double> E(6 * 1080), H(6 * 1080);
+    orc_eval_fields(m, freq, AJ, AdivJ, pts.data(), 1080, E.data(), H.data());
+    for (int i = 0; i < 1080; i++) {
+        const cd* e = reinterpret_cast<const cd*>(&E[6 * i]);
+        const cd* h = reinterpret_cast<const cd*>(&H[6 * i]);
+        cd hc[3] = {std::conj(h[0]), std::conj(h[1]), std::conj(h[2])};
+        cd S[3] = {0.5 * (e[1] * hc[2] - e[2] * hc[1]), 0.5 * (e[2] * hc[0] - e[0] * hc[2]),
+                   0.5 * (e[0] * hc[1] - e[1] * hc[0])};
+        cd SS = std::conj(S[0]) * S[0] + std::conj(S[1]) * S[1] + std::conj(S[2]) * S[2];
+        double Prad = std::real(std::sqrt(SS));
+        const double* P = &pts[3 * i];
+        double Rn = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2]);
+        gains[i] = 4 * M_PI * Rn * Rn * Prad / pwr;
+    }
+    return 0;
+}
+
+/*
+ * RCS of a scattered field at (R, theta_s, phi_s): bistatic_RCS_unit_Ei
+ * src/emw_ana_bistatic_rcs.cpp:62-82.  out = {sigma_VV, sigma_HH}.
+ */
+void orc_rcs_unit_Ei(const double* Es_in, double theta_s, double phi_s, double R, double* out)
+{
+    const cd* Es = reinterpret_cast<const cd*>(Es_in);
+    double th[3] = {std::cos(theta_s) * std::cos(phi_s), std::cos(theta_s) * std::sin(phi_s),
+                    -std::sin(theta_s)};
+    double ph[3] = {-std::sin(phi_s), std::cos(phi_s), 0.0};
+    cd Et = th[0] * Es[0] + th[1] * Es[1] + th[2] * Es[2];
+    cd Ep = ph[0] * Es[0] + ph[1] * Es[1] + ph[2] * Es[2];
+    out[0] = 4.0 * M_PI * R * R * std::norm(Et);
+    out[1] = 4.0 * M_PI * R * R * std::norm(Ep);
+}
+
+/* sph2rect: src/utils.h:53-60 */
+void orc_sph2rect(double R, double theta, double phi, double* p)
+{
+    p[0] = R * std::cos(phi) * std::sin(theta);
+    p[1] = R * std::sin(phi) * std::sin(theta);
+    p[2] = R * std::cos(theta);
+}
+
+/*
+ * Bistatic RCS table: src/emw_ana_bistatic_rcs.cpp:96-117.  359 rows (i = 0..358) of
+ * {VV_theta, HH_theta, VV_phi, HH_phi}; the phi cut point is sph2rect(R, 0, i deg) (:110).
+ */
+int orc_bistatic(const orc_mesh* m, double freq, const double* AJ, const double* AdivJ, double R,
+                 double* table)
+{
+    std::vector<double> pts(3 * 718);
+    double deg2rad = M_PI / 180.0;
+    for (int i = 0; i < 359; i++) {
+        orc_sph2rect(R, deg2rad * i, 0.0, &pts[3 * (2 * i)]);
+        orc_sph2rect(R, 0.0, deg2rad * i, &pts[3 * (2 * i + 1)]);
+    }
+    std::vector<double> E(6 * 718), H(6 * 718);
+    orc_eval_fields(m, freq, AJ, AdivJ, pts.data(), 718, E.data(), H.data());
+    for (int i = 0; i < 359; i++) {
+        orc_rcs_unit_Ei(&E[6 * (2 * i)], deg2rad * i, 0.0, R, &table[4 * i]);
+        orc_rcs_unit_Ei(&E[6 * (2 * i + 1)], 0.0, deg2rad * i, R, &table[4 * i + 2]);
+    }
+    return 0;
+}
+
+} // extern "C"
