@@ -64,10 +64,10 @@ template <typename T> __host__ __device__ inline T cabs1(cx<T> a)
 namespace frico {
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 void set_error(const char* fmt, ...);
-// Raises a kernel's dynamic-LDS limit to the 160 KB of a gfx950 CU.
-inline hipError_t allow_full_lds(const void* func)
+// Raises a kernel's dynamic-LDS limit to `bytes` (gfx950 CUs have 160 KB, shared with static LDS).
+inline hipError_t allow_lds(const void* func, size_t bytes)
 {
-    return hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    return hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 } // namespace frico
 

@@ -580,7 +580,8 @@ int launch_fill_quad(const DevMesh& m, double k, double inv_ksq, int prec, void*
         if (m.q == 7) {
             hipLaunchKernelGGL((tripair_f64<7, double>), g1, dim3(256), sh, st, m, k, inv_ksq, (c128*)Cbuf);
         } else {
-            FRICO_HIP_CHECK(allow_full_lds(reinterpret_cast<const void*>(&tripair_f64<0, double>)));
+            if (sh > 64 * 1024)
+                FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&tripair_f64<0, double>), sh));
             hipLaunchKernelGGL((tripair_f64<0, double>), g1, dim3(256), sh, st, m, k, inv_ksq, (c128*)Cbuf);
         }
     } else {
@@ -588,7 +589,8 @@ int launch_fill_quad(const DevMesh& m, double k, double inv_ksq, int prec, void*
         if (m.q == 7) {
             hipLaunchKernelGGL((tripair_f32<7>), g1, dim3(256), sh, st, m, k, inv_ksq, (c64*)Cbuf);
         } else {
-            FRICO_HIP_CHECK(allow_full_lds(reinterpret_cast<const void*>(&tripair_f32<0>)));
+            if (sh > 64 * 1024)
+                FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&tripair_f32<0>), sh));
             hipLaunchKernelGGL((tripair_f32<0>), g1, dim3(256), sh, st, m, k, inv_ksq, (c64*)Cbuf);
         }
     }

@@ -583,7 +583,9 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         return set_error("lu: block size out of range"), -1;
     const int R = panel_rows<T>(nb);
     const int64_t npan = (n + nb - 1) / nb;
-    FRICO_HIP_CHECK(allow_full_lds(reinterpret_cast<const void*>(&panel_factor<T>)));
+    const size_t sh_max = ((size_t)nb * R + nb) * sizeof(cx<T>);
+    if (sh_max > 64 * 1024)
+        FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&panel_factor<T>), sh_max));
     if ((int64_t)wk.ncounters < npan)
         return set_error("lu: sync workspace too small"), -1;
     hipLaunchKernelGGL(zero_u32, dim3((unsigned)((npan + 255) / 256)), dim3(256), 0, st, wk.counters, npan);

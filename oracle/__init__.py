@@ -252,3 +252,86 @@ def bistatic_analysis(sm, freq, theta, phi, eth=1.0, eph=1.0, degree=5, approx=F
     AJ, AdivJ, _ = m.scatter(I)
     tab = m.bistatic(freq, AJ, AdivJ, R)
     return {"mesh": m, "I": I, "V": V, "table": tab, "info": info}
+
+
+# ---- adjoint reference (no counterpart in the reference code) -------------------------------
+
+def _adj_sigs():
+    L = lib()
+    L.orc_adjoint_contract.argtypes = [C.c_void_p, C.c_double, C.c_int, _d, _d, _d]
+    L.orc_fields_vjp.argtypes = [C.c_void_p, C.c_double, _d, _d, _d, _d, _d, _d]
+    L.orc_length_grad.argtypes = [C.c_void_p, C.c_int64, _d]
+    return L
+
+
+def adjoint_contract(om, freq, degree, I, W):
+    """g[v, c] = sum_ij W_j I_i dZ(j,i)/dx_{v,c} (complex), quadrature-mode Z."""
+    L = _adj_sigs()
+    I = np.ascontiguousarray(I, np.complex128)
+    W = np.ascontiguousarray(W, np.complex128)
+    g = np.zeros(om.sm.nv * 3, np.complex128)
+    rc = L.orc_adjoint_contract(om.h, freq, degree, _p(I), _p(W), _p(g))
+    if rc:
+        raise ValueError("bad degree")
+    return g.reshape(-1, 3)
+
+
+def antenna_gradients(sm, freq, degree=5, obs=(0.0, 0.0, 10.0), voltage=1.0):
+    """FP64 CPU reference of the adjoint: input impedance Zin and gain G at the observation point
+    `obs`, with dZin/dx (complex) and dG/dx (real) for every vertex coordinate.
+
+    Forward: Z I = V (delta gap), Zin = V / sum_gap l s I, G = 4 pi |obs|^2 |S| / P with
+    S = E x conj(H) / 2 (src/emw_ana_antenna.cpp:33-105).  Adjoint: Z^T lam = c (c = l s on the
+    gap), Z^T mu = d|S|/dI (Wirtinger); dI_port = (dc) I + lam^T (dV - dZ I)."""
+    L = _adj_sigs()
+    m = OracleMesh(sm)
+    signs = m.reorient(sm.feed)
+    s = np.ones(m.N) if signs is None else signs
+    Z = m.fill(freq, degree)
+    V = m.rhs_deltagap(freq, sm.feed, voltage, signs)
+    I, info = lu_solve(Z, V)
+    gap = np.isin(m.iface, sm.feed) & (m.has_iface != 0)
+    c = np.where(gap, m.length * s, 0.0).astype(np.complex128)
+    Iport = c @ I
+    Vg = complex(voltage)
+    zin = Vg / Iport
+    omega = 2 * math.pi * freq
+    mu0 = 1.256637061435917e-06
+    # fields at the observation point and |S|
+    AJ, AdivJ, _ = m.scatter(I)
+    obs = np.asarray(obs, np.float64)
+    E, H = m.fields(freq, AJ, AdivJ, obs)
+    E, H = E[0], H[0]
+    S = 0.5 * np.cross(E, np.conj(H))
+    absS = math.sqrt(float(np.real(np.vdot(S, S))))
+    P = (0.5 * Vg * np.conj(Iport)).real
+    G = 4 * math.pi * float(obs @ obs) * absS / P
+    # Wirtinger derivatives of |S| w.r.t. E and H
+    eE = 0.5 * np.cross(np.conj(H), np.conj(S)) / (2 * absS)
+    eH = 0.5 * np.cross(S, np.conj(E)) / (2 * absS)
+    gx_f = np.zeros(sm.nv * 3, np.complex128)
+    gI = np.zeros(m.N, np.complex128)
+    L.orc_fields_vjp(m.h, freq, _p(np.ascontiguousarray(I)), _p(obs), _p(np.ascontiguousarray(eE)),
+                     _p(np.ascontiguousarray(eH)), _p(gx_f), _p(gI))
+    lam = np.linalg.solve(Z.T, c)
+    mu = np.linalg.solve(Z.T, gI)
+    Clam = adjoint_contract(m, freq, degree, I, lam).reshape(-1)
+    Cmu = adjoint_contract(m, freq, degree, I, mu).reshape(-1)
+    # edge-length terms of V and of the port weights
+    dc_I = np.zeros(sm.nv * 3, np.complex128)
+    lam_dV = np.zeros(sm.nv * 3, np.complex128)
+    mu_dV = np.zeros(sm.nv * 3, np.complex128)
+    gl = np.zeros(sm.nv * 3)
+    for n in np.nonzero(gap)[0]:
+        L.orc_length_grad(m.h, int(n), _p(gl))
+        dVn = -1j * Vg * s[n] / (omega * mu0)  # dV_n / dl_n
+        dc_I += s[n] * I[n] * gl
+        lam_dV += lam[n] * dVn * gl
+        mu_dV += mu[n] * dVn * gl
+    dIport = dc_I + lam_dV - Clam
+    dzin = -Vg / Iport ** 2 * dIport
+    dP = (0.5 * Vg * np.conj(dIport)).real
+    dabsS = 2 * (gx_f.real) + 2 * (mu_dV - Cmu).real
+    dG = 4 * math.pi * float(obs @ obs) / P * dabsS - G / P * dP
+    return {"zin": zin, "gain": G, "dzin": dzin.reshape(-1, 3), "dgain": dG.reshape(-1, 3), "I": I,
+            "lam": lam, "mu": mu, "N": m.N}

@@ -941,4 +941,392 @@ int orc_bistatic(const orc_mesh* m, double freq, const double* AJ, const double*
     return 0;
 }
 
+
+/* ---------------------------------------------------------------------------------------------
+ * Adjoint support (no counterpart in the reference).  Derivatives of the quadrature-mode Z with
+ * respect to vertex coordinates, differentiated by hand through the reference's per-RWG-pair
+ * loop (src/emw_solver.cpp:56-141): quadrature points are linear in the vertices (barycentric),
+ * weights scale with the triangle area (Kahan/Heron == cross-product area mathematically),
+ * A-/A+ are cross-product areas and the edge length is hypot(dx, dy).
+ * ------------------------------------------------------------------------------------------- */
+} // extern "C"
+
+namespace {
+
+struct dqpoint {
+    vec3 p;
+    double w;
+    double beta[3];      /* p = sum_k beta[k] * V[tri[k]] */
+    vec3 dw[3];          /* d w / d V[tri[k]] */
+};
+
+/* gradient of the area of triangle (a, b, c): d/da = 0.5 n_hat x (c - b), cyclic */
+void area_grad(const vec3& a, const vec3& b, const vec3& c, vec3 g[3])
+{
+    vec3 n = cross(b - a, c - a);
+    double nn = norm(n);
+    if (nn == 0) {
+        g[0] = g[1] = g[2] = {0, 0, 0};
+        return;
+    }
+    vec3 nh = n / nn;
+    g[0] = cross(nh, c - b) * 0.5;
+    g[1] = cross(nh, a - c) * 0.5;
+    g[2] = cross(nh, b - a) * 0.5;
+}
+
+/* Dunavant rule on corners q_j = sum_k M[j][k] P_k of triangle P, with derivative data. */
+void dunavant_d(int degree, const vec3 P[3], const double M[3][3], std::vector<dqpoint>& out)
+{
+    vec3 q[3];
+    for (int j = 0; j < 3; j++)
+        q[j] = P[0] * M[j][0] + P[1] * M[j][1] + P[2] * M[j][2];
+    auto base = dunavant(degree, q[0], q[1], q[2]);
+    double area = area_kahan(q[0], q[1], q[2]);
+    vec3 ga[3];
+    area_grad(q[0], q[1], q[2], ga);
+    int rule = degree == 0 ? 0 : degree - 1;
+    size_t idx = 0;
+    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++) {
+        const auto& d = frico_dunavant_orbits[o];
+        double perm[6][3] = {{d.a, d.b, d.c}, {d.b, d.c, d.a}, {d.b, d.a, d.c}};
+        int np = d.orbit;
+        if (np == 6) {
+            double p6[6][3] = {{d.a, d.b, d.c}, {d.a, d.c, d.b}, {d.b, d.a, d.c},
+                               {d.b, d.c, d.a}, {d.c, d.a, d.b}, {d.c, d.b, d.a}};
+            std::memcpy(perm, p6, sizeof p6);
+        }
+        for (int s = 0; s < np; s++, idx++) {
+            dqpoint dp;
+            dp.p = base[idx].p;
+            dp.w = base[idx].w;
+            double wref = d.w;
+            for (int k = 0; k < 3; k++) {
+                dp.beta[k] = perm[s][0] * M[0][k] + perm[s][1] * M[1][k] + perm[s][2] * M[2][k];
+                vec3 g = ga[0] * M[0][k] + ga[1] * M[1][k] + ga[2] * M[2][k];
+                dp.dw[k] = g * wref;
+            }
+            (void)area;
+            out.push_back(dp);
+        }
+    }
+}
+
+void rules_d(const orc_mesh& m, int64_t t, int degree, bool split, std::vector<dqpoint>& out)
+{
+    const auto& tr = m.T[t];
+    vec3 P[3] = {m.V[tr[0]], m.V[tr[1]], m.V[tr[2]]};
+    out.clear();
+    if (!split) {
+        const double I3[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+        dunavant_d(degree, P, I3, out);
+        return;
+    }
+    const double t3 = 1.0 / 3.0;
+    const double S0[3][3] = {{1, 0, 0}, {0, 1, 0}, {t3, t3, t3}};
+    const double S1[3][3] = {{0, 1, 0}, {0, 0, 1}, {t3, t3, t3}};
+    const double S2[3][3] = {{0, 0, 1}, {1, 0, 0}, {t3, t3, t3}};
+    dunavant_d(degree, P, S0, out);
+    dunavant_d(degree, P, S1, out);
+    dunavant_d(degree, P, S2, out);
+}
+
+struct cvec3acc {
+    cd x = 0, y = 0, z = 0;
+    void add(const vec3& v, cd s)
+    {
+        x += v.x * s;
+        y += v.y * s;
+        z += v.z * s;
+    }
+};
+
+} // namespace
+
+extern "C" {
+
+/*
+ * grad[3*v + c] (complex) += sum_{i,j} W_j I_i dZ(j,i)/dx_{v,c} for the quadrature-mode Z.
+ * grad is nv*3 complex (interleaved), overwritten.
+ */
+int orc_adjoint_contract(orc_mesh* m, double freq, int degree, const double* Iin, const double* Win, double* gout)
+{
+    if (degree < 0 || degree > FRICO_DUNAVANT_MAX_DEGREE)
+        return -2;
+    const int64_t N = (int64_t)m->bf.size();
+    const cd* I = reinterpret_cast<const cd*>(Iin);
+    const cd* W = reinterpret_cast<const cd*>(Win);
+    const size_t nv = m->V.size();
+    double omega = 2.0 * M_PI * freq;
+    double k = omega * std::sqrt(MU0 * EPS0);
+    double inv_ksq = 1. / (omega * omega * MU0 * EPS0);
+    const cd jk{0, k};
+    std::vector<cd> total(3 * nv, cd(0));
+#pragma omp parallel
+    {
+        std::vector<cd> G(3 * nv, cd(0));
+        std::vector<dqpoint> qi, qj;
+        auto addv = [&](int64_t v, const cvec3acc& a) {
+            G[3 * v] += a.x;
+            G[3 * v + 1] += a.y;
+            G[3 * v + 2] += a.z;
+        };
+        auto edge_grad = [&](const rwg& b, cd s) {
+            /* d hypot(dx, dy): d/d v1 = (dx, dy, 0)/l, d/d v0 = -(dx, dy, 0)/l */
+            const edge_t& e = m->edges[b.edge_index];
+            vec3 d = m->V[e.iv1] - m->V[e.iv0];
+            if (b.length == 0)
+                return;
+            vec3 g{d.x / b.length, d.y / b.length, 0.0};
+            cvec3acc a;
+            a.add(g, s);
+            addv(e.iv1, a);
+            cvec3acc a2;
+            a2.add(g, -s);
+            addv(e.iv0, a2);
+        };
+        auto tri_area_grad = [&](int64_t t, cd s) {
+            const auto& tr = m->T[t];
+            vec3 g[3];
+            area_grad(m->V[tr[0]], m->V[tr[1]], m->V[tr[2]], g);
+            for (int kk = 0; kk < 3; kk++) {
+                cvec3acc a;
+                a.add(g[kk], s);
+                addv(tr[kk], a);
+            }
+        };
+        const int evmap[3] = {2, 0, 1};
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t ii = 0; ii < N; ii++) {
+            const rwg& ibf = m->bf[ii];
+            for (int64_t jj = 0; jj < N; jj++) {
+                const rwg& jbf = m->bf[jj];
+                const cd om = W[jbf.matrix_index] * I[ibf.matrix_index];
+                if (om == cd(0))
+                    continue;
+                double prodl = 0.25 * ibf.length * jbf.length * M_1_PI;
+                for (int a = 0; a < 2; a++) {
+                    int64_t P = a == 0 ? ibf.itminus : ibf.itplus;
+                    const vec3 fi = a == 0 ? ibf.pminus : ibf.pplus;
+                    int64_t fiv = m->T[P][evmap[a == 0 ? ibf.loc_eminus : ibf.loc_eplus]];
+                    double Ai = a == 0 ? ibf.Aminus : ibf.Aplus;
+                    rules_d(*m, P, degree, false, qi);
+                    for (int b = 0; b < 2; b++) {
+                        int64_t Q = b == 0 ? jbf.itminus : jbf.itplus;
+                        bool split = (Q == ibf.itminus) || (Q == ibf.itplus);
+                        const vec3 fj = b == 0 ? jbf.pminus : jbf.pplus;
+                        int64_t fjv = m->T[Q][evmap[b == 0 ? jbf.loc_eminus : jbf.loc_eplus]];
+                        double Aj = b == 0 ? jbf.Aminus : jbf.Aplus;
+                        rules_d(*m, Q, degree, split, qj);
+                        const double sg = (a == b) ? 1.0 : -1.0;
+                        const double invAA = 1.0 / (Ai * Aj);
+                        /* S = sum w w h g and the coefficient vectors of its derivative */
+                        cd S = 0;
+                        std::vector<cvec3acc> gP(3), gQ(3);
+                        cvec3acc gfi, gfj;
+                        for (const auto& p : qi) {
+                            cvec3acc grp;
+                            cd dwp = 0;
+                            vec3 rhop = p.p - fi;
+                            for (const auto& r : qj) {
+                                vec3 rhor = r.p - fj;
+                                vec3 dv = p.p - r.p;
+                                double R = norm(dv);
+                                cd g = std::exp(cd{0, -k * R}) / R;
+                                cd gp = -(jk + 1.0 / R) * g; /* dg/dR */
+                                double h = 0.25 * dot(rhop, rhor) - inv_ksq;
+                                double ww = p.w * r.w;
+                                S += ww * h * g;
+                                /* d/dr_p: ww (0.25 rho_r g + h g' dv/R);  d/dr_r: ww (0.25 rho_p g - h g' dv/R) */
+                                grp.add(rhor, ww * 0.25 * g);
+                                grp.add(dv, ww * h * gp / R);
+                                cvec3acc grr;
+                                grr.add(rhop, ww * 0.25 * g);
+                                grr.add(dv, -ww * h * gp / R);
+                                gfi.add(rhor, -ww * 0.25 * g);
+                                gfj.add(rhop, -ww * 0.25 * g);
+                                dwp += r.w * h * g;
+                                cd dwr = p.w * h * g;
+                                for (int kk = 0; kk < 3; kk++) {
+                                    gQ[kk].x += grr.x * r.beta[kk];
+                                    gQ[kk].y += grr.y * r.beta[kk];
+                                    gQ[kk].z += grr.z * r.beta[kk];
+                                    gQ[kk].add(r.dw[kk], dwr);
+                                }
+                            }
+                            for (int kk = 0; kk < 3; kk++) {
+                                gP[kk].x += grp.x * p.beta[kk];
+                                gP[kk].y += grp.y * p.beta[kk];
+                                gP[kk].z += grp.z * p.beta[kk];
+                                gP[kk].add(p.dw[kk], dwp);
+                            }
+                        }
+                        /* T = sg prodl invAA S */
+                        const cd c0 = om * sg * prodl * invAA;
+                        auto scale_add = [&](int64_t v, const cvec3acc& a) {
+                            cvec3acc s;
+                            s.x = a.x * c0;
+                            s.y = a.y * c0;
+                            s.z = a.z * c0;
+                            addv(v, s);
+                        };
+                        for (int kk = 0; kk < 3; kk++) {
+                            scale_add(m->T[P][kk], gP[kk]);
+                            scale_add(m->T[Q][kk], gQ[kk]);
+                        }
+                        scale_add(fiv, gfi);
+                        scale_add(fjv, gfj);
+                        /* areas: -T (dAi/Ai + dAj/Aj) */
+                        const cd Tv = om * sg * prodl * invAA * S;
+                        tri_area_grad(P, -Tv / Ai);
+                        tri_area_grad(Q, -Tv / Aj);
+                        /* lengths: prodl' = 0.25/pi (l_j dl_i + l_i dl_j) */
+                        const cd Tl = om * sg * invAA * S * 0.25 * M_1_PI;
+                        edge_grad(ibf, Tl * jbf.length);
+                        edge_grad(jbf, Tl * ibf.length);
+                    }
+                }
+            }
+        }
+#pragma omp critical
+        for (size_t x = 0; x < 3 * nv; x++)
+            total[x] += G[x];
+    }
+    std::memcpy(gout, total.data(), total.size() * sizeof(cd));
+    return 0;
+}
+
+
+/*
+ * Vector-Jacobian product of the field sums at one point (src/emw_postpro_common.cpp:31-66 with
+ * the scatter of src/emw_solver.h:137-164) for the current vector I:
+ *   gx[3v+c] = sum_d eE_d dE_d/dx_{v,c} + eH_d dH_d/dx_{v,c}   (I held fixed)
+ *   gI[n]    = sum_d eE_d dE_d/dI_n     + eH_d dH_d/dI_n
+ * E_t = -j w mu0 (AJ_t g - D_t phi vR), H_t = (vR x AJ_t) psi with vR = pt - c_t,
+ * g = e^{-jkR}/(4 pi R), phi = (1+jkR) e^{-jkR}/(4 pi k^2 R^3), psi = -(1+jkR) e^{+jkR}/(4 pi R^3).
+ */
+int orc_fields_vjp(const orc_mesh* m, double freq, const double* Iin, const double* ptin, const double* eEin,
+                   const double* eHin, double* gxout, double* gIout)
+{
+    const cd* I = reinterpret_cast<const cd*>(Iin);
+    const cd* eE = reinterpret_cast<const cd*>(eEin);
+    const cd* eH = reinterpret_cast<const cd*>(eHin);
+    cd* gx = reinterpret_cast<cd*>(gxout);
+    cd* gI = reinterpret_cast<cd*>(gIout);
+    const size_t nv = m->V.size(), T = m->T.size();
+    std::fill(gx, gx + 3 * nv, cd(0));
+    std::fill(gI, gI + m->bf.size(), cd(0));
+    const double omega = 2.0 * M_PI * freq;
+    const double k = omega * std::sqrt(MU0 * EPS0);
+    const cd jwm{0.0, omega * MU0};
+    const cd jk{0.0, k};
+    vec3 pt{ptin[0], ptin[1], ptin[2]};
+    std::vector<cd> AJ(3 * T, cd(0)), D(T, cd(0));
+    for (const auto& b : m->bf) {
+        for (int s = 0; s < 2; s++) {
+            int64_t t = s == 0 ? b.itminus : b.itplus;
+            double sg = s == 0 ? -1.0 : 1.0;
+            vec3 c = tri_bar(*m, t), f = s == 0 ? b.pminus : b.pplus;
+            vec3 al = c - f;
+            cd In = I[b.matrix_index];
+            AJ[3 * t] += sg * b.length * al.x * 0.5 * In;
+            AJ[3 * t + 1] += sg * b.length * al.y * 0.5 * In;
+            AJ[3 * t + 2] += sg * b.length * al.z * 0.5 * In;
+            D[t] += sg * b.length * In;
+        }
+    }
+    std::vector<cd> cAJ(3 * T), cD(T);
+    std::vector<vec3> dummy;
+    std::vector<cd> gc(3 * T, cd(0)); /* cotangent of the centroid c_t */
+    for (size_t t = 0; t < T; t++) {
+        vec3 c = tri_bar(*m, (int64_t)t);
+        vec3 v = pt - c;
+        double R = norm(v);
+        cd e = std::exp(cd{0, -k * R});
+        cd g = e / (4 * M_PI * R);
+        cd gp = -(jk + 1.0 / R) * g;
+        cd phi = (1.0 + jk * R) * e / (4 * M_PI * k * k * R * R * R);
+        cd vphi = e / (4 * M_PI * k * k * R * R * R);
+        cd phip = vphi * (-3.0 / R + k * k * R - 3.0 * jk);
+        cd ep = std::exp(cd{0, k * R});
+        cd psi = -(1.0 + jk * R) * ep / (4 * M_PI * R * R * R);
+        cd vpsi = ep / (4 * M_PI * R * R * R);
+        cd psip = vpsi * (3.0 / R + k * k * R + jk);
+        cd A[3] = {AJ[3 * t], AJ[3 * t + 1], AJ[3 * t + 2]};
+        double vr[3] = {v.x, v.y, v.z};
+        cd eEv = eE[0] * v.x + eE[1] * v.y + eE[2] * v.z;
+        cd eEA = eE[0] * A[0] + eE[1] * A[1] + eE[2] * A[2];
+        /* eH x vR and eH . (vR x A) */
+        cd eHxv[3] = {eH[1] * v.z - eH[2] * v.y, eH[2] * v.x - eH[0] * v.z, eH[0] * v.y - eH[1] * v.x};
+        cd vxA[3] = {v.y * A[2] - v.z * A[1], v.z * A[0] - v.x * A[2], v.x * A[1] - v.y * A[0]};
+        cd eHvxA = eH[0] * vxA[0] + eH[1] * vxA[1] + eH[2] * vxA[2];
+        cd Axe[3] = {A[1] * eH[2] - A[2] * eH[1], A[2] * eH[0] - A[0] * eH[2], A[0] * eH[1] - A[1] * eH[0]};
+        for (int d = 0; d < 3; d++)
+            cAJ[3 * t + d] = -jwm * g * eE[d] + psi * eHxv[d];
+        cD[t] = jwm * phi * eEv;
+        for (int d = 0; d < 3; d++) {
+            cd dE = -jwm * (eEA * gp * vr[d] / R - D[t] * (phip * eEv * vr[d] / R + phi * eE[d]));
+            cd dH = psi * Axe[d] + psip * vr[d] / R * eHvxA;
+            gc[3 * t + d] -= dE + dH; /* vR = pt - c */
+        }
+    }
+    auto addv = [&](int64_t vv, const cd g3[3]) {
+        gx[3 * vv] += g3[0];
+        gx[3 * vv + 1] += g3[1];
+        gx[3 * vv + 2] += g3[2];
+    };
+    for (const auto& b : m->bf) {
+        cd In = I[b.matrix_index];
+        cd dl = 0;
+        for (int s = 0; s < 2; s++) {
+            int64_t t = s == 0 ? b.itminus : b.itplus;
+            double sg = s == 0 ? -1.0 : 1.0;
+            vec3 c = tri_bar(*m, t), f = s == 0 ? b.pminus : b.pplus;
+            vec3 al = c - f;
+            cd ca = cAJ[3 * t] * al.x + cAJ[3 * t + 1] * al.y + cAJ[3 * t + 2] * al.z;
+            gI[b.matrix_index] += sg * b.length * 0.5 * ca + sg * b.length * cD[t];
+            dl += sg * 0.5 * ca * In + sg * In * cD[t];
+            /* AJ += s l (c - f)/2 I: centroid and free vertex */
+            cd gcc[3], gf[3];
+            for (int d = 0; d < 3; d++) {
+                gcc[d] = sg * b.length * 0.5 * In * cAJ[3 * t + d];
+                gf[d] = -gcc[d];
+                gc[3 * t + d] += gcc[d];
+            }
+            const int evmap[3] = {2, 0, 1};
+            int lo = s == 0 ? b.loc_eminus : b.loc_eplus;
+            addv(m->T[t][evmap[lo]], gf);
+        }
+        const edge_t& e = m->edges[b.edge_index];
+        vec3 d = m->V[e.iv1] - m->V[e.iv0];
+        if (b.length > 0) {
+            cd g1[3] = {dl * (d.x / b.length), dl * (d.y / b.length), 0.0};
+            cd g0[3] = {-g1[0], -g1[1], 0.0};
+            addv(e.iv1, g1);
+            addv(e.iv0, g0);
+        }
+    }
+    for (size_t t = 0; t < T; t++) {
+        cd g3[3] = {gc[3 * t] / 3.0, gc[3 * t + 1] / 3.0, gc[3 * t + 2] / 3.0};
+        for (int kk = 0; kk < 3; kk++)
+            addv(m->T[t][kk], g3);
+    }
+    return 0;
+}
+
+/* d length(edge of RWG n) / d x: writes gl[3*nv] real for one RWG (zero elsewhere). */
+int orc_length_grad(const orc_mesh* m, int64_t n, double* gl)
+{
+    std::fill(gl, gl + 3 * m->V.size(), 0.0);
+    const rwg& b = m->bf[n];
+    const edge_t& e = m->edges[b.edge_index];
+    vec3 d = m->V[e.iv1] - m->V[e.iv0];
+    if (b.length == 0)
+        return 0;
+    gl[3 * e.iv1] = d.x / b.length;
+    gl[3 * e.iv1 + 1] = d.y / b.length;
+    gl[3 * e.iv0] = -d.x / b.length;
+    gl[3 * e.iv0 + 1] = -d.y / b.length;
+    return 0;
+}
 } // extern "C"

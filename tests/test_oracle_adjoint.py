@@ -1,0 +1,66 @@
+"""Pins the FP64 CPU adjoint (oracle.antenna_gradients / oracle.adjoint_contract) by central finite
+differences of the oracle's forward path.  The adjoint has no counterpart in the reference; its
+forward pieces are the reference's (src/emw_solver.cpp:45-142, src/emw_ana_antenna.cpp:33-105)."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from frico_amd import meshes
+
+F0 = 300e6
+
+
+def _perturbed(sm, V):
+    return meshes.SurfaceMesh(V, sm.triangles, sm.tags, sm.bedges, sm.btags, feed=sm.feed)
+
+
+@pytest.mark.parametrize("sm", [meshes.strip_dipole(0.3, 0.04, 4, 2), meshes.icosphere(0.4, 1)],
+                         ids=lambda s: s.name)
+def test_contraction_matches_central_differences(sm):
+    om = oracle.OracleMesh(sm)
+    rng = np.random.default_rng(1)
+    I = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    W = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    g = oracle.adjoint_contract(om, F0, 5, I, W)
+    h = 1e-6
+    coords = [(v, c) for v in range(0, sm.nv, max(1, sm.nv // 5)) for c in range(3)]
+    for v, c in coords:
+        Vp = sm.vertices.copy()
+        Vp[v, c] += h
+        Vm = sm.vertices.copy()
+        Vm[v, c] -= h
+        Zp = oracle.OracleMesh(_perturbed(sm, Vp)).fill(F0, 5)
+        Zm = oracle.OracleMesh(_perturbed(sm, Vm)).fill(F0, 5)
+        fd = W @ ((Zp - Zm) / (2 * h)) @ I
+        assert abs(fd - g[v, c]) <= 1e-7 * np.max(np.abs(g))
+
+
+def test_impedance_and_gain_gradients_match_central_differences():
+    lam = meshes.wavelength(F0)
+    sm = meshes.bowtie(0.25 * lam, 60.0, 4, 3)
+    r = oracle.antenna_gradients(sm, F0, 5, obs=(0.0, 0.0, 10.0))
+    h = 1e-6 * lam
+
+    def fwd(V):
+        a = oracle.antenna_analysis(_perturbed(sm, V), F0, degree=5)
+        m = a["mesh"]
+        AJ, AdivJ, _ = m.scatter(a["I"])
+        E, H = m.fields(F0, AJ, AdivJ, np.array([0.0, 0.0, 10.0]))
+        S = 0.5 * np.cross(E[0], np.conj(H[0]))
+        return a["port"]["impedance"], 4 * math.pi * 100.0 * math.sqrt(np.real(np.vdot(S, S))) / a["port"]["power"].real
+
+    z0, g0 = fwd(sm.vertices)
+    assert abs(z0 - r["zin"]) <= 1e-12 * abs(z0)
+    assert abs(g0 - r["gain"]) <= 1e-12 * abs(g0)
+    for v in range(0, sm.nv, 3):
+        for c in range(3):
+            Vp = sm.vertices.copy()
+            Vp[v, c] += h
+            Vm = sm.vertices.copy()
+            Vm[v, c] -= h
+            zp, gp = fwd(Vp)
+            zm, gm = fwd(Vm)
+            assert abs((zp - zm) / (2 * h) - r["dzin"][v, c]) <= 1e-6 * np.max(np.abs(r["dzin"]))
+            assert abs((gp - gm) / (2 * h) - r["dgain"][v, c]) <= 1e-6 * np.max(np.abs(r["dgain"]))
