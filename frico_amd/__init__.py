@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib, meshes
 from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C128)
 
-__all__ = ["Context", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
+__all__ = ["Context", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR"]
 
 
@@ -260,3 +260,47 @@ class Context:
         self._check(self._L.frico_monostatic_sweep(self._h, _p(f), f.size, _modes(mode), th.size, _p(th), _p(ph),
                                                    eth, eph, R, _form(form), tol, maxit, _p(sig), infos))
         return sig, [SolveInfo(i.iterations, bool(i.converged), i.max_rel_residual) for i in infos]
+
+    # ---- adjoint (autograd-Function-shaped pair) ---------------------------------------------
+    def adjoint_forward(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0)):
+        """Quadrature-mode antenna analysis kept for a backward pass: returns (Zin, gain at obs)."""
+        ifa = np.ascontiguousarray(ifaces, np.int32)
+        ob = np.ascontiguousarray(obs, np.float64)
+        out = np.zeros(4)
+        v = complex(voltage)
+        self._check(self._L.frico_adjoint_forward(self._h, float(freq), _p(ifa, C.c_int32), len(ifa), v.real, v.imag,
+                                                  _p(ob), _p(out)))
+        return complex(out[0], out[1]), float(out[2])
+
+    def adjoint_backward(self, g_zin_re=0.0, g_zin_im=0.0, g_gain=0.0):
+        """Vertex-coordinate gradient of g_re Re(Zin) + g_im Im(Zin) + g_gain G, shape (nv, 3)."""
+        g = np.zeros(self.mesh.nv * 3)
+        self._check(self._L.frico_adjoint_backward(self._h, float(g_zin_re), float(g_zin_im), float(g_gain), _p(g)))
+        return g.reshape(-1, 3)
+
+    def antenna_gradients(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0)):
+        """Zin, G and their gradients: dZin (complex, (nv,3)) and dG (real, (nv,3))."""
+        zin, gain = self.adjoint_forward(freq, ifaces, voltage, obs)
+        dre = self.adjoint_backward(1.0, 0.0, 0.0)
+        dim = self.adjoint_backward(0.0, 1.0, 0.0)
+        dg = self.adjoint_backward(0.0, 0.0, 1.0)
+        return {"zin": zin, "gain": gain, "dzin": dre + 1j * dim, "dgain": dg}
+
+
+class AntennaResponse:
+    """Forward/backward pair shaped like torch.autograd.Function over the C ABI:
+    forward(vertices) -> [Re Zin, Im Zin, G]; backward(grad_out[3]) -> d/d vertices."""
+
+    def __init__(self, ctx: "Context", sm, freq, degree=5, voltage=1.0, obs=(0.0, 0.0, 10.0)):
+        self.ctx, self.sm, self.freq, self.degree, self.voltage, self.obs = ctx, sm, freq, degree, voltage, obs
+
+    def forward(self, vertices):
+        sm = meshes.SurfaceMesh(np.asarray(vertices, np.float64).reshape(-1, 3), self.sm.triangles, self.sm.tags,
+                                self.sm.bedges, self.sm.btags, feed=self.sm.feed)
+        self.ctx.load_mesh(sm, self.degree)
+        zin, g = self.ctx.adjoint_forward(self.freq, sm.feed, self.voltage, self.obs)
+        return np.array([zin.real, zin.imag, g])
+
+    def backward(self, grad_out):
+        go = np.asarray(grad_out, np.float64)
+        return self.ctx.adjoint_backward(go[0], go[1], go[2])

@@ -1,21 +1,9 @@
 // C ABI of the MI355X EFIE library: context, mesh ingestion, fill, RHS, factor/solve, fields.
-#include "../../include/frico_gpu.h"
+#include "ctx.hpp"
 
-#include "common.hpp"
-#include "lu.hpp"
-#include "mesh_dev.hpp"
-
-#include <algorithm>
-#include <array>
-#include <cmath>
 #include <cstdarg>
 #include <cstdio>
-#include <cstring>
 #include <list>
-#include <string>
-#include <vector>
-
-#include "dunavant_rules.inc"
 
 namespace {
 thread_local std::string g_err;
@@ -39,172 +27,9 @@ int hip_fail(hipError_t e, const char* what, const char* file, int line)
     return e == hipErrorOutOfMemory ? FRICO_ERR_NOMEM : FRICO_ERR_HIP;
 }
 
-template <typename T> struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    int ensure(size_t count)
-    {
-        if (count <= n && p)
-            return 0;
-        release();
-        if (count == 0)
-            return 0;
-        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
-        if (e != hipSuccess) {
-            p = nullptr;
-            n = 0;
-            set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
-            return FRICO_ERR_NOMEM;
-        }
-        n = count;
-        return 0;
-    }
-    int upload(const T* h, size_t count, hipStream_t st)
-    {
-        int rc = ensure(count);
-        if (rc)
-            return rc;
-        if (count)
-            FRICO_HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
-        return 0;
-    }
-    void release()
-    {
-        if (p)
-            (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    ~DevBuf() { release(); }
-};
-
-namespace {
-
-struct vec3 {
-    double x, y, z;
-};
-inline vec3 operator+(vec3 a, vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
-inline vec3 operator-(vec3 a, vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
-inline vec3 operator*(vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
-inline vec3 operator/(vec3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
-inline double dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
-inline vec3 cross(vec3 a, vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
-inline double norm(vec3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
-
-struct QP {
-    vec3 p;
-    double w;
-};
-
-// Heron's formula in Kahan's stable ordering (src/quadratures.cpp:41-57).
-double kahan_area(vec3 p0, vec3 p1, vec3 p2)
-{
-    std::array<double, 3> l{norm(p1 - p0), norm(p2 - p0), norm(p2 - p1)};
-    std::sort(l.begin(), l.end());
-    const double a = l[2], b = l[1], c = l[0];
-    return 0.25 * std::sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)));
-}
-
-// Symmetric Dunavant rule on a triangle: each orbit expands to 1, 3 or 6 barycentric
-// permutations in a fixed order; weights are scaled by the triangle area
-// (src/quadratures.cpp:321-372).
-void dunavant(int degree, vec3 p0, vec3 p1, vec3 p2, std::vector<QP>& out)
-{
-    const int rule = degree == 0 ? 0 : degree - 1;
-    const double area = kahan_area(p0, p1, p2);
-    auto at = [&](double a, double b, double c) { return p0 * a + p1 * b + p2 * c; };
-    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++) {
-        const auto& d = frico_dunavant_orbits[o];
-        const double a = d.a, b = d.b, c = d.c, w = d.w * area;
-        if (d.orbit == 1) {
-            out.push_back({at(a, b, c), w});
-        } else if (d.orbit == 3) {
-            out.push_back({at(a, b, c), w});
-            out.push_back({at(b, c, a), w});
-            out.push_back({at(b, a, c), w});
-        } else {
-            out.push_back({at(a, b, c), w});
-            out.push_back({at(a, c, b), w});
-            out.push_back({at(b, a, c), w});
-            out.push_back({at(b, c, a), w});
-            out.push_back({at(c, a, b), w});
-            out.push_back({at(c, b, a), w});
-        }
-    }
-}
-
-struct Neighbours {
-    int64_t tm = -1, tp = -1;
-    int lm = -1, lp = -1;
-    bool has_iface = false;
-    int iface = 0;
-};
-
-} // namespace
-
-struct HostMesh {
-    int64_t nv = 0, nt = 0;
-    std::vector<vec3> V;
-    std::vector<std::array<int64_t, 3>> T;
-    std::vector<int> tag;
-    std::vector<std::pair<int64_t, int64_t>> edges;
-    std::vector<frico_rwg> bf;
-    int degree = -1, q = 0;
-    // device-side arrays prepared on the host
-    std::vector<double> cen, area, sqrtA, alpha, tri_sgn, rwg_len, rwg_Am, rwg_Ap;
-    std::vector<double4> qreg, qsub;
-    std::vector<int> tri_rwg, rwg_t, rwg_le;
-    std::vector<int2> near;
-};
-
 } // namespace frico
 
 using namespace frico;
-
-struct frico_ctx {
-    int device = 0;
-    hipStream_t st = nullptr;
-    int nb = 64;
-    int max_groups = 0;
-    bool profiling = false;
-    bool have_mesh = false;
-    HostMesh hm;
-    int64_t N = 0;
-    // device mesh
-    DevBuf<double> d_cen, d_area, d_sqrtA, d_alpha, d_tri_sgn, d_rwg_len, d_rwg_Am, d_rwg_Ap;
-    DevBuf<double4> d_qreg, d_qsub;
-    DevBuf<int> d_tri_rwg, d_rwg_t, d_rwg_le;
-    DevBuf<int2> d_near;
-    DevMesh dm{};
-    // matrix state
-    DevBuf<c128> Z64;
-    DevBuf<c64> Z32, LU32;
-    DevBuf<unsigned char> Cbuf;
-    int zprec = -1;        // precision of the current Z (-1: none)
-    bool z64_factored = false;
-    int factored = -1;     // form of the current factorisation (-1: none)
-    int nb_factored = 64;  // block width the factors were computed with
-    DevBuf<int> ipiv;
-    // lu workspace
-    DevBuf<unsigned> counters;
-    DevBuf<int> flags;
-    DevBuf<double> cand_val;
-    DevBuf<int> cand_idx;
-    DevBuf<unsigned char> cand_row, diag_row;
-    // rhs / solution
-    int64_t nrhs = 0;
-    bool have_x = false;
-    DevBuf<c128> B, X, R;
-    DevBuf<c64> D32;
-    DevBuf<double> norms;
-    // fields
-    DevBuf<c128> TS, EH;
-    DevBuf<double> pts;
-    DevBuf<int2> req;
-    DevBuf<PlaneWaveDev> waves;
-    // profiling
-    KernelTimer t_fill, t_rhs, t_factor, t_solve, t_refine, t_gemm, t_tripair;
-};
 
 namespace {
 
@@ -575,6 +400,26 @@ int frico_mesh_load(frico_ctx* c, const double* vertices, int64_t nv, const int6
             hm.near.push_back(int2{(int)i, js[x]});
     }
 
+    // adjoint data: triangle vertices and barycentric weights of every rule point
+    hm.tri_v.resize(3 * nt);
+    for (int64_t t = 0; t < nt; t++)
+        for (int kk = 0; kk < 3; kk++)
+            hm.tri_v[3 * t + kk] = (int)hm.T[t][kk];
+    {
+        std::vector<std::array<double, 3>> bc;
+        dunavant_bary(degree, bc);
+        hm.qbeta.clear();
+        for (auto& b : bc)
+            hm.qbeta.insert(hm.qbeta.end(), b.begin(), b.end());
+        hm.qsub_beta.clear();
+        const int corners[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+        for (int s = 0; s < 3; s++)
+            for (auto& b : bc)
+                for (int kk = 0; kk < 3; kk++)
+                    hm.qsub_beta.push_back((kk == corners[s][0] ? b[0] : 0.0) + (kk == corners[s][1] ? b[1] : 0.0) +
+                                           b[2] / 3.0);
+    }
+
     // upload
     hipStream_t st = c->st;
     if ((rc = c->d_cen.upload(hm.cen.data(), hm.cen.size(), st)) || (rc = c->d_area.upload(hm.area.data(), hm.area.size(), st)) ||
@@ -589,7 +434,10 @@ int frico_mesh_load(frico_ctx* c, const double* vertices, int64_t nv, const int6
         (rc = c->d_tri_rwg.upload(hm.tri_rwg.data(), hm.tri_rwg.size(), st)) ||
         (rc = c->d_rwg_t.upload(hm.rwg_t.data(), hm.rwg_t.size(), st)) ||
         (rc = c->d_rwg_le.upload(hm.rwg_le.data(), hm.rwg_le.size(), st)) ||
-        (rc = c->d_near.upload(hm.near.data(), hm.near.size(), st)))
+        (rc = c->d_near.upload(hm.near.data(), hm.near.size(), st)) ||
+        (rc = c->d_tri_v.upload(hm.tri_v.data(), hm.tri_v.size(), st)) ||
+        (rc = c->d_qbeta.upload(hm.qbeta.data(), hm.qbeta.size(), st)) ||
+        (rc = c->d_qsub_beta.upload(hm.qsub_beta.data(), hm.qsub_beta.size(), st)))
         return rc;
     FRICO_HIP_CHECK(hipStreamSynchronize(st));
     c->hm = std::move(hm);
@@ -612,10 +460,14 @@ int frico_mesh_load(frico_ctx* c, const double* vertices, int64_t nv, const int6
     m.rwg_len = c->d_rwg_len.p;
     m.rwg_Am = c->d_rwg_Am.p;
     m.rwg_Ap = c->d_rwg_Ap.p;
+    c->am.tri_v = c->d_tri_v.p;
+    c->am.qbeta = c->d_qbeta.p;
+    c->am.qsub_beta = c->d_qsub_beta.p;
     c->zprec = -1;
     c->factored = -1;
     c->nrhs = 0;
     c->have_x = false;
+    c->adj = frico_ctx::AdjointState{};
     return 0;
 }
 

@@ -1,0 +1,234 @@
+// Context, host mesh and device-buffer definitions shared by the C-ABI sources.
+#pragma once
+
+#include "../../include/frico_gpu.h"
+
+#include "adjoint.hpp"
+#include "common.hpp"
+#include "lu.hpp"
+#include "mesh_dev.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dunavant_rules.inc"
+
+namespace frico {
+
+template <typename T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    int ensure(size_t count)
+    {
+        if (count <= n && p)
+            return 0;
+        release();
+        if (count == 0)
+            return 0;
+        hipError_t e = hipMalloc((void**)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            set_error("hipMalloc of %zu bytes failed: %s", count * sizeof(T), hipGetErrorString(e));
+            return FRICO_ERR_NOMEM;
+        }
+        n = count;
+        return 0;
+    }
+    int upload(const T* h, size_t count, hipStream_t st)
+    {
+        int rc = ensure(count);
+        if (rc)
+            return rc;
+        if (count)
+            FRICO_HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st));
+        return 0;
+    }
+    void release()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+namespace geom {
+
+struct vec3 {
+    double x, y, z;
+};
+inline vec3 operator+(vec3 a, vec3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline vec3 operator-(vec3 a, vec3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline vec3 operator*(vec3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline vec3 operator/(vec3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline double dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline vec3 cross(vec3 a, vec3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+inline double norm(vec3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+
+struct QP {
+    vec3 p;
+    double w;
+};
+
+// Heron's formula in Kahan's stable ordering (src/quadratures.cpp:41-57).
+inline double kahan_area(vec3 p0, vec3 p1, vec3 p2)
+{
+    std::array<double, 3> l{norm(p1 - p0), norm(p2 - p0), norm(p2 - p1)};
+    std::sort(l.begin(), l.end());
+    const double a = l[2], b = l[1], c = l[0];
+    return 0.25 * std::sqrt((a + (b + c)) * (c - (a - b)) * (c + (a - b)) * (a + (b - c)));
+}
+
+// Symmetric Dunavant rule on a triangle: each orbit expands to 1, 3 or 6 barycentric
+// permutations in a fixed order; weights are scaled by the triangle area
+// (src/quadratures.cpp:321-372).
+inline void dunavant(int degree, vec3 p0, vec3 p1, vec3 p2, std::vector<QP>& out)
+{
+    const int rule = degree == 0 ? 0 : degree - 1;
+    const double area = kahan_area(p0, p1, p2);
+    auto at = [&](double a, double b, double c) { return p0 * a + p1 * b + p2 * c; };
+    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++) {
+        const auto& d = frico_dunavant_orbits[o];
+        const double a = d.a, b = d.b, c = d.c, w = d.w * area;
+        if (d.orbit == 1) {
+            out.push_back({at(a, b, c), w});
+        } else if (d.orbit == 3) {
+            out.push_back({at(a, b, c), w});
+            out.push_back({at(b, c, a), w});
+            out.push_back({at(b, a, c), w});
+        } else {
+            out.push_back({at(a, b, c), w});
+            out.push_back({at(a, c, b), w});
+            out.push_back({at(b, a, c), w});
+            out.push_back({at(b, c, a), w});
+            out.push_back({at(c, a, b), w});
+            out.push_back({at(c, b, a), w});
+        }
+    }
+}
+
+// Barycentric coordinates of the rule points, in the expansion order of dunavant().
+inline void dunavant_bary(int degree, std::vector<std::array<double, 3>>& out)
+{
+    const int rule = degree == 0 ? 0 : degree - 1;
+    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++) {
+        const auto& d = frico_dunavant_orbits[o];
+        const double a = d.a, b = d.b, c = d.c;
+        if (d.orbit == 1) {
+            out.push_back({a, b, c});
+        } else if (d.orbit == 3) {
+            out.push_back({a, b, c});
+            out.push_back({b, c, a});
+            out.push_back({b, a, c});
+        } else {
+            out.push_back({a, b, c});
+            out.push_back({a, c, b});
+            out.push_back({b, a, c});
+            out.push_back({b, c, a});
+            out.push_back({c, a, b});
+            out.push_back({c, b, a});
+        }
+    }
+}
+
+struct Neighbours {
+    int64_t tm = -1, tp = -1;
+    int lm = -1, lp = -1;
+    bool has_iface = false;
+    int iface = 0;
+};
+
+} // namespace geom
+using namespace geom;
+
+struct HostMesh {
+    int64_t nv = 0, nt = 0;
+    std::vector<vec3> V;
+    std::vector<std::array<int64_t, 3>> T;
+    std::vector<int> tag;
+    std::vector<std::pair<int64_t, int64_t>> edges;
+    std::vector<frico_rwg> bf;
+    int degree = -1, q = 0;
+    // device-side arrays prepared on the host
+    std::vector<double> cen, area, sqrtA, alpha, tri_sgn, rwg_len, rwg_Am, rwg_Ap;
+    std::vector<double4> qreg, qsub;
+    std::vector<int> tri_rwg, rwg_t, rwg_le;
+    std::vector<int2> near;
+    // adjoint: triangle vertices and barycentric weights of the rule points
+    std::vector<int> tri_v;
+    std::vector<double> qbeta, qsub_beta;
+};
+
+} // namespace frico
+
+using namespace frico;
+
+struct frico_ctx {
+    int device = 0;
+    hipStream_t st = nullptr;
+    int nb = 64;
+    int max_groups = 0;
+    bool profiling = false;
+    bool have_mesh = false;
+    HostMesh hm;
+    int64_t N = 0;
+    // device mesh
+    DevBuf<double> d_cen, d_area, d_sqrtA, d_alpha, d_tri_sgn, d_rwg_len, d_rwg_Am, d_rwg_Ap;
+    DevBuf<double4> d_qreg, d_qsub;
+    DevBuf<int> d_tri_rwg, d_rwg_t, d_rwg_le;
+    DevBuf<int2> d_near;
+    DevMesh dm{};
+    // matrix state
+    DevBuf<c128> Z64;
+    DevBuf<c64> Z32, LU32;
+    DevBuf<unsigned char> Cbuf;
+    int zprec = -1;        // precision of the current Z (-1: none)
+    bool z64_factored = false;
+    int factored = -1;     // form of the current factorisation (-1: none)
+    int nb_factored = 64;  // block width the factors were computed with
+    DevBuf<int> ipiv;
+    // lu workspace
+    DevBuf<unsigned> counters;
+    DevBuf<int> flags;
+    DevBuf<double> cand_val;
+    DevBuf<int> cand_idx;
+    DevBuf<unsigned char> cand_row, diag_row;
+    // rhs / solution
+    int64_t nrhs = 0;
+    bool have_x = false;
+    DevBuf<c128> B, X, R;
+    DevBuf<c64> D32;
+    DevBuf<double> norms;
+    // fields
+    DevBuf<c128> TS, EH;
+    DevBuf<double> pts;
+    DevBuf<int2> req;
+    DevBuf<PlaneWaveDev> waves;
+    // profiling
+    KernelTimer t_fill, t_rhs, t_factor, t_solve, t_refine, t_gemm, t_tripair;
+    // adjoint
+    DevBuf<int> d_tri_v;
+    DevBuf<double> d_qbeta, d_qsub_beta, d_gvert;
+    DevBuf<c128> adjW, aw, bw;
+    AdjMesh am{};
+    struct AdjointState {
+        bool forward_done = false, grads_done = false;
+        double freq = 0;
+        std::vector<int32_t> ifaces;
+        std::complex<double> Vg = 0, Iport = 0, zin = 0;
+        double obs[3] = {0, 0, 0};
+        std::vector<double> signs;
+        std::vector<std::complex<double>> I, V, c, E, H;
+        double absS = 0, P = 0, gain = 0;
+        std::vector<std::complex<double>> dzin; // 3 nv
+        std::vector<double> dgain;              // 3 nv
+    } adj;
+};
+

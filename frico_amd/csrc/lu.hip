@@ -19,8 +19,10 @@
 namespace frico {
 
 // ----------------------------------------------------------------------------------------------
-// MFMA wrappers.  Operand lane layout for the 16x16x4 shapes: lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]; the 4 accumulator values of lane l are D[4*(l/16)+r][l%16], r = 0..3.
+// MFMA wrappers.  Operand lane layout of both 16x16x4 shapes: lane l supplies A[l%16][l/16] and
+// B[l/16][l%16].  Accumulator value r of lane l holds D[row][l%16] with
+//   f32: row = 4*(l/16) + r        f64: row = 4*r + l/16
+// (measured on gfx950 with one-hot operands).
 // ----------------------------------------------------------------------------------------------
 template <typename T> struct Mfma;
 template <> struct Mfma<double> {
@@ -29,6 +31,7 @@ template <> struct Mfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int acc_row(int lane, int r) { return 4 * r + (lane >> 4); }
 };
 template <> struct Mfma<float> {
     typedef float acc_t __attribute__((ext_vector_type(4)));
@@ -36,6 +39,7 @@ template <> struct Mfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    __device__ static inline int acc_row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
 // ----------------------------------------------------------------------------------------------
@@ -46,7 +50,7 @@ template <> struct Mfma<float> {
 // ----------------------------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16, GWM = 2, GWN = 2;
 
-template <typename T, typename TA>
+template <typename T, typename TA, bool TRANSA = false>
 __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
                                                  const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
                                                  int64_t ldc)
@@ -74,10 +78,11 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
 #pragma unroll
         for (int s = 0; s < LA; s++) {
             const int e = tid + 256 * s;
-            const int r = e % GBM, kk = e / GBM;
+            // consecutive threads walk the contiguous dimension of A in memory
+            const int r = TRANSA ? e / GBK : e % GBM, kk = TRANSA ? e % GBK : e / GBM;
             const int64_t gr = m0 + r, gk = k0 + kk;
             if (gr < M && gk < K) {
-                const cx<TA> v = A[gr + gk * lda];
+                const cx<TA> v = TRANSA ? A[gk + gr * lda] : A[gr + gk * lda];
                 ra[s] = cx<T>{(T)v.re, (T)v.im};
             } else {
                 ra[s] = cx<T>{0, 0};
@@ -95,7 +100,10 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
 #pragma unroll
         for (int s = 0; s < LA; s++) {
             const int e = tid + 256 * s;
-            sA[e / GBM][e % GBM] = ra[s];
+            if (TRANSA)
+                sA[e % GBK][e / GBK] = ra[s];
+            else
+                sA[e / GBM][e % GBM] = ra[s];
         }
 #pragma unroll
         for (int s = 0; s < LB; s++) {
@@ -141,7 +149,7 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
                 continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t col = n0 + wn * GWN * 16 + b * 16 + 4 * (lane >> 4) + r;
+                const int64_t col = n0 + wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
                 if (col < N) {
                     cx<T>* p = C + row + col * ldc;
                     cx<T> v = *p;
@@ -399,11 +407,29 @@ __global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda,
     }
 }
 
+template <typename T>
+__global__ __launch_bounds__(256) void laswp_reverse(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols,
+                                                     int64_t k, int w, const int* __restrict__ ipiv)
+{
+    const int64_t j = col0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (j >= col0 + ncols)
+        return;
+    cx<T>* col = A + j * lda;
+    for (int c = w - 1; c >= 0; c--) {
+        const int64_t p = ipiv[k + c];
+        if (p != k + c) {
+            cx<T> t = col[k + c];
+            col[k + c] = col[p];
+            col[p] = t;
+        }
+    }
+}
+
 // ----------------------------------------------------------------------------------------------
 // In-place triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block.
 // UPPER=false: unit lower (forward).  UPPER=true: non-unit upper (backward).
 // ----------------------------------------------------------------------------------------------
-template <typename T, bool UPPER>
+template <typename T, bool UPPER, bool TRANSD = false, bool UNIT = !UPPER>
 __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict__ D, int64_t ldd, cx<T>* __restrict__ B,
                                                   int64_t ldb, int64_t ncols)
 {
@@ -414,13 +440,18 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
     const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
     for (int e = tid; e < 64 * 64; e += 256) {
         const int r = e % 64, c = e / 64;
-        sD[r][c] = (r < w && c < w) ? D[r + (int64_t)c * ldd] : cx<T>{0, 0};
+        sD[r][c] = (r < w && c < w) ? (TRANSD ? D[c + (int64_t)r * ldd] : D[r + (int64_t)c * ldd]) : cx<T>{0, 0};
         sX[r][c] = (r < w && c < nc) ? B[r + (c0 + c) * ldb] : cx<T>{0, 0};
     }
     __syncthreads();
     const int col = tid & 63, rg = tid >> 6;
     if (!UPPER) {
         for (int r = 0; r < w; r++) {
+            if (!UNIT) {
+                if (rg == 0)
+                    sX[r][col] = cdiv(sX[r][col], sD[r][r]);
+                __syncthreads();
+            }
             const cx<T> x = sX[r][col];
             for (int rr = r + 1 + rg; rr < w; rr += 4) {
                 const cx<T> l = sD[rr][r];
@@ -431,7 +462,7 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
         }
     } else {
         for (int r = w - 1; r >= 0; r--) {
-            if (rg == 0)
+            if (rg == 0 && !UNIT)
                 sX[r][col] = cdiv(sX[r][col], sD[r][r]);
             __syncthreads();
             const cx<T> x = sX[r][col];
@@ -679,6 +710,54 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
     return 0;
 }
 
+// X = Z^{-T} B with the factors of lu_factor (deferred left interchanges):
+//   Z^{-1} = U^{-1} E_last S_last ... E_1 S_1,  E_p: [B_p <- L11^{-1} B_p ; B_below -= L21 B_p]
+//   Z^{-T} = S_1^T E_1^T ... S_last^T E_last^T U^{-T}
+//   U^T y = b forward (lower, non-unit), then per panel p descending:
+//   B_p <- L11^{-T} (B_p - L21^T B_below), then undo panel p's interchanges in reverse order.
+template <typename T>
+int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
+                        int64_t nrhs, hipStream_t st)
+{
+    const int64_t npan = (n + nb - 1) / nb;
+    // U^T y = b: block rows ascending; y_p = U_pp^{-T} (b_p - U(0:k, p)^T y(0:k))
+    for (int64_t p = 0; p < npan; p++) {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        for (int s = 0; s < w; s += 64) {
+            const int ws = (w - s) < 64 ? (w - s) : 64;
+            const int64_t above = k + s;
+            if (above > 0) {
+                const int64_t gx = (ws + GBM - 1) / GBM, gy = (nrhs + GBN - 1) / GBN;
+                hipLaunchKernelGGL((cgemm_sub<T, T, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
+                                   nrhs, above, A + (k + s) * lda, lda, B, ldb, B + (k + s), ldb);
+            }
+            hipLaunchKernelGGL((trsm_block<T, false, true, false>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
+                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+        }
+    }
+    for (int64_t p = npan - 1; p >= 0; p--) {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        for (int s = ((w - 1) / 64) * 64; s >= 0; s -= 64) {
+            const int ws = (w - s) < 64 ? (w - s) : 64;
+            const int64_t below = n - (k + s + ws);
+            if (below > 0) {
+                const int64_t gx = (ws + GBM - 1) / GBM, gy = (nrhs + GBN - 1) / GBN;
+                hipLaunchKernelGGL((cgemm_sub<T, T, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
+                                   nrhs, below, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s + ws), ldb, B + (k + s),
+                                   ldb);
+            }
+            hipLaunchKernelGGL((trsm_block<T, true, true, true>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
+                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+        }
+        hipLaunchKernelGGL((laswp_reverse<T>), dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0,
+                           nrhs, k, w, ipiv);
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_convert_c128_c64(const c128* X, int64_t ldx, c64* Y, int64_t ldy, int64_t rows, int64_t cols, hipStream_t st)
 {
     for (int64_t c0 = 0; c0 < cols; c0 += 65535) {
@@ -713,6 +792,10 @@ template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipS
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
 template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t);
 template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t);
+template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t,
+                                         hipStream_t);
+template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,
+                                        hipStream_t);
 template int panel_rows<double>(int);
 template int panel_rows<float>(int);
 

@@ -181,6 +181,21 @@ int frico_monostatic_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int m
                            double R, int form, double tol, int maxit, double* sigma_out,
                            frico_solve_info* info_out);
 
+/*
+ * Adjoint pair (no counterpart in the reference; it differentiates the antenna analysis of
+ * src/emw_ana_antenna.cpp:33-105 through compute_matrix, src/emw_solver.cpp:45-142).
+ * frico_adjoint_forward: quadrature-mode fill at `freq` (Dunavant degree 1, 2, 4 or 5 of the loaded
+ *   mesh), delta-gap excitation on ifaces, complex128 LU and solve; out[0..2] = Re Zin, Im Zin and
+ *   the gain G = 4 pi |obs|^2 |S| / Re(V conj(I_port)/2) at the point obs[3].
+ * frico_adjoint_backward: grad[3*v + c] = g_zin_re dRe(Zin)/dx + g_zin_im dIm(Zin)/dx + g_gain dG/dx
+ *   for every vertex coordinate, from two solves with Z^T on the forward factors and the
+ *   device contraction of dZ/dx with the forward and adjoint currents.
+ */
+int frico_adjoint_forward(frico_ctx* ctx, double freq, const int32_t* ifaces, int nif, double v_re,
+                          double v_im, const double* obs, double* out);
+int frico_adjoint_backward(frico_ctx* ctx, double g_zin_re, double g_zin_im, double g_gain,
+                           double* grad);
+
 #ifdef __cplusplus
 }
 #endif

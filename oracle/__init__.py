@@ -61,6 +61,7 @@ def lib():
         L.orc_gain.argtypes = [C.c_void_p, C.c_double, _d, _d, C.c_double, C.c_double, _d]
         L.orc_rcs_unit_Ei.argtypes = [_d, C.c_double, C.c_double, C.c_double, _d]
         L.orc_bistatic.argtypes = [C.c_void_p, C.c_double, _d, _d, C.c_double, _d]
+        L.orc_sph2rect.argtypes = [C.c_double, C.c_double, C.c_double, _d]
         _lib = L
     return _lib
 
