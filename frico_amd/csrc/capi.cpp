@@ -2,6 +2,7 @@
 #include "ctx.hpp"
 
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <list>
 
@@ -63,8 +64,9 @@ void phase_end(frico_ctx* c, KernelTimer& t)
 int ensure_lu_work(frico_ctx* c)
 {
     const int64_t npan = (c->N + 31) / 32 + 1;
+    const int gmax = std::min(c->max_groups, LU_MAX_GROUPS);
     int rc;
-    if ((rc = c->counters.ensure((size_t)npan)))
+    if ((rc = c->counters.ensure((size_t)(npan * 2 * gmax))))
         return rc;
     if ((rc = c->flags.ensure(2)))
         return rc;
@@ -76,6 +78,14 @@ int ensure_lu_work(frico_ctx* c)
         return rc;
     if ((rc = c->diag_row.ensure((size_t)2 * LU_NB_MAX * sizeof(c128))))
         return rc;
+    if ((rc = c->dinv.ensure((size_t)((c->N + 63) / 64) * 2 * 4096 * sizeof(c128))))
+        return rc;
+    const size_t nev = (size_t)(2 * npan + 1);
+    while (c->lu_events.size() < nev) {
+        hipEvent_t e;
+        FRICO_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->lu_events.push_back(e);
+    }
     return 0;
 }
 
@@ -89,6 +99,12 @@ LuWork lu_work(frico_ctx* c)
     w.cand_idx = c->cand_idx.p;
     w.cand_row = c->cand_row.p;
     w.diag_row = c->diag_row.p;
+    w.dinv = c->dinv.p;
+    if (c->lookahead) {
+        w.side = c->side;
+        w.events = c->lu_events.data();
+        w.nevents = (int)c->lu_events.size();
+    }
     w.max_groups = std::min(c->max_groups, LU_MAX_GROUPS);
     return w;
 }
@@ -123,10 +139,14 @@ int frico_ctx_create(int device, frico_ctx** out)
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess)
         e = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking);
+    if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
     if (e != hipSuccess) {
         delete c;
         return hip_fail(e, "stream create", __FILE__, __LINE__);
     }
+    if (const char* la = std::getenv("FRICO_LU_LOOKAHEAD"))
+        c->lookahead = std::atoi(la) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -141,6 +161,10 @@ int frico_ctx_destroy(frico_ctx* c)
         return 0;
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->st);
+    (void)hipStreamSynchronize(c->side);
+    for (auto e : c->lu_events)
+        (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->side);
     (void)hipStreamDestroy(c->st);
     delete c;
     return 0;
@@ -887,7 +911,7 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     if (c->factored == FRICO_SOLVE_C128) {
         phase_begin(c, c->t_solve);
         FRICO_HIP_CHECK(hipMemcpyAsync(c->X.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
-        rc = lu_solve<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->X.p, N, nr, c->st);
+        rc = lu_solve<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->X.p, N, nr, c->st, c->dinv.p);
         phase_end(c, c->t_solve);
         if (rc)
             return rc;
@@ -913,7 +937,7 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
         phase_begin(c, tt);
         if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
             return rc;
-        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->D32.p, N, nr, c->st)))
+        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p)))
             return rc;
         if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
             return rc;

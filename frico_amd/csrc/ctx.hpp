@@ -173,6 +173,9 @@ using namespace frico;
 struct frico_ctx {
     int device = 0;
     hipStream_t st = nullptr;
+    hipStream_t side = nullptr;      // look-ahead stream of the LU (default priority)
+    std::vector<hipEvent_t> lu_events;
+    bool lookahead = true;
     int nb = 64;
     int max_groups = 0;
     bool profiling = false;
@@ -199,7 +202,7 @@ struct frico_ctx {
     DevBuf<int> flags;
     DevBuf<double> cand_val;
     DevBuf<int> cand_idx;
-    DevBuf<unsigned char> cand_row, diag_row;
+    DevBuf<unsigned char> cand_row, diag_row, dinv;
     // rhs / solution
     int64_t nrhs = 0;
     bool have_x = false;

@@ -37,6 +37,37 @@ template <> struct QN<0> {
     __device__ static int get(int q) { return q; }
 };
 
+// (sin x, cos x) for the small per-point phase of a triangle pair: Taylor series through x^17 and
+// x^18 for |x| <= 1 (truncation below 1e-17), the library sincos otherwise.
+__device__ inline void small_sincos(double x, double& sn, double& cs)
+{
+    if (fabs(x) <= 1.0) {
+        const double x2 = x * x;
+        double s = 1.0 / 355687428096000.0;        // 1/17!
+        s = fma(s, -x2, 1.0 / 1307674368000.0);     // 1/15!
+        s = fma(s, -x2, 1.0 / 6227020800.0);        // 1/13!
+        s = fma(s, -x2, 1.0 / 39916800.0);          // 1/11!
+        s = fma(s, -x2, 1.0 / 362880.0);            // 1/9!
+        s = fma(s, -x2, 1.0 / 5040.0);              // 1/7!
+        s = fma(s, -x2, 1.0 / 120.0);               // 1/5!
+        s = fma(s, -x2, 1.0 / 6.0);                 // 1/3!
+        s = fma(s, -x2, 1.0);
+        double c = 1.0 / 6402373705728000.0;       // 1/18!
+        c = fma(c, -x2, 1.0 / 20922789888000.0);    // 1/16!
+        c = fma(c, -x2, 1.0 / 87178291200.0);       // 1/14!
+        c = fma(c, -x2, 1.0 / 479001600.0);         // 1/12!
+        c = fma(c, -x2, 1.0 / 3628800.0);           // 1/10!
+        c = fma(c, -x2, 1.0 / 40320.0);             // 1/8!
+        c = fma(c, -x2, 1.0 / 720.0);               // 1/6!
+        c = fma(c, -x2, 1.0 / 24.0);                // 1/4!
+        c = fma(c, -x2, 0.5);                       // 1/2!
+        sn = x * s;
+        cs = fma(c, -x2, 1.0);
+    } else {
+        sincos(x, &sn, &cs);
+    }
+}
+
 template <int QC, typename OutT>
 __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double inv_ksq, cx<OutT>* __restrict__ C)
 {
@@ -60,6 +91,9 @@ __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double i
     const double dx = m.cen[3 * P] - m.cen[3 * Q];
     const double dy = m.cen[3 * P + 1] - m.cen[3 * Q + 1];
     const double dz = m.cen[3 * P + 2] - m.cen[3 * Q + 2];
+    // exp(-jkR) = exp(-jk R0) exp(-jk (R - R0)) with R0 the centroid distance: the per-point factor
+    // has a small argument, the pair factor is applied once to the moments.
+    const double R0 = sqrt(dx * dx + dy * dy + dz * dz);
 
     // Q's points in registers when the rule size is a compile-time constant.
     double4 vq[QC ? QC : 1];
@@ -87,10 +121,11 @@ __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double i
                 v = sQ[tq * q + r];
             const double rx = ex - v.x, ry = ey - v.y, rz = ez - v.z;
             const double R2 = rx * rx + ry * ry + rz * rz;
-            const double R = sqrt(R2);
+            const double rinv = R2 > 0 ? rsqrt(R2) : 0.0;
+            const double R = R2 * rinv;
             double sn, cs;
-            sincos(k * R, &sn, &cs);
-            const double f = R2 > 0 ? v.w / R : 0.0;
+            small_sincos(k * (R - R0), sn, cs);
+            const double f = v.w * rinv;
             const double gr = cs * f, gi = -sn * f;
             hr += gr;
             hi += gi;
@@ -118,6 +153,22 @@ __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double i
         svi[2] += w * vzi;
         suvr += w * (u.x * vxr + u.y * vyr + u.z * vzr);
         suvi += w * (u.x * vxi + u.y * vyi + u.z * vzi);
+    }
+    {
+        // rotate the moments by the pair phase exp(-jk R0)
+        double ps, pc;
+        sincos(k * R0, &ps, &pc);
+        auto rot = [&](double& a, double& b) {
+            const double t = a * pc + b * ps;
+            b = b * pc - a * ps;
+            a = t;
+        };
+        rot(s0r, s0i);
+        rot(suvr, suvi);
+        for (int c = 0; c < 3; c++) {
+            rot(sur[c], sui[c]);
+            rot(svr[c], svi[c]);
+        }
     }
     const double invAA = 1.0 / (m.area[P] * m.area[Q]);
     const double* aP = m.alpha + 9 * (size_t)P;

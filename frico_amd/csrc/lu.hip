@@ -162,152 +162,140 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
 }
 
 // ----------------------------------------------------------------------------------------------
-// Panel factorisation with a grid barrier per column.
+// Panel factorisation.  G workgroups, workgroup g owns panel rows [g*R, g*R + R) in LDS, one row
+// per thread.  Per column every workgroup publishes its local pivot candidate (value, row index,
+// row contents) and raises its flag; after all G flags are up every workgroup picks the same
+// winner (first max of |re|+|im|) and performs the interchange on its own rows.
 // ----------------------------------------------------------------------------------------------
 struct PanelSync {
-    unsigned* counter; // one counter per panel launch, zeroed before the factorisation
-    int* abort_flag;   // set if a barrier wait exceeds its spin limit
-    double* cand_val;  // [2][G]
-    int* cand_idx;     // [2][G]
-    void* cand_row;    // [2][G][wmax] complex
-    void* diag_row;    // [2][wmax] complex
-    int* info;         // first zero pivot (1-based global column), 0 if none
+    unsigned* flags;  // [npanels][2][gmax] publication flags, zeroed before the factorisation
+    int gmax;         // flag / candidate stride per parity
+    int* abort_flag;  // set if a wait exceeds its spin limit
+    double* cand_val; // [2][gmax]
+    int* cand_idx;    // [2][gmax]
+    void* cand_row;   // [2][gmax][wmax] complex
+    void* diag_row;   // [2][wmax] complex
+    int* info;        // first zero pivot (1-based global column), 0 if none
 };
 
 constexpr long SPIN_LIMIT = 40000000; // ~seconds; a wait this long means a missing workgroup
 
-__device__ inline bool grid_barrier(unsigned* counter, unsigned target, int* abort_flag)
+// Wave-level wait until flags[x] >= target for every x < G (each lane polls its own flags).
+__device__ inline bool wait_flags(const unsigned* fl, int G, unsigned target, int* abort_flag, int lane)
 {
-    __syncthreads();
-    __shared__ int aborted;
-    if (threadIdx.x == 0) {
-        __threadfence();
-        __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int ok = 1;
+    for (int x = lane; x < G && ok; x += 64) {
         long spins = 0;
-        int ab = 0;
-        while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(fl + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins > SPIN_LIMIT) {
                 __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ab = 1;
+                ok = 0;
                 break;
             }
-            if ((spins & 1023) == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                ab = 1;
+            if ((spins & 255) == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                ok = 0;
                 break;
             }
         }
-        __threadfence();
-        aborted = ab;
     }
-    __syncthreads();
-    return aborted == 0;
+    return __all(ok);
+}
+
+__device__ inline void argmax_merge(double& v, int& i, double ov, int oi)
+{
+    if (ov > v || (ov == v && oi < i)) {
+        v = ov;
+        i = oi;
+    }
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
                                                     int R, int G, int* __restrict__ ipiv, PanelSync ps,
-                                                    unsigned* counter)
+                                                    unsigned* __restrict__ flags)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw); // [w][R] column-major
     cx<T>* prow = slab + (size_t)w * R;                // [w]
     __shared__ double rv[4];
     __shared__ int ri[4];
-    __shared__ int s_piv;
+    __shared__ int s_piv, s_gw, s_abort;
     __shared__ double s_pval;
 
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t r0 = (int64_t)g * R;                 // first local row of this workgroup (panel-relative)
-    const int nr = (int)((m - r0) < R ? (m - r0) : R); // rows owned
-    cx<T>* Ap = A + k + k * lda;                       // panel origin
-
-    for (int e = tid; e < w * R; e += 256) {
-        const int c = e / R, r = e % R;
-        slab[e] = r < nr ? Ap[(r0 + r) + (int64_t)c * lda] : cx<T>{0, 0};
-    }
+    const int r0 = g * R;                                      // first panel row of this workgroup
+    const int nr = (int)((m - r0) < R ? (m - r0) : R);         // rows owned (one per thread)
+    cx<T>* Ap = A + k + k * lda;                               // panel origin
+    for (int c = 0; c < w; c++)
+        if (tid < nr)
+            slab[(size_t)c * R + tid] = Ap[(r0 + tid) + (int64_t)c * lda];
     __syncthreads();
 
+    const int gmax = ps.gmax;
     double* cval = ps.cand_val;
     int* cidx = ps.cand_idx;
     cx<T>* crow = reinterpret_cast<cx<T>*>(ps.cand_row);
     cx<T>* drow = reinterpret_cast<cx<T>*>(ps.diag_row);
+    const bool own = tid < nr;
+    const int grow = r0 + tid; // panel-relative row of this thread
 
     for (int c = 0; c < w; c++) {
         const int par = c & 1;
-        // ---- local pivot candidate among rows >= c (panel-relative)
-        double best = -1.0;
-        int bidx = 0x7fffffff;
-        for (int r = tid; r < nr; r += 256) {
-            const int64_t gr = r0 + r;
-            if (gr < c)
-                continue;
-            const double v = (double)cabs1(slab[(size_t)c * R + r]);
-            if (v > best) { // rows visited in increasing order per thread: first max kept
-                best = v;
-                bidx = (int)gr;
-            }
+        // ---- local pivot candidate among rows >= c
+        double v = -1.0;
+        int vi = 0x7fffffff;
+        if (own && grow >= c) {
+            v = (double)cabs1(slab[(size_t)c * R + tid]);
+            vi = grow;
         }
 #pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const double ov = __shfl_xor(best, o, 64);
-            const int oi = __shfl_xor(bidx, o, 64);
-            if (ov > best || (ov == best && oi < bidx)) {
-                best = ov;
-                bidx = oi;
-            }
-        }
+        for (int o = 32; o > 0; o >>= 1)
+            argmax_merge(v, vi, __shfl_xor(v, o, 64), __shfl_xor(vi, o, 64));
         if (lane == 0) {
-            rv[wv] = best;
-            ri[wv] = bidx;
+            rv[wv] = v;
+            ri[wv] = vi;
         }
         __syncthreads();
-        if (tid == 0) {
-            for (int x = 1; x < 4; x++)
-                if (rv[x] > rv[0] || (rv[x] == rv[0] && ri[x] < ri[0])) {
-                    rv[0] = rv[x];
-                    ri[0] = ri[x];
-                }
-        }
-        __syncthreads();
-        const double lbest = rv[0];
-        const int lidx = ri[0];
-
+        double lb = rv[0];
+        int li = ri[0];
+        for (int x = 1; x < 4; x++)
+            argmax_merge(lb, li, rv[x], ri[x]);
+        int p;
+        double pval;
         if (G > 1) {
-            // ---- publish candidate (value, index, row) and, if owned, the diagonal row
+            // ---- publish candidate, and the diagonal row if owned, then raise this group's flag
             if (tid == 0) {
-                cval[par * G + g] = lbest;
-                cidx[par * G + g] = lidx;
+                cval[par * gmax + g] = lb;
+                cidx[par * gmax + g] = li;
             }
-            if (lbest >= 0.0) {
-                const int lr = lidx - (int)r0;
-                for (int j = tid; j < w; j += 256)
-                    crow[((size_t)par * G + g) * w + j] = slab[(size_t)j * R + lr];
+            if (lb >= 0.0 && tid < w)
+                crow[((size_t)par * gmax + g) * w + tid] = slab[(size_t)tid * R + (li - r0)];
+            if (c >= r0 && c < r0 + nr && tid < w)
+                drow[(size_t)par * w + tid] = slab[(size_t)tid * R + (c - r0)];
+            __syncthreads();
+            if (tid == 0) {
+                __threadfence();
+                __hip_atomic_store(flags + par * gmax + g, (unsigned)(c + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            const bool own_diag = (c >= r0) && (c < r0 + nr);
-            if (own_diag)
-                for (int j = tid; j < w; j += 256)
-                    drow[(size_t)par * w + j] = slab[(size_t)j * R + (c - r0)];
-            if (!grid_barrier(counter, (unsigned)(c + 1) * (unsigned)G, ps.abort_flag))
-                return;
-            // ---- global winner (first max), identical in every workgroup
             if (wv == 0) {
+                const bool ok = wait_flags(flags + par * gmax, G, (unsigned)(c + 1), ps.abort_flag, lane);
+                __threadfence();
                 double b = -1.0;
                 int bi = 0x7fffffff, bg = 0;
                 for (int x = lane; x < G; x += 64) {
-                    const double v = cval[par * G + x];
-                    const int ii = cidx[par * G + x];
-                    if (v > b || (v == b && ii < bi)) {
-                        b = v;
-                        bi = ii;
+                    const double cv = cval[par * gmax + x];
+                    const int ci = cidx[par * gmax + x];
+                    if (cv > b || (cv == b && ci < bi)) {
+                        b = cv;
+                        bi = ci;
                         bg = x;
                     }
                 }
 #pragma unroll
                 for (int o = 32; o > 0; o >>= 1) {
                     const double ov = __shfl_xor(b, o, 64);
-                    const int oi = __shfl_xor(bi, o, 64);
-                    const int og = __shfl_xor(bg, o, 64);
+                    const int oi = __shfl_xor(bi, o, 64), og = __shfl_xor(bg, o, 64);
                     if (ov > b || (ov == b && oi < bi)) {
                         b = ov;
                         bi = oi;
@@ -317,73 +305,60 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 if (lane == 0) {
                     s_pval = b;
                     s_piv = bi;
-                    ri[0] = bg;
+                    s_gw = bg;
+                    s_abort = ok ? 0 : 1;
                 }
             }
             __syncthreads();
-            const int gw = ri[0];
-            for (int j = tid; j < w; j += 256)
-                prow[j] = crow[((size_t)par * G + gw) * w + j];
+            if (s_abort)
+                return;
+            p = s_piv;
+            pval = s_pval;
+            if (tid < w)
+                prow[tid] = crow[((size_t)par * gmax + s_gw) * w + tid];
             __syncthreads();
-            const int p = s_piv;
-            if (s_pval > 0.0) {
-                if (c >= r0 && c < r0 + nr)
-                    for (int j = tid; j < w; j += 256)
-                        slab[(size_t)j * R + (c - r0)] = prow[j];
-                if (p != c && p >= r0 && p < r0 + nr)
-                    for (int j = tid; j < w; j += 256)
-                        slab[(size_t)j * R + (p - r0)] = drow[(size_t)par * w + j];
+            if (pval > 0.0) {
+                if (c >= r0 && c < r0 + nr && tid < w)
+                    slab[(size_t)tid * R + (c - r0)] = prow[tid];
+                if (p != c && p >= r0 && p < r0 + nr && tid < w)
+                    slab[(size_t)tid * R + (p - r0)] = drow[(size_t)par * w + tid];
             }
+            __syncthreads();
         } else {
-            // single workgroup: swap in LDS directly
-            if (tid == 0) {
-                s_piv = lidx;
-                s_pval = lbest;
+            p = li;
+            pval = lb;
+            if (pval > 0.0 && p != c && tid < w) {
+                const cx<T> t = slab[(size_t)tid * R + c];
+                slab[(size_t)tid * R + c] = slab[(size_t)tid * R + p];
+                slab[(size_t)tid * R + p] = t;
             }
             __syncthreads();
-            const int p = s_piv;
-            if (s_pval > 0.0 && p != c)
-                for (int j = tid; j < w; j += 256) {
-                    cx<T> t = slab[(size_t)j * R + c];
-                    slab[(size_t)j * R + c] = slab[(size_t)j * R + p];
-                    slab[(size_t)j * R + p] = t;
-                }
+            if (tid < w)
+                prow[tid] = slab[(size_t)tid * R + c];
             __syncthreads();
-            for (int j = tid; j < w; j += 256)
-                prow[j] = slab[(size_t)j * R + c];
         }
-        __syncthreads();
-        const int p = s_piv;
         if (g == 0 && tid == 0) {
-            ipiv[k + c] = (int)(k + (s_pval > 0.0 ? p : c));
-            if (!(s_pval > 0.0) && *ps.info == 0)
+            ipiv[k + c] = (int)(k + (pval > 0.0 ? p : c));
+            if (!(pval > 0.0) && *ps.info == 0)
                 *ps.info = (int)(k + c + 1);
         }
-        if (s_pval > 0.0) {
-            // ---- scale the column below the diagonal and update the rest of the panel
-            const cx<T> piv = prow[c];
-            const cx<T> rp = cdiv(cx<T>{1, 0}, piv);
-            const int rstart = (int)((c + 1 - r0) > 0 ? (c + 1 - r0) : 0);
-            for (int r = rstart + tid; r < nr; r += 256)
-                slab[(size_t)c * R + r] = slab[(size_t)c * R + r] * rp;
-            __syncthreads();
-            const int rows = nr - rstart, cols = w - c - 1;
-            for (int e = tid; e < rows * cols; e += 256) {
-                const int r = rstart + e % rows, j = c + 1 + e / rows;
-                const cx<T> l = slab[(size_t)c * R + r];
+        // ---- scale this thread's row and apply the rank-1 update to the rest of its row
+        if (pval > 0.0 && own && grow > c) {
+            const cx<T> rp = cdiv(cx<T>{1, 0}, prow[c]);
+            const cx<T> l = slab[(size_t)c * R + tid] * rp;
+            slab[(size_t)c * R + tid] = l;
+            for (int j = c + 1; j < w; j++) {
                 const cx<T> u = prow[j];
-                cx<T>& a = slab[(size_t)j * R + r];
+                cx<T>& a = slab[(size_t)j * R + tid];
                 a.re -= l.re * u.re - l.im * u.im;
                 a.im -= l.re * u.im + l.im * u.re;
             }
         }
-        __syncthreads();
     }
-    for (int e = tid; e < w * R; e += 256) {
-        const int c = e / R, r = e % R;
-        if (r < nr)
-            Ap[(r0 + r) + (int64_t)c * lda] = slab[e];
-    }
+    __syncthreads();
+    for (int c = 0; c < w; c++)
+        if (tid < nr)
+            Ap[(r0 + tid) + (int64_t)c * lda] = slab[(size_t)c * R + tid];
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -479,6 +454,97 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
         if (r < w && c < nc)
             B[r + (c0 + c) * ldb] = sX[r][c];
     }
+}
+
+// Inverses of the 64 x 64 diagonal chunks of the factors, for the solves: chunk ch covers rows and
+// columns [64 ch, 64 ch + 64).  Linv = L_cc^{-1} (unit lower), Uinv = U_cc^{-1} (upper).  Stored
+// dense column-major, 64 x 64 per chunk: inv[(2 ch + 0) * 4096] = Linv, [(2 ch + 1) * 4096] = Uinv.
+template <typename T>
+__global__ __launch_bounds__(256) void diag_inverse(const cx<T>* __restrict__ A, int64_t n, int64_t lda,
+                                                    cx<T>* __restrict__ inv)
+{
+    __shared__ cx<T> sD[64][65];
+    __shared__ cx<T> sX[64][65];
+    const int64_t ch = blockIdx.x, k = ch * 64;
+    const int w = (int)((n - k) < 64 ? (n - k) : 64);
+    const int tid = threadIdx.x, col = tid & 63, rg = tid >> 6;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e % 64, c = e / 64;
+        sD[r][c] = (r < w && c < w) ? A[(k + r) + (k + c) * lda] : cx<T>{0, 0};
+        sX[r][c] = cx<T>{(T)(r == c && r < w ? 1 : 0), 0};
+    }
+    __syncthreads();
+    for (int r = 0; r < w; r++) {
+        const cx<T> x = sX[r][col];
+        for (int rr = r + 1 + rg; rr < w; rr += 4) {
+            const cx<T> l = sD[rr][r];
+            sX[rr][col].re -= l.re * x.re - l.im * x.im;
+            sX[rr][col].im -= l.re * x.im + l.im * x.re;
+        }
+        __syncthreads();
+    }
+    cx<T>* Li = inv + (size_t)(2 * ch) * 4096;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e % 64, c = e / 64;
+        Li[e] = sX[r][c];
+        sX[r][c] = cx<T>{(T)(r == c && r < w ? 1 : 0), 0};
+    }
+    __syncthreads();
+    for (int r = w - 1; r >= 0; r--) {
+        if (rg == 0)
+            sX[r][col] = cdiv(sX[r][col], sD[r][r]);
+        __syncthreads();
+        const cx<T> x = sX[r][col];
+        for (int rr = rg; rr < r; rr += 4) {
+            const cx<T> u = sD[rr][r];
+            sX[rr][col].re -= u.re * x.re - u.im * x.im;
+            sX[rr][col].im -= u.re * x.im + u.im * x.re;
+        }
+        __syncthreads();
+    }
+    cx<T>* Ui = inv + (size_t)(2 * ch + 1) * 4096;
+    for (int e = tid; e < 64 * 64; e += 256)
+        Ui[e] = sX[e % 64][e / 64];
+}
+
+// B (w x ncols, leading dimension ldb) <- Tm B with a dense 64 x 64 Tm (column-major), in place;
+// one workgroup per 64 columns, every thread computes 16 rows of one column.
+template <typename T>
+__global__ __launch_bounds__(256) void trmm_block(int w, const cx<T>* __restrict__ Tm, cx<T>* __restrict__ B, int64_t ldb,
+                                                  int64_t ncols)
+{
+    __shared__ cx<T> sT[64][65];
+    __shared__ cx<T> sX[64][65];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * 64;
+    const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e % 64, c = e / 64;
+        sT[r][c] = (r < w && c < w) ? Tm[r + 64 * c] : cx<T>{0, 0};
+        sX[r][c] = (r < w && c < nc) ? B[r + (c0 + c) * ldb] : cx<T>{0, 0};
+    }
+    __syncthreads();
+    const int col = tid & 63, rg = tid >> 6;
+    cx<T> acc[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        acc[i] = cx<T>{0, 0};
+    for (int j = 0; j < w; j++) {
+        const cx<T> x = sX[j][col];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const cx<T> t = sT[rg + 4 * i][j];
+            acc[i].re += t.re * x.re - t.im * x.im;
+            acc[i].im += t.re * x.im + t.im * x.re;
+        }
+    }
+    if (col < nc)
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = rg + 4 * i;
+            if (r < w)
+                B[r + (c0 + col) * ldb] = acc[i];
+        }
 }
 
 __global__ void zero_u32(unsigned* p, int64_t n)
@@ -600,13 +666,45 @@ int launch_symmetrize(c128* Z, int64_t n, int64_t ldz, hipStream_t st)
 
 template <typename T> int panel_rows(int w)
 {
-    // slab of R rows x w complex plus the pivot row must fit in 160 KB of LDS with headroom
+    // one row per thread (at most 256) and the slab of R rows x w complex plus the pivot row in LDS
     const size_t budget = 144 * 1024;
     int R = (int)(budget / ((size_t)w * sizeof(cx<T>))) - 1;
     R = R >= 256 ? 256 : (R >= 128 ? 128 : (R >= 64 ? 64 : 32));
     return R;
 }
 
+// Update of columns [c0, c0 + nc) by step k (panel of width w): row interchanges, U12 = L11^{-1} A12
+// and A22 -= L21 U12.
+template <typename T>
+static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int64_t k, int w, int64_t c0, int64_t nc,
+                       hipStream_t st, KernelTimer* kt)
+{
+    if (nc <= 0)
+        return 0;
+    hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A, lda, c0, nc, k, w, ipiv);
+    for (int s = 0; s < w; s += 64) {
+        const int ws = (w - s) < 64 ? (w - s) : 64;
+        cx<T>* D = A + (k + s) + (k + s) * lda;
+        cx<T>* Bb = A + (k + s) + c0 * lda;
+        hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nc + 63) / 64)), dim3(256), 0, st, ws, D, lda, Bb, lda,
+                           nc);
+        if (s + ws < w) {
+            int rc = gemm_sub<T>(w - s - ws, nc, ws, A + (k + s + ws) + (k + s) * lda, lda, Bb, lda,
+                                 A + (k + s + ws) + c0 * lda, lda, st, nullptr);
+            if (rc)
+                return rc;
+        }
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda, A + (k + w) + c0 * lda, lda,
+                       st, kt);
+}
+
+// Right-looking blocked LU.  With a side stream the update of step p is split: the columns of panel
+// p+1 are updated on the main stream, which then factors panel p+1 while the side stream updates
+// the remaining columns (look-ahead of depth one).  Dependencies are carried by events:
+//   main: panel p -> [P_p] -> (wait W_{p-1}) -> narrow update p -> panel p+1
+//   side: (wait P_p) -> wide update p -> [W_p]
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer)
 {
@@ -617,62 +715,92 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     const size_t sh_max = ((size_t)nb * R + nb) * sizeof(cx<T>);
     if (sh_max > 64 * 1024)
         FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&panel_factor<T>), sh_max));
-    if ((int64_t)wk.ncounters < npan)
+    const int gmax = wk.max_groups;
+    if ((int64_t)wk.ncounters < npan * 2 * gmax)
         return set_error("lu: sync workspace too small"), -1;
-    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((npan + 255) / 256)), dim3(256), 0, st, wk.counters, npan);
+    const int64_t nfl = npan * 2 * gmax;
+    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((nfl + 255) / 256)), dim3(256), 0, st, wk.counters, nfl);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
     FRICO_HIP_CHECK(hipGetLastError());
     PanelSync ps{};
+    ps.gmax = gmax;
     ps.abort_flag = wk.flags;
     ps.info = wk.flags + 1;
     ps.cand_val = wk.cand_val;
     ps.cand_idx = wk.cand_idx;
     ps.cand_row = wk.cand_row;
     ps.diag_row = wk.diag_row;
-    for (int64_t p = 0; p < npan; p++) {
+    auto panel = [&](int64_t p) -> int {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
         const int64_t m = n - k;
         const int G = (int)((m + R - 1) / R);
-        if (G > wk.max_groups)
-            return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, wk.max_groups), -1;
+        if (G > gmax)
+            return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), -1;
         const size_t sh = ((size_t)w * R + w) * sizeof(cx<T>);
         hipLaunchKernelGGL((panel_factor<T>), dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
-                           wk.counters + p);
+                           wk.counters + p * 2 * gmax);
         FRICO_HIP_CHECK(hipGetLastError());
-        const int64_t nright = n - k - w;
-        if (nright > 0) {
-            hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((nright + 255) / 256)), dim3(256), 0, st, A, lda, k + w,
-                               nright, k, w, ipiv);
-            // U12 = L11^{-1} A12, in chunks of 64 rows of the block when nb > 64
-            for (int s = 0; s < w; s += 64) {
-                const int ws = (w - s) < 64 ? (w - s) : 64;
-                cx<T>* D = A + (k + s) + (k + s) * lda;
-                cx<T>* Bb = A + (k + s) + (k + w) * lda;
-                hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nright + 63) / 64)), dim3(256), 0, st, ws, D, lda,
-                                   Bb, lda, nright);
-                if (s + ws < w) {
-                    int rc = gemm_sub<T>(w - s - ws, nright, ws, A + (k + s + ws) + (k + s) * lda, lda, Bb, lda,
-                                         A + (k + s + ws) + (k + w) * lda, lda, st, nullptr);
-                    if (rc)
-                        return rc;
-                }
-            }
-            FRICO_HIP_CHECK(hipGetLastError());
-            int rc = gemm_sub<T>(nright, nright, w, A + (k + w) + k * lda, lda, A + k + (k + w) * lda, lda,
-                                 A + (k + w) + (k + w) * lda, lda, st, gemm_timer);
-            if (rc)
+        return 0;
+    };
+    const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
+    int rc;
+    if (!lookahead) {
+        for (int64_t p = 0; p < npan; p++) {
+            const int64_t k = p * nb;
+            const int w = (int)((n - k) < nb ? (n - k) : nb);
+            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, ipiv, k, w, k + w, n - k - w, st, gemm_timer)))
                 return rc;
         }
+    } else {
+        hipEvent_t* evP = wk.events;
+        hipEvent_t* evW = wk.events + npan;
+        hipEvent_t ev0 = wk.events[2 * npan];
+        FRICO_HIP_CHECK(hipEventRecord(ev0, st));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, ev0, 0));
+        if ((rc = panel(0)))
+            return rc;
+        bool prev_wide = false;
+        for (int64_t p = 0; p < npan; p++) {
+            const int64_t k = p * nb;
+            const int w = (int)((n - k) < nb ? (n - k) : nb);
+            const int64_t nright = n - k - w;
+            if (nright <= 0)
+                break;
+            const int64_t nnext = nright < nb ? nright : nb;
+            const int64_t nrest = nright - nnext;
+            FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
+            if (nrest > 0) {
+                FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
+                if ((rc = step_update<T>(A, n, lda, ipiv, k, w, k + w + nnext, nrest, wk.side, gemm_timer)))
+                    return rc;
+                FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
+            }
+            if (prev_wide)
+                FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
+            if ((rc = step_update<T>(A, n, lda, ipiv, k, w, k + w, nnext, st, gemm_timer)) || (rc = panel(p + 1)))
+                return rc;
+            prev_wide = nrest > 0;
+        }
+        // the main stream continues only after all side work
+        FRICO_HIP_CHECK(hipEventRecord(ev0, wk.side));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(st, ev0, 0));
+    }
+    if (wk.dinv) {
+        const int64_t nch = (n + 63) / 64;
+        hipLaunchKernelGGL((diag_inverse<T>), dim3((unsigned)nch), dim3(256), 0, st, A, n, lda, (cx<T>*)wk.dinv);
+        FRICO_HIP_CHECK(hipGetLastError());
     }
     return 0;
 }
 
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st)
+             hipStream_t st, const void* dinv)
 {
     const int64_t npan = (n + nb - 1) / nb;
+    const cx<T>* inv = (const cx<T>*)dinv;
+    const unsigned gcols = (unsigned)((nrhs + 63) / 64);
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
@@ -680,8 +808,14 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
                            w, ipiv);
         for (int s = 0; s < w; s += 64) {
             const int ws = (w - s) < 64 ? (w - s) : 64;
-            hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
-                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+            // the diagonal sub-blocks of a chunk inverse are the inverses of the chunk's diagonal sub-blocks
+            const int64_t ch = (k + s) / 64, off = (k + s) % 64;
+            if (inv)
+                hipLaunchKernelGGL((trmm_block<T>), dim3(gcols), dim3(256), 0, st, ws,
+                                   inv + (size_t)(2 * ch) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
+            else
+                hipLaunchKernelGGL((trsm_block<T, false>), dim3(gcols), dim3(256), 0, st, ws, A + (k + s) + (k + s) * lda, lda,
+                                   B + (k + s), ldb, nrhs);
             const int64_t below = n - (k + s + ws);
             if (below > 0) {
                 int rc = gemm_sub<T>(below, nrhs, ws, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s), ldb,
@@ -696,8 +830,13 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
         const int w = (int)((n - k) < nb ? (n - k) : nb);
         for (int s = ((w - 1) / 64) * 64; s >= 0; s -= 64) {
             const int ws = (w - s) < 64 ? (w - s) : 64;
-            hipLaunchKernelGGL((trsm_block<T, true>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
-                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+            const int64_t ch = (k + s) / 64, off = (k + s) % 64;
+            if (inv)
+                hipLaunchKernelGGL((trmm_block<T>), dim3(gcols), dim3(256), 0, st, ws,
+                                   inv + (size_t)(2 * ch + 1) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
+            else
+                hipLaunchKernelGGL((trsm_block<T, true>), dim3(gcols), dim3(256), 0, st, ws, A + (k + s) + (k + s) * lda, lda,
+                                   B + (k + s), ldb, nrhs);
             const int64_t above = k + s;
             if (above > 0) {
                 int rc = gemm_sub<T>(above, nrhs, ws, A + (k + s) * lda, lda, B + (k + s), ldb, B, ldb, st, nullptr);
@@ -790,8 +929,10 @@ template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, con
                              hipStream_t, KernelTimer*);
 template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
-template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t);
-template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t);
+template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t,
+                              const void*);
+template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t,
+                             const void*);
 template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t,
                                          hipStream_t);
 template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,

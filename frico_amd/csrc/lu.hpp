@@ -20,6 +20,10 @@ struct LuWork {
     void* cand_row = nullptr;
     void* diag_row = nullptr;
     int max_groups = 0; // co-resident workgroup limit of the panel kernel
+    void* dinv = nullptr; // [ceil(n/64)][2][64*64] diagonal-chunk inverses, written at the end of lu_factor
+    hipStream_t side = nullptr;    // second stream (default priority) for the look-ahead, or null
+    hipEvent_t* events = nullptr;  // >= 2 * npanels + 1 events (timing disabled)
+    int nevents = 0;
 };
 
 template <typename T>
@@ -31,9 +35,11 @@ int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, c
 template <typename T> int panel_rows(int w);
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
+// dinv: the 64 x 64 diagonal-chunk inverses written by lu_factor (LuWork::dinv), or null to use
+// in-LDS triangular solves.
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st);
+             hipStream_t st, const void* dinv);
 template <typename T>
 int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
                         int64_t nrhs, hipStream_t st);
