@@ -13,6 +13,7 @@
 //   trsm_block     U12 = L11^{-1} A12 (unit lower, in LDS).
 //   cgemm_sub      A22 -= L21 U12, complex GEMM on MFMA (v_mfma_f64_16x16x4f64 or
 //                  v_mfma_f32_16x16x4f32), 64x64 complex output tile per workgroup.
+#include "../../include/frico_gpu.h"
 #include "common.hpp"
 #include "lu.hpp"
 
@@ -736,7 +737,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         const int64_t m = n - k;
         const int G = (int)((m + R - 1) / R);
         if (G > gmax)
-            return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), -1;
+            return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), FRICO_ERR_LIMIT;
         const size_t sh = ((size_t)w * R + w) * sizeof(cx<T>);
         hipLaunchKernelGGL((panel_factor<T>), dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
                            wk.counters + p * 2 * gmax);
