@@ -147,6 +147,8 @@ int frico_ctx_create(int device, frico_ctx** out)
     }
     if (const char* la = std::getenv("FRICO_LU_LOOKAHEAD"))
         c->lookahead = std::atoi(la) != 0;
+    if (const char* fs = std::getenv("FRICO_LU_FUSED_SOLVE"))
+        c->fused_solve = std::atoi(fs) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -833,6 +835,18 @@ int frico_factor(frico_ctx* c, int form)
         return rc;
     const int64_t N = c->N;
     LuWork wk = lu_work(c);
+    // W/V products for the fused solves: one more N x N matrix of the factor's type, taken only
+    // when it fits with room to spare
+    c->wv_ok = false;
+    if (c->fused_solve && c->nb % 64 == 0 && N > 64) {
+        const size_t bytes = (size_t)(N * N) * (form == FRICO_SOLVE_C128 ? sizeof(c128) : sizeof(c64));
+        size_t fr = 0, tot = 0;
+        if (c->wv.n >= bytes || (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > bytes + c->wv.n + ((size_t)2 << 30))) {
+            if ((rc = c->wv.ensure(bytes)))
+                return rc;
+            wk.wv = c->wv.p;
+        }
+    }
     if (form == FRICO_SOLVE_C128) {
         if (c->zprec != FRICO_C128 || c->z64_factored)
             return set_error("complex128 factorisation needs an unfactored complex128 Z"), FRICO_ERR_STATE;
@@ -861,6 +875,7 @@ int frico_factor(frico_ctx* c, int form)
         return rc;
     c->factored = form;
     c->nb_factored = c->nb;
+    c->wv_ok = wk.wv != nullptr;
     return read_lu_flags(c);
 }
 
@@ -911,7 +926,10 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     if (c->factored == FRICO_SOLVE_C128) {
         phase_begin(c, c->t_solve);
         FRICO_HIP_CHECK(hipMemcpyAsync(c->X.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
-        rc = lu_solve<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->X.p, N, nr, c->st, c->dinv.p);
+        if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c128))))
+            return rc;
+        rc = lu_solve<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->X.p, N, nr, c->st, c->dinv.p,
+                              c->wv_ok ? c->wv.p : nullptr, c->ysolve.p);
         phase_end(c, c->t_solve);
         if (rc)
             return rc;
@@ -925,6 +943,8 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
         return set_error("maxit must be >= 1"), -3;
     if ((rc = c->R.ensure((size_t)(N * nr))) || (rc = c->D32.ensure((size_t)(N * nr))))
         return rc;
+    if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c64))))
+        return rc;
     std::vector<double> bn, rn;
     if ((rc = col_norms(c, c->B.p, nr, bn)))
         return rc;
@@ -937,7 +957,8 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
         phase_begin(c, tt);
         if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
             return rc;
-        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p)))
+        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
+                                  c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
             return rc;
         if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
             return rc;

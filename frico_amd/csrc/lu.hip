@@ -44,33 +44,39 @@ template <> struct Mfma<float> {
 };
 
 // ----------------------------------------------------------------------------------------------
-// C[M x N] -= A[M x K] * B[K x N], complex, column-major.
-// Workgroup = 4 waves (2 x 2), tile BM x BN complex, wave tile (WM*16) x (WN*16).
+// 64 x 64 complex output tiles on MFMA.  Workgroup = 4 waves (2 x 2), wave tile (WM*16) x (WN*16).
 // The MFMA computes the transposed product (B^T A^T) so that the 16 lanes of an accumulator row
-// hold 16 consecutive rows of C: stores are contiguous down a column.
+// hold 16 consecutive rows of the output: stores are contiguous down a column.
 // ----------------------------------------------------------------------------------------------
 constexpr int GBM = 64, GBN = 64, GBK = 16, GWM = 2, GWN = 2;
 
-template <typename T, typename TA, bool TRANSA = false>
-__global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
-                                                 const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
-                                                 int64_t ldc)
+template <typename T> struct GemmAcc {
+    typename Mfma<T>::acc_t re[GWM][GWN], im[GWM][GWN];
+};
+
+template <typename T> struct GemmSmem {
+    cx<T> a[GBK][GBM + 1];
+    cx<T> b[GBK][GBN + 1];
+};
+
+// acc = A_t[0:mr, 0:K] * B_t[0:K, 0:nc] (entries outside are zero), with
+//   A_t(r, kk) = TRANSA ? A[kk + r lda] : A[r + kk lda],   B_t(kk, c) = B[kk + c ldb].
+// With BLDS the B operand comes from LDS instead: B_t(kk, c) = Bs[kk][c].
+template <typename T, typename TA, bool TRANSA, bool BLDS = false>
+__device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
+                                 const cx<T>* __restrict__ B, int64_t ldb, GemmAcc<T>& acc, GemmSmem<T>& sm,
+                                 const cx<T> (*Bs)[65] = nullptr)
 {
     using MF = Mfma<T>;
     using acc_t = typename MF::acc_t;
-    __shared__ cx<T> sA[GBK][GBM + 1];
-    __shared__ cx<T> sB[GBK][GBN + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
-    const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
-
-    acc_t accr[GWM][GWN], acci[GWM][GWN];
 #pragma unroll
     for (int a = 0; a < GWM; a++)
 #pragma unroll
         for (int b = 0; b < GWN; b++) {
-            accr[a][b] = acc_t{0, 0, 0, 0};
-            acci[a][b] = acc_t{0, 0, 0, 0};
+            acc.re[a][b] = acc_t{0, 0, 0, 0};
+            acc.im[a][b] = acc_t{0, 0, 0, 0};
         }
 
     constexpr int LA = GBM * GBK / 256, LB = GBK * GBN / 256;
@@ -81,9 +87,9 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
             const int e = tid + 256 * s;
             // consecutive threads walk the contiguous dimension of A in memory
             const int r = TRANSA ? e / GBK : e % GBM, kk = TRANSA ? e % GBK : e / GBM;
-            const int64_t gr = m0 + r, gk = k0 + kk;
-            if (gr < M && gk < K) {
-                const cx<TA> v = TRANSA ? A[gk + gr * lda] : A[gr + gk * lda];
+            const int64_t gk = k0 + kk;
+            if (r < mr && gk < K) {
+                const cx<TA> v = TRANSA ? A[gk + r * lda] : A[r + gk * lda];
                 ra[s] = cx<T>{(T)v.re, (T)v.im};
             } else {
                 ra[s] = cx<T>{0, 0};
@@ -93,8 +99,11 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
         for (int s = 0; s < LB; s++) {
             const int e = tid + 256 * s;
             const int kk = e % GBK, c = e / GBK;
-            const int64_t gk = k0 + kk, gc = n0 + c;
-            rb[s] = (gk < K && gc < N) ? B[gk + gc * ldb] : cx<T>{0, 0};
+            const int64_t gk = k0 + kk;
+            if (BLDS)
+                rb[s] = (gk < K && c < nc) ? Bs[gk][c] : cx<T>{0, 0};
+            else
+                rb[s] = (gk < K && c < nc) ? B[gk + c * ldb] : cx<T>{0, 0};
         }
     };
     auto sstore = [&]() {
@@ -102,14 +111,14 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
         for (int s = 0; s < LA; s++) {
             const int e = tid + 256 * s;
             if (TRANSA)
-                sA[e % GBK][e / GBK] = ra[s];
+                sm.a[e % GBK][e / GBK] = ra[s];
             else
-                sA[e / GBM][e % GBM] = ra[s];
+                sm.a[e / GBM][e % GBM] = ra[s];
         }
 #pragma unroll
         for (int s = 0; s < LB; s++) {
             const int e = tid + 256 * s;
-            sB[e % GBK][e / GBK] = rb[s];
+            sm.b[e % GBK][e / GBK] = rb[s];
         }
     };
 
@@ -125,41 +134,111 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
             cx<T> af[GWM], bf[GWN];
 #pragma unroll
             for (int a = 0; a < GWM; a++)
-                af[a] = sA[kk + (lane >> 4)][wm * GWM * 16 + a * 16 + (lane & 15)];
+                af[a] = sm.a[kk + (lane >> 4)][wm * GWM * 16 + a * 16 + (lane & 15)];
 #pragma unroll
             for (int b = 0; b < GWN; b++)
-                bf[b] = sB[kk + (lane >> 4)][wn * GWN * 16 + b * 16 + (lane & 15)];
+                bf[b] = sm.b[kk + (lane >> 4)][wn * GWN * 16 + b * 16 + (lane & 15)];
 #pragma unroll
             for (int a = 0; a < GWM; a++)
 #pragma unroll
                 for (int b = 0; b < GWN; b++) {
                     // D^T = B^T A^T:  re += br*ar - bi*ai ; im += br*ai + bi*ar
-                    accr[a][b] = MF::op(bf[b].re, af[a].re, accr[a][b]);
-                    accr[a][b] = MF::op(-bf[b].im, af[a].im, accr[a][b]);
-                    acci[a][b] = MF::op(bf[b].re, af[a].im, acci[a][b]);
-                    acci[a][b] = MF::op(bf[b].im, af[a].re, acci[a][b]);
+                    acc.re[a][b] = MF::op(bf[b].re, af[a].re, acc.re[a][b]);
+                    acc.re[a][b] = MF::op(-bf[b].im, af[a].im, acc.re[a][b]);
+                    acc.im[a][b] = MF::op(bf[b].re, af[a].im, acc.im[a][b]);
+                    acc.im[a][b] = MF::op(bf[b].im, af[a].re, acc.im[a][b]);
                 }
         }
     }
+}
+
+// C_t[0:mr, 0:nc] -= acc (assign = false) or C_t = acc (assign = true); C_t(r, c) = C[r + c ldc].
+template <typename T>
+__device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C, int64_t ldc, const GemmAcc<T>& acc,
+                                  bool assign)
+{
+    using MF = Mfma<T>;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv & 1, wn = wv >> 1;
 #pragma unroll
     for (int a = 0; a < GWM; a++)
 #pragma unroll
         for (int b = 0; b < GWN; b++) {
-            const int64_t row = m0 + wm * GWM * 16 + a * 16 + (lane & 15);
-            if (row >= M)
+            const int row = wm * GWM * 16 + a * 16 + (lane & 15);
+            if (row >= mr)
                 continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t col = n0 + wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
-                if (col < N) {
-                    cx<T>* p = C + row + col * ldc;
-                    cx<T> v = *p;
-                    v.re -= accr[a][b][r];
-                    v.im -= acci[a][b][r];
-                    *p = v;
+                const int col = wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
+                if (col < nc) {
+                    cx<T>* p = C + row + (int64_t)col * ldc;
+                    if (assign) {
+                        *p = cx<T>{acc.re[a][b][r], acc.im[a][b][r]};
+                    } else {
+                        cx<T> v = *p;
+                        v.re -= acc.re[a][b][r];
+                        v.im -= acc.im[a][b][r];
+                        *p = v;
+                    }
                 }
             }
         }
+}
+
+// C[M x N] -= op(A)[M x K] * B[K x N], complex, column-major; op(A) = A or A^T.
+template <typename T, typename TA, bool TRANSA = false>
+__global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
+                                                 const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+                                                 int64_t ldc)
+{
+    __shared__ GemmSmem<T> sm;
+    const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
+    GemmAcc<T> acc;
+    gemm_tile<T, TA, TRANSA>(M - m0, N - n0, K, TRANSA ? A + m0 * lda : A + m0, lda, B + n0 * ldb, ldb, acc, sm);
+    gemm_store<T>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, false);
+}
+
+// W blocks for the forward solves: for row tile t > chunk j (64 x 64 each)
+//   W(t, j) = L(t, j) Linv_j,  stored at the same place in WV (leading dimension ldw).
+// Grid (row tiles, chunks).  (The backward solve does not use U's diagonal-block inverses: a
+// product with an explicit U_jj^{-1} loses accuracy in proportion to cond(U_jj) on EFIE
+// matrices, so U_jj is applied by substitution.)
+template <typename T>
+__global__ __launch_bounds__(256) void make_w(int64_t n, const cx<T>* __restrict__ A, int64_t lda,
+                                              const cx<T>* __restrict__ inv, cx<T>* __restrict__ WV, int64_t ldw)
+{
+    const int64_t t = blockIdx.x, j = blockIdx.y;
+    if (t <= j)
+        return;
+    __shared__ GemmSmem<T> sm;
+    const int64_t m0 = t * 64, k = j * 64;
+    const int64_t w = (n - k) < 64 ? (n - k) : 64;
+    GemmAcc<T> acc;
+    gemm_tile<T, T, false>(n - m0, w, w, A + m0 + k * lda, lda, inv + (size_t)(2 * j) * 4096, 64, acc, sm);
+    gemm_store<T>(n - m0, w, WV + m0 + k * ldw, ldw, acc, true);
+}
+
+// One chunk of a blocked triangular solve as a single product.  Rows [r0, r0 + M) in 64-row tiles;
+// tile ct holds the chunk's own rows and computes Xo = inv * Y (assign), every other tile computes
+// C -= WV(rows, chunk) * Y.  Y is the chunk's K x nrhs right-hand side (leading dimension ldy);
+// it is only read, and C and Xo do not overlap it.
+template <typename T>
+__global__ __launch_bounds__(256) void solve_chunk(int64_t r0, int64_t M, int64_t nrhs, int64_t K, int64_t ct,
+                                                   const cx<T>* __restrict__ inv, const cx<T>* __restrict__ WVc,
+                                                   int64_t ldw, const cx<T>* __restrict__ Y, int64_t ldy,
+                                                   cx<T>* __restrict__ C, int64_t ldc, cx<T>* __restrict__ Xo,
+                                                   int64_t ldx)
+{
+    __shared__ GemmSmem<T> sm;
+    const int64_t bx = blockIdx.x, m0 = bx * GBM, n0 = (int64_t)blockIdx.y * GBN;
+    const bool own = bx == ct;
+    const cx<T>* At = own ? inv : WVc + r0 + m0;
+    GemmAcc<T> acc;
+    gemm_tile<T, T, false>(M - m0, nrhs - n0, K, At, own ? 64 : ldw, Y + n0 * ldy, ldy, acc, sm);
+    if (own)
+        gemm_store<T>(M - m0, nrhs - n0, Xo + r0 + m0 + n0 * ldx, ldx, acc, true);
+    else
+        gemm_store<T>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, acc, false);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -363,24 +442,132 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
 }
 
 // ----------------------------------------------------------------------------------------------
-// Row interchanges of one panel applied to columns [col0, col0+ncols).
+// Row interchanges of one panel (rows k .. k+w-1 swapped in turn with ipiv[k + c]) applied to
+// columns [col0, col0 + ncols).  The sequence of swaps is first reduced to the permutation it
+// makes of the rows it touches: wave 0 replays the swaps on row labels in LDS, with the rows below
+// the panel held one per lane and found by ballot.  Each touched row then receives its final
+// content in one gather, all reads before all writes within a column.  One workgroup per 64
+// columns.
 // ----------------------------------------------------------------------------------------------
+// Slot access for laswp's replay: slot sl lives in sr<sl / 64> of lane sl % 64 (sl wave-uniform).
+#define LASWP_GET(sl)                                                                                       \
+    ((sl) < 64 ? __builtin_amdgcn_readlane(sr0, (sl) & 63)                                                  \
+               : (sl) < 128 ? __builtin_amdgcn_readlane(sr1, (sl) & 63)                                     \
+                            : (sl) < 192 ? __builtin_amdgcn_readlane(sr2, (sl) & 63)                        \
+                                         : __builtin_amdgcn_readlane(sr3, (sl) & 63))
+#define LASWP_PUT(sl, v)                                                                                    \
+    do {                                                                                                    \
+        if (lane == ((sl) & 63)) {                                                                          \
+            if ((sl) < 64)                                                                                  \
+                sr0 = (v);                                                                                  \
+            else if ((sl) < 128)                                                                            \
+                sr1 = (v);                                                                                  \
+            else if ((sl) < 192)                                                                            \
+                sr2 = (v);                                                                                  \
+            else                                                                                            \
+                sr3 = (v);                                                                                  \
+        }                                                                                                   \
+    } while (0)
+
+constexpr int LASWP_COLS = 64;
+
 template <typename T>
 __global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols, int64_t k,
                                              int w, const int* __restrict__ ipiv)
 {
-    const int64_t j = col0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (j >= col0 + ncols)
-        return;
-    cx<T>* col = A + j * lda;
-    for (int c = 0; c < w; c++) {
-        const int64_t p = ipiv[k + c];
-        if (p != k + c) {
-            cx<T> t = col[k + c];
-            col[k + c] = col[p];
-            col[p] = t;
-        }
+    __shared__ int s_row[2 * LU_NB_MAX]; // row of slot s
+    __shared__ int s_src[2 * LU_NB_MAX]; // row whose content ends in slot s
+    __shared__ int s_cnt;
+    const int tid = threadIdx.x;
+    for (int s = tid; s < w; s += 256) {
+        s_row[s] = (int)k + s;
+        s_src[s] = (int)k + s;
     }
+    __syncthreads();
+    if (tid < 64) {
+        // Slot s (< 256) has its content source in register sr[s / 64] of lane s % 64; slots are
+        // addressed with wave-uniform indices, so reads are readlanes and the swap chain never
+        // waits on memory.
+        const int lane = tid;
+        const int pv0 = lane < w ? ipiv[k + lane] : 0; // pivots of the panel, two per lane (w <= 128)
+        const int pv1 = lane + 64 < w ? ipiv[k + 64 + lane] : 0;
+        int sr0 = (int)k + lane, sr1 = (int)k + 64 + lane, sr2 = -1, sr3 = -1;
+        int e0 = -1, e1 = -1; // rows of the extra slots w + lane and w + 64 + lane
+        int cnt = 0;
+        const int kw = (int)k + w;
+        for (int c = 0; c < w; c++) {
+            const int p = c < 64 ? __builtin_amdgcn_readlane(pv0, c) : __builtin_amdgcn_readlane(pv1, c - 64);
+            if (p == (int)k + c)
+                continue;
+            int sp;
+            if (p < kw) {
+                sp = p - (int)k;
+            } else {
+                const unsigned long long m0 = __ballot(e0 == p), m1 = __ballot(e1 == p);
+                if (m0) {
+                    sp = w + __ffsll((long long)m0) - 1;
+                } else if (m1) {
+                    sp = w + 64 + __ffsll((long long)m1) - 1;
+                } else {
+                    sp = w + cnt;
+                    if (cnt < 64) {
+                        if (lane == cnt)
+                            e0 = p;
+                    } else if (lane == cnt - 64) {
+                        e1 = p;
+                    }
+                    LASWP_PUT(sp, p);
+                    if (lane == 0)
+                        s_row[sp] = p;
+                    cnt++;
+                }
+            }
+            const int a = LASWP_GET(c), b = LASWP_GET(sp);
+            LASWP_PUT(c, b);
+            LASWP_PUT(sp, a);
+        }
+        const int L = w + cnt;
+        if (lane < L)
+            s_src[lane] = sr0;
+        if (64 + lane < L)
+            s_src[64 + lane] = sr1;
+        if (128 + lane < L)
+            s_src[128 + lane] = sr2;
+        if (192 + lane < L)
+            s_src[192 + lane] = sr3;
+        if (lane == 0)
+            s_cnt = cnt;
+    }
+    __syncthreads();
+    const int L = w + s_cnt;
+    const int64_t c0 = col0 + (int64_t)blockIdx.x * LASWP_COLS;
+    const int nc = (int)((col0 + ncols - c0) < LASWP_COLS ? (col0 + ncols - c0) : LASWP_COLS);
+    constexpr int CB = 8; // columns per pass, staged through LDS
+    __shared__ cx<T> buf[2 * LU_NB_MAX * CB];
+    for (int cb = 0; cb < nc; cb += CB) {
+        for (int e = tid; e < L * CB; e += 256) {
+            const int sl = e % L, cc = cb + e / L;
+            if (cc < nc && s_src[sl] != s_row[sl])
+                buf[e] = A[(int64_t)s_src[sl] + (c0 + cc) * lda];
+        }
+        __syncthreads();
+        for (int e = tid; e < L * CB; e += 256) {
+            const int sl = e % L, cc = cb + e / L;
+            if (cc < nc && s_src[sl] != s_row[sl])
+                A[(int64_t)s_row[sl] + (c0 + cc) * lda] = buf[e];
+        }
+        __syncthreads();
+    }
+}
+
+template <typename T>
+static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, int64_t k, int w, const int* ipiv,
+                         hipStream_t st)
+{
+    if (ncols <= 0)
+        return;
+    hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
+                       col0, ncols, k, w, ipiv);
 }
 
 template <typename T>
@@ -402,58 +589,89 @@ __global__ __launch_bounds__(256) void laswp_reverse(cx<T>* __restrict__ A, int6
 }
 
 // ----------------------------------------------------------------------------------------------
-// In-place triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block.
-// UPPER=false: unit lower (forward).  UPPER=true: non-unit upper (backward).
+// Triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block: Bout = D^{-1} Bin
+// (Bout may equal Bin).  UPPER=false: lower (forward), UPPER=true: upper (backward); TRANSD uses
+// D^T; UNIT takes the diagonal as one.  One workgroup per 64 columns; thread (col, rg) keeps rows
+// rg + 4 i of its column in registers, and each row step is one broadcast through LDS (double
+// buffered, one barrier per row).
 // ----------------------------------------------------------------------------------------------
 template <typename T, bool UPPER, bool TRANSD = false, bool UNIT = !UPPER>
-__global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict__ D, int64_t ldd, cx<T>* __restrict__ B,
-                                                  int64_t ldb, int64_t ncols)
+__global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict__ D, int64_t ldd, const cx<T>* Bin,
+                                                  int64_t ldi, cx<T>* Bout, int64_t ldo, int64_t ncols)
 {
     __shared__ cx<T> sD[64][65];
-    __shared__ cx<T> sX[64][65];
-    const int tid = threadIdx.x;
+    __shared__ cx<T> xs[2][64];
+    const int tid = threadIdx.x, col = tid & 63, rg = tid >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * 64;
     const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
     for (int e = tid; e < 64 * 64; e += 256) {
         const int r = e % 64, c = e / 64;
-        sD[r][c] = (r < w && c < w) ? (TRANSD ? D[c + (int64_t)r * ldd] : D[r + (int64_t)c * ldd]) : cx<T>{0, 0};
-        sX[r][c] = (r < w && c < nc) ? B[r + (c0 + c) * ldb] : cx<T>{0, 0};
+        sD[r][c] = (r < w && c < w) ? (TRANSD ? D[c + (int64_t)r * ldd] : D[r + (int64_t)c * ldd])
+                                    : cx<T>{(T)(r == c ? 1 : 0), 0};
+    }
+    cx<T> acc[16];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++) {
+        const int r = rg + 4 * i;
+        acc[i] = (r < w && col < nc) ? Bin[r + (c0 + col) * ldi] : cx<T>{0, 0};
     }
     __syncthreads();
-    const int col = tid & 63, rg = tid >> 6;
     if (!UPPER) {
-        for (int r = 0; r < w; r++) {
-            if (!UNIT) {
-                if (rg == 0)
-                    sX[r][col] = cdiv(sX[r][col], sD[r][r]);
-                __syncthreads();
+#pragma clang loop unroll(full)
+        for (int i = 0; i < 16; i++) {
+#pragma clang loop unroll(full)
+            for (int q = 0; q < 4; q++) {
+                const int r = 4 * i + q;
+                if (r < w) {
+                    if (rg == q) {
+                        if (!UNIT)
+                            acc[i] = cdiv(acc[i], sD[r][r]);
+                        xs[r & 1][col] = acc[i];
+                    }
+                    __syncthreads();
+                    const cx<T> x = xs[r & 1][col];
+#pragma clang loop unroll(full)
+                    for (int i2 = i; i2 < 16; i2++) {
+                        if (i2 > i || rg > q) {
+                            const cx<T> l = sD[rg + 4 * i2][r];
+                            acc[i2].re -= l.re * x.re - l.im * x.im;
+                            acc[i2].im -= l.re * x.im + l.im * x.re;
+                        }
+                    }
+                }
             }
-            const cx<T> x = sX[r][col];
-            for (int rr = r + 1 + rg; rr < w; rr += 4) {
-                const cx<T> l = sD[rr][r];
-                sX[rr][col].re -= l.re * x.re - l.im * x.im;
-                sX[rr][col].im -= l.re * x.im + l.im * x.re;
-            }
-            __syncthreads();
         }
     } else {
-        for (int r = w - 1; r >= 0; r--) {
-            if (rg == 0 && !UNIT)
-                sX[r][col] = cdiv(sX[r][col], sD[r][r]);
-            __syncthreads();
-            const cx<T> x = sX[r][col];
-            for (int rr = rg; rr < r; rr += 4) {
-                const cx<T> u = sD[rr][r];
-                sX[rr][col].re -= u.re * x.re - u.im * x.im;
-                sX[rr][col].im -= u.re * x.im + u.im * x.re;
+#pragma clang loop unroll(full)
+        for (int i = 15; i >= 0; i--) {
+#pragma clang loop unroll(full)
+            for (int q = 3; q >= 0; q--) {
+                const int r = 4 * i + q;
+                if (r < w) {
+                    if (rg == q) {
+                        if (!UNIT)
+                            acc[i] = cdiv(acc[i], sD[r][r]);
+                        xs[r & 1][col] = acc[i];
+                    }
+                    __syncthreads();
+                    const cx<T> x = xs[r & 1][col];
+#pragma clang loop unroll(full)
+                    for (int i2 = 0; i2 <= i; i2++) {
+                        if (i2 < i || rg < q) {
+                            const cx<T> u = sD[rg + 4 * i2][r];
+                            acc[i2].re -= u.re * x.re - u.im * x.im;
+                            acc[i2].im -= u.re * x.im + u.im * x.re;
+                        }
+                    }
+                }
             }
-            __syncthreads();
         }
     }
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e % 64, c = e / 64;
-        if (r < w && c < nc)
-            B[r + (c0 + c) * ldb] = sX[r][c];
+#pragma clang loop unroll(full)
+    for (int i = 0; i < 16; i++) {
+        const int r = rg + 4 * i;
+        if (r < w && col < nc)
+            Bout[r + (c0 + col) * ldo] = acc[i];
     }
 }
 
@@ -682,13 +900,13 @@ static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int64_
 {
     if (nc <= 0)
         return 0;
-    hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, st, A, lda, c0, nc, k, w, ipiv);
+    launch_laswp<T>(A, lda, c0, nc, k, w, ipiv, st);
     for (int s = 0; s < w; s += 64) {
         const int ws = (w - s) < 64 ? (w - s) : 64;
         cx<T>* D = A + (k + s) + (k + s) * lda;
         cx<T>* Bb = A + (k + s) + c0 * lda;
         hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nc + 63) / 64)), dim3(256), 0, st, ws, D, lda, Bb, lda,
-                           nc);
+                           Bb, lda, nc);
         if (s + ws < w) {
             int rc = gemm_sub<T>(w - s - ws, nc, ws, A + (k + s + ws) + (k + s) * lda, lda, Bb, lda,
                                  A + (k + s + ws) + c0 * lda, lda, st, nullptr);
@@ -791,22 +1009,65 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         const int64_t nch = (n + 63) / 64;
         hipLaunchKernelGGL((diag_inverse<T>), dim3((unsigned)nch), dim3(256), 0, st, A, n, lda, (cx<T>*)wk.dinv);
         FRICO_HIP_CHECK(hipGetLastError());
+        if (wk.wv && nb % 64 == 0 && nch > 1) {
+            hipLaunchKernelGGL((make_w<T>), dim3((unsigned)nch, (unsigned)nch), dim3(256), 0, st, n, A, lda,
+                               (const cx<T>*)wk.dinv, (cx<T>*)wk.wv, n);
+            FRICO_HIP_CHECK(hipGetLastError());
+        }
     }
+    return 0;
+}
+
+// Fused form of the solve: per 64-row chunk j (k = 64 j, width w)
+//   forward:  y_j = Linv_j b_j  into Y,  b_below -= W(:, j) b_j      (after the panel's interchanges)
+//   backward: x_j = U_jj^{-1} y_j (substitution) into B,  y_above -= U(above, j) x_j
+// forward: one launch per chunk plus one laswp per panel; backward: a substitution and a product.
+template <typename T>
+static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
+                          int64_t nrhs, hipStream_t st, const cx<T>* inv, const cx<T>* WV, cx<T>* Y)
+{
+    const int64_t npan = (n + nb - 1) / nb, nch = (n + 63) / 64;
+    const unsigned gy = (unsigned)((nrhs + GBN - 1) / GBN);
+    for (int64_t p = 0; p < npan; p++) {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        launch_laswp<T>(B, ldb, 0, nrhs, k, w, ipiv, st);
+        for (int s = 0; s < w; s += 64) {
+            const int64_t kc = k + s, ws = (n - kc) < 64 ? (n - kc) : 64, ch = kc / 64;
+            const unsigned gx = (unsigned)((n - kc + GBM - 1) / GBM);
+            hipLaunchKernelGGL((solve_chunk<T>), dim3(gx, gy), dim3(256), 0, st, kc, n - kc, nrhs, ws, (int64_t)0,
+                               inv + (size_t)(2 * ch) * 4096, WV + kc * n, n, B + kc, ldb, B, ldb, Y, n);
+        }
+    }
+    const unsigned gcols = (unsigned)((nrhs + 63) / 64);
+    for (int64_t ch = nch - 1; ch >= 0; ch--) {
+        const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
+        // x_j = U_jj^{-1} y_j into B, then y_above -= U(above, j) x_j
+        hipLaunchKernelGGL((trsm_block<T, true>), dim3(gcols), dim3(256), 0, st, (int)ws, A + kc + kc * lda, lda, Y + kc,
+                           n, B + kc, ldb, nrhs);
+        if (kc > 0) {
+            int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, B + kc, ldb, Y, n, st, nullptr);
+            if (rc)
+                return rc;
+        }
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st, const void* dinv)
+             hipStream_t st, const void* dinv, const void* wv, void* ybuf)
 {
     const int64_t npan = (n + nb - 1) / nb;
     const cx<T>* inv = (const cx<T>*)dinv;
+    if (inv && wv && ybuf && nb % 64 == 0 && n > 64)
+        return lu_solve_fused<T>(A, n, lda, ipiv, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf);
     const unsigned gcols = (unsigned)((nrhs + 63) / 64);
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
-        hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0, nrhs, k,
-                           w, ipiv);
+        launch_laswp<T>(B, ldb, 0, nrhs, k, w, ipiv, st);
         for (int s = 0; s < w; s += 64) {
             const int ws = (w - s) < 64 ? (w - s) : 64;
             // the diagonal sub-blocks of a chunk inverse are the inverses of the chunk's diagonal sub-blocks
@@ -816,7 +1077,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
                                    inv + (size_t)(2 * ch) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
             else
                 hipLaunchKernelGGL((trsm_block<T, false>), dim3(gcols), dim3(256), 0, st, ws, A + (k + s) + (k + s) * lda, lda,
-                                   B + (k + s), ldb, nrhs);
+                                   B + (k + s), ldb, B + (k + s), ldb, nrhs);
             const int64_t below = n - (k + s + ws);
             if (below > 0) {
                 int rc = gemm_sub<T>(below, nrhs, ws, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s), ldb,
@@ -837,7 +1098,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
                                    inv + (size_t)(2 * ch + 1) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
             else
                 hipLaunchKernelGGL((trsm_block<T, true>), dim3(gcols), dim3(256), 0, st, ws, A + (k + s) + (k + s) * lda, lda,
-                                   B + (k + s), ldb, nrhs);
+                                   B + (k + s), ldb, B + (k + s), ldb, nrhs);
             const int64_t above = k + s;
             if (above > 0) {
                 int rc = gemm_sub<T>(above, nrhs, ws, A + (k + s) * lda, lda, B + (k + s), ldb, B, ldb, st, nullptr);
@@ -873,7 +1134,7 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
                                    nrhs, above, A + (k + s) * lda, lda, B, ldb, B + (k + s), ldb);
             }
             hipLaunchKernelGGL((trsm_block<T, false, true, false>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
-                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs);
         }
     }
     for (int64_t p = npan - 1; p >= 0; p--) {
@@ -889,7 +1150,7 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
                                    ldb);
             }
             hipLaunchKernelGGL((trsm_block<T, true, true, true>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
-                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, nrhs);
+                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs);
         }
         hipLaunchKernelGGL((laswp_reverse<T>), dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0,
                            nrhs, k, w, ipiv);
@@ -931,9 +1192,9 @@ template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, con
 template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
 template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t,
-                              const void*);
+                              const void*, const void*, void*);
 template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t,
-                             const void*);
+                             const void*, const void*, void*);
 template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t,
                                          hipStream_t);
 template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,

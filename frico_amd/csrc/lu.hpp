@@ -21,6 +21,9 @@ struct LuWork {
     void* diag_row = nullptr;
     int max_groups = 0; // co-resident workgroup limit of the panel kernel
     void* dinv = nullptr; // [ceil(n/64)][2][64*64] diagonal-chunk inverses, written at the end of lu_factor
+    // n x n (leading dimension n) products W = L(t, j) Linv_j below and V = U(t, j) Uinv_j above each
+    // diagonal chunk, written at the end of lu_factor when non-null (needs dinv and nb % 64 == 0)
+    void* wv = nullptr;
     hipStream_t side = nullptr;    // second stream (default priority) for the look-ahead, or null
     hipEvent_t* events = nullptr;  // >= 2 * npanels + 1 events (timing disabled)
     int nevents = 0;
@@ -36,10 +39,11 @@ template <typename T> int panel_rows(int w);
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
 // dinv: the 64 x 64 diagonal-chunk inverses written by lu_factor (LuWork::dinv), or null to use
-// in-LDS triangular solves.
+// in-LDS triangular solves.  wv: the W/V products of lu_factor (LuWork::wv), or null; with wv the
+// solve runs one product per 64-row chunk and needs ybuf, an n x nrhs scratch (leading dimension n).
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st, const void* dinv);
+             hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr);
 template <typename T>
 int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
                         int64_t nrhs, hipStream_t st);

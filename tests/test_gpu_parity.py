@@ -198,8 +198,9 @@ def test_antenna_parity_dipole(gpu_ctx, mode):
     print(mode, "Zin gpu", out["impedance"], "oracle", zo)
     assert abs(out["impedance"] - zo) / abs(zo) <= 1e-10
     assert relmax(out["gain"], ref["gain"]) <= 1e-10
-    # refinement form: report the error against the oracle, hold it to 1e-9
-    out32 = gpu_ctx.antenna_sweep([F0], sm.feed, mode=mode, form="c64_ir", tol=1e-13, maxit=10)[0]
+    # refinement form: report the error against the oracle, hold it to 1e-9 (the refinement target
+    # 1e-12 sits just above this matrix's FP64 residual floor of ~1e-13)
+    out32 = gpu_ctx.antenna_sweep([F0], sm.feed, mode=mode, form="c64_ir", tol=1e-12, maxit=10)[0]
     e32 = abs(out32["impedance"] - zo) / abs(zo)
     print(mode, "c64_ir Zin error vs oracle", e32, "residual", out32["residual"])
     assert e32 <= 1e-9
