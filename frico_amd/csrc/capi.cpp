@@ -80,6 +80,8 @@ int ensure_lu_work(frico_ctx* c)
         return rc;
     if ((rc = c->dinv.ensure((size_t)((c->N + 63) / 64) * 2 * 4096 * sizeof(c128))))
         return rc;
+    if ((rc = c->perm.ensure((size_t)npan * lu_perm_stride())))
+        return rc;
     const size_t nev = (size_t)(2 * npan + 1);
     while (c->lu_events.size() < nev) {
         hipEvent_t e;
@@ -100,6 +102,8 @@ LuWork lu_work(frico_ctx* c)
     w.cand_row = c->cand_row.p;
     w.diag_row = c->diag_row.p;
     w.dinv = c->dinv.p;
+    w.perm = c->perm.p;
+    w.nperm = (int64_t)(c->perm.n / lu_perm_stride());
     if (c->lookahead) {
         w.side = c->side;
         w.events = c->lu_events.data();
@@ -928,7 +932,7 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
         FRICO_HIP_CHECK(hipMemcpyAsync(c->X.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
         if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c128))))
             return rc;
-        rc = lu_solve<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->X.p, N, nr, c->st, c->dinv.p,
+        rc = lu_solve<double>(c->Z64.p, N, N, c->perm.p, c->nb_factored, c->X.p, N, nr, c->st, c->dinv.p,
                               c->wv_ok ? c->wv.p : nullptr, c->ysolve.p);
         phase_end(c, c->t_solve);
         if (rc)
@@ -957,7 +961,7 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
         phase_begin(c, tt);
         if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
             return rc;
-        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
+        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
                                   c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
             return rc;
         if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))

@@ -203,6 +203,7 @@ struct frico_ctx {
     DevBuf<double> cand_val;
     DevBuf<int> cand_idx;
     DevBuf<unsigned char> cand_row, diag_row, dinv;
+    DevBuf<int> perm;             // row-interchange lists of the last factorisation
     DevBuf<unsigned char> wv;     // W/V products of the last factorisation (fused solves)
     DevBuf<unsigned char> ysolve; // n x nrhs scratch of the fused solves
     bool wv_ok = false;           // wv holds the products of the current factors

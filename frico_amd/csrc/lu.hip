@@ -242,6 +242,145 @@ __global__ __launch_bounds__(256) void solve_chunk(int64_t r0, int64_t M, int64_
 }
 
 // ----------------------------------------------------------------------------------------------
+// Row interchanges.  The swaps of one panel (rows k .. k+w-1 exchanged in turn with ipiv[k + c])
+// are reduced once, when the panel is factored, to the permutation they make of the rows they
+// touch: a list of (row, source row) pairs, meaning "row receives the content source held before
+// the panel".  Applying a panel's interchanges to any set of columns is then one gather.
+//
+// Replay (one wavefront): slot s < w is panel row k + s, slots w, w+1, ... are the rows below the
+// panel in order of first appearance (held one per lane, found by ballot); slot s keeps the row
+// whose content it currently holds in register sr<s / 64> of lane s % 64.  Slot indices are
+// wave-uniform, so every access is a readlane and the chain of swaps never waits on memory.
+// ----------------------------------------------------------------------------------------------
+constexpr int PERM_STRIDE = 1 + 2 * 2 * LU_NB_MAX; // ints per panel: count, then (row, src) pairs
+
+// Slot access for the replay: slot sl lives in sr<sl / 64> of lane sl % 64 (sl wave-uniform).
+#define PERM_GET(sl)                                                                                        \
+    ((sl) < 64 ? __builtin_amdgcn_readlane(sr0, (sl) & 63)                                                  \
+               : (sl) < 128 ? __builtin_amdgcn_readlane(sr1, (sl) & 63)                                     \
+                            : (sl) < 192 ? __builtin_amdgcn_readlane(sr2, (sl) & 63)                        \
+                                         : __builtin_amdgcn_readlane(sr3, (sl) & 63))
+#define PERM_PUT(sl, v)                                                                                     \
+    do {                                                                                                    \
+        if (lane == ((sl) & 63)) {                                                                          \
+            if ((sl) < 64)                                                                                  \
+                sr0 = (v);                                                                                  \
+            else if ((sl) < 128)                                                                            \
+                sr1 = (v);                                                                                  \
+            else if ((sl) < 192)                                                                            \
+                sr2 = (v);                                                                                  \
+            else                                                                                            \
+                sr3 = (v);                                                                                  \
+        }                                                                                                   \
+    } while (0)
+
+// Called by one full wavefront.  piv[c] (LDS or global) = absolute pivot row of panel column c.
+// s_row must hold >= 2 w ints of scratch.  Writes the list of moved rows to out (PERM_STRIDE ints).
+__device__ inline void replay_panel_swaps(int k, int w, const int* piv, int* s_row, int* __restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int pv0 = lane < w ? piv[lane] : 0; // pivots, two per lane (w <= 128)
+    const int pv1 = lane + 64 < w ? piv[64 + lane] : 0;
+    int sr0 = k + lane, sr1 = k + 64 + lane, sr2 = -1, sr3 = -1;
+    s_row[lane] = k + lane;
+    s_row[64 + lane] = k + 64 + lane;
+    int e0 = -1, e1 = -1; // rows of the extra slots w + lane and w + 64 + lane
+    int cnt = 0;
+    const int kw = k + w;
+    for (int c = 0; c < w; c++) {
+        const int p = c < 64 ? __builtin_amdgcn_readlane(pv0, c) : __builtin_amdgcn_readlane(pv1, c - 64);
+        if (p == k + c)
+            continue;
+        int sp;
+        if (p < kw) {
+            sp = p - k;
+        } else {
+            const unsigned long long m0 = __ballot(e0 == p), m1 = __ballot(e1 == p);
+            if (m0) {
+                sp = w + __ffsll((long long)m0) - 1;
+            } else if (m1) {
+                sp = w + 64 + __ffsll((long long)m1) - 1;
+            } else {
+                sp = w + cnt;
+                if (cnt < 64) {
+                    if (lane == cnt)
+                        e0 = p;
+                } else if (lane == cnt - 64) {
+                    e1 = p;
+                }
+                PERM_PUT(sp, p);
+                cnt++;
+            }
+        }
+        const int a = PERM_GET(c), b = PERM_GET(sp);
+        PERM_PUT(c, b);
+        PERM_PUT(sp, a);
+    }
+    // rows of the extra slots
+    if (lane < cnt)
+        s_row[w + lane] = e0;
+    if (lane + 64 < cnt)
+        s_row[w + 64 + lane] = e1;
+    // compact the moved slots (source differs from the slot's row) into the list
+    const int L = w + cnt;
+    int base = 0;
+    for (int j = 0; j < 4; j++) {
+        const int sl = 64 * j + lane;
+        const int src = j == 0 ? sr0 : j == 1 ? sr1 : j == 2 ? sr2 : sr3;
+        const int row = sl < L ? s_row[sl] : -1;
+        const bool moved = sl < L && src != row;
+        const unsigned long long m = __ballot(moved);
+        if (moved) {
+            const int pos = base + __popcll(m & ((1ull << lane) - 1ull));
+            out[1 + 2 * pos] = row;
+            out[2 + 2 * pos] = src;
+        }
+        base += __popcll(m);
+    }
+    if (lane == 0)
+        out[0] = base;
+}
+
+// Applies one panel's list (perm, PERM_STRIDE ints) to columns [col0, col0 + ncols).
+// One workgroup per LASWP_COLS columns; all reads of a column precede its writes (through LDS).
+constexpr int LASWP_COLS = 8;
+
+template <typename T>
+__global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols,
+                                             const int* __restrict__ perm)
+{
+    __shared__ int2 s_mv[2 * LU_NB_MAX];
+    __shared__ cx<T> buf[2 * LU_NB_MAX * LASWP_COLS];
+    const int tid = threadIdx.x;
+    const int L = perm[0];
+    for (int e = tid; e < L; e += 256)
+        s_mv[e] = int2{perm[1 + 2 * e], perm[2 + 2 * e]};
+    __syncthreads();
+    const int64_t c0 = col0 + (int64_t)blockIdx.x * LASWP_COLS;
+    const int nc = (int)((col0 + ncols - c0) < LASWP_COLS ? (col0 + ncols - c0) : LASWP_COLS);
+    for (int e = tid; e < L * LASWP_COLS; e += 256) {
+        const int m = e % L, cc = e / L;
+        if (cc < nc)
+            buf[e] = A[(int64_t)s_mv[m].y + (c0 + cc) * lda];
+    }
+    __syncthreads();
+    for (int e = tid; e < L * LASWP_COLS; e += 256) {
+        const int m = e % L, cc = e / L;
+        if (cc < nc)
+            A[(int64_t)s_mv[m].x + (c0 + cc) * lda] = buf[e];
+    }
+}
+
+template <typename T>
+static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, const int* perm, hipStream_t st)
+{
+    if (ncols <= 0)
+        return;
+    hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
+                       col0, ncols, perm);
+}
+
+// ----------------------------------------------------------------------------------------------
 // Panel factorisation.  G workgroups, workgroup g owns panel rows [g*R, g*R + R) in LDS, one row
 // per thread.  Per column every workgroup publishes its local pivot candidate (value, row index,
 // row contents) and raises its flag; after all G flags are up every workgroup picks the same
@@ -293,7 +432,7 @@ __device__ inline void argmax_merge(double& v, int& i, double ov, int oi)
 template <typename T>
 __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
                                                     int R, int G, int* __restrict__ ipiv, PanelSync ps,
-                                                    unsigned* __restrict__ flags)
+                                                    unsigned* __restrict__ flags, int* __restrict__ perm)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw); // [w][R] column-major
@@ -302,6 +441,8 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
     __shared__ int ri[4];
     __shared__ int s_piv, s_gw, s_abort;
     __shared__ double s_pval;
+    __shared__ int s_pivots[LU_NB_MAX];   // absolute pivot rows of the panel (workgroup 0)
+    __shared__ int s_scratch[2 * LU_NB_MAX];
 
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r0 = g * R;                                      // first panel row of this workgroup
@@ -419,6 +560,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
         }
         if (g == 0 && tid == 0) {
             ipiv[k + c] = (int)(k + (pval > 0.0 ? p : c));
+            s_pivots[c] = (int)(k + (pval > 0.0 ? p : c));
             if (!(pval > 0.0) && *ps.info == 0)
                 *ps.info = (int)(k + c + 1);
         }
@@ -436,138 +578,11 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
         }
     }
     __syncthreads();
+    if (g == 0 && wv == 0)
+        replay_panel_swaps((int)k, w, s_pivots, s_scratch, perm);
     for (int c = 0; c < w; c++)
         if (tid < nr)
             Ap[(r0 + tid) + (int64_t)c * lda] = slab[(size_t)c * R + tid];
-}
-
-// ----------------------------------------------------------------------------------------------
-// Row interchanges of one panel (rows k .. k+w-1 swapped in turn with ipiv[k + c]) applied to
-// columns [col0, col0 + ncols).  The sequence of swaps is first reduced to the permutation it
-// makes of the rows it touches: wave 0 replays the swaps on row labels in LDS, with the rows below
-// the panel held one per lane and found by ballot.  Each touched row then receives its final
-// content in one gather, all reads before all writes within a column.  One workgroup per 64
-// columns.
-// ----------------------------------------------------------------------------------------------
-// Slot access for laswp's replay: slot sl lives in sr<sl / 64> of lane sl % 64 (sl wave-uniform).
-#define LASWP_GET(sl)                                                                                       \
-    ((sl) < 64 ? __builtin_amdgcn_readlane(sr0, (sl) & 63)                                                  \
-               : (sl) < 128 ? __builtin_amdgcn_readlane(sr1, (sl) & 63)                                     \
-                            : (sl) < 192 ? __builtin_amdgcn_readlane(sr2, (sl) & 63)                        \
-                                         : __builtin_amdgcn_readlane(sr3, (sl) & 63))
-#define LASWP_PUT(sl, v)                                                                                    \
-    do {                                                                                                    \
-        if (lane == ((sl) & 63)) {                                                                          \
-            if ((sl) < 64)                                                                                  \
-                sr0 = (v);                                                                                  \
-            else if ((sl) < 128)                                                                            \
-                sr1 = (v);                                                                                  \
-            else if ((sl) < 192)                                                                            \
-                sr2 = (v);                                                                                  \
-            else                                                                                            \
-                sr3 = (v);                                                                                  \
-        }                                                                                                   \
-    } while (0)
-
-constexpr int LASWP_COLS = 64;
-
-template <typename T>
-__global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols, int64_t k,
-                                             int w, const int* __restrict__ ipiv)
-{
-    __shared__ int s_row[2 * LU_NB_MAX]; // row of slot s
-    __shared__ int s_src[2 * LU_NB_MAX]; // row whose content ends in slot s
-    __shared__ int s_cnt;
-    const int tid = threadIdx.x;
-    for (int s = tid; s < w; s += 256) {
-        s_row[s] = (int)k + s;
-        s_src[s] = (int)k + s;
-    }
-    __syncthreads();
-    if (tid < 64) {
-        // Slot s (< 256) has its content source in register sr[s / 64] of lane s % 64; slots are
-        // addressed with wave-uniform indices, so reads are readlanes and the swap chain never
-        // waits on memory.
-        const int lane = tid;
-        const int pv0 = lane < w ? ipiv[k + lane] : 0; // pivots of the panel, two per lane (w <= 128)
-        const int pv1 = lane + 64 < w ? ipiv[k + 64 + lane] : 0;
-        int sr0 = (int)k + lane, sr1 = (int)k + 64 + lane, sr2 = -1, sr3 = -1;
-        int e0 = -1, e1 = -1; // rows of the extra slots w + lane and w + 64 + lane
-        int cnt = 0;
-        const int kw = (int)k + w;
-        for (int c = 0; c < w; c++) {
-            const int p = c < 64 ? __builtin_amdgcn_readlane(pv0, c) : __builtin_amdgcn_readlane(pv1, c - 64);
-            if (p == (int)k + c)
-                continue;
-            int sp;
-            if (p < kw) {
-                sp = p - (int)k;
-            } else {
-                const unsigned long long m0 = __ballot(e0 == p), m1 = __ballot(e1 == p);
-                if (m0) {
-                    sp = w + __ffsll((long long)m0) - 1;
-                } else if (m1) {
-                    sp = w + 64 + __ffsll((long long)m1) - 1;
-                } else {
-                    sp = w + cnt;
-                    if (cnt < 64) {
-                        if (lane == cnt)
-                            e0 = p;
-                    } else if (lane == cnt - 64) {
-                        e1 = p;
-                    }
-                    LASWP_PUT(sp, p);
-                    if (lane == 0)
-                        s_row[sp] = p;
-                    cnt++;
-                }
-            }
-            const int a = LASWP_GET(c), b = LASWP_GET(sp);
-            LASWP_PUT(c, b);
-            LASWP_PUT(sp, a);
-        }
-        const int L = w + cnt;
-        if (lane < L)
-            s_src[lane] = sr0;
-        if (64 + lane < L)
-            s_src[64 + lane] = sr1;
-        if (128 + lane < L)
-            s_src[128 + lane] = sr2;
-        if (192 + lane < L)
-            s_src[192 + lane] = sr3;
-        if (lane == 0)
-            s_cnt = cnt;
-    }
-    __syncthreads();
-    const int L = w + s_cnt;
-    const int64_t c0 = col0 + (int64_t)blockIdx.x * LASWP_COLS;
-    const int nc = (int)((col0 + ncols - c0) < LASWP_COLS ? (col0 + ncols - c0) : LASWP_COLS);
-    constexpr int CB = 8; // columns per pass, staged through LDS
-    __shared__ cx<T> buf[2 * LU_NB_MAX * CB];
-    for (int cb = 0; cb < nc; cb += CB) {
-        for (int e = tid; e < L * CB; e += 256) {
-            const int sl = e % L, cc = cb + e / L;
-            if (cc < nc && s_src[sl] != s_row[sl])
-                buf[e] = A[(int64_t)s_src[sl] + (c0 + cc) * lda];
-        }
-        __syncthreads();
-        for (int e = tid; e < L * CB; e += 256) {
-            const int sl = e % L, cc = cb + e / L;
-            if (cc < nc && s_src[sl] != s_row[sl])
-                A[(int64_t)s_row[sl] + (c0 + cc) * lda] = buf[e];
-        }
-        __syncthreads();
-    }
-}
-
-template <typename T>
-static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, int64_t k, int w, const int* ipiv,
-                         hipStream_t st)
-{
-    if (ncols <= 0)
-        return;
-    hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
-                       col0, ncols, k, w, ipiv);
 }
 
 template <typename T>
@@ -591,88 +606,98 @@ __global__ __launch_bounds__(256) void laswp_reverse(cx<T>* __restrict__ A, int6
 // ----------------------------------------------------------------------------------------------
 // Triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block: Bout = D^{-1} Bin
 // (Bout may equal Bin).  UPPER=false: lower (forward), UPPER=true: upper (backward); TRANSD uses
-// D^T; UNIT takes the diagonal as one.  One workgroup per 64 columns; thread (col, rg) keeps rows
-// rg + 4 i of its column in registers, and each row step is one broadcast through LDS (double
-// buffered, one barrier per row).
+// D^T; UNIT takes the diagonal as one.  One workgroup per TRSM_COLS columns: thread (rg, col),
+// rg = tid % 16, keeps rows rg + 16 i of its column in registers, so loads and stores walk down
+// the columns; each row step is one broadcast through LDS (double buffered, one barrier per row).
 // ----------------------------------------------------------------------------------------------
+constexpr int TRSM_COLS = 16;
+
 template <typename T, bool UPPER, bool TRANSD = false, bool UNIT = !UPPER>
 __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict__ D, int64_t ldd, const cx<T>* Bin,
                                                   int64_t ldi, cx<T>* Bout, int64_t ldo, int64_t ncols)
 {
     __shared__ cx<T> sD[64][65];
-    __shared__ cx<T> xs[2][64];
-    const int tid = threadIdx.x, col = tid & 63, rg = tid >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * 64;
-    const int nc = (int)((ncols - c0) < 64 ? (ncols - c0) : 64);
+    __shared__ cx<T> xs[2][TRSM_COLS];
+    const int tid = threadIdx.x, rg = tid & 15, col = tid >> 4;
+    const int64_t c0 = (int64_t)blockIdx.x * TRSM_COLS;
+    const bool live = c0 + col < ncols;
     for (int e = tid; e < 64 * 64; e += 256) {
         const int r = e % 64, c = e / 64;
         sD[r][c] = (r < w && c < w) ? (TRANSD ? D[c + (int64_t)r * ldd] : D[r + (int64_t)c * ldd])
                                     : cx<T>{(T)(r == c ? 1 : 0), 0};
     }
-    cx<T> acc[16];
+    cx<T> acc[4];
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 16; i++) {
-        const int r = rg + 4 * i;
-        acc[i] = (r < w && col < nc) ? Bin[r + (c0 + col) * ldi] : cx<T>{0, 0};
+    for (int i = 0; i < 4; i++) {
+        const int r = rg + 16 * i;
+        acc[i] = (r < w && live) ? Bin[r + (c0 + col) * ldi] : cx<T>{0, 0};
     }
     __syncthreads();
     if (!UPPER) {
 #pragma clang loop unroll(full)
-        for (int i = 0; i < 16; i++) {
+        for (int i = 0; i < 4; i++) {
+            for (int q = 0; q < 16; q++) {
+                const int r = 16 * i + q;
+                if (r >= w)
+                    break;
+                if (rg == q) {
+                    if (!UNIT)
+                        acc[i] = cdiv(acc[i], sD[r][r]);
+                    xs[r & 1][col] = acc[i];
+                }
+                __syncthreads();
+                const cx<T> x = xs[r & 1][col];
 #pragma clang loop unroll(full)
-            for (int q = 0; q < 4; q++) {
-                const int r = 4 * i + q;
-                if (r < w) {
-                    if (rg == q) {
-                        if (!UNIT)
-                            acc[i] = cdiv(acc[i], sD[r][r]);
-                        xs[r & 1][col] = acc[i];
-                    }
-                    __syncthreads();
-                    const cx<T> x = xs[r & 1][col];
-#pragma clang loop unroll(full)
-                    for (int i2 = i; i2 < 16; i2++) {
-                        if (i2 > i || rg > q) {
-                            const cx<T> l = sD[rg + 4 * i2][r];
-                            acc[i2].re -= l.re * x.re - l.im * x.im;
-                            acc[i2].im -= l.re * x.im + l.im * x.re;
-                        }
+                for (int i2 = i; i2 < 4; i2++) {
+                    if (i2 > i || rg > q) {
+                        const cx<T> l = sD[rg + 16 * i2][r];
+                        acc[i2].re -= l.re * x.re - l.im * x.im;
+                        acc[i2].im -= l.re * x.im + l.im * x.re;
                     }
                 }
             }
         }
     } else {
 #pragma clang loop unroll(full)
-        for (int i = 15; i >= 0; i--) {
+        for (int i = 3; i >= 0; i--) {
+            for (int q = 15; q >= 0; q--) {
+                const int r = 16 * i + q;
+                if (r >= w)
+                    continue;
+                if (rg == q) {
+                    if (!UNIT)
+                        acc[i] = cdiv(acc[i], sD[r][r]);
+                    xs[r & 1][col] = acc[i];
+                }
+                __syncthreads();
+                const cx<T> x = xs[r & 1][col];
 #pragma clang loop unroll(full)
-            for (int q = 3; q >= 0; q--) {
-                const int r = 4 * i + q;
-                if (r < w) {
-                    if (rg == q) {
-                        if (!UNIT)
-                            acc[i] = cdiv(acc[i], sD[r][r]);
-                        xs[r & 1][col] = acc[i];
-                    }
-                    __syncthreads();
-                    const cx<T> x = xs[r & 1][col];
-#pragma clang loop unroll(full)
-                    for (int i2 = 0; i2 <= i; i2++) {
-                        if (i2 < i || rg < q) {
-                            const cx<T> u = sD[rg + 4 * i2][r];
-                            acc[i2].re -= u.re * x.re - u.im * x.im;
-                            acc[i2].im -= u.re * x.im + u.im * x.re;
-                        }
+                for (int i2 = 0; i2 <= i; i2++) {
+                    if (i2 < i || rg < q) {
+                        const cx<T> u = sD[rg + 16 * i2][r];
+                        acc[i2].re -= u.re * x.re - u.im * x.im;
+                        acc[i2].im -= u.re * x.im + u.im * x.re;
                     }
                 }
             }
         }
     }
 #pragma clang loop unroll(full)
-    for (int i = 0; i < 16; i++) {
-        const int r = rg + 4 * i;
-        if (r < w && col < nc)
+    for (int i = 0; i < 4; i++) {
+        const int r = rg + 16 * i;
+        if (r < w && live)
             Bout[r + (c0 + col) * ldo] = acc[i];
     }
+}
+
+template <typename T, bool UPPER, bool TRANSD = false, bool UNIT = !UPPER>
+static void launch_trsm(int w, const cx<T>* D, int64_t ldd, const cx<T>* Bin, int64_t ldi, cx<T>* Bout, int64_t ldo,
+                        int64_t ncols, hipStream_t st)
+{
+    if (ncols <= 0)
+        return;
+    hipLaunchKernelGGL((trsm_block<T, UPPER, TRANSD, UNIT>), dim3((unsigned)((ncols + TRSM_COLS - 1) / TRSM_COLS)),
+                       dim3(256), 0, st, w, D, ldd, Bin, ldi, Bout, ldo, ncols);
 }
 
 // Inverses of the 64 x 64 diagonal chunks of the factors, for the solves: chunk ch covers rows and
@@ -895,18 +920,17 @@ template <typename T> int panel_rows(int w)
 // Update of columns [c0, c0 + nc) by step k (panel of width w): row interchanges, U12 = L11^{-1} A12
 // and A22 -= L21 U12.
 template <typename T>
-static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int64_t k, int w, int64_t c0, int64_t nc,
+static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, int64_t k, int w, int64_t c0, int64_t nc,
                        hipStream_t st, KernelTimer* kt)
 {
     if (nc <= 0)
         return 0;
-    launch_laswp<T>(A, lda, c0, nc, k, w, ipiv, st);
+    launch_laswp<T>(A, lda, c0, nc, perm, st);
     for (int s = 0; s < w; s += 64) {
         const int ws = (w - s) < 64 ? (w - s) : 64;
         cx<T>* D = A + (k + s) + (k + s) * lda;
         cx<T>* Bb = A + (k + s) + c0 * lda;
-        hipLaunchKernelGGL((trsm_block<T, false>), dim3((unsigned)((nc + 63) / 64)), dim3(256), 0, st, ws, D, lda, Bb, lda,
-                           Bb, lda, nc);
+        launch_trsm<T, false>(ws, D, lda, Bb, lda, Bb, lda, nc, st);
         if (s + ws < w) {
             int rc = gemm_sub<T>(w - s - ws, nc, ws, A + (k + s + ws) + (k + s) * lda, lda, Bb, lda,
                                  A + (k + s + ws) + c0 * lda, lda, st, nullptr);
@@ -937,6 +961,8 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     const int gmax = wk.max_groups;
     if ((int64_t)wk.ncounters < npan * 2 * gmax)
         return set_error("lu: sync workspace too small"), -1;
+    if (!wk.perm || wk.nperm < npan)
+        return set_error("lu: interchange-list workspace too small"), -1;
     const int64_t nfl = npan * 2 * gmax;
     hipLaunchKernelGGL(zero_u32, dim3((unsigned)((nfl + 255) / 256)), dim3(256), 0, st, wk.counters, nfl);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
@@ -958,7 +984,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
             return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), FRICO_ERR_LIMIT;
         const size_t sh = ((size_t)w * R + w) * sizeof(cx<T>);
         hipLaunchKernelGGL((panel_factor<T>), dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
-                           wk.counters + p * 2 * gmax);
+                           wk.counters + p * 2 * gmax, wk.perm + p * PERM_STRIDE);
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
     };
@@ -968,7 +994,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         for (int64_t p = 0; p < npan; p++) {
             const int64_t k = p * nb;
             const int w = (int)((n - k) < nb ? (n - k) : nb);
-            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, ipiv, k, w, k + w, n - k - w, st, gemm_timer)))
+            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, wk.perm + p * PERM_STRIDE, k, w, k + w, n - k - w, st, gemm_timer)))
                 return rc;
         }
     } else {
@@ -991,13 +1017,13 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
             FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
             if (nrest > 0) {
                 FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
-                if ((rc = step_update<T>(A, n, lda, ipiv, k, w, k + w + nnext, nrest, wk.side, gemm_timer)))
+                if ((rc = step_update<T>(A, n, lda, wk.perm + p * PERM_STRIDE, k, w, k + w + nnext, nrest, wk.side, gemm_timer)))
                     return rc;
                 FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
             }
             if (prev_wide)
                 FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
-            if ((rc = step_update<T>(A, n, lda, ipiv, k, w, k + w, nnext, st, gemm_timer)) || (rc = panel(p + 1)))
+            if ((rc = step_update<T>(A, n, lda, wk.perm + p * PERM_STRIDE, k, w, k + w, nnext, st, gemm_timer)) || (rc = panel(p + 1)))
                 return rc;
             prev_wide = nrest > 0;
         }
@@ -1023,7 +1049,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
 //   backward: x_j = U_jj^{-1} y_j (substitution) into B,  y_above -= U(above, j) x_j
 // forward: one launch per chunk plus one laswp per panel; backward: a substitution and a product.
 template <typename T>
-static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
+static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb,
                           int64_t nrhs, hipStream_t st, const cx<T>* inv, const cx<T>* WV, cx<T>* Y)
 {
     const int64_t npan = (n + nb - 1) / nb, nch = (n + 63) / 64;
@@ -1031,7 +1057,7 @@ static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* ipi
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
-        launch_laswp<T>(B, ldb, 0, nrhs, k, w, ipiv, st);
+        launch_laswp<T>(B, ldb, 0, nrhs, perm + p * PERM_STRIDE, st);
         for (int s = 0; s < w; s += 64) {
             const int64_t kc = k + s, ws = (n - kc) < 64 ? (n - kc) : 64, ch = kc / 64;
             const unsigned gx = (unsigned)((n - kc + GBM - 1) / GBM);
@@ -1043,8 +1069,7 @@ static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* ipi
     for (int64_t ch = nch - 1; ch >= 0; ch--) {
         const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
         // x_j = U_jj^{-1} y_j into B, then y_above -= U(above, j) x_j
-        hipLaunchKernelGGL((trsm_block<T, true>), dim3(gcols), dim3(256), 0, st, (int)ws, A + kc + kc * lda, lda, Y + kc,
-                           n, B + kc, ldb, nrhs);
+        launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, Y + kc, n, B + kc, ldb, nrhs, st);
         if (kc > 0) {
             int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, B + kc, ldb, Y, n, st, nullptr);
             if (rc)
@@ -1056,18 +1081,18 @@ static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* ipi
 }
 
 template <typename T>
-int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
+int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
              hipStream_t st, const void* dinv, const void* wv, void* ybuf)
 {
     const int64_t npan = (n + nb - 1) / nb;
     const cx<T>* inv = (const cx<T>*)dinv;
     if (inv && wv && ybuf && nb % 64 == 0 && n > 64)
-        return lu_solve_fused<T>(A, n, lda, ipiv, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf);
+        return lu_solve_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf);
     const unsigned gcols = (unsigned)((nrhs + 63) / 64);
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
-        launch_laswp<T>(B, ldb, 0, nrhs, k, w, ipiv, st);
+        launch_laswp<T>(B, ldb, 0, nrhs, perm + p * PERM_STRIDE, st);
         for (int s = 0; s < w; s += 64) {
             const int ws = (w - s) < 64 ? (w - s) : 64;
             // the diagonal sub-blocks of a chunk inverse are the inverses of the chunk's diagonal sub-blocks
@@ -1076,8 +1101,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
                 hipLaunchKernelGGL((trmm_block<T>), dim3(gcols), dim3(256), 0, st, ws,
                                    inv + (size_t)(2 * ch) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
             else
-                hipLaunchKernelGGL((trsm_block<T, false>), dim3(gcols), dim3(256), 0, st, ws, A + (k + s) + (k + s) * lda, lda,
-                                   B + (k + s), ldb, B + (k + s), ldb, nrhs);
+                launch_trsm<T, false>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
             const int64_t below = n - (k + s + ws);
             if (below > 0) {
                 int rc = gemm_sub<T>(below, nrhs, ws, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s), ldb,
@@ -1097,8 +1121,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx
                 hipLaunchKernelGGL((trmm_block<T>), dim3(gcols), dim3(256), 0, st, ws,
                                    inv + (size_t)(2 * ch + 1) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
             else
-                hipLaunchKernelGGL((trsm_block<T, true>), dim3(gcols), dim3(256), 0, st, ws, A + (k + s) + (k + s) * lda, lda,
-                                   B + (k + s), ldb, B + (k + s), ldb, nrhs);
+                launch_trsm<T, true>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
             const int64_t above = k + s;
             if (above > 0) {
                 int rc = gemm_sub<T>(above, nrhs, ws, A + (k + s) * lda, lda, B + (k + s), ldb, B, ldb, st, nullptr);
@@ -1133,8 +1156,7 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
                 hipLaunchKernelGGL((cgemm_sub<T, T, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
                                    nrhs, above, A + (k + s) * lda, lda, B, ldb, B + (k + s), ldb);
             }
-            hipLaunchKernelGGL((trsm_block<T, false, true, false>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
-                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs);
+            launch_trsm<T, false, true, false>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
         }
     }
     for (int64_t p = npan - 1; p >= 0; p--) {
@@ -1149,8 +1171,7 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
                                    nrhs, below, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s + ws), ldb, B + (k + s),
                                    ldb);
             }
-            hipLaunchKernelGGL((trsm_block<T, true, true, true>), dim3((unsigned)((nrhs + 63) / 64)), dim3(256), 0, st, ws,
-                               A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs);
+            launch_trsm<T, true, true, true>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
         }
         hipLaunchKernelGGL((laswp_reverse<T>), dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0,
                            nrhs, k, w, ipiv);
@@ -1199,6 +1220,7 @@ template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const in
                                          hipStream_t);
 template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,
                                         hipStream_t);
+int lu_perm_stride() { return PERM_STRIDE; }
 template int panel_rows<double>(int);
 template int panel_rows<float>(int);
 

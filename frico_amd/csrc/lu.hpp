@@ -24,6 +24,9 @@ struct LuWork {
     // n x n (leading dimension n) products W = L(t, j) Linv_j below and V = U(t, j) Uinv_j above each
     // diagonal chunk, written at the end of lu_factor when non-null (needs dinv and nb % 64 == 0)
     void* wv = nullptr;
+    // per panel: the row-interchange list written by the panel kernel (lu_perm_stride() ints each)
+    int* perm = nullptr;
+    int64_t nperm = 0; // panels it can hold
     hipStream_t side = nullptr;    // second stream (default priority) for the look-ahead, or null
     hipEvent_t* events = nullptr;  // >= 2 * npanels + 1 events (timing disabled)
     int nevents = 0;
@@ -38,11 +41,14 @@ int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, c
 template <typename T> int panel_rows(int w);
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
+// ints per panel of LuWork::perm
+int lu_perm_stride();
+// perm: the interchange lists written by lu_factor (LuWork::perm).
 // dinv: the 64 x 64 diagonal-chunk inverses written by lu_factor (LuWork::dinv), or null to use
 // in-LDS triangular solves.  wv: the W/V products of lu_factor (LuWork::wv), or null; with wv the
 // solve runs one product per 64-row chunk and needs ybuf, an n x nrhs scratch (leading dimension n).
 template <typename T>
-int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
+int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
              hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr);
 template <typename T>
 int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
