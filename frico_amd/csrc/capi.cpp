@@ -66,17 +66,9 @@ int ensure_lu_work(frico_ctx* c)
     const int64_t npan = (c->N + 31) / 32 + 1;
     const int gmax = std::min(c->max_groups, LU_MAX_GROUPS);
     int rc;
-    if ((rc = c->counters.ensure((size_t)(npan * 2 * gmax))))
+    if ((rc = c->llw.ensure(lu_exchange_words(gmax))))
         return rc;
     if ((rc = c->flags.ensure(2)))
-        return rc;
-    if ((rc = c->cand_val.ensure(2 * LU_MAX_GROUPS)))
-        return rc;
-    if ((rc = c->cand_idx.ensure(2 * LU_MAX_GROUPS)))
-        return rc;
-    if ((rc = c->cand_row.ensure((size_t)2 * LU_MAX_GROUPS * LU_NB_MAX * sizeof(c128))))
-        return rc;
-    if ((rc = c->diag_row.ensure((size_t)2 * LU_NB_MAX * sizeof(c128))))
         return rc;
     if ((rc = c->dinv.ensure((size_t)((c->N + 63) / 64) * 2 * 4096 * sizeof(c128))))
         return rc;
@@ -94,13 +86,9 @@ int ensure_lu_work(frico_ctx* c)
 LuWork lu_work(frico_ctx* c)
 {
     LuWork w;
-    w.counters = c->counters.p;
-    w.ncounters = (int64_t)c->counters.n;
+    w.ll = c->llw.p;
+    w.nll = c->llw.n;
     w.flags = c->flags.p;
-    w.cand_val = c->cand_val.p;
-    w.cand_idx = c->cand_idx.p;
-    w.cand_row = c->cand_row.p;
-    w.diag_row = c->diag_row.p;
     w.dinv = c->dinv.p;
     w.perm = c->perm.p;
     w.nperm = (int64_t)(c->perm.n / lu_perm_stride());

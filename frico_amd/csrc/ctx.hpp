@@ -198,13 +198,11 @@ struct frico_ctx {
     int nb_factored = 64;  // block width the factors were computed with
     DevBuf<int> ipiv;
     // lu workspace
-    DevBuf<unsigned> counters;
+    DevBuf<unsigned long long> llw; // panel exchange area
     DevBuf<int> flags;
-    DevBuf<double> cand_val;
-    DevBuf<int> cand_idx;
-    DevBuf<unsigned char> cand_row, diag_row, dinv;
+    DevBuf<unsigned char> dinv;
     DevBuf<int> perm;             // row-interchange lists of the last factorisation
-    DevBuf<unsigned char> wv;     // W/V products of the last factorisation (fused solves)
+    DevBuf<unsigned char> wv;     // W products of the last factorisation (fused solves)
     DevBuf<unsigned char> ysolve; // n x nrhs scratch of the fused solves
     bool wv_ok = false;           // wv holds the products of the current factors
     bool fused_solve = true;

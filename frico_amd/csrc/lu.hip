@@ -383,42 +383,66 @@ static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, con
 // ----------------------------------------------------------------------------------------------
 // Panel factorisation.  G workgroups, workgroup g owns panel rows [g*R, g*R + R) in LDS, one row
 // per thread.  Per column every workgroup publishes its local pivot candidate (value, row index,
-// row contents) and raises its flag; after all G flags are up every workgroup picks the same
-// winner (first max of |re|+|im|) and performs the interchange on its own rows.
+// row contents) and, if it owns it, the current diagonal row; every workgroup then reads all G
+// candidates, picks the same winner (first max of |re|+|im|), reads the winner's row and performs
+// the interchange on its own rows.
+//
+// The exchange area holds 64-bit words {sequence number (high 32 bits), payload (low 32 bits)},
+// written and read with relaxed agent-scope atomics.  A reader accepts a word only when it carries
+// the current column's sequence number (k + c + 1, increasing over the factorisation), so no
+// ordering between different words is needed and no fence (which on gfx950 writes back and
+// invalidates the XCD's whole L2) is issued.  Two slots per group alternate by column parity: a
+// group rewrites slot c & 1 only after every group has published column c + 1, i.e. after every
+// reader of column c is done.
 // ----------------------------------------------------------------------------------------------
+constexpr int LL_HDR = 4;                 // header words per group: value lo, value hi, row
+constexpr int LL_ROWW = 4 * LU_NB_MAX;    // words per published row (complex128: 4 per element)
+
+__host__ __device__ inline size_t ll_words(int gmax)
+{
+    return (size_t)2 * gmax * LL_HDR + (size_t)2 * gmax * LL_ROWW + (size_t)2 * LL_ROWW;
+}
+
 struct PanelSync {
-    unsigned* flags;  // [npanels][2][gmax] publication flags, zeroed before the factorisation
-    int gmax;         // flag / candidate stride per parity
-    int* abort_flag;  // set if a wait exceeds its spin limit
-    double* cand_val; // [2][gmax]
-    int* cand_idx;    // [2][gmax]
-    void* cand_row;   // [2][gmax][wmax] complex
-    void* diag_row;   // [2][wmax] complex
-    int* info;        // first zero pivot (1-based global column), 0 if none
+    unsigned long long* words; // ll_words(gmax) words, zeroed before the factorisation
+    int gmax;
+    int* abort_flag; // set if a wait exceeds its spin limit
+    int* info;       // first zero pivot (1-based global column), 0 if none
+    __device__ unsigned long long* hdr(int par, int g) const { return words + ((size_t)par * gmax + g) * LL_HDR; }
+    __device__ unsigned long long* row(int par, int g) const
+    {
+        return words + (size_t)2 * gmax * LL_HDR + ((size_t)par * gmax + g) * LL_ROWW;
+    }
+    __device__ unsigned long long* diag(int par) const
+    {
+        return words + (size_t)2 * gmax * LL_HDR + (size_t)2 * gmax * LL_ROWW + (size_t)par * LL_ROWW;
+    }
 };
 
 constexpr long SPIN_LIMIT = 40000000; // ~seconds; a wait this long means a missing workgroup
 
-// Wave-level wait until flags[x] >= target for every x < G (each lane polls its own flags).
-__device__ inline bool wait_flags(const unsigned* fl, int G, unsigned target, int* abort_flag, int lane)
+__device__ inline void ll_put(unsigned long long* p, unsigned seq, unsigned payload)
 {
-    int ok = 1;
-    for (int x = lane; x < G && ok; x += 64) {
-        long spins = 0;
-        while (__hip_atomic_load(fl + x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-            __builtin_amdgcn_s_sleep(1);
-            if (++spins > SPIN_LIMIT) {
-                __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                ok = 0;
-                break;
-            }
-            if ((spins & 255) == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-                ok = 0;
-                break;
-            }
-        }
+    __hip_atomic_store(p, ((unsigned long long)seq << 32) | payload, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline unsigned long long ll_load(const unsigned long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Bounded wait of one lane: returns false (and raises the abort flag) after SPIN_LIMIT polls, or
+// when another workgroup has raised it.
+__device__ inline bool ll_spin(long& spins, int* abort_flag)
+{
+    if (++spins > SPIN_LIMIT) {
+        __hip_atomic_store(abort_flag, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return false;
     }
-    return __all(ok);
+    if ((spins & 255) == 0 && __hip_atomic_load(abort_flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        return false;
+    __builtin_amdgcn_s_sleep(1);
+    return true;
 }
 
 __device__ inline void argmax_merge(double& v, int& i, double ov, int oi)
@@ -432,17 +456,19 @@ __device__ inline void argmax_merge(double& v, int& i, double ov, int oi)
 template <typename T>
 __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
                                                     int R, int G, int* __restrict__ ipiv, PanelSync ps,
-                                                    unsigned* __restrict__ flags, int* __restrict__ perm)
+                                                    int* __restrict__ perm)
 {
     extern __shared__ unsigned char smem_raw[];
     cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw); // [w][R] column-major
-    cx<T>* prow = slab + (size_t)w * R;                // [w]
+    cx<T>* prow = slab + (size_t)w * R;                // [w] pivot row
+    cx<T>* drow = prow + w;                            // [w] previous content of row c
     __shared__ double rv[4];
     __shared__ int ri[4];
-    __shared__ int s_piv, s_gw, s_abort;
+    __shared__ int s_piv, s_gw;
     __shared__ double s_pval;
-    __shared__ int s_pivots[LU_NB_MAX];   // absolute pivot rows of the panel (workgroup 0)
+    __shared__ int s_pivots[LU_NB_MAX]; // absolute pivot rows of the panel (workgroup 0)
     __shared__ int s_scratch[2 * LU_NB_MAX];
+    constexpr int CW = (int)(sizeof(cx<T>) / 4); // 32-bit words per complex element
 
     const int g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r0 = g * R;                                      // first panel row of this workgroup
@@ -453,13 +479,9 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             slab[(size_t)c * R + tid] = Ap[(r0 + tid) + (int64_t)c * lda];
     __syncthreads();
 
-    const int gmax = ps.gmax;
-    double* cval = ps.cand_val;
-    int* cidx = ps.cand_idx;
-    cx<T>* crow = reinterpret_cast<cx<T>*>(ps.cand_row);
-    cx<T>* drow = reinterpret_cast<cx<T>*>(ps.diag_row);
     const bool own = tid < nr;
     const int grow = r0 + tid; // panel-relative row of this thread
+    const int RW = w * CW;     // words of a published row
 
     for (int c = 0; c < w; c++) {
         const int par = c & 1;
@@ -485,32 +507,59 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
         int p;
         double pval;
         if (G > 1) {
-            // ---- publish candidate, and the diagonal row if owned, then raise this group's flag
+            const unsigned seq = (unsigned)(k + c + 1);
+            // ---- publish: header, candidate row, and the diagonal row if this group holds row c
             if (tid == 0) {
-                cval[par * gmax + g] = lb;
-                cidx[par * gmax + g] = li;
+                const unsigned long long vb = (unsigned long long)__double_as_longlong(lb);
+                unsigned long long* h = ps.hdr(par, g);
+                ll_put(h + 0, seq, (unsigned)vb);
+                ll_put(h + 1, seq, (unsigned)(vb >> 32));
+                ll_put(h + 2, seq, (unsigned)li);
             }
-            if (lb >= 0.0 && tid < w)
-                crow[((size_t)par * gmax + g) * w + tid] = slab[(size_t)tid * R + (li - r0)];
-            if (c >= r0 && c < r0 + nr && tid < w)
-                drow[(size_t)par * w + tid] = slab[(size_t)tid * R + (c - r0)];
-            __syncthreads();
-            if (tid == 0) {
-                __threadfence();
-                __hip_atomic_store(flags + par * gmax + g, (unsigned)(c + 1), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
+            if (lb >= 0.0)
+                for (int t = tid; t < RW; t += 256)
+                    ll_put(ps.row(par, g) + t, seq,
+                           reinterpret_cast<const unsigned*>(&slab[(size_t)(t / CW) * R + (li - r0)])[t % CW]);
+            if (c >= r0 && c < r0 + nr)
+                for (int t = tid; t < RW; t += 256)
+                    ll_put(ps.diag(par) + t, seq,
+                           reinterpret_cast<const unsigned*>(&slab[(size_t)(t / CW) * R + (c - r0)])[t % CW]);
+            // ---- all candidates (wave 0, up to 4 groups per lane, all words in flight together)
+            bool ok = true;
             if (wv == 0) {
-                const bool ok = wait_flags(flags + par * gmax, G, (unsigned)(c + 1), ps.abort_flag, lane);
-                __threadfence();
+                unsigned long long hv[4][3];
+                long spins = 0;
+                for (;;) {
+                    bool ready = true;
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const int x = lane + 64 * j;
+                        if (x < G) {
+                            const unsigned long long* h = ps.hdr(par, x);
+#pragma unroll
+                            for (int q = 0; q < 3; q++) {
+                                hv[j][q] = ll_load(h + q);
+                                ready = ready && (unsigned)(hv[j][q] >> 32) == seq;
+                            }
+                        }
+                    }
+                    if (ready || !(ok = ll_spin(spins, ps.abort_flag)))
+                        break;
+                }
                 double b = -1.0;
                 int bi = 0x7fffffff, bg = 0;
-                for (int x = lane; x < G; x += 64) {
-                    const double cv = cval[par * gmax + x];
-                    const int ci = cidx[par * gmax + x];
-                    if (cv > b || (cv == b && ci < bi)) {
-                        b = cv;
-                        bi = ci;
-                        bg = x;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int x = lane + 64 * j;
+                    if (x < G) {
+                        const double cv = __longlong_as_double(
+                            (long long)(((hv[j][1] & 0xffffffffull) << 32) | (hv[j][0] & 0xffffffffull)));
+                        const int ci = (int)(unsigned)hv[j][2];
+                        if (cv > b || (cv == b && ci < bi)) {
+                            b = cv;
+                            bi = ci;
+                            bg = x;
+                        }
                     }
                 }
 #pragma unroll
@@ -527,22 +576,37 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                     s_pval = b;
                     s_piv = bi;
                     s_gw = bg;
-                    s_abort = ok ? 0 : 1;
                 }
             }
-            __syncthreads();
-            if (s_abort)
+            if (__syncthreads_or(!ok))
                 return;
             p = s_piv;
             pval = s_pval;
-            if (tid < w)
-                prow[tid] = crow[((size_t)par * gmax + s_gw) * w + tid];
-            __syncthreads();
+            // ---- winner row, and the previous row c when row p is ours (both words in flight)
+            const bool need_diag = pval > 0.0 && p != c && p >= r0 && p < r0 + nr;
+            const unsigned long long* wrow = ps.row(par, s_gw);
+            for (int t = tid; t < RW && ok; t += 256) {
+                long spins = 0;
+                for (;;) {
+                    const unsigned long long a = ll_load(wrow + t);
+                    const unsigned long long d = need_diag ? ll_load(ps.diag(par) + t) : ((unsigned long long)seq << 32);
+                    if ((unsigned)(a >> 32) == seq && (unsigned)(d >> 32) == seq) {
+                        reinterpret_cast<unsigned*>(prow)[t] = (unsigned)a;
+                        if (need_diag)
+                            reinterpret_cast<unsigned*>(drow)[t] = (unsigned)d;
+                        break;
+                    }
+                    if (!(ok = ll_spin(spins, ps.abort_flag)))
+                        break;
+                }
+            }
+            if (__syncthreads_or(!ok))
+                return;
             if (pval > 0.0) {
                 if (c >= r0 && c < r0 + nr && tid < w)
                     slab[(size_t)tid * R + (c - r0)] = prow[tid];
-                if (p != c && p >= r0 && p < r0 + nr && tid < w)
-                    slab[(size_t)tid * R + (p - r0)] = drow[(size_t)par * w + tid];
+                if (need_diag && tid < w)
+                    slab[(size_t)tid * R + (p - r0)] = drow[tid];
             }
             __syncthreads();
         } else {
@@ -585,6 +649,8 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             Ap[(r0 + tid) + (int64_t)c * lda] = slab[(size_t)c * R + tid];
 }
 
+// Row interchanges of one panel undone (swaps applied in reverse order) on columns [col0, col0 + ncols);
+// used by the transposed solve.
 template <typename T>
 __global__ __launch_bounds__(256) void laswp_reverse(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols,
                                                      int64_t k, int w, const int* __restrict__ ipiv)
@@ -955,26 +1021,23 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         return set_error("lu: block size out of range"), -1;
     const int R = panel_rows<T>(nb);
     const int64_t npan = (n + nb - 1) / nb;
-    const size_t sh_max = ((size_t)nb * R + nb) * sizeof(cx<T>);
+    const size_t sh_max = ((size_t)nb * R + 2 * nb) * sizeof(cx<T>);
     if (sh_max > 64 * 1024)
         FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&panel_factor<T>), sh_max));
     const int gmax = wk.max_groups;
-    if ((int64_t)wk.ncounters < npan * 2 * gmax)
-        return set_error("lu: sync workspace too small"), -1;
+    if (!wk.ll || wk.nll < ll_words(gmax))
+        return set_error("lu: exchange workspace too small"), -1;
     if (!wk.perm || wk.nperm < npan)
         return set_error("lu: interchange-list workspace too small"), -1;
-    const int64_t nfl = npan * 2 * gmax;
-    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((nfl + 255) / 256)), dim3(256), 0, st, wk.counters, nfl);
+    const int64_t nz = 2 * (int64_t)ll_words(gmax);
+    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.ll, nz);
     hipLaunchKernelGGL(zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
     FRICO_HIP_CHECK(hipGetLastError());
     PanelSync ps{};
+    ps.words = wk.ll;
     ps.gmax = gmax;
     ps.abort_flag = wk.flags;
     ps.info = wk.flags + 1;
-    ps.cand_val = wk.cand_val;
-    ps.cand_idx = wk.cand_idx;
-    ps.cand_row = wk.cand_row;
-    ps.diag_row = wk.diag_row;
     auto panel = [&](int64_t p) -> int {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
@@ -982,9 +1045,9 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         const int G = (int)((m + R - 1) / R);
         if (G > gmax)
             return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), FRICO_ERR_LIMIT;
-        const size_t sh = ((size_t)w * R + w) * sizeof(cx<T>);
+        const size_t sh = ((size_t)w * R + 2 * w) * sizeof(cx<T>);
         hipLaunchKernelGGL((panel_factor<T>), dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
-                           wk.counters + p * 2 * gmax, wk.perm + p * PERM_STRIDE);
+                           wk.perm + p * PERM_STRIDE);
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
     };
@@ -1221,6 +1284,7 @@ template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const in
 template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,
                                         hipStream_t);
 int lu_perm_stride() { return PERM_STRIDE; }
+size_t lu_exchange_words(int gmax) { return ll_words(gmax); }
 template int panel_rows<double>(int);
 template int panel_rows<float>(int);
 

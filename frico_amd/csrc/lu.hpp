@@ -12,17 +12,14 @@ constexpr int LU_MAX_GROUPS = 512;
 
 // Device workspace of the panel factorisation.
 struct LuWork {
-    unsigned* counters = nullptr; // one barrier counter per panel
-    int64_t ncounters = 0;
+    // panel exchange area (lu_exchange_words(max_groups) 64-bit words), zeroed by lu_factor
+    unsigned long long* ll = nullptr;
+    size_t nll = 0;
     int* flags = nullptr; // [0] abort, [1] info
-    double* cand_val = nullptr;
-    int* cand_idx = nullptr;
-    void* cand_row = nullptr;
-    void* diag_row = nullptr;
-    int max_groups = 0; // co-resident workgroup limit of the panel kernel
+    int max_groups = 0;   // co-resident workgroup limit of the panel kernel
     void* dinv = nullptr; // [ceil(n/64)][2][64*64] diagonal-chunk inverses, written at the end of lu_factor
-    // n x n (leading dimension n) products W = L(t, j) Linv_j below and V = U(t, j) Uinv_j above each
-    // diagonal chunk, written at the end of lu_factor when non-null (needs dinv and nb % 64 == 0)
+    // W = L(t, j) Linv_j below each diagonal chunk (n x n, leading dimension n), written at the end of
+    // lu_factor when non-null (needs dinv and nb % 64 == 0)
     void* wv = nullptr;
     // per panel: the row-interchange list written by the panel kernel (lu_perm_stride() ints each)
     int* perm = nullptr;
@@ -43,6 +40,8 @@ template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
 // ints per panel of LuWork::perm
 int lu_perm_stride();
+// 64-bit words of LuWork::ll for gmax workgroups
+size_t lu_exchange_words(int gmax);
 // perm: the interchange lists written by lu_factor (LuWork::perm).
 // dinv: the 64 x 64 diagonal-chunk inverses written by lu_factor (LuWork::dinv), or null to use
 // in-LDS triangular solves.  wv: the W/V products of lu_factor (LuWork::wv), or null; with wv the
