@@ -453,6 +453,50 @@ __device__ inline void argmax_merge(double& v, int& i, double ov, int oi)
     }
 }
 
+// Wavefront reductions with DPP (quad_perm, row_ror, row_bcast): the result is read from lane 63
+// and returned to every lane.
+template <int CTRL, int RMASK> __device__ inline double dpp_f64(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp((int)b, (int)b, CTRL, RMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(b >> 32), (int)(b >> 32), CTRL, RMASK, 0xf, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ inline double wave_max(double v)
+{
+    v = fmax(v, dpp_f64<0xb1, 0xf>(v));  // quad_perm [1,0,3,2]
+    v = fmax(v, dpp_f64<0x4e, 0xf>(v));  // quad_perm [2,3,0,1]
+    v = fmax(v, dpp_f64<0x124, 0xf>(v)); // row_ror:4
+    v = fmax(v, dpp_f64<0x128, 0xf>(v)); // row_ror:8
+    v = fmax(v, dpp_f64<0x142, 0xa>(v)); // row_bcast:15
+    v = fmax(v, dpp_f64<0x143, 0xc>(v)); // row_bcast:31
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, 63), hi = __builtin_amdgcn_readlane((int)(b >> 32), 63);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+__device__ inline int wave_min(int v)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0xb1, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x4e, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x124, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x128, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x142, 0xa, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// First maximum over a wavefront of (value, index) pairs: the largest value, and among equal values
+// the smallest index; also returns the lane that holds it.
+__device__ inline void wave_argmax(double v, int vi, double& bv, int& bi, int& blane)
+{
+    bv = wave_max(v);
+    bi = wave_min(v == bv ? vi : 0x7fffffff);
+    const unsigned long long m = __ballot(v == bv && vi == bi);
+    blane = __ffsll((long long)m) - 1;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
                                                     int R, int G, int* __restrict__ ipiv, PanelSync ps,
@@ -492,12 +536,14 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             v = (double)cabs1(slab[(size_t)c * R + tid]);
             vi = grow;
         }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1)
-            argmax_merge(v, vi, __shfl_xor(v, o, 64), __shfl_xor(vi, o, 64));
-        if (lane == 0) {
-            rv[wv] = v;
-            ri[wv] = vi;
+        {
+            double bv;
+            int bi, bl;
+            wave_argmax(v, vi, bv, bi, bl);
+            if (lane == 0) {
+                rv[wv] = bv;
+                ri[wv] = bi;
+            }
         }
         __syncthreads();
         double lb = rv[0];
@@ -562,20 +608,14 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                         }
                     }
                 }
-#pragma unroll
-                for (int o = 32; o > 0; o >>= 1) {
-                    const double ov = __shfl_xor(b, o, 64);
-                    const int oi = __shfl_xor(bi, o, 64), og = __shfl_xor(bg, o, 64);
-                    if (ov > b || (ov == b && oi < bi)) {
-                        b = ov;
-                        bi = oi;
-                        bg = og;
-                    }
-                }
+                double wb;
+                int wi, wl;
+                wave_argmax(b, bi, wb, wi, wl);
+                const int wg = __builtin_amdgcn_readlane(bg, wl);
                 if (lane == 0) {
-                    s_pval = b;
-                    s_piv = bi;
-                    s_gw = bg;
+                    s_pval = wb;
+                    s_piv = wi;
+                    s_gw = wg;
                 }
             }
             if (__syncthreads_or(!ok))
@@ -633,7 +673,25 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             const cx<T> rp = cdiv(cx<T>{1, 0}, prow[c]);
             const cx<T> l = slab[(size_t)c * R + tid] * rp;
             slab[(size_t)c * R + tid] = l;
-            for (int j = c + 1; j < w; j++) {
+            int j = c + 1;
+            // eight columns at a time, all LDS reads of a group in flight together
+            for (; j + 8 <= w; j += 8) {
+                cx<T> a[8], u[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    a[q] = slab[(size_t)(j + q) * R + tid];
+                    u[q] = prow[j + q];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    a[q].re -= l.re * u[q].re - l.im * u[q].im;
+                    a[q].im -= l.re * u[q].im + l.im * u[q].re;
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    slab[(size_t)(j + q) * R + tid] = a[q];
+            }
+            for (; j < w; j++) {
                 const cx<T> u = prow[j];
                 cx<T>& a = slab[(size_t)j * R + tid];
                 a.re -= l.re * u.re - l.im * u.im;
