@@ -16,7 +16,7 @@ import numpy as np
 from . import _lib, meshes
 from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C128)
 
-__all__ = ["Context", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
+__all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR"]
 
 
@@ -304,3 +304,72 @@ class AntennaResponse:
     def backward(self, grad_out):
         go = np.asarray(grad_out, np.float64)
         return self.ctx.adjoint_backward(go[0], go[1], go[2])
+
+
+class Batch:
+    """Many meshes, each at many frequencies, swept together on one device (frico_batch_*).
+
+    The reference sweeps one mesh at a time (do_sweep, src/emw_ana_antenna.cpp:236-295); here the
+    meshes run largest first on `workers` concurrent contexts, with HIP-graph replay of the
+    complex64 LU and refinement sweeps and a per-size-class choice of the LU block width."""
+
+    def __init__(self, device: int = 0, workers: int = 8):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        self._check(self._L.frico_batch_create(device, workers, C.byref(h)))
+        self._h = h
+        self.meshes = []
+
+    def _check(self, rc, ok=(0,)):
+        return Context._check(self, rc, ok)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.frico_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_mesh(self, sm: meshes.SurfaceMesh, degree: int = 5, feed=None) -> int:
+        fd = np.ascontiguousarray(sm.feed if feed is None else feed, np.int32)
+        rc = self._L.frico_batch_add_mesh(self._h, _p(sm.vertices), sm.nv, _p(sm.triangles, C.c_int64),
+                                          _p(sm.tags, C.c_int32), sm.nt, _p(sm.bedges, C.c_int64),
+                                          _p(sm.btags, C.c_int32), sm.bedges.shape[0], degree, _p(fd, C.c_int32),
+                                          len(fd))
+        if rc < 0:
+            self._check(rc)
+        self.meshes.append(sm)
+        return rc
+
+    def antenna(self, freqs, fill="c64", form="c64_ir", tol=1e-10, maxit=10, graphs=True, autotune=True,
+                voltage=1.0):
+        """Input impedance Zin[m, f] and the solve status of every (mesh, frequency)."""
+        f = np.ascontiguousarray(np.atleast_1d(freqs), np.float64)
+        nm = len(self.meshes)
+        z = np.zeros((nm, f.size, 2))
+        infos = (_lib.frico_solve_info * (nm * f.size))()
+        flags = (_lib.BATCH_GRAPHS if graphs else 0) | (_lib.BATCH_AUTOTUNE if autotune else 0)
+        v = complex(voltage)
+        self._check(self._L.frico_batch_antenna(self._h, _p(f), f.size, {"c128": C128, "c64": C64}[fill],
+                                                _form(form), tol, maxit, flags, v.real, v.imag, _p(z), infos))
+        info = [[SolveInfo(infos[m * f.size + i].iterations, bool(infos[m * f.size + i].converged),
+                           infos[m * f.size + i].max_rel_residual) for i in range(f.size)] for m in range(nm)]
+        return z[..., 0] + 1j * z[..., 1], info
+
+    def block_sizes(self):
+        """(block width per mesh, number of meshes that replayed graphs) of the last run."""
+        nb = np.zeros(len(self.meshes), np.int32)
+        n = self._L.frico_batch_block_sizes(self._h, _p(nb, C.c_int32))
+        if n < 0:
+            self._check(n)
+        return nb, n

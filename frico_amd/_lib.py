@@ -21,6 +21,7 @@ FRICO_ERR_LIMIT = -1006
 FILL_QUAD, FILL_APPROX = 0, 1
 C128, C64 = 0, 1
 SOLVE_C128, SOLVE_C64_IR = 0, 1
+BATCH_GRAPHS, BATCH_AUTOTUNE = 1, 2
 
 
 class frico_rwg(C.Structure):
@@ -87,6 +88,14 @@ SIGNATURES = {
     "frico_monostatic_sweep": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_int64, _d, _d, C.c_double,
                                          C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _d,
                                          C.POINTER(frico_solve_info)]),
+    "frico_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
+    "frico_batch_destroy": (C.c_int, [_vp]),
+    "frico_batch_add_mesh": (C.c_int, [_vp, _d, C.c_int64, _i64, _i32, C.c_int64, _i64, _i32, C.c_int64, C.c_int,
+                                       _i32, C.c_int]),
+    "frico_batch_num_meshes": (C.c_int, [_vp]),
+    "frico_batch_antenna": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                      C.c_double, _d, C.POINTER(frico_solve_info)]),
+    "frico_batch_block_sizes": (C.c_int, [_vp, _i32]),
 }
 
 _lib = None

@@ -1,0 +1,423 @@
+// Batched uneven antenna workloads: many meshes, each at many frequencies, on one GPU.
+//
+// The reference runs one mesh at a time and its frequencies one after another
+// (do_sweep, src/emw_ana_antenna.cpp:236-295; init_sweep, src/emw_solver.cpp:460-471).  Here the
+// unit of work is one mesh at all of its frequencies:
+//   - units are ordered by decreasing N (longest processing time first) and taken by W worker
+//     threads, each with its own context (stream, workspace), so W systems are in flight at once;
+//   - per unit the worker loads the mesh, orients the delta gap once, and picks the LU block width
+//     from a table filled by timing the candidates on the first system of each size class;
+//   - per frequency: impedance fill and delta-gap RHS are enqueued eagerly; the complex64
+//     conversion, the LU and the first refinement sweep are one HIP graph, built on the second
+//     frequency from the recorded kernel launches and replayed for the rest, and every further
+//     sweep is a second graph.  The graphs are assembled node by node on the worker's thread
+//     (no stream capture), so the other workers' launches and allocations cannot disturb them.
+// Co-residency: a panel launch needs ceil(N / R) workgroups resident at once (one per CU), so the
+// worker count is capped at CUs / ceil(Nmax / R).
+#include "ctx.hpp"
+#include "solver.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <string>
+#include <thread>
+
+using namespace frico;
+
+struct frico_batch {
+    struct Mesh {
+        std::vector<double> V;
+        std::vector<int64_t> T;
+        std::vector<int32_t> tags;
+        std::vector<int64_t> iface;
+        std::vector<int32_t> itags;
+        std::vector<int32_t> feed;
+        int degree = 5;
+        int64_t N = 0;
+    };
+    int device = 0;
+    int cus = 0;
+    std::vector<frico_ctx*> ctx;
+    std::vector<Mesh> meshes;
+    std::vector<int32_t> mesh_nb; // block width used for each mesh by the last run
+    std::atomic<int> graph_units{0}; // units of the last run that replayed graphs
+    std::mutex mu;
+    std::map<int64_t, int> tuned; // size class (N / 256) -> block width
+};
+
+namespace {
+
+using cd = std::complex<double>;
+
+// Internal edges (shared by exactly two triangles) = RWG unknowns, for ordering only.
+int64_t count_internal_edges(const int64_t* tri, int64_t nt)
+{
+    std::vector<std::pair<int64_t, int64_t>> e;
+    e.reserve((size_t)(3 * nt));
+    for (int64_t t = 0; t < nt; t++)
+        for (int k = 0; k < 3; k++) {
+            const int64_t a = tri[3 * t + k], b = tri[3 * t + (k + 1) % 3];
+            e.emplace_back(std::min(a, b), std::max(a, b));
+        }
+    std::sort(e.begin(), e.end());
+    int64_t n = 0;
+    for (size_t i = 0; i < e.size();) {
+        size_t j = i;
+        while (j < e.size() && e[j] == e[i])
+            j++;
+        n += (j - i) == 2;
+        i = j;
+    }
+    return n;
+}
+
+struct RunOpts {
+    const double* freqs;
+    int nfreq;
+    int fill_prec;
+    int form;
+    double tol;
+    int maxit;
+    bool graphs;
+    bool autotune;
+    cd voltage;
+};
+
+struct Graph {
+    hipGraph_t g = nullptr;
+    hipGraphExec_t x = nullptr;
+    ~Graph()
+    {
+        if (x)
+            (void)hipGraphExecDestroy(x);
+        if (g)
+            (void)hipGraphDestroy(g);
+    }
+};
+
+// Builds an executable graph from the kernels fn launches on the context's stream (recorded, not
+// run: see GraphRecorder).  fn must enqueue kernels only.
+template <typename F> int record(frico_ctx* c, Graph& gr, F fn)
+{
+    GraphRecorder rec;
+    rec.st = c->st;
+    FRICO_HIP_CHECK(hipGraphCreate(&rec.graph, 0));
+    graph_recorder() = &rec;
+    const int rc = fn();
+    graph_recorder() = nullptr;
+    if (rc || rec.err != hipSuccess) {
+        (void)hipGraphDestroy(rec.graph);
+        return rc ? rc : hip_fail(rec.err, "hipGraphAddKernelNode", __FILE__, __LINE__);
+    }
+    gr.g = rec.graph;
+    FRICO_HIP_CHECK(hipGraphInstantiate(&gr.x, gr.g, nullptr, nullptr, 0));
+    return 0;
+}
+
+// Eager factor + solve (used for the first frequency, for tuning and when graphs are off).
+int solve_eager(frico_ctx* c, const RunOpts& o, frico_solve_info* info)
+{
+    int rc = frico_factor(c, o.form);
+    if (rc)
+        return rc;
+    return frico_solve(c, o.tol, o.maxit, info);
+}
+
+// Complex64 LU + refinement by graph replay; g1 = conversion + LU + first sweep, g2 = one sweep.
+int solve_graphs(frico_ctx* c, const RunOpts& o, Graph& g1, Graph& g2, frico_solve_info* info)
+{
+    LuWork wk;
+    int rc = factor_prepare(c, o.form, wk, false);
+    if (rc || (rc = refine_prepare(c)))
+        return rc;
+    std::vector<double> bn;
+    if ((rc = col_norms(c, c->B.p, c->nrhs, bn)))
+        return rc;
+    if (!g1.x && ((rc = record(c, g1, [&] {
+                      int r = factor_enqueue(c, o.form, wk);
+                      return r ? r : refine_enqueue(c, true);
+                  })) ||
+                  (rc = record(c, g2, [&] { return refine_enqueue(c, false); }))))
+        return rc;
+    FRICO_HIP_CHECK(hipGraphLaunch(g1.x, c->st));
+    if ((rc = factor_finish(c, o.form, wk)))
+        return rc;
+    frico_solve_info inf{0, 0, -1.0};
+    for (int it = 1; it <= o.maxit; it++) {
+        if (it > 1)
+            FRICO_HIP_CHECK(hipGraphLaunch(g2.x, c->st));
+        double worst = 0;
+        if ((rc = read_refine_norms(c, bn, &worst)))
+            return rc;
+        inf.iterations = it;
+        inf.max_rel_residual = worst;
+        if (worst <= o.tol) {
+            inf.converged = 1;
+            break;
+        }
+    }
+    c->have_x = true;
+    *info = inf;
+    if (!inf.converged)
+        return set_error("refinement stopped at relative residual %.3e > %.3e after %d sweeps", inf.max_rel_residual,
+                         o.tol, inf.iterations),
+               FRICO_ERR_NOCONV;
+    return 0;
+}
+
+// Block width for this size class: the table entry, or (autotune) the faster of the candidates
+// on the current system, timed as factor + solve.
+int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, int* nb_out)
+{
+    const int64_t cls = c->N / 256;
+    {
+        std::lock_guard<std::mutex> lock(b->mu);
+        auto it = b->tuned.find(cls);
+        if (it != b->tuned.end() || !o.autotune) {
+            *nb_out = it != b->tuned.end() ? it->second : 64;
+            return 0;
+        }
+    }
+    const int cand[2] = {64, 32};
+    double best = 1e300;
+    int best_nb = 64;
+    for (int nb : cand) {
+        int rc = frico_ctx_set_block(c, nb);
+        if (rc)
+            return rc;
+        frico_solve_info inf{};
+        // one untimed pass (workspace allocation, first launches), then the timed one
+        for (int pass = 0; pass < 2; pass++) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if ((rc = solve_eager(c, o, &inf)))
+                return rc;
+            const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            if (pass == 1 && t < best) {
+                best = t;
+                best_nb = nb;
+            }
+        }
+    }
+    std::lock_guard<std::mutex> lock(b->mu);
+    b->tuned.emplace(cls, best_nb);
+    *nb_out = b->tuned[cls];
+    return 0;
+}
+
+int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin, frico_solve_info* infos)
+{
+    const frico_batch::Mesh& m = b->meshes[mi];
+    int rc = frico_mesh_load(c, m.V.data(), (int64_t)(m.V.size() / 3), m.T.data(), m.tags.data(),
+                             (int64_t)m.tags.size(), m.iface.empty() ? nullptr : m.iface.data(),
+                             m.itags.empty() ? nullptr : m.itags.data(), (int64_t)m.itags.size(), m.degree);
+    if (rc)
+        return rc;
+    const int64_t N = c->N;
+    // reorient_deltagap_edges once per sweep (src/emw_ana_antenna.cpp:238)
+    std::vector<double> signs((size_t)N);
+    int applied = 0;
+    if ((rc = frico_reorient_deltagap(c, m.feed.data(), (int)m.feed.size(), 0, signs.data(), &applied)))
+        return rc;
+    std::vector<frico_rwg> bf((size_t)N);
+    if ((rc = frico_rwg_table(c, bf.data())))
+        return rc;
+    std::vector<cd> I((size_t)N);
+    Graph g1, g2;
+    int nb = 64;
+    for (int f = 0; f < o.nfreq; f++) {
+        const double freq = o.freqs[f];
+        if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, o.fill_prec)) ||
+            (rc = frico_rhs_deltagap(c, freq, m.feed.data(), (int)m.feed.size(), o.voltage.real(), o.voltage.imag(),
+                                     applied ? signs.data() : nullptr)))
+            return rc;
+        frico_solve_info& inf = infos[(size_t)mi * o.nfreq + f];
+        if (f == 0) {
+            if ((rc = pick_block(b, c, o, &nb)) || (rc = frico_ctx_set_block(c, nb)))
+                return rc;
+            b->mesh_nb[mi] = nb;
+            rc = solve_eager(c, o, &inf);
+        } else if (o.graphs && o.form == FRICO_SOLVE_C64_IR) {
+            rc = solve_graphs(c, o, g1, g2, &inf);
+        } else {
+            rc = solve_eager(c, o, &inf);
+        }
+        if (rc)
+            return rc;
+        if (f == o.nfreq - 1 && g1.x)
+            b->graph_units++;
+        // compute_port_values (src/emw_ana_antenna.cpp:76-105): I_port = sum l I sign over the feed
+        if ((rc = frico_solution_get(c, reinterpret_cast<double*>(I.data()), N)))
+            return rc;
+        cd ip = 0.0;
+        for (int64_t n = 0; n < N; n++) {
+            if (!bf[n].has_interface)
+                continue;
+            for (int32_t itf : m.feed)
+                if (bf[n].interface_tag == itf)
+                    ip += bf[n].length * I[n] * (applied ? signs[n] : 1.0);
+        }
+        const cd z = o.voltage / ip;
+        zin[2 * ((size_t)mi * o.nfreq + f)] = z.real();
+        zin[2 * ((size_t)mi * o.nfreq + f) + 1] = z.imag();
+    }
+    return 0;
+}
+
+} // namespace
+
+extern "C" {
+
+int frico_batch_create(int device, int workers, frico_batch** out)
+{
+    if (!out)
+        return set_error("null output pointer"), -3;
+    if (workers < 1)
+        return set_error("workers must be >= 1"), -2;
+    auto* b = new frico_batch;
+    b->device = device;
+    for (int w = 0; w < workers; w++) {
+        frico_ctx* c = nullptr;
+        const int rc = frico_ctx_create(device, &c);
+        if (rc) {
+            for (auto* x : b->ctx)
+                frico_ctx_destroy(x);
+            delete b;
+            return rc;
+        }
+        c->lookahead = false;
+        b->ctx.push_back(c);
+    }
+    b->cus = b->ctx[0]->max_groups;
+    *out = b;
+    return 0;
+}
+
+int frico_batch_destroy(frico_batch* b)
+{
+    if (!b)
+        return 0;
+    for (auto* c : b->ctx)
+        frico_ctx_destroy(c);
+    delete b;
+    return 0;
+}
+
+int frico_batch_add_mesh(frico_batch* b, const double* vertices, int64_t nv, const int64_t* triangles,
+                         const int32_t* tri_tags, int64_t nt, const int64_t* iface_edges, const int32_t* iface_tags,
+                         int64_t nif, int degree, const int32_t* feed, int nfeed)
+{
+    if (!b)
+        return set_error("null batch"), -1;
+    if (!vertices || nv < 3 || !triangles || !tri_tags || nt < 1)
+        return set_error("empty mesh"), -2;
+    if (!feed || nfeed < 1)
+        return set_error("no feed interface"), -11;
+    frico_batch::Mesh m;
+    m.V.assign(vertices, vertices + 3 * nv);
+    m.T.assign(triangles, triangles + 3 * nt);
+    m.tags.assign(tri_tags, tri_tags + nt);
+    if (nif > 0 && iface_edges && iface_tags) {
+        m.iface.assign(iface_edges, iface_edges + 2 * nif);
+        m.itags.assign(iface_tags, iface_tags + nif);
+    }
+    m.feed.assign(feed, feed + nfeed);
+    m.degree = degree;
+    m.N = count_internal_edges(triangles, nt);
+    b->meshes.push_back(std::move(m));
+    return (int)b->meshes.size() - 1;
+}
+
+int frico_batch_num_meshes(const frico_batch* b) { return b ? (int)b->meshes.size() : -1; }
+
+int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill_prec, int form, double tol, int maxit,
+                        int flags, double v_re, double v_im, double* zin, frico_solve_info* info)
+{
+    if (!b)
+        return set_error("null batch"), -1;
+    if (!freqs || nfreq < 1)
+        return set_error("no frequencies"), -2;
+    if (fill_prec != FRICO_C128 && fill_prec != FRICO_C64)
+        return set_error("unknown fill precision %d", fill_prec), -4;
+    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR)
+        return set_error("unknown factorisation form %d", form), -5;
+    if (form == FRICO_SOLVE_C128 && fill_prec != FRICO_C128)
+        return set_error("the complex128 form needs a complex128 fill"), -4;
+    if (maxit < 1)
+        return set_error("maxit must be >= 1"), -7;
+    if (!zin || !info)
+        return set_error("null output"), -11;
+    const int nm = (int)b->meshes.size();
+    if (nm == 0)
+        return 0;
+    RunOpts o{freqs, nfreq, fill_prec, form, tol, maxit, (flags & FRICO_BATCH_GRAPHS) != 0,
+              (flags & FRICO_BATCH_AUTOTUNE) != 0, cd(v_re, v_im)};
+    // longest processing time first
+    std::vector<int> order((size_t)nm);
+    for (int i = 0; i < nm; i++)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int c) { return b->meshes[a].N > b->meshes[c].N; });
+    // co-residency of the panel launches of all workers
+    int64_t nmax = b->meshes[order[0]].N;
+    const int R = form == FRICO_SOLVE_C128 ? panel_rows<double>(64) : panel_rows<float>(64);
+    const int64_t gmax = std::max<int64_t>(1, (nmax + R - 1) / R);
+    const int W = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)b->ctx.size(), b->cus / gmax));
+    b->mesh_nb.assign((size_t)nm, 0);
+    b->graph_units = 0;
+    for (int i = 0; i < nm * nfreq; i++)
+        info[i] = frico_solve_info{0, 0, -1.0};
+
+    std::atomic<int> next{0};
+    std::atomic<int> failed{0};
+    std::mutex emu;
+    int first_rc = 0;
+    std::string first_msg;
+    auto worker = [&](int w) {
+        frico_ctx* c = b->ctx[w];
+        if (hipSetDevice(b->device) != hipSuccess) {
+            failed = 1;
+            return;
+        }
+        for (;;) {
+            if (failed.load())
+                return;
+            const int k = next.fetch_add(1);
+            if (k >= nm)
+                return;
+            const int rc = run_unit(b, c, order[k], o, zin, info);
+            if (rc) {
+                std::lock_guard<std::mutex> lock(emu);
+                if (!first_rc) {
+                    first_rc = rc;
+                    first_msg = std::string("mesh ") + std::to_string(order[k]) + ": " + frico_last_error();
+                }
+                failed = 1;
+                return;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 0; w < W; w++)
+        th.emplace_back(worker, w);
+    for (auto& t : th)
+        t.join();
+    if (first_rc)
+        return set_error("%s", first_msg.c_str()), first_rc;
+    if (failed)
+        return set_error("a batch worker could not select the device"), FRICO_ERR_HIP;
+    return 0;
+}
+
+int frico_batch_block_sizes(const frico_batch* b, int32_t* nb)
+{
+    if (!b || !nb)
+        return -1;
+    for (size_t i = 0; i < b->mesh_nb.size(); i++)
+        nb[i] = b->mesh_nb[i];
+    return b->graph_units.load();
+}
+
+} // extern "C"

@@ -1,10 +1,13 @@
 // C ABI of the MI355X EFIE library: context, mesh ingestion, fill, RHS, factor/solve, fields.
 #include "ctx.hpp"
+#include "solver.hpp"
 
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
 #include <list>
+#include <mutex>
+#include <tuple>
 
 namespace {
 thread_local std::string g_err;
@@ -20,6 +23,30 @@ void set_error(const char* fmt, ...)
     vsnprintf(buf, sizeof buf, fmt, ap);
     va_end(ap);
     g_err = buf;
+}
+
+GraphRecorder*& graph_recorder()
+{
+    thread_local GraphRecorder* r = nullptr;
+    return r;
+}
+
+hipError_t allow_lds(const void* func, size_t bytes)
+{
+    static std::mutex mu;
+    static std::vector<std::tuple<int, const void*, size_t>> done;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess)
+        return e;
+    std::lock_guard<std::mutex> lock(mu);
+    for (auto& d : done)
+        if (std::get<0>(d) == dev && std::get<1>(d) == func && std::get<2>(d) >= bytes)
+            return hipSuccess;
+    e = hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e == hipSuccess)
+        done.emplace_back(dev, func, bytes);
+    return e;
 }
 
 int hip_fail(hipError_t e, const char* what, const char* file, int line)
@@ -814,20 +841,35 @@ static int read_lu_flags(frico_ctx* c)
     return h[1];
 }
 
-int frico_factor(frico_ctx* c, int form)
+} // extern "C"
+
+namespace frico {
+
+int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
 {
-    if (check_ctx(c))
-        return -1;
     if (c->zprec < 0)
         return set_error("no matrix to factor"), FRICO_ERR_STATE;
     if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR)
         return set_error("unknown factorisation form %d", form), -2;
+    if (form == FRICO_SOLVE_C128 && (c->zprec != FRICO_C128 || c->z64_factored))
+        return set_error("complex128 factorisation needs an unfactored complex128 Z"), FRICO_ERR_STATE;
+    if (form == FRICO_SOLVE_C64_IR && c->zprec == FRICO_C128 && c->z64_factored)
+        return set_error("Z was overwritten by a complex128 factorisation"), FRICO_ERR_STATE;
     int rc = set_dev(c);
     if (rc || (rc = ensure_lu_work(c)) || (rc = c->ipiv.ensure((size_t)c->N)))
         return rc;
     const int64_t N = c->N;
-    LuWork wk = lu_work(c);
-    // W/V products for the fused solves: one more N x N matrix of the factor's type, taken only
+    if (form == FRICO_SOLVE_C64_IR && (rc = c->LU32.ensure((size_t)(N * N))))
+        return rc;
+    c->factored = -1; // the factors are about to be overwritten
+    c->nb_factored = c->nb;
+    wk = lu_work(c);
+    if (!lookahead) {
+        wk.side = nullptr;
+        wk.events = nullptr;
+        wk.nevents = 0;
+    }
+    // W products for the fused solves: one more N x N matrix of the factor's type, taken only
     // when it fits with room to spare
     c->wv_ok = false;
     if (c->fused_solve && c->nb % 64 == 0 && N > 64) {
@@ -839,39 +881,39 @@ int frico_factor(frico_ctx* c, int form)
             wk.wv = c->wv.p;
         }
     }
+    c->wv_ok = wk.wv != nullptr;
+    return 0;
+}
+
+int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
+{
+    const int64_t N = c->N;
+    int rc;
+    phase_begin(c, c->t_factor);
     if (form == FRICO_SOLVE_C128) {
-        if (c->zprec != FRICO_C128 || c->z64_factored)
-            return set_error("complex128 factorisation needs an unfactored complex128 Z"), FRICO_ERR_STATE;
-        phase_begin(c, c->t_factor);
         rc = lu_factor<double>(wk, c->Z64.p, N, N, c->ipiv.p, c->nb, c->st, c->profiling ? &c->t_gemm : nullptr);
-        phase_end(c, c->t_factor);
-        c->z64_factored = true;
     } else {
-        if ((rc = c->LU32.ensure((size_t)(N * N))))
-            return rc;
-        if (c->zprec == FRICO_C128 && c->z64_factored)
-            return set_error("Z was overwritten by a complex128 factorisation"), FRICO_ERR_STATE;
-        phase_begin(c, c->t_factor);
         if (c->zprec == FRICO_C128)
             rc = launch_convert_c128_c64(c->Z64.p, N, c->LU32.p, N, N, N, c->st);
-        else {
-            const hipError_t e =
-                hipMemcpyAsync(c->LU32.p, c->Z32.p, (size_t)(N * N) * sizeof(c64), hipMemcpyDeviceToDevice, c->st);
-            rc = e == hipSuccess ? 0 : hip_fail(e, "copy Z32", __FILE__, __LINE__);
-        }
+        else
+            rc = launch_copy_bytes(c->Z32.p, c->LU32.p, (size_t)(N * N) * sizeof(c64), c->st);
         if (!rc)
             rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb, c->st, c->profiling ? &c->t_gemm : nullptr);
-        phase_end(c, c->t_factor);
     }
-    if (rc)
-        return rc;
+    phase_end(c, c->t_factor);
+    return rc;
+}
+
+int factor_finish(frico_ctx* c, int form, const LuWork& wk)
+{
+    if (form == FRICO_SOLVE_C128)
+        c->z64_factored = true;
     c->factored = form;
-    c->nb_factored = c->nb;
     c->wv_ok = wk.wv != nullptr;
     return read_lu_flags(c);
 }
 
-static int col_norms(frico_ctx* c, const c128* X, int64_t cols, std::vector<double>& out)
+int col_norms(frico_ctx* c, const c128* X, int64_t cols, std::vector<double>& out)
 {
     int rc;
     if ((rc = c->norms.ensure((size_t)cols)))
@@ -886,6 +928,21 @@ static int col_norms(frico_ctx* c, const c128* X, int64_t cols, std::vector<doub
     return 0;
 }
 
+} // namespace frico
+
+extern "C" {
+
+int frico_factor(frico_ctx* c, int form)
+{
+    if (check_ctx(c))
+        return -1;
+    LuWork wk;
+    int rc = factor_prepare(c, form, wk, true);
+    if (rc || (rc = factor_enqueue(c, form, wk)))
+        return rc;
+    return factor_finish(c, form, wk);
+}
+
 // R = B - Z X with the kept (unfactored) Z.
 static int residual(frico_ctx* c, KernelTimer* kt)
 {
@@ -893,7 +950,8 @@ static int residual(frico_ctx* c, KernelTimer* kt)
     int rc;
     if ((rc = c->R.ensure((size_t)(N * nr))))
         return rc;
-    FRICO_HIP_CHECK(hipMemcpyAsync(c->R.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
+    if ((rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st)))
+        return rc;
     if (c->zprec == FRICO_C128) {
         if (c->z64_factored)
             return set_error("residual needs the unfactored Z"), FRICO_ERR_STATE;
@@ -901,6 +959,65 @@ static int residual(frico_ctx* c, KernelTimer* kt)
     }
     return gemm_sub_mixed(N, nr, N, c->Z32.p, N, c->X.p, N, c->R.p, N, c->st, kt);
 }
+
+} // extern "C"
+
+namespace frico {
+
+int refine_prepare(frico_ctx* c)
+{
+    const int64_t N = c->N, nr = c->nrhs;
+    int rc;
+    if ((rc = c->R.ensure((size_t)(N * nr))) || (rc = c->D32.ensure((size_t)(N * nr))) ||
+        (rc = c->norms.ensure((size_t)nr)))
+        return rc;
+    if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c64))))
+        return rc;
+    return 0;
+}
+
+int refine_enqueue(frico_ctx* c, bool first)
+{
+    const int64_t N = c->N, nr = c->nrhs;
+    int rc;
+    KernelTimer& tt = first ? c->t_solve : c->t_refine;
+    phase_begin(c, tt);
+    if (first && ((rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)) ||
+                  (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st))))
+        return rc;
+    if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
+        return rc;
+    if ((rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
+                              c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
+        return rc;
+    if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
+        return rc;
+    phase_end(c, tt);
+    phase_begin(c, c->t_refine);
+    if ((rc = residual(c, nullptr)))
+        return rc;
+    phase_end(c, c->t_refine);
+    return launch_col_norm2(c->R.p, N, N, nr, c->norms.p, c->st);
+}
+
+int read_refine_norms(frico_ctx* c, const std::vector<double>& bn, double* worst)
+{
+    const int64_t nr = c->nrhs;
+    std::vector<double> rn((size_t)nr);
+    FRICO_HIP_CHECK(hipMemcpyAsync(rn.data(), c->norms.p, (size_t)nr * sizeof(double), hipMemcpyDeviceToHost, c->st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
+    double w = 0;
+    for (int64_t j = 0; j < nr; j++) {
+        const double r = std::sqrt(rn[j]);
+        w = std::max(w, bn[j] > 0 ? r / bn[j] : r);
+    }
+    *worst = w;
+    return 0;
+}
+
+} // namespace frico
+
+extern "C" {
 
 int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
 {
@@ -917,7 +1034,8 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     frico_solve_info inf{0, 1, -1.0};
     if (c->factored == FRICO_SOLVE_C128) {
         phase_begin(c, c->t_solve);
-        FRICO_HIP_CHECK(hipMemcpyAsync(c->X.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
+        if ((rc = launch_copy_bytes(c->B.p, c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)))
+            return rc;
         if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c128))))
             return rc;
         rc = lu_solve<double>(c->Z64.p, N, N, c->perm.p, c->nb_factored, c->X.p, N, nr, c->st, c->dinv.p,
@@ -933,39 +1051,16 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     }
     if (maxit < 1)
         return set_error("maxit must be >= 1"), -3;
-    if ((rc = c->R.ensure((size_t)(N * nr))) || (rc = c->D32.ensure((size_t)(N * nr))))
+    if ((rc = refine_prepare(c)))
         return rc;
-    if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c64))))
-        return rc;
-    std::vector<double> bn, rn;
+    std::vector<double> bn;
     if ((rc = col_norms(c, c->B.p, nr, bn)))
         return rc;
-    // X = 0, R = B
-    FRICO_HIP_CHECK(hipMemsetAsync(c->X.p, 0, (size_t)(N * nr) * sizeof(c128), c->st));
-    FRICO_HIP_CHECK(hipMemcpyAsync(c->R.p, c->B.p, (size_t)(N * nr) * sizeof(c128), hipMemcpyDeviceToDevice, c->st));
     inf.converged = 0;
     for (int it = 1; it <= maxit; it++) {
-        KernelTimer& tt = it == 1 ? c->t_solve : c->t_refine;
-        phase_begin(c, tt);
-        if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
-            return rc;
-        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
-                                  c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
-            return rc;
-        if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
-            return rc;
-        phase_end(c, tt);
-        phase_begin(c, c->t_refine);
-        if ((rc = residual(c, nullptr)))
-            return rc;
-        phase_end(c, c->t_refine);
-        if ((rc = col_norms(c, c->R.p, nr, rn)))
-            return rc;
         double worst = 0;
-        for (int64_t j = 0; j < nr; j++) {
-            const double rel = bn[j] > 0 ? rn[j] / bn[j] : rn[j];
-            worst = std::max(worst, rel);
-        }
+        if ((rc = refine_enqueue(c, it == 1)) || (rc = read_refine_norms(c, bn, &worst)))
+            return rc;
         inf.iterations = it;
         inf.max_rel_residual = worst;
         if (worst <= tol) {

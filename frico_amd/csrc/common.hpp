@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 namespace frico {
@@ -65,10 +67,57 @@ namespace frico {
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 void set_error(const char* fmt, ...);
 // Raises a kernel's dynamic-LDS limit to `bytes` (gfx950 CUs have 160 KB, shared with static LDS).
-inline hipError_t allow_lds(const void* func, size_t bytes)
+// Each (device, kernel) is set once, so that later launches, including launches under HIP graph
+// capture, make no attribute call.
+hipError_t allow_lds(const void* func, size_t bytes);
+
+// Graph building without stream capture.  While a recorder is installed on the calling thread,
+// every launch_kernel on the recorder's stream becomes a kernel node of recorder.graph, each node
+// depending on the previous one, instead of running.  Recording is per thread and touches no
+// stream state, so other threads keep launching and allocating freely.  Only kernel launches are
+// recorded: a sequence recorded this way must not contain copies, memsets or event operations.
+struct GraphRecorder {
+    hipStream_t st = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphNode_t last = nullptr;
+    hipError_t err = hipSuccess;
+};
+GraphRecorder*& graph_recorder(); // this thread's recorder, or null
+
+template <typename... P, typename... A, size_t... I>
+inline hipError_t launch_kernel_impl(void (*k)(P...), dim3 grid, dim3 block, size_t sh, hipStream_t st,
+                                     std::index_sequence<I...>, A&&... a)
 {
-    return hipFuncSetAttribute(func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    std::tuple<std::decay_t<P>...> vals(static_cast<std::decay_t<P>>(std::forward<A>(a))...);
+    void* argv[sizeof...(P) > 0 ? sizeof...(P) : 1] = {(void*)&std::get<I>(vals)...};
+    GraphRecorder* r = graph_recorder();
+    if (!r || r->st != st)
+        return hipLaunchKernel(reinterpret_cast<const void*>(k), grid, block, argv, sh, st);
+    hipKernelNodeParams p{};
+    p.func = reinterpret_cast<void*>(k);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = (unsigned)sh;
+    p.kernelParams = argv;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    const hipError_t e = hipGraphAddKernelNode(&node, r->graph, r->last ? &r->last : nullptr, r->last ? 1 : 0, &p);
+    if (e == hipSuccess)
+        r->last = node;
+    else if (r->err == hipSuccess)
+        r->err = e;
+    return e;
 }
+
+// Launches kernel k on st, or appends it to this thread's graph recorder for st.  A failed launch
+// is reported by the next hipGetLastError like hipLaunchKernelGGL's.
+template <typename... P, typename... A>
+inline void launch_kernel(void (*k)(P...), dim3 grid, dim3 block, size_t sh, hipStream_t st, A&&... a)
+{
+    static_assert(sizeof...(P) == sizeof...(A), "kernel argument count");
+    (void)launch_kernel_impl(k, grid, block, sh, st, std::index_sequence_for<P...>{}, std::forward<A>(a)...);
+}
+
 } // namespace frico
 
 namespace frico {

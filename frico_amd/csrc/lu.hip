@@ -376,7 +376,7 @@ static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, con
 {
     if (ncols <= 0)
         return;
-    hipLaunchKernelGGL((laswp<T>), dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
+    launch_kernel(&laswp<T>, dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
                        col0, ncols, perm);
 }
 
@@ -820,7 +820,7 @@ static void launch_trsm(int w, const cx<T>* D, int64_t ldd, const cx<T>* Bin, in
 {
     if (ncols <= 0)
         return;
-    hipLaunchKernelGGL((trsm_block<T, UPPER, TRANSD, UNIT>), dim3((unsigned)((ncols + TRSM_COLS - 1) / TRSM_COLS)),
+    launch_kernel(&trsm_block<T, UPPER, TRANSD, UNIT>, dim3((unsigned)((ncols + TRSM_COLS - 1) / TRSM_COLS)),
                        dim3(256), 0, st, w, D, ldd, Bin, ldi, Bout, ldo, ncols);
 }
 
@@ -922,6 +922,34 @@ __global__ void zero_u32(unsigned* p, int64_t n)
         p[i] = 0;
 }
 
+__global__ void copy_u32(const unsigned* __restrict__ src, unsigned* __restrict__ dst, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        dst[i] = src[i];
+}
+
+// Clears / copies device memory with kernels rather than memset / memcpy nodes, so that the
+// sequences that use them can be captured into HIP graphs.  Sizes are multiples of 4 bytes.
+int launch_zero_bytes(void* p, size_t bytes, hipStream_t st)
+{
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n > 0)
+        launch_kernel(&zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (unsigned*)p, n);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st)
+{
+    const int64_t n = (int64_t)(bytes / 4);
+    if (n > 0)
+        launch_kernel(&copy_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, (const unsigned*)src,
+                           (unsigned*)dst, n);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 // ----------------------------------------------------------------------------------------------
 // Elementwise helpers for mixed precision and refinement.
 // ----------------------------------------------------------------------------------------------
@@ -985,7 +1013,7 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
         kt->begin(st);
-    hipLaunchKernelGGL((cgemm_sub<T, T>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+    launch_kernel(&cgemm_sub<T, T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K);
     FRICO_HIP_CHECK(hipGetLastError());
@@ -1002,7 +1030,7 @@ int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, c
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
         kt->begin(st);
-    hipLaunchKernelGGL((cgemm_sub<double, float>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B,
+    launch_kernel(&cgemm_sub<double, float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B,
                        ldb, C, ldc);
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K);
@@ -1027,7 +1055,7 @@ int launch_symmetrize(c128* Z, int64_t n, int64_t ldz, hipStream_t st)
 {
     if (n > 65535)
         return set_error("symmetrize: n too large"), -1;
-    hipLaunchKernelGGL(symmetrize_k, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, st, Z, n, ldz);
+    launch_kernel(&symmetrize_k, dim3((unsigned)((n + 255) / 256), (unsigned)n), dim3(256), 0, st, Z, n, ldz);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -1088,8 +1116,8 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     if (!wk.perm || wk.nperm < npan)
         return set_error("lu: interchange-list workspace too small"), -1;
     const int64_t nz = 2 * (int64_t)ll_words(gmax);
-    hipLaunchKernelGGL(zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.ll, nz);
-    hipLaunchKernelGGL(zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
+    launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.ll, nz);
+    launch_kernel(&zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
     FRICO_HIP_CHECK(hipGetLastError());
     PanelSync ps{};
     ps.words = wk.ll;
@@ -1104,7 +1132,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         if (G > gmax)
             return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), FRICO_ERR_LIMIT;
         const size_t sh = ((size_t)w * R + 2 * w) * sizeof(cx<T>);
-        hipLaunchKernelGGL((panel_factor<T>), dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
+        launch_kernel(&panel_factor<T>, dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
                            wk.perm + p * PERM_STRIDE);
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
@@ -1154,10 +1182,10 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     }
     if (wk.dinv) {
         const int64_t nch = (n + 63) / 64;
-        hipLaunchKernelGGL((diag_inverse<T>), dim3((unsigned)nch), dim3(256), 0, st, A, n, lda, (cx<T>*)wk.dinv);
+        launch_kernel(&diag_inverse<T>, dim3((unsigned)nch), dim3(256), 0, st, A, n, lda, (cx<T>*)wk.dinv);
         FRICO_HIP_CHECK(hipGetLastError());
         if (wk.wv && nb % 64 == 0 && nch > 1) {
-            hipLaunchKernelGGL((make_w<T>), dim3((unsigned)nch, (unsigned)nch), dim3(256), 0, st, n, A, lda,
+            launch_kernel(&make_w<T>, dim3((unsigned)nch, (unsigned)nch), dim3(256), 0, st, n, A, lda,
                                (const cx<T>*)wk.dinv, (cx<T>*)wk.wv, n);
             FRICO_HIP_CHECK(hipGetLastError());
         }
@@ -1182,7 +1210,7 @@ static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* per
         for (int s = 0; s < w; s += 64) {
             const int64_t kc = k + s, ws = (n - kc) < 64 ? (n - kc) : 64, ch = kc / 64;
             const unsigned gx = (unsigned)((n - kc + GBM - 1) / GBM);
-            hipLaunchKernelGGL((solve_chunk<T>), dim3(gx, gy), dim3(256), 0, st, kc, n - kc, nrhs, ws, (int64_t)0,
+            launch_kernel(&solve_chunk<T>, dim3(gx, gy), dim3(256), 0, st, kc, n - kc, nrhs, ws, (int64_t)0,
                                inv + (size_t)(2 * ch) * 4096, WV + kc * n, n, B + kc, ldb, B, ldb, Y, n);
         }
     }
@@ -1219,7 +1247,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx
             // the diagonal sub-blocks of a chunk inverse are the inverses of the chunk's diagonal sub-blocks
             const int64_t ch = (k + s) / 64, off = (k + s) % 64;
             if (inv)
-                hipLaunchKernelGGL((trmm_block<T>), dim3(gcols), dim3(256), 0, st, ws,
+                launch_kernel(&trmm_block<T>, dim3(gcols), dim3(256), 0, st, ws,
                                    inv + (size_t)(2 * ch) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
             else
                 launch_trsm<T, false>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
@@ -1239,7 +1267,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx
             const int ws = (w - s) < 64 ? (w - s) : 64;
             const int64_t ch = (k + s) / 64, off = (k + s) % 64;
             if (inv)
-                hipLaunchKernelGGL((trmm_block<T>), dim3(gcols), dim3(256), 0, st, ws,
+                launch_kernel(&trmm_block<T>, dim3(gcols), dim3(256), 0, st, ws,
                                    inv + (size_t)(2 * ch + 1) * 4096 + off + off * 64, B + (k + s), ldb, nrhs);
             else
                 launch_trsm<T, true>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
@@ -1274,7 +1302,7 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
             const int64_t above = k + s;
             if (above > 0) {
                 const int64_t gx = (ws + GBM - 1) / GBM, gy = (nrhs + GBN - 1) / GBN;
-                hipLaunchKernelGGL((cgemm_sub<T, T, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
+                launch_kernel(&cgemm_sub<T, T, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
                                    nrhs, above, A + (k + s) * lda, lda, B, ldb, B + (k + s), ldb);
             }
             launch_trsm<T, false, true, false>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
@@ -1288,13 +1316,13 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
             const int64_t below = n - (k + s + ws);
             if (below > 0) {
                 const int64_t gx = (ws + GBM - 1) / GBM, gy = (nrhs + GBN - 1) / GBN;
-                hipLaunchKernelGGL((cgemm_sub<T, T, true>), dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
+                launch_kernel(&cgemm_sub<T, T, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
                                    nrhs, below, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s + ws), ldb, B + (k + s),
                                    ldb);
             }
             launch_trsm<T, true, true, true>(ws, A + (k + s) + (k + s) * lda, lda, B + (k + s), ldb, B + (k + s), ldb, nrhs, st);
         }
-        hipLaunchKernelGGL((laswp_reverse<T>), dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0,
+        launch_kernel(&laswp_reverse<T>, dim3((unsigned)((nrhs + 255) / 256)), dim3(256), 0, st, B, ldb, (int64_t)0,
                            nrhs, k, w, ipiv);
     }
     FRICO_HIP_CHECK(hipGetLastError());
@@ -1305,7 +1333,7 @@ int launch_convert_c128_c64(const c128* X, int64_t ldx, c64* Y, int64_t ldy, int
 {
     for (int64_t c0 = 0; c0 < cols; c0 += 65535) {
         const int64_t nc = (cols - c0) < 65535 ? (cols - c0) : 65535;
-        hipLaunchKernelGGL((convert_mat<double, float>), dim3((unsigned)((rows + 255) / 256), (unsigned)nc), dim3(256), 0, st,
+        launch_kernel(&convert_mat<double, float>, dim3((unsigned)((rows + 255) / 256), (unsigned)nc), dim3(256), 0, st,
                            X + c0 * ldx, ldx, Y + c0 * ldy, ldy, rows);
     }
     FRICO_HIP_CHECK(hipGetLastError());
@@ -1314,7 +1342,7 @@ int launch_convert_c128_c64(const c128* X, int64_t ldx, c64* Y, int64_t ldy, int
 
 int launch_add_correction(c128* X, int64_t ldx, const c64* D, int64_t ldd, int64_t rows, int64_t cols, hipStream_t st)
 {
-    hipLaunchKernelGGL(add_correction, dim3((unsigned)((rows + 255) / 256), (unsigned)cols), dim3(256), 0, st, X, ldx, D,
+    launch_kernel(&add_correction, dim3((unsigned)((rows + 255) / 256), (unsigned)cols), dim3(256), 0, st, X, ldx, D,
                        ldd, rows);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
@@ -1322,7 +1350,7 @@ int launch_add_correction(c128* X, int64_t ldx, const c64* D, int64_t ldd, int64
 
 int launch_col_norm2(const c128* X, int64_t ldx, int64_t rows, int64_t cols, double* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(col_norm2, dim3((unsigned)cols), dim3(256), 0, st, X, ldx, rows, out);
+    launch_kernel(&col_norm2, dim3((unsigned)cols), dim3(256), 0, st, X, ldx, rows, out);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }

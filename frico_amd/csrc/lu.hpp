@@ -38,6 +38,8 @@ int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, c
 template <typename T> int panel_rows(int w);
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
+int launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
+int launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st);
 // ints per panel of LuWork::perm
 int lu_perm_stride();
 // 64-bit words of LuWork::ll for gmax workgroups

@@ -196,6 +196,40 @@ int frico_adjoint_forward(frico_ctx* ctx, double freq, const int32_t* ifaces, in
 int frico_adjoint_backward(frico_ctx* ctx, double g_zin_re, double g_zin_im, double g_gain,
                            double* grad);
 
+/*
+ * Batched uneven antenna workloads (the reference runs one mesh and one frequency at a time:
+ * do_sweep, src/emw_ana_antenna.cpp:236-295, over init_sweep's frequencies,
+ * src/emw_solver.cpp:460-471).  A batch owns `workers` contexts on one device; meshes are added
+ * once and then swept together.
+ * frico_batch_add_mesh: mesh arrays as for frico_mesh_load plus the feed interface tags; returns
+ *   the mesh index (>= 0) or an error.
+ * frico_batch_antenna: for every mesh and every freqs[f], quadrature-mode fill (fill_prec
+ *   FRICO_C128 or FRICO_C64), delta-gap excitation of v on the feed (oriented once per mesh,
+ *   src/emw_ana_antenna.cpp:238), factor + solve in `form`, and the input impedance
+ *   zin[2*(m*nfreq + f) ..] = V / I_port (compute_port_values, src/emw_ana_antenna.cpp:76-105);
+ *   info[m*nfreq + f] is the solve status.  Meshes run largest first on up to `workers` concurrent
+ *   contexts (fewer when ceil(Nmax / R) panel workgroups per context would exceed the CUs).
+ *   FRICO_BATCH_GRAPHS replays a captured HIP graph per mesh for the complex64 LU and each
+ *   refinement sweep; FRICO_BATCH_AUTOTUNE picks the LU block width per size class (N / 256) by
+ *   timing the candidates on the first system of that class.
+ * frico_batch_block_sizes: nb[m] = block width used for mesh m by the last frico_batch_antenna;
+ *   returns the number of meshes whose frequencies after the first ran as graph replays.
+ */
+#define FRICO_BATCH_GRAPHS 1
+#define FRICO_BATCH_AUTOTUNE 2
+typedef struct frico_batch frico_batch;
+int frico_batch_create(int device, int workers, frico_batch** out);
+int frico_batch_destroy(frico_batch* batch);
+int frico_batch_add_mesh(frico_batch* batch, const double* vertices, int64_t nv,
+                         const int64_t* triangles, const int32_t* tri_tags, int64_t nt,
+                         const int64_t* iface_edges, const int32_t* iface_tags, int64_t nif,
+                         int degree, const int32_t* feed, int nfeed);
+int frico_batch_num_meshes(const frico_batch* batch);
+int frico_batch_antenna(frico_batch* batch, const double* freqs, int nfreq, int fill_prec, int form,
+                        double tol, int maxit, int flags, double v_re, double v_im, double* zin,
+                        frico_solve_info* info);
+int frico_batch_block_sizes(const frico_batch* batch, int32_t* nb);
+
 #ifdef __cplusplus
 }
 #endif
