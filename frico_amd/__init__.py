@@ -366,6 +366,10 @@ class Batch:
                            infos[m * f.size + i].max_rel_residual) for i in range(f.size)] for m in range(nm)]
         return z[..., 0] + 1j * z[..., 1], info
 
+    def workers_used(self) -> int:
+        """Concurrent contexts of the last run (capped by GPU_MAX_HW_QUEUES and co-residency)."""
+        return int(self._L.frico_batch_workers_used(self._h))
+
     def block_sizes(self):
         """(block width per mesh, number of meshes that replayed graphs) of the last run."""
         nb = np.zeros(len(self.meshes), np.int32)

@@ -93,6 +93,7 @@ SIGNATURES = {
     "frico_batch_add_mesh": (C.c_int, [_vp, _d, C.c_int64, _i64, _i32, C.c_int64, _i64, _i32, C.c_int64, C.c_int,
                                        _i32, C.c_int]),
     "frico_batch_num_meshes": (C.c_int, [_vp]),
+    "frico_batch_workers_used": (C.c_int, [_vp]),
     "frico_batch_antenna": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
                                       C.c_double, _d, C.POINTER(frico_solve_info)]),
     "frico_batch_block_sizes": (C.c_int, [_vp, _i32]),

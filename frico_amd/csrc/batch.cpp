@@ -13,13 +13,15 @@
 //     sweep is a second graph.  The graphs are assembled node by node on the worker's thread
 //     (no stream capture), so the other workers' launches and allocations cannot disturb them.
 // Co-residency: a panel launch needs ceil(N / R) workgroups resident at once (one per CU), so the
-// worker count is capped at CUs / ceil(Nmax / R).
+// worker count is capped at CUs / ceil(Nmax / R); it is also capped at the process's hardware
+// queues (GPU_MAX_HW_QUEUES, 4 by default), beyond which streams serialise behind each other.
 #include "ctx.hpp"
 #include "solver.hpp"
 
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdlib>
 #include <complex>
 #include <map>
 #include <mutex>
@@ -45,6 +47,7 @@ struct frico_batch {
     std::vector<Mesh> meshes;
     std::vector<int32_t> mesh_nb; // block width used for each mesh by the last run
     std::atomic<int> graph_units{0}; // units of the last run that replayed graphs
+    int workers_used = 0;            // concurrent contexts of the last run
     std::mutex mu;
     std::map<int64_t, int> tuned; // size class (N / 256) -> block width
 };
@@ -333,6 +336,8 @@ int frico_batch_add_mesh(frico_batch* b, const double* vertices, int64_t nv, con
 
 int frico_batch_num_meshes(const frico_batch* b) { return b ? (int)b->meshes.size() : -1; }
 
+int frico_batch_workers_used(const frico_batch* b) { return b ? b->workers_used : -1; }
+
 int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill_prec, int form, double tol, int maxit,
                         int flags, double v_re, double v_im, double* zin, frico_solve_info* info)
 {
@@ -364,7 +369,13 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
     int64_t nmax = b->meshes[order[0]].N;
     const int R = form == FRICO_SOLVE_C128 ? panel_rows<double>(64) : panel_rows<float>(64);
     const int64_t gmax = std::max<int64_t>(1, (nmax + R - 1) / R);
-    const int W = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)b->ctx.size(), b->cus / gmax));
+    // streams beyond the process's hardware queues share queues and serialise behind each other
+    int hwq = 4; // HIP's default
+    if (const char* q = std::getenv("GPU_MAX_HW_QUEUES"))
+        hwq = std::max(1, std::atoi(q));
+    const int W = (int)std::max<int64_t>(
+        1, std::min<int64_t>(std::min<int64_t>((int64_t)b->ctx.size(), hwq), b->cus / gmax));
+    b->workers_used = W;
     b->mesh_nb.assign((size_t)nm, 0);
     b->graph_units = 0;
     for (int i = 0; i < nm * nfreq; i++)

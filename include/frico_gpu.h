@@ -225,6 +225,9 @@ int frico_batch_add_mesh(frico_batch* batch, const double* vertices, int64_t nv,
                          const int64_t* iface_edges, const int32_t* iface_tags, int64_t nif,
                          int degree, const int32_t* feed, int nfeed);
 int frico_batch_num_meshes(const frico_batch* batch);
+/* Concurrent contexts used by the last frico_batch_antenna: min(workers, GPU_MAX_HW_QUEUES
+ * (default 4), CUs / ceil(Nmax / R)). */
+int frico_batch_workers_used(const frico_batch* batch);
 int frico_batch_antenna(frico_batch* batch, const double* freqs, int nfreq, int fill_prec, int form,
                         double tol, int maxit, int flags, double v_re, double v_im, double* zin,
                         frico_solve_info* info);
