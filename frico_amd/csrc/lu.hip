@@ -80,7 +80,12 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
         }
 
     constexpr int LA = GBM * GBK / 256, LB = GBK * GBN / 256;
-    cx<T> ra[LA], rb[LB];
+    // Raw loaded values and their in-range masks; the mask is applied when the values go to LDS, so
+    // that the prefetch of the next step is not waited for before this step's products.  Loads are
+    // unconditional (out-of-range lanes read element 0), so a step's loads are in flight together.
+    cx<TA> ra[LA];
+    cx<T> rb[LB];
+    bool oka[LA], okb[LB];
     auto gload = [&](int64_t k0) {
 #pragma unroll
         for (int s = 0; s < LA; s++) {
@@ -88,37 +93,37 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
             // consecutive threads walk the contiguous dimension of A in memory
             const int r = TRANSA ? e / GBK : e % GBM, kk = TRANSA ? e % GBK : e / GBM;
             const int64_t gk = k0 + kk;
-            if (r < mr && gk < K) {
-                const cx<TA> v = TRANSA ? A[gk + r * lda] : A[r + gk * lda];
-                ra[s] = cx<T>{(T)v.re, (T)v.im};
-            } else {
-                ra[s] = cx<T>{0, 0};
-            }
+            const int64_t ok = r < mr && gk < K;
+            oka[s] = ok;
+            ra[s] = A[ok * (TRANSA ? gk + r * lda : r + gk * lda)];
         }
 #pragma unroll
         for (int s = 0; s < LB; s++) {
             const int e = tid + 256 * s;
             const int kk = e % GBK, c = e / GBK;
             const int64_t gk = k0 + kk;
+            const int64_t ok = gk < K && c < nc;
+            okb[s] = ok;
             if (BLDS)
-                rb[s] = (gk < K && c < nc) ? Bs[gk][c] : cx<T>{0, 0};
+                rb[s] = ok ? Bs[gk][c] : cx<T>{0, 0};
             else
-                rb[s] = (gk < K && c < nc) ? B[gk + c * ldb] : cx<T>{0, 0};
+                rb[s] = B[ok * (gk + c * ldb)];
         }
     };
     auto sstore = [&]() {
 #pragma unroll
         for (int s = 0; s < LA; s++) {
             const int e = tid + 256 * s;
+            const cx<T> v = oka[s] ? cx<T>{(T)ra[s].re, (T)ra[s].im} : cx<T>{0, 0};
             if (TRANSA)
-                sm.a[e % GBK][e / GBK] = ra[s];
+                sm.a[e % GBK][e / GBK] = v;
             else
-                sm.a[e / GBM][e % GBM] = ra[s];
+                sm.a[e / GBM][e % GBM] = v;
         }
 #pragma unroll
         for (int s = 0; s < LB; s++) {
             const int e = tid + 256 * s;
-            sm.b[e % GBK][e / GBK] = rb[s];
+            sm.b[e % GBK][e / GBK] = okb[s] ? rb[s] : cx<T>{0, 0};
         }
     };
 
@@ -152,10 +157,35 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
     }
 }
 
-// C_t[0:mr, 0:nc] -= acc (assign = false) or C_t = acc (assign = true); C_t(r, c) = C[r + c ldc].
+// The output tile's current values, read before the product so that their latency overlaps it.
+template <typename T> struct GemmC {
+    cx<T> v[GWM][GWN][4];
+};
+
+template <typename T>
+__device__ inline void gemm_cload(int64_t mr, int64_t nc, const cx<T>* __restrict__ C, int64_t ldc, GemmC<T>& cv)
+{
+    using MF = Mfma<T>;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv & 1, wn = wv >> 1;
+#pragma unroll
+    for (int a = 0; a < GWM; a++)
+#pragma unroll
+        for (int b = 0; b < GWN; b++) {
+            const int row = wm * GWM * 16 + a * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int col = wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
+                const int64_t ok = row < mr && col < nc;
+                cv.v[a][b][r] = C[ok * (row + (int64_t)col * ldc)];
+            }
+        }
+}
+
+// C_t[0:mr, 0:nc] = cv - acc (cv non-null) or C_t = acc (cv null); C_t(r, c) = C[r + c ldc].
 template <typename T>
 __device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C, int64_t ldc, const GemmAcc<T>& acc,
-                                  bool assign)
+                                  const GemmC<T>* cv)
 {
     using MF = Mfma<T>;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -172,10 +202,10 @@ __device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C,
                 const int col = wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
                 if (col < nc) {
                     cx<T>* p = C + row + (int64_t)col * ldc;
-                    if (assign) {
+                    if (!cv) {
                         *p = cx<T>{acc.re[a][b][r], acc.im[a][b][r]};
                     } else {
-                        cx<T> v = *p;
+                        cx<T> v = cv->v[a][b][r];
                         v.re -= acc.re[a][b][r];
                         v.im -= acc.im[a][b][r];
                         *p = v;
@@ -194,8 +224,10 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
     __shared__ GemmSmem<T> sm;
     const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
     GemmAcc<T> acc;
+    GemmC<T> cv;
+    gemm_cload<T>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, cv);
     gemm_tile<T, TA, TRANSA>(M - m0, N - n0, K, TRANSA ? A + m0 * lda : A + m0, lda, B + n0 * ldb, ldb, acc, sm);
-    gemm_store<T>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, false);
+    gemm_store<T>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, &cv);
 }
 
 // W blocks for the forward solves: for row tile t > chunk j (64 x 64 each)
@@ -215,7 +247,7 @@ __global__ __launch_bounds__(256) void make_w(int64_t n, const cx<T>* __restrict
     const int64_t w = (n - k) < 64 ? (n - k) : 64;
     GemmAcc<T> acc;
     gemm_tile<T, T, false>(n - m0, w, w, A + m0 + k * lda, lda, inv + (size_t)(2 * j) * 4096, 64, acc, sm);
-    gemm_store<T>(n - m0, w, WV + m0 + k * ldw, ldw, acc, true);
+    gemm_store<T>(n - m0, w, WV + m0 + k * ldw, ldw, acc, nullptr);
 }
 
 // One chunk of a blocked triangular solve as a single product.  Rows [r0, r0 + M) in 64-row tiles;
@@ -234,11 +266,14 @@ __global__ __launch_bounds__(256) void solve_chunk(int64_t r0, int64_t M, int64_
     const bool own = bx == ct;
     const cx<T>* At = own ? inv : WVc + r0 + m0;
     GemmAcc<T> acc;
+    GemmC<T> cv;
+    if (!own)
+        gemm_cload<T>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, cv);
     gemm_tile<T, T, false>(M - m0, nrhs - n0, K, At, own ? 64 : ldw, Y + n0 * ldy, ldy, acc, sm);
     if (own)
-        gemm_store<T>(M - m0, nrhs - n0, Xo + r0 + m0 + n0 * ldx, ldx, acc, true);
+        gemm_store<T>(M - m0, nrhs - n0, Xo + r0 + m0 + n0 * ldx, ldx, acc, nullptr);
     else
-        gemm_store<T>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, acc, false);
+        gemm_store<T>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, acc, &cv);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -741,6 +776,7 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
                                                   int64_t ldi, cx<T>* Bout, int64_t ldo, int64_t ncols)
 {
     __shared__ cx<T> sD[64][65];
+    __shared__ cx<T> rdiag[64]; // 1 / D[r][r], computed once so the row steps only multiply
     __shared__ cx<T> xs[2][TRSM_COLS];
     const int tid = threadIdx.x, rg = tid & 15, col = tid >> 4;
     const int64_t c0 = (int64_t)blockIdx.x * TRSM_COLS;
@@ -757,6 +793,11 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
         acc[i] = (r < w && live) ? Bin[r + (c0 + col) * ldi] : cx<T>{0, 0};
     }
     __syncthreads();
+    if (!UNIT) {
+        if (tid < 64)
+            rdiag[tid] = cdiv(cx<T>{1, 0}, sD[tid][tid]);
+        __syncthreads();
+    }
     if (!UPPER) {
 #pragma clang loop unroll(full)
         for (int i = 0; i < 4; i++) {
@@ -766,7 +807,7 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
                     break;
                 if (rg == q) {
                     if (!UNIT)
-                        acc[i] = cdiv(acc[i], sD[r][r]);
+                        acc[i] = acc[i] * rdiag[r];
                     xs[r & 1][col] = acc[i];
                 }
                 __syncthreads();
@@ -790,7 +831,7 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
                     continue;
                 if (rg == q) {
                     if (!UNIT)
-                        acc[i] = cdiv(acc[i], sD[r][r]);
+                        acc[i] = acc[i] * rdiag[r];
                     xs[r & 1][col] = acc[i];
                 }
                 __syncthreads();
