@@ -162,9 +162,15 @@ template <typename T> struct GemmC {
     cx<T> v[GWM][GWN][4];
 };
 
+// Complex128 tiles read C in the epilogue instead: holding it through the product would cost the
+// registers of a second wave per SIMD.
+template <typename T> constexpr bool kPrefetchC = sizeof(T) == 4;
+
 template <typename T>
 __device__ inline void gemm_cload(int64_t mr, int64_t nc, const cx<T>* __restrict__ C, int64_t ldc, GemmC<T>& cv)
 {
+    if (!kPrefetchC<T>)
+        return;
     using MF = Mfma<T>;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
@@ -200,17 +206,18 @@ __device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C,
 #pragma unroll
             for (int r = 0; r < 4; r++) {
                 const int col = wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
-                if (col < nc) {
-                    cx<T>* p = C + row + (int64_t)col * ldc;
-                    if (!cv) {
-                        *p = cx<T>{acc.re[a][b][r], acc.im[a][b][r]};
-                    } else {
-                        cx<T> v = cv->v[a][b][r];
-                        v.re -= acc.re[a][b][r];
-                        v.im -= acc.im[a][b][r];
-                        *p = v;
-                    }
+                const int64_t ok = col < nc;
+                cx<T>* p = C + row + (int64_t)col * ldc;
+                cx<T> v;
+                if (cv) {
+                    v = kPrefetchC<T> ? cv->v[a][b][r] : C[ok * (row + (int64_t)col * ldc)];
+                    v.re -= acc.re[a][b][r];
+                    v.im -= acc.im[a][b][r];
+                } else {
+                    v = cx<T>{acc.re[a][b][r], acc.im[a][b][r]};
                 }
+                if (ok)
+                    *p = v;
             }
         }
 }
