@@ -175,7 +175,7 @@ struct frico_ctx {
     hipStream_t st = nullptr;
     hipStream_t side = nullptr;      // look-ahead stream of the LU (default priority)
     std::vector<hipEvent_t> lu_events;
-    bool lookahead = false; // FRICO_LU_LOOKAHEAD=1 enables the look-ahead stream
+    bool lookahead = true; // FRICO_LU_LOOKAHEAD=0 disables the look-ahead stream
     int nb = 64;
     int max_groups = 0;
     bool profiling = false;

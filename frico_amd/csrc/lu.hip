@@ -1108,11 +1108,14 @@ int launch_symmetrize(c128* Z, int64_t n, int64_t ldz, hipStream_t st)
     return 0;
 }
 
-template <typename T> int panel_rows(int w)
+// Rows per panel workgroup: one row per thread (at most 256), slab of R rows x w complex plus two
+// rows in LDS.  With shared_cu the slab stays within ~72 KB, which leaves a CU room for the
+// trailing-update workgroups it already holds, so that the panel runs beside the look-ahead update
+// instead of waiting for CUs to drain; otherwise it may use ~144 KB.
+template <typename T> int panel_rows(int w, bool shared_cu)
 {
-    // one row per thread (at most 256) and the slab of R rows x w complex plus the pivot row in LDS
-    const size_t budget = 144 * 1024;
-    int R = (int)(budget / ((size_t)w * sizeof(cx<T>))) - 1;
+    const size_t budget = (size_t)(shared_cu ? 72 : 144) * 1024;
+    int R = (int)(budget / ((size_t)w * sizeof(cx<T>))) - 2;
     R = R >= 256 ? 256 : (R >= 128 ? 128 : (R >= 64 ? 64 : 32));
     return R;
 }
@@ -1153,8 +1156,11 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
 {
     if (nb > LU_NB_MAX || nb < 1)
         return set_error("lu: block size out of range"), -1;
-    const int R = panel_rows<T>(nb);
     const int64_t npan = (n + nb - 1) / nb;
+    const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
+    int R = panel_rows<T>(nb, lookahead);
+    if ((n + R - 1) / R > wk.max_groups)
+        R = panel_rows<T>(nb, false);
     const size_t sh_max = ((size_t)nb * R + 2 * nb) * sizeof(cx<T>);
     if (sh_max > 64 * 1024)
         FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&panel_factor<T>), sh_max));
@@ -1185,7 +1191,6 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
     };
-    const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
     int rc;
     if (!lookahead) {
         for (int64_t p = 0; p < npan; p++) {
@@ -1419,7 +1424,7 @@ template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*
                                         hipStream_t);
 int lu_perm_stride() { return PERM_STRIDE; }
 size_t lu_exchange_words(int gmax) { return ll_words(gmax); }
-template int panel_rows<double>(int);
-template int panel_rows<float>(int);
+template int panel_rows<double>(int, bool);
+template int panel_rows<float>(int, bool);
 
 } // namespace frico

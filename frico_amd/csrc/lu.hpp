@@ -35,7 +35,7 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
 // Residual GEMM with a complex64-stored A and FP64 arithmetic: C -= double(A) * B.
 int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const c128* B, int64_t ldb, c128* C,
                    int64_t ldc, hipStream_t st, KernelTimer* kt);
-template <typename T> int panel_rows(int w);
+template <typename T> int panel_rows(int w, bool shared_cu = false);
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
 int launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
