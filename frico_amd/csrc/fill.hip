@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double i
     double svr[3] = {0, 0, 0}, svi[3] = {0, 0, 0};
     double suvr = 0, suvi = 0;
     (void)QMAX;
-#pragma unroll(QC ? QC : 1)
+#pragma unroll 1
     for (int p = 0; p < q; p++) {
         const double4 u = sP[tp * q + p];
         const double ex = dx + u.x, ey = dy + u.y, ez = dz + u.z;
@@ -237,7 +237,7 @@ __global__ __launch_bounds__(256) void tripair_f32(DevMesh m, double k, double i
     }
     float s0r = 0, s0i = 0, sur[3] = {0, 0, 0}, sui[3] = {0, 0, 0}, svr[3] = {0, 0, 0},
           svi[3] = {0, 0, 0}, suvr = 0, suvi = 0;
-#pragma unroll(QC ? QC : 1)
+#pragma unroll 1
     for (int p = 0; p < q; p++) {
         const float4 u = sP[tp * q + p];
         float hr = 0, hi = 0, vxr = 0, vxi = 0, vyr = 0, vyi = 0, vzr = 0, vzi = 0;
