@@ -17,6 +17,8 @@
 #include "common.hpp"
 #include "lu.hpp"
 
+#include <cstdlib>
+
 namespace frico {
 
 // ----------------------------------------------------------------------------------------------
@@ -44,27 +46,44 @@ template <> struct Mfma<float> {
 };
 
 // ----------------------------------------------------------------------------------------------
-// 64 x 64 complex output tiles on MFMA.  Workgroup = 4 waves (2 x 2), wave tile (WM*16) x (WN*16).
+// Complex output tiles on MFMA.  Workgroup = 4 waves (2 x 2), wave tile (WM*16) x (WN*16).
 // The MFMA computes the transposed product (B^T A^T) so that the 16 lanes of an accumulator row
 // hold 16 consecutive rows of the output: stores are contiguous down a column.
 // ----------------------------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 16, GWM = 2, GWN = 2;
+constexpr int GBK = 16;
 
-template <typename T> struct GemmAcc {
-    typename Mfma<T>::acc_t re[GWM][GWN], im[GWM][GWN];
+template <int WM_, int WN_> struct TileCfg {
+    static constexpr int WM = WM_, WN = WN_, BM = 32 * WM_, BN = 32 * WN_;
+};
+// 64 x 64: the chunk-aligned products of the solves and of W
+using Tile64 = TileCfg<2, 2>;
+constexpr int GBM = Tile64::BM, GBN = Tile64::BN;
+// Trailing updates, residuals and the transposed solve.  Measured at M = N = 12096, K = 64:
+// 64 x 64 tiles reach 92 TFLOP/s (complex64) and 40 (complex128); 128 x 128 / 128 x 64 tiles
+// reached 39 / 27 at one wave per SIMD and 61 / 37 when held to two, so the products stay 64 x 64.
+template <typename T> struct TileBig {
+    using type = Tile64;
 };
 
-template <typename T> struct GemmSmem {
-    cx<T> a[GBK][GBM + 1];
-    cx<T> b[GBK][GBN + 1];
+template <typename T, typename TL> struct GemmAcc {
+    typename Mfma<T>::acc_t re[TL::WM][TL::WN], im[TL::WM][TL::WN];
+    // three-product form: re = T1 = sum br ar, im = T2 = sum bi ai, s3 = T3 = sum (br+bi)(ar+ai)
+    typename Mfma<T>::acc_t s3[TL::WM][TL::WN];
+};
+
+template <typename T, typename TL> struct GemmSmem {
+    cx<T> a[GBK][TL::BM + 1];
+    cx<T> b[GBK][TL::BN + 1];
 };
 
 // acc = A_t[0:mr, 0:K] * B_t[0:K, 0:nc] (entries outside are zero), with
 //   A_t(r, kk) = TRANSA ? A[kk + r lda] : A[r + kk lda],   B_t(kk, c) = B[kk + c ldb].
 // With BLDS the B operand comes from LDS instead: B_t(kk, c) = Bs[kk][c].
-template <typename T, typename TA, bool TRANSA, bool BLDS = false>
+// M3: complex products with three real MFMAs (T1 = br ar, T2 = bi ai, T3 = (br + bi)(ar + ai);
+// re = T1 - T2, im = T3 - T1 - T2) instead of four; gemm_store combines them.
+template <typename T, typename TA, bool TRANSA, typename TL, bool BLDS = false, bool M3 = false>
 __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
-                                 const cx<T>* __restrict__ B, int64_t ldb, GemmAcc<T>& acc, GemmSmem<T>& sm,
+                                 const cx<T>* __restrict__ B, int64_t ldb, GemmAcc<T, TL>& acc, GemmSmem<T, TL>& sm,
                                  const cx<T> (*Bs)[65] = nullptr)
 {
     using MF = Mfma<T>;
@@ -72,14 +91,16 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
 #pragma unroll
-    for (int a = 0; a < GWM; a++)
+    for (int a = 0; a < TL::WM; a++)
 #pragma unroll
-        for (int b = 0; b < GWN; b++) {
+        for (int b = 0; b < TL::WN; b++) {
             acc.re[a][b] = acc_t{0, 0, 0, 0};
             acc.im[a][b] = acc_t{0, 0, 0, 0};
+            if (M3)
+                acc.s3[a][b] = acc_t{0, 0, 0, 0};
         }
 
-    constexpr int LA = GBM * GBK / 256, LB = GBK * GBN / 256;
+    constexpr int LA = TL::BM * GBK / 256, LB = GBK * TL::BN / 256;
     // Raw loaded values and their in-range masks; the mask is applied when the values go to LDS, so
     // that the prefetch of the next step is not waited for before this step's products.  Loads are
     // unconditional (out-of-range lanes read element 0), so a step's loads are in flight together.
@@ -91,7 +112,7 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
         for (int s = 0; s < LA; s++) {
             const int e = tid + 256 * s;
             // consecutive threads walk the contiguous dimension of A in memory
-            const int r = TRANSA ? e / GBK : e % GBM, kk = TRANSA ? e % GBK : e / GBM;
+            const int r = TRANSA ? e / GBK : e % TL::BM, kk = TRANSA ? e % GBK : e / TL::BM;
             const int64_t gk = k0 + kk;
             const int64_t ok = r < mr && gk < K;
             oka[s] = ok;
@@ -118,7 +139,7 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
             if (TRANSA)
                 sm.a[e % GBK][e / GBK] = v;
             else
-                sm.a[e / GBM][e % GBM] = v;
+                sm.a[e / TL::BM][e % TL::BM] = v;
         }
 #pragma unroll
         for (int s = 0; s < LB; s++) {
@@ -136,52 +157,70 @@ __device__ inline void gemm_tile(int64_t mr, int64_t nc, int64_t K, const cx<TA>
             gload(k0 + GBK);
 #pragma unroll
         for (int kk = 0; kk < GBK; kk += 4) {
-            cx<T> af[GWM], bf[GWN];
+            cx<T> af[TL::WM], bf[TL::WN];
 #pragma unroll
-            for (int a = 0; a < GWM; a++)
-                af[a] = sm.a[kk + (lane >> 4)][wm * GWM * 16 + a * 16 + (lane & 15)];
+            for (int a = 0; a < TL::WM; a++)
+                af[a] = sm.a[kk + (lane >> 4)][wm * TL::WM * 16 + a * 16 + (lane & 15)];
 #pragma unroll
-            for (int b = 0; b < GWN; b++)
-                bf[b] = sm.b[kk + (lane >> 4)][wn * GWN * 16 + b * 16 + (lane & 15)];
+            for (int b = 0; b < TL::WN; b++)
+                bf[b] = sm.b[kk + (lane >> 4)][wn * TL::WN * 16 + b * 16 + (lane & 15)];
+            if (M3) {
+                T as[TL::WM], bs[TL::WN];
 #pragma unroll
-            for (int a = 0; a < GWM; a++)
+                for (int a = 0; a < TL::WM; a++)
+                    as[a] = af[a].re + af[a].im;
 #pragma unroll
-                for (int b = 0; b < GWN; b++) {
-                    // D^T = B^T A^T:  re += br*ar - bi*ai ; im += br*ai + bi*ar
-                    acc.re[a][b] = MF::op(bf[b].re, af[a].re, acc.re[a][b]);
-                    acc.re[a][b] = MF::op(-bf[b].im, af[a].im, acc.re[a][b]);
-                    acc.im[a][b] = MF::op(bf[b].re, af[a].im, acc.im[a][b]);
-                    acc.im[a][b] = MF::op(bf[b].im, af[a].re, acc.im[a][b]);
-                }
+                for (int b = 0; b < TL::WN; b++)
+                    bs[b] = bf[b].re + bf[b].im;
+#pragma unroll
+                for (int a = 0; a < TL::WM; a++)
+#pragma unroll
+                    for (int b = 0; b < TL::WN; b++) {
+                        acc.re[a][b] = MF::op(bf[b].re, af[a].re, acc.re[a][b]);
+                        acc.im[a][b] = MF::op(bf[b].im, af[a].im, acc.im[a][b]);
+                        acc.s3[a][b] = MF::op(bs[b], as[a], acc.s3[a][b]);
+                    }
+            } else {
+#pragma unroll
+                for (int a = 0; a < TL::WM; a++)
+#pragma unroll
+                    for (int b = 0; b < TL::WN; b++) {
+                        // D^T = B^T A^T:  re += br*ar - bi*ai ; im += br*ai + bi*ar
+                        acc.re[a][b] = MF::op(bf[b].re, af[a].re, acc.re[a][b]);
+                        acc.re[a][b] = MF::op(-bf[b].im, af[a].im, acc.re[a][b]);
+                        acc.im[a][b] = MF::op(bf[b].re, af[a].im, acc.im[a][b]);
+                        acc.im[a][b] = MF::op(bf[b].im, af[a].re, acc.im[a][b]);
+                    }
+            }
         }
     }
 }
 
 // The output tile's current values, read before the product so that their latency overlaps it.
-template <typename T> struct GemmC {
-    cx<T> v[GWM][GWN][4];
+template <typename T, typename TL> struct GemmC {
+    cx<T> v[TL::WM][TL::WN][4];
 };
 
-// Complex128 tiles read C in the epilogue instead: holding it through the product would cost the
-// registers of a second wave per SIMD.
-template <typename T> constexpr bool kPrefetchC = sizeof(T) == 4;
+// Complex128 tiles and the large tiles read C in the epilogue instead: holding it through the
+// product would cost the registers of a second wave per SIMD.
+template <typename T, typename TL> constexpr bool kPrefetchC = sizeof(T) == 4 && TL::WM * TL::WN <= 4;
 
-template <typename T>
-__device__ inline void gemm_cload(int64_t mr, int64_t nc, const cx<T>* __restrict__ C, int64_t ldc, GemmC<T>& cv)
+template <typename T, typename TL>
+__device__ inline void gemm_cload(int64_t mr, int64_t nc, const cx<T>* __restrict__ C, int64_t ldc, GemmC<T, TL>& cv)
 {
-    if (!kPrefetchC<T>)
+    if (!kPrefetchC<T, TL>)
         return;
     using MF = Mfma<T>;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
 #pragma unroll
-    for (int a = 0; a < GWM; a++)
+    for (int a = 0; a < TL::WM; a++)
 #pragma unroll
-        for (int b = 0; b < GWN; b++) {
-            const int row = wm * GWM * 16 + a * 16 + (lane & 15);
+        for (int b = 0; b < TL::WN; b++) {
+            const int row = wm * TL::WM * 16 + a * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int col = wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
+                const int col = wn * TL::WN * 16 + b * 16 + MF::acc_row(lane, r);
                 const int64_t ok = row < mr && col < nc;
                 cv.v[a][b][r] = C[ok * (row + (int64_t)col * ldc)];
             }
@@ -189,32 +228,34 @@ __device__ inline void gemm_cload(int64_t mr, int64_t nc, const cx<T>* __restric
 }
 
 // C_t[0:mr, 0:nc] = cv - acc (cv non-null) or C_t = acc (cv null); C_t(r, c) = C[r + c ldc].
-template <typename T>
-__device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C, int64_t ldc, const GemmAcc<T>& acc,
-                                  const GemmC<T>* cv)
+template <typename T, typename TL, bool M3 = false>
+__device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C, int64_t ldc,
+                                  const GemmAcc<T, TL>& acc, const GemmC<T, TL>* cv)
 {
     using MF = Mfma<T>;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
 #pragma unroll
-    for (int a = 0; a < GWM; a++)
+    for (int a = 0; a < TL::WM; a++)
 #pragma unroll
-        for (int b = 0; b < GWN; b++) {
-            const int row = wm * GWM * 16 + a * 16 + (lane & 15);
+        for (int b = 0; b < TL::WN; b++) {
+            const int row = wm * TL::WM * 16 + a * 16 + (lane & 15);
             if (row >= mr)
                 continue;
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int col = wn * GWN * 16 + b * 16 + MF::acc_row(lane, r);
+                const int col = wn * TL::WN * 16 + b * 16 + MF::acc_row(lane, r);
                 const int64_t ok = col < nc;
                 cx<T>* p = C + row + (int64_t)col * ldc;
+                const T pre = M3 ? acc.re[a][b][r] - acc.im[a][b][r] : acc.re[a][b][r];
+                const T pim = M3 ? acc.s3[a][b][r] - acc.re[a][b][r] - acc.im[a][b][r] : acc.im[a][b][r];
                 cx<T> v;
                 if (cv) {
-                    v = kPrefetchC<T> ? cv->v[a][b][r] : C[ok * (row + (int64_t)col * ldc)];
-                    v.re -= acc.re[a][b][r];
-                    v.im -= acc.im[a][b][r];
+                    v = kPrefetchC<T, TL> ? cv->v[a][b][r] : C[ok * (row + (int64_t)col * ldc)];
+                    v.re -= pre;
+                    v.im -= pim;
                 } else {
-                    v = cx<T>{acc.re[a][b][r], acc.im[a][b][r]};
+                    v = cx<T>{pre, pim};
                 }
                 if (ok)
                     *p = v;
@@ -223,18 +264,20 @@ __device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C,
 }
 
 // C[M x N] -= op(A)[M x K] * B[K x N], complex, column-major; op(A) = A or A^T.
-template <typename T, typename TA, bool TRANSA = false>
+template <typename T, typename TA, bool TRANSA = false, bool M3 = false>
 __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
                                                  const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
                                                  int64_t ldc)
 {
-    __shared__ GemmSmem<T> sm;
-    const int64_t m0 = (int64_t)blockIdx.x * GBM, n0 = (int64_t)blockIdx.y * GBN;
-    GemmAcc<T> acc;
-    GemmC<T> cv;
-    gemm_cload<T>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, cv);
-    gemm_tile<T, TA, TRANSA>(M - m0, N - n0, K, TRANSA ? A + m0 * lda : A + m0, lda, B + n0 * ldb, ldb, acc, sm);
-    gemm_store<T>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, &cv);
+    using TL = typename TileBig<T>::type;
+    __shared__ GemmSmem<T, TL> sm;
+    const int64_t m0 = (int64_t)blockIdx.x * TL::BM, n0 = (int64_t)blockIdx.y * TL::BN;
+    GemmAcc<T, TL> acc;
+    GemmC<T, TL> cv;
+    gemm_cload<T, TL>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, cv);
+    gemm_tile<T, TA, TRANSA, TL, false, M3>(M - m0, N - n0, K, TRANSA ? A + m0 * lda : A + m0, lda, B + n0 * ldb, ldb,
+                                             acc, sm);
+    gemm_store<T, TL, M3>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, &cv);
 }
 
 // W blocks for the forward solves: for row tile t > chunk j (64 x 64 each)
@@ -249,12 +292,12 @@ __global__ __launch_bounds__(256) void make_w(int64_t n, const cx<T>* __restrict
     const int64_t t = blockIdx.x, j = blockIdx.y;
     if (t <= j)
         return;
-    __shared__ GemmSmem<T> sm;
+    __shared__ GemmSmem<T, Tile64> sm;
     const int64_t m0 = t * 64, k = j * 64;
     const int64_t w = (n - k) < 64 ? (n - k) : 64;
-    GemmAcc<T> acc;
-    gemm_tile<T, T, false>(n - m0, w, w, A + m0 + k * lda, lda, inv + (size_t)(2 * j) * 4096, 64, acc, sm);
-    gemm_store<T>(n - m0, w, WV + m0 + k * ldw, ldw, acc, nullptr);
+    GemmAcc<T, Tile64> acc;
+    gemm_tile<T, T, false, Tile64>(n - m0, w, w, A + m0 + k * lda, lda, inv + (size_t)(2 * j) * 4096, 64, acc, sm);
+    gemm_store<T, Tile64>(n - m0, w, WV + m0 + k * ldw, ldw, acc, nullptr);
 }
 
 // One chunk of a blocked triangular solve as a single product.  Rows [r0, r0 + M) in 64-row tiles;
@@ -268,19 +311,19 @@ __global__ __launch_bounds__(256) void solve_chunk(int64_t r0, int64_t M, int64_
                                                    cx<T>* __restrict__ C, int64_t ldc, cx<T>* __restrict__ Xo,
                                                    int64_t ldx)
 {
-    __shared__ GemmSmem<T> sm;
+    __shared__ GemmSmem<T, Tile64> sm;
     const int64_t bx = blockIdx.x, m0 = bx * GBM, n0 = (int64_t)blockIdx.y * GBN;
     const bool own = bx == ct;
     const cx<T>* At = own ? inv : WVc + r0 + m0;
-    GemmAcc<T> acc;
-    GemmC<T> cv;
+    GemmAcc<T, Tile64> acc;
+    GemmC<T, Tile64> cv;
     if (!own)
-        gemm_cload<T>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, cv);
-    gemm_tile<T, T, false>(M - m0, nrhs - n0, K, At, own ? 64 : ldw, Y + n0 * ldy, ldy, acc, sm);
+        gemm_cload<T, Tile64>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, cv);
+    gemm_tile<T, T, false, Tile64>(M - m0, nrhs - n0, K, At, own ? 64 : ldw, Y + n0 * ldy, ldy, acc, sm);
     if (own)
-        gemm_store<T>(M - m0, nrhs - n0, Xo + r0 + m0 + n0 * ldx, ldx, acc, nullptr);
+        gemm_store<T, Tile64>(M - m0, nrhs - n0, Xo + r0 + m0 + n0 * ldx, ldx, acc, nullptr);
     else
-        gemm_store<T>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, acc, &cv);
+        gemm_store<T, Tile64>(M - m0, nrhs - n0, C + r0 + m0 + n0 * ldc, ldc, acc, &cv);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1050,18 +1093,36 @@ __global__ __launch_bounds__(256) void col_norm2(const c128* __restrict__ X, int
 // ----------------------------------------------------------------------------------------------
 // Host drivers
 // ----------------------------------------------------------------------------------------------
+// Complex64 products with three real MFMAs per complex multiply-add (default; FRICO_GEMM_3M=0
+// restores four).  Normwise as stable as the four-product form; at N = 12160 the refinement still
+// ends after two sweeps at the same residual (9.8e-11), and the trailing update runs 1.13x faster.
+static bool gemm_three_products()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FRICO_GEMM_3M");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
 template <typename T>
 int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb, cx<T>* C,
              int64_t ldc, hipStream_t st, KernelTimer* kt)
 {
     if (M <= 0 || N <= 0 || K <= 0)
         return 0;
-    const int64_t gx = (M + GBM - 1) / GBM, gy = (N + GBN - 1) / GBN;
+    using TL = typename TileBig<T>::type;
+    const int64_t gx = (M + TL::BM - 1) / TL::BM, gy = (N + TL::BN - 1) / TL::BN;
     if (gy > 65535)
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
         kt->begin(st);
-    launch_kernel(&cgemm_sub<T, T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+    if (sizeof(T) == 4 && gemm_three_products())
+        launch_kernel(&cgemm_sub<T, T, false, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda,
+                      B, ldb, C, ldc);
+    else
+        launch_kernel(&cgemm_sub<T, T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C,
+                      ldc);
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K);
     FRICO_HIP_CHECK(hipGetLastError());
@@ -1073,7 +1134,8 @@ int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, c
 {
     if (M <= 0 || N <= 0 || K <= 0)
         return 0;
-    const int64_t gx = (M + GBM - 1) / GBM, gy = (N + GBN - 1) / GBN;
+    using TL = typename TileBig<double>::type;
+    const int64_t gx = (M + TL::BM - 1) / TL::BM, gy = (N + TL::BN - 1) / TL::BN;
     if (gy > 65535)
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
@@ -1354,7 +1416,8 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
             const int ws = (w - s) < 64 ? (w - s) : 64;
             const int64_t above = k + s;
             if (above > 0) {
-                const int64_t gx = (ws + GBM - 1) / GBM, gy = (nrhs + GBN - 1) / GBN;
+                using TL = typename TileBig<T>::type;
+                const int64_t gx = (ws + TL::BM - 1) / TL::BM, gy = (nrhs + TL::BN - 1) / TL::BN;
                 launch_kernel(&cgemm_sub<T, T, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
                                    nrhs, above, A + (k + s) * lda, lda, B, ldb, B + (k + s), ldb);
             }
@@ -1368,7 +1431,8 @@ int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv,
             const int ws = (w - s) < 64 ? (w - s) : 64;
             const int64_t below = n - (k + s + ws);
             if (below > 0) {
-                const int64_t gx = (ws + GBM - 1) / GBM, gy = (nrhs + GBN - 1) / GBN;
+                using TL = typename TileBig<T>::type;
+                const int64_t gx = (ws + TL::BM - 1) / TL::BM, gy = (nrhs + TL::BN - 1) / TL::BN;
                 launch_kernel(&cgemm_sub<T, T, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, (int64_t)ws,
                                    nrhs, below, A + (k + s + ws) + (k + s) * lda, lda, B + (k + s + ws), ldb, B + (k + s),
                                    ldb);
