@@ -1093,9 +1093,12 @@ __global__ __launch_bounds__(256) void col_norm2(const c128* __restrict__ X, int
 // ----------------------------------------------------------------------------------------------
 // Host drivers
 // ----------------------------------------------------------------------------------------------
-// Complex64 products with three real MFMAs per complex multiply-add (default; FRICO_GEMM_3M=0
-// restores four).  Normwise as stable as the four-product form; at N = 12160 the refinement still
-// ends after two sweeps at the same residual (9.8e-11), and the trailing update runs 1.13x faster.
+// Complex products with three real MFMAs per complex multiply-add instead of four.  Normwise as
+// stable as the four-product form.  Used for complex64 (the LU of the refinement form) unless
+// FRICO_GEMM_3M=0: at N = 12160 the refinement still ends after two sweeps at the same residual
+// (9.8e-11) and the trailing update runs 1.13x faster.  Complex128 keeps four products: its
+// three-product tile needs 286 registers per lane (one wave per SIMD) and ran the K = 64 update
+// at 27 TFLOP/s against 40, and the K = 12160 residual product at the same speed.
 static bool gemm_three_products()
 {
     static const bool on = [] {
@@ -1117,12 +1120,15 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
         kt->begin(st);
-    if (sizeof(T) == 4 && gemm_three_products())
-        launch_kernel(&cgemm_sub<T, T, false, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda,
-                      B, ldb, C, ldc);
-    else
-        launch_kernel(&cgemm_sub<T, T>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C,
-                      ldc);
+    const dim3 grid((unsigned)gx, (unsigned)gy);
+    if constexpr (sizeof(T) == 4) {
+        if (gemm_three_products())
+            launch_kernel(&cgemm_sub<T, T, false, true>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+        else
+            launch_kernel(&cgemm_sub<T, T>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+    } else {
+        launch_kernel(&cgemm_sub<T, T>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+    }
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K);
     FRICO_HIP_CHECK(hipGetLastError());
