@@ -10,7 +10,7 @@
 //   laswp          applies the panel's interchanges to the columns right of the panel.  Columns
 //                  left of the panel are not swapped; the forward solve applies each panel's
 //                  interchanges just before using that panel's L, which is equivalent.
-//   trsm_block     U12 = L11^{-1} A12 (unit lower, in LDS).
+//   trsm_block     U12 = L11^{-1} A12 (unit lower; one wavefront per column group, lane per row).
 //   cgemm_sub      A22 -= L21 U12, complex GEMM on MFMA (v_mfma_f64_16x16x4f64 or
 //                  v_mfma_f32_16x16x4f32), 64x64 complex output tile per workgroup.
 #include "../../include/frico_gpu.h"
@@ -472,6 +472,12 @@ static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, con
 // candidates, picks the same winner (first max of |re|+|im|), reads the winner's row and performs
 // the interchange on its own rows.
 //
+// Column c's rank-1 update is split: each workgroup first updates column c + 1 alone, finds and
+// publishes its candidate for c + 1 (the published rows carry the rest of the update, computed as
+// they are written), and only then updates columns c + 2 .. w - 1, while the other workgroups'
+// candidates travel.  The pivot sequence and the arithmetic of every entry are those of the
+// unsplit right-looking order.
+//
 // The exchange area holds 64-bit words {sequence number (high 32 bits), payload (low 32 bits)},
 // written and read with relaxed agent-scope atomics.  A reader accepts a word only when it carries
 // the current column's sequence number (k + c + 1, increasing over the factorisation), so no
@@ -582,6 +588,14 @@ __device__ inline void wave_argmax(double v, int vi, double& bv, int& bi, int& b
     blane = __ffsll((long long)m) - 1;
 }
 
+// 32-bit word q of a complex value, in memory order (re low word first).
+__device__ inline unsigned cx_word(cx<float> x, int q) { return __float_as_uint(q == 0 ? x.re : x.im); }
+__device__ inline unsigned cx_word(cx<double> x, int q)
+{
+    const long long b = __double_as_longlong(q < 2 ? x.re : x.im);
+    return (q & 1) ? (unsigned)((unsigned long long)b >> 32) : (unsigned)b;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
                                                     int R, int G, int* __restrict__ ipiv, PanelSync ps,
@@ -603,58 +617,88 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
     const int r0 = g * R;                                      // first panel row of this workgroup
     const int nr = (int)((m - r0) < R ? (m - r0) : R);         // rows owned (one per thread)
     cx<T>* Ap = A + k + k * lda;                               // panel origin
-    for (int c = 0; c < w; c++)
-        if (tid < nr)
-            slab[(size_t)c * R + tid] = Ap[(r0 + tid) + (int64_t)c * lda];
+    if (tid < nr) {
+        // sixteen columns of loads in flight per thread
+        const cx<T>* src = Ap + r0 + tid;
+        int c0 = 0;
+        for (; c0 + 16 <= w; c0 += 16) {
+            cx<T> v[16];
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                v[q] = src[(int64_t)(c0 + q) * lda];
+#pragma unroll
+            for (int q = 0; q < 16; q++)
+                slab[(size_t)(c0 + q) * R + tid] = v[q];
+        }
+        for (; c0 < w; c0++)
+            slab[(size_t)c0 * R + tid] = src[(int64_t)c0 * lda];
+    }
     __syncthreads();
 
     const bool own = tid < nr;
     const int grow = r0 + tid; // panel-relative row of this thread
     const int RW = w * CW;     // words of a published row
 
-    for (int c = 0; c < w; c++) {
-        const int par = c & 1;
-        // ---- local pivot candidate among rows >= c
-        double v = -1.0;
-        int vi = 0x7fffffff;
-        if (own && grow >= c) {
-            v = (double)cabs1(slab[(size_t)c * R + tid]);
-            vi = grow;
-        }
-        {
-            double bv;
-            int bi, bl;
-            wave_argmax(v, vi, bv, bi, bl);
-            if (lane == 0) {
-                rv[wv] = bv;
-                ri[wv] = bi;
-            }
+    // Local pivot candidate of column c: the first maximum of v over the workgroup (v < 0 for rows
+    // that do not take part); every thread returns with the same (lb, li).
+    double lb;
+    int li;
+    auto candidate = [&](double v, int vi) {
+        double bv;
+        int bi, bl;
+        wave_argmax(v, vi, bv, bi, bl);
+        if (lane == 0) {
+            rv[wv] = bv;
+            ri[wv] = bi;
         }
         __syncthreads();
-        double lb = rv[0];
-        int li = ri[0];
+        lb = rv[0];
+        li = ri[0];
         for (int x = 1; x < 4; x++)
             argmax_merge(lb, li, rv[x], ri[x]);
+    };
+    // Entry (j, r) of the slab with the rank-1 update of column cp applied when that update is
+    // still pending on column j (pend: the update of column cp has reached only column cp + 1).
+    bool pend = false;
+    int cp = 0;
+    auto current = [&](int j, int r) {
+        cx<T> x = slab[(size_t)j * R + r];
+        if (pend && j > cp + 1) {
+            const cx<T> l = slab[(size_t)cp * R + r], u = prow[j];
+            x.re -= l.re * u.re - l.im * u.im;
+            x.im -= l.re * u.im + l.im * u.re;
+        }
+        return x;
+    };
+    // Publish column cn: header, candidate row, and row cn itself if this workgroup holds it.
+    auto publish = [&](int cn) {
+        const int pb = cn & 1;
+        const unsigned sq = (unsigned)(k + cn + 1);
+        if (tid == 0) {
+            const unsigned long long vb = (unsigned long long)__double_as_longlong(lb);
+            unsigned long long* h = ps.hdr(pb, g);
+            ll_put(h + 0, sq, (unsigned)vb);
+            ll_put(h + 1, sq, (unsigned)(vb >> 32));
+            ll_put(h + 2, sq, (unsigned)li);
+        }
+        if (lb >= 0.0)
+            for (int t = tid; t < RW; t += 256)
+                ll_put(ps.row(pb, g) + t, sq, cx_word(current(t / CW, li - r0), t % CW));
+        if (cn >= r0 && cn < r0 + nr)
+            for (int t = tid; t < RW; t += 256)
+                ll_put(ps.diag(pb) + t, sq, cx_word(current(t / CW, cn - r0), t % CW));
+    };
+
+    candidate(own ? (double)cabs1(slab[tid]) : -1.0, own ? grow : 0x7fffffff);
+    if (G > 1)
+        publish(0);
+
+    for (int c = 0; c < w; c++) {
+        const int par = c & 1;
         int p;
         double pval;
         if (G > 1) {
             const unsigned seq = (unsigned)(k + c + 1);
-            // ---- publish: header, candidate row, and the diagonal row if this group holds row c
-            if (tid == 0) {
-                const unsigned long long vb = (unsigned long long)__double_as_longlong(lb);
-                unsigned long long* h = ps.hdr(par, g);
-                ll_put(h + 0, seq, (unsigned)vb);
-                ll_put(h + 1, seq, (unsigned)(vb >> 32));
-                ll_put(h + 2, seq, (unsigned)li);
-            }
-            if (lb >= 0.0)
-                for (int t = tid; t < RW; t += 256)
-                    ll_put(ps.row(par, g) + t, seq,
-                           reinterpret_cast<const unsigned*>(&slab[(size_t)(t / CW) * R + (li - r0)])[t % CW]);
-            if (c >= r0 && c < r0 + nr)
-                for (int t = tid; t < RW; t += 256)
-                    ll_put(ps.diag(par) + t, seq,
-                           reinterpret_cast<const unsigned*>(&slab[(size_t)(t / CW) * R + (c - r0)])[t % CW]);
             // ---- all candidates (wave 0, up to 4 groups per lane, all words in flight together)
             bool ok = true;
             if (wv == 0) {
@@ -735,6 +779,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             }
             __syncthreads();
         } else {
+            __syncthreads(); // the previous column's update is complete
             p = li;
             pval = lb;
             if (pval > 0.0 && p != c && tid < w) {
@@ -753,12 +798,38 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             if (!(pval > 0.0) && *ps.info == 0)
                 *ps.info = (int)(k + c + 1);
         }
-        // ---- scale this thread's row and apply the rank-1 update to the rest of its row
-        if (pval > 0.0 && own && grow > c) {
+        // ---- multiplier of this thread's row and the update of column c + 1 only; the next
+        // column's candidate is published before the rest of the rank-1 update, which then runs
+        // while the other workgroups' candidates are on their way
+        const bool mine = pval > 0.0 && own && grow > c;
+        cx<T> l{};
+        if (mine) {
             const cx<T> rp = cdiv(cx<T>{1, 0}, prow[c]);
-            const cx<T> l = slab[(size_t)c * R + tid] * rp;
+            l = slab[(size_t)c * R + tid] * rp;
             slab[(size_t)c * R + tid] = l;
-            int j = c + 1;
+        }
+        if (c + 1 < w) {
+            double v = -1.0;
+            if (own && grow > c) {
+                cx<T> a1 = slab[(size_t)(c + 1) * R + tid];
+                if (mine) {
+                    const cx<T> u = prow[c + 1];
+                    a1.re -= l.re * u.re - l.im * u.im;
+                    a1.im -= l.re * u.im + l.im * u.re;
+                    slab[(size_t)(c + 1) * R + tid] = a1;
+                }
+                v = (double)cabs1(a1);
+            }
+            candidate(v, v >= 0.0 ? grow : 0x7fffffff);
+            if (G > 1) {
+                pend = pval > 0.0;
+                cp = c;
+                publish(c + 1);
+                __syncthreads(); // published rows are read before the update below rewrites them
+            }
+        }
+        if (mine) {
+            int j = c + 2;
             // eight columns at a time, all LDS reads of a group in flight together
             for (; j + 8 <= w; j += 8) {
                 cx<T> a[8], u[8];
@@ -787,6 +858,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
     __syncthreads();
     if (g == 0 && wv == 0)
         replay_panel_swaps((int)k, w, s_pivots, s_scratch, perm);
+#pragma unroll 16
     for (int c = 0; c < w; c++)
         if (tid < nr)
             Ap[(r0 + tid) + (int64_t)c * lda] = slab[(size_t)c * R + tid];
@@ -813,96 +885,71 @@ __global__ __launch_bounds__(256) void laswp_reverse(cx<T>* __restrict__ A, int6
 }
 
 // ----------------------------------------------------------------------------------------------
-// Triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block: Bout = D^{-1} Bin
-// (Bout may equal Bin).  UPPER=false: lower (forward), UPPER=true: upper (backward); TRANSD uses
-// D^T; UNIT takes the diagonal as one.  One workgroup per TRSM_COLS columns: thread (rg, col),
-// rg = tid % 16, keeps rows rg + 16 i of its column in registers, so loads and stores walk down
-// the columns; each row step is one broadcast through LDS (double buffered, one barrier per row).
+// Triangular solve with a w x w diagonal block (w <= 64) on a w x ncols block: Bout = E^{-1} Bin
+// (Bout may equal Bin), E = D, or D^T with TRANSD; UPPER: E upper (backward substitution), else
+// lower (forward); UNIT takes the diagonal as one.
+//
+// One wavefront per NC columns, lane j holding row j of each: the row steps run in registers with
+// no barrier.  Step i reads x_i = b_i / E_ii from lane i (readlane) and subtracts E_ji x_i from
+// every row j that depends on it; the coefficients sit in LDS, sE[i][j] = E_ji for the dependent
+// rows and zero elsewhere (the diagonal included), so lane i's b_i is left alone after its step
+// and every lane scales by 1 / E_jj once at the end.  Row j of sE is stored with its columns
+// permuted (slot j ^ i), so both the coalesced load and the transposed one are free of bank
+// conflicts.
 // ----------------------------------------------------------------------------------------------
-constexpr int TRSM_COLS = 16;
+constexpr int TRSM_WAVES = 4;
 
-template <typename T, bool UPPER, bool TRANSD = false, bool UNIT = !UPPER>
+__device__ inline cx<float> read_lane(cx<float> v, int i)
+{
+    return {__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.re), i)),
+            __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.im), i))};
+}
+__device__ inline double read_lane(double v, int i)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, i), hi = __builtin_amdgcn_readlane((int)(b >> 32), i);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+__device__ inline cx<double> read_lane(cx<double> v, int i) { return {read_lane(v.re, i), read_lane(v.im, i)}; }
+
+template <typename T, bool UPPER, bool TRANSD, bool UNIT, int NC>
 __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict__ D, int64_t ldd, const cx<T>* Bin,
                                                   int64_t ldi, cx<T>* Bout, int64_t ldo, int64_t ncols)
 {
-    __shared__ cx<T> sD[64][65];
-    __shared__ cx<T> rdiag[64]; // 1 / D[r][r], computed once so the row steps only multiply
-    __shared__ cx<T> xs[2][TRSM_COLS];
-    const int tid = threadIdx.x, rg = tid & 15, col = tid >> 4;
-    const int64_t c0 = (int64_t)blockIdx.x * TRSM_COLS;
-    const bool live = c0 + col < ncols;
+    __shared__ cx<T> sE[64][64];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e % 64, c = e / 64;
-        sD[r][c] = (r < w && c < w) ? (TRANSD ? D[c + (int64_t)r * ldd] : D[r + (int64_t)c * ldd])
-                                    : cx<T>{(T)(r == c ? 1 : 0), 0};
+        const int r = e & 63, c = e >> 6;                 // D(r, c); r fastest, coalesced
+        const int i = TRANSD ? r : c, j = TRANSD ? c : r; // E_ji
+        const bool dep = UPPER ? j < i : j > i;
+        sE[i][j ^ i] = (dep && r < w && c < w) ? D[r + (int64_t)c * ldd] : cx<T>{0, 0};
     }
-    cx<T> acc[4];
-#pragma clang loop unroll(full)
-    for (int i = 0; i < 4; i++) {
-        const int r = rg + 16 * i;
-        acc[i] = (r < w && live) ? Bin[r + (c0 + col) * ldi] : cx<T>{0, 0};
-    }
+    cx<T> rd{1, 0};
+    if (!UNIT && lane < w)
+        rd = cdiv(cx<T>{1, 0}, D[lane + (int64_t)lane * ldd]);
+    const int64_t cb = ((int64_t)blockIdx.x * TRSM_WAVES + wv) * NC; // first column of this wavefront
+    cx<T> b[NC];
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+        b[q] = (lane < w && cb + q < ncols) ? Bin[lane + (cb + q) * ldi] : cx<T>{0, 0};
     __syncthreads();
-    if (!UNIT) {
-        if (tid < 64)
-            rdiag[tid] = cdiv(cx<T>{1, 0}, sD[tid][tid]);
-        __syncthreads();
-    }
-    if (!UPPER) {
-#pragma clang loop unroll(full)
-        for (int i = 0; i < 4; i++) {
-            for (int q = 0; q < 16; q++) {
-                const int r = 16 * i + q;
-                if (r >= w)
-                    break;
-                if (rg == q) {
-                    if (!UNIT)
-                        acc[i] = acc[i] * rdiag[r];
-                    xs[r & 1][col] = acc[i];
-                }
-                __syncthreads();
-                const cx<T> x = xs[r & 1][col];
-#pragma clang loop unroll(full)
-                for (int i2 = i; i2 < 4; i2++) {
-                    if (i2 > i || rg > q) {
-                        const cx<T> l = sD[rg + 16 * i2][r];
-                        acc[i2].re -= l.re * x.re - l.im * x.im;
-                        acc[i2].im -= l.re * x.im + l.im * x.re;
-                    }
-                }
-            }
-        }
-    } else {
-#pragma clang loop unroll(full)
-        for (int i = 3; i >= 0; i--) {
-            for (int q = 15; q >= 0; q--) {
-                const int r = 16 * i + q;
-                if (r >= w)
-                    continue;
-                if (rg == q) {
-                    if (!UNIT)
-                        acc[i] = acc[i] * rdiag[r];
-                    xs[r & 1][col] = acc[i];
-                }
-                __syncthreads();
-                const cx<T> x = xs[r & 1][col];
-#pragma clang loop unroll(full)
-                for (int i2 = 0; i2 <= i; i2++) {
-                    if (i2 < i || rg < q) {
-                        const cx<T> u = sD[rg + 16 * i2][r];
-                        acc[i2].re -= u.re * x.re - u.im * x.im;
-                        acc[i2].im -= u.re * x.im + u.im * x.re;
-                    }
-                }
+#pragma unroll
+    for (int s = 0; s < 64; s++) {
+        const int i = UPPER ? 63 - s : s;
+        if (i < w) {
+            const cx<T> e = sE[i][lane ^ i];
+#pragma unroll
+            for (int q = 0; q < NC; q++) {
+                const cx<T> x = read_lane(UNIT ? b[q] : b[q] * rd, i);
+                b[q].re -= e.re * x.re - e.im * x.im;
+                b[q].im -= e.re * x.im + e.im * x.re;
             }
         }
     }
-#pragma clang loop unroll(full)
-    for (int i = 0; i < 4; i++) {
-        const int r = rg + 16 * i;
-        if (r < w && live)
-            Bout[r + (c0 + col) * ldo] = acc[i];
-    }
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+        if (lane < w && cb + q < ncols)
+            Bout[lane + (cb + q) * ldo] = UNIT ? b[q] : b[q] * rd;
 }
 
 template <typename T, bool UPPER, bool TRANSD = false, bool UNIT = !UPPER>
@@ -911,8 +958,17 @@ static void launch_trsm(int w, const cx<T>* D, int64_t ldd, const cx<T>* Bin, in
 {
     if (ncols <= 0)
         return;
-    launch_kernel(&trsm_block<T, UPPER, TRANSD, UNIT>, dim3((unsigned)((ncols + TRSM_COLS - 1) / TRSM_COLS)),
-                       dim3(256), 0, st, w, D, ldd, Bin, ldi, Bout, ldo, ncols);
+    // wide blocks (the factorisation's row of U) share each coefficient load over four columns a
+    // wavefront; narrow ones (the solves' right-hand sides) take one column a wavefront
+    if (ncols > 1024) {
+        constexpr int NC = 4;
+        launch_kernel(&trsm_block<T, UPPER, TRANSD, UNIT, NC>,
+                      dim3((unsigned)((ncols + TRSM_WAVES * NC - 1) / (TRSM_WAVES * NC))), dim3(256), 0, st, w, D, ldd,
+                      Bin, ldi, Bout, ldo, ncols);
+    } else {
+        launch_kernel(&trsm_block<T, UPPER, TRANSD, UNIT, 1>, dim3((unsigned)((ncols + TRSM_WAVES - 1) / TRSM_WAVES)),
+                      dim3(256), 0, st, w, D, ldd, Bin, ldi, Bout, ldo, ncols);
+    }
 }
 
 // Inverses of the 64 x 64 diagonal chunks of the factors, for the solves: chunk ch covers rows and
