@@ -64,6 +64,11 @@ constexpr int GBM = Tile64::BM, GBN = Tile64::BN;
 template <typename T> struct TileBig {
     using type = Tile64;
 };
+// Minimum workgroups per CU asked of the compiler for the product kernels.  Complex128 tiles held to
+// two (166 VGPRs, no accumulation registers: three waves per SIMD instead of two) ran the K = 64
+// update at 49.8 TFLOP/s against 39.7 and the K = 12160 residual at 54.7 against 48.7; complex64
+// tiles already reach three waves per SIMD.
+template <typename T> constexpr int kGemmOcc = sizeof(T) == 8 ? 2 : 1;
 
 template <typename T, typename TL> struct GemmAcc {
     typename Mfma<T>::acc_t re[TL::WM][TL::WN], im[TL::WM][TL::WN];
@@ -265,7 +270,7 @@ __device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C,
 
 // C[M x N] -= op(A)[M x K] * B[K x N], complex, column-major; op(A) = A or A^T.
 template <typename T, typename TA, bool TRANSA = false, bool M3 = false>
-__global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
+__global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
                                                  const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
                                                  int64_t ldc)
 {
@@ -286,7 +291,7 @@ __global__ __launch_bounds__(256) void cgemm_sub(int64_t M, int64_t N, int64_t K
 // product with an explicit U_jj^{-1} loses accuracy in proportion to cond(U_jj) on EFIE
 // matrices, so U_jj is applied by substitution.)
 template <typename T>
-__global__ __launch_bounds__(256) void make_w(int64_t n, const cx<T>* __restrict__ A, int64_t lda,
+__global__ __launch_bounds__(256, kGemmOcc<T>) void make_w(int64_t n, const cx<T>* __restrict__ A, int64_t lda,
                                               const cx<T>* __restrict__ inv, cx<T>* __restrict__ WV, int64_t ldw)
 {
     const int64_t t = blockIdx.x, j = blockIdx.y;
@@ -305,7 +310,7 @@ __global__ __launch_bounds__(256) void make_w(int64_t n, const cx<T>* __restrict
 // C -= WV(rows, chunk) * Y.  Y is the chunk's K x nrhs right-hand side (leading dimension ldy);
 // it is only read, and C and Xo do not overlap it.
 template <typename T>
-__global__ __launch_bounds__(256) void solve_chunk(int64_t r0, int64_t M, int64_t nrhs, int64_t K, int64_t ct,
+__global__ __launch_bounds__(256, kGemmOcc<T>) void solve_chunk(int64_t r0, int64_t M, int64_t nrhs, int64_t K, int64_t ct,
                                                    const cx<T>* __restrict__ inv, const cx<T>* __restrict__ WVc,
                                                    int64_t ldw, const cx<T>* __restrict__ Y, int64_t ldy,
                                                    cx<T>* __restrict__ C, int64_t ldc, cx<T>* __restrict__ Xo,
@@ -602,8 +607,9 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                                                     int* __restrict__ perm)
 {
     extern __shared__ unsigned char smem_raw[];
-    cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw); // [w][R] column-major
-    cx<T>* prow = slab + (size_t)w * R;                // [w] pivot row
+    const int RS = R + 1; // slab column stride: a row of the slab spans every LDS bank
+    cx<T>* slab = reinterpret_cast<cx<T>*>(smem_raw); // [w][RS] column-major
+    cx<T>* prow = slab + (size_t)w * RS;               // [w] pivot row
     cx<T>* drow = prow + w;                            // [w] previous content of row c
     __shared__ double rv[4];
     __shared__ int ri[4];
@@ -628,10 +634,10 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 v[q] = src[(int64_t)(c0 + q) * lda];
 #pragma unroll
             for (int q = 0; q < 16; q++)
-                slab[(size_t)(c0 + q) * R + tid] = v[q];
+                slab[(size_t)(c0 + q) * RS + tid] = v[q];
         }
         for (; c0 < w; c0++)
-            slab[(size_t)c0 * R + tid] = src[(int64_t)c0 * lda];
+            slab[(size_t)c0 * RS + tid] = src[(int64_t)c0 * lda];
     }
     __syncthreads();
 
@@ -662,9 +668,9 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
     bool pend = false;
     int cp = 0;
     auto current = [&](int j, int r) {
-        cx<T> x = slab[(size_t)j * R + r];
+        cx<T> x = slab[(size_t)j * RS + r];
         if (pend && j > cp + 1) {
-            const cx<T> l = slab[(size_t)cp * R + r], u = prow[j];
+            const cx<T> l = slab[(size_t)cp * RS + r], u = prow[j];
             x.re -= l.re * u.re - l.im * u.im;
             x.im -= l.re * u.im + l.im * u.re;
         }
@@ -773,9 +779,9 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 return;
             if (pval > 0.0) {
                 if (c >= r0 && c < r0 + nr && tid < w)
-                    slab[(size_t)tid * R + (c - r0)] = prow[tid];
+                    slab[(size_t)tid * RS + (c - r0)] = prow[tid];
                 if (need_diag && tid < w)
-                    slab[(size_t)tid * R + (p - r0)] = drow[tid];
+                    slab[(size_t)tid * RS + (p - r0)] = drow[tid];
             }
             __syncthreads();
         } else {
@@ -783,13 +789,13 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
             p = li;
             pval = lb;
             if (pval > 0.0 && p != c && tid < w) {
-                const cx<T> t = slab[(size_t)tid * R + c];
-                slab[(size_t)tid * R + c] = slab[(size_t)tid * R + p];
-                slab[(size_t)tid * R + p] = t;
+                const cx<T> t = slab[(size_t)tid * RS + c];
+                slab[(size_t)tid * RS + c] = slab[(size_t)tid * RS + p];
+                slab[(size_t)tid * RS + p] = t;
             }
             __syncthreads();
             if (tid < w)
-                prow[tid] = slab[(size_t)tid * R + c];
+                prow[tid] = slab[(size_t)tid * RS + c];
             __syncthreads();
         }
         if (g == 0 && tid == 0) {
@@ -805,18 +811,18 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
         cx<T> l{};
         if (mine) {
             const cx<T> rp = cdiv(cx<T>{1, 0}, prow[c]);
-            l = slab[(size_t)c * R + tid] * rp;
-            slab[(size_t)c * R + tid] = l;
+            l = slab[(size_t)c * RS + tid] * rp;
+            slab[(size_t)c * RS + tid] = l;
         }
         if (c + 1 < w) {
             double v = -1.0;
             if (own && grow > c) {
-                cx<T> a1 = slab[(size_t)(c + 1) * R + tid];
+                cx<T> a1 = slab[(size_t)(c + 1) * RS + tid];
                 if (mine) {
                     const cx<T> u = prow[c + 1];
                     a1.re -= l.re * u.re - l.im * u.im;
                     a1.im -= l.re * u.im + l.im * u.re;
-                    slab[(size_t)(c + 1) * R + tid] = a1;
+                    slab[(size_t)(c + 1) * RS + tid] = a1;
                 }
                 v = (double)cabs1(a1);
             }
@@ -835,7 +841,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 cx<T> a[8], u[8];
 #pragma unroll
                 for (int q = 0; q < 8; q++) {
-                    a[q] = slab[(size_t)(j + q) * R + tid];
+                    a[q] = slab[(size_t)(j + q) * RS + tid];
                     u[q] = prow[j + q];
                 }
 #pragma unroll
@@ -845,11 +851,11 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 }
 #pragma unroll
                 for (int q = 0; q < 8; q++)
-                    slab[(size_t)(j + q) * R + tid] = a[q];
+                    slab[(size_t)(j + q) * RS + tid] = a[q];
             }
             for (; j < w; j++) {
                 const cx<T> u = prow[j];
-                cx<T>& a = slab[(size_t)j * R + tid];
+                cx<T>& a = slab[(size_t)j * RS + tid];
                 a.re -= l.re * u.re - l.im * u.im;
                 a.im -= l.re * u.im + l.im * u.re;
             }
@@ -861,7 +867,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
 #pragma unroll 16
     for (int c = 0; c < w; c++)
         if (tid < nr)
-            Ap[(r0 + tid) + (int64_t)c * lda] = slab[(size_t)c * R + tid];
+            Ap[(r0 + tid) + (int64_t)c * lda] = slab[(size_t)c * RS + tid];
 }
 
 // Row interchanges of one panel undone (swaps applied in reverse order) on columns [col0, col0 + ncols);
@@ -1239,7 +1245,7 @@ int launch_symmetrize(c128* Z, int64_t n, int64_t ldz, hipStream_t st)
 template <typename T> int panel_rows(int w, bool shared_cu)
 {
     const size_t budget = (size_t)(shared_cu ? 72 : 144) * 1024;
-    int R = (int)(budget / ((size_t)w * sizeof(cx<T>))) - 2;
+    int R = (int)(budget / ((size_t)w * sizeof(cx<T>))) - 3; // slab padding, pivot and previous rows
     R = R >= 256 ? 256 : (R >= 128 ? 128 : (R >= 64 ? 64 : 32));
     return R;
 }
@@ -1285,7 +1291,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     int R = panel_rows<T>(nb, lookahead);
     if ((n + R - 1) / R > wk.max_groups)
         R = panel_rows<T>(nb, false);
-    const size_t sh_max = ((size_t)nb * R + 2 * nb) * sizeof(cx<T>);
+    const size_t sh_max = ((size_t)nb * (R + 1) + 2 * nb) * sizeof(cx<T>);
     if (sh_max > 64 * 1024)
         FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&panel_factor<T>), sh_max));
     const int gmax = wk.max_groups;
@@ -1309,7 +1315,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         const int G = (int)((m + R - 1) / R);
         if (G > gmax)
             return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), FRICO_ERR_LIMIT;
-        const size_t sh = ((size_t)w * R + 2 * w) * sizeof(cx<T>);
+        const size_t sh = ((size_t)w * (R + 1) + 2 * w) * sizeof(cx<T>);
         launch_kernel(&panel_factor<T>, dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
                            wk.perm + p * PERM_STRIDE);
         FRICO_HIP_CHECK(hipGetLastError());
