@@ -128,7 +128,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--nb", type=int, default=64)
+    ap.add_argument("--nb", type=int, default=128)
     ap.add_argument("--n", type=int, default=64, help="plate cells per side (64 -> N=12160)")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")

@@ -90,7 +90,7 @@ void phase_end(frico_ctx* c, KernelTimer& t)
 
 int ensure_lu_work(frico_ctx* c)
 {
-    const int64_t npan = (c->N + 31) / 32 + 1;
+    const int64_t npan = (c->N + 31) / 32 + 3; // panels at the smallest block width, two sub-panel lists
     const int gmax = std::min(c->max_groups, LU_MAX_GROUPS);
     int rc;
     if ((rc = c->llw.ensure(lu_exchange_words(gmax))))
@@ -238,11 +238,16 @@ int frico_ctx_set_block(frico_ctx* c, int nb)
 {
     if (check_ctx(c))
         return -1;
-    if (nb != 32 && nb != 64 && nb != 128)
-        return set_error("block size must be 32, 64 or 128"), -2;
+    if (nb != 0 && nb != 32 && nb != 64 && nb != 128)
+        return set_error("block size must be 0 (automatic), 32, 64 or 128"), -2;
     c->nb = nb;
     return 0;
 }
+
+// Block width of the next factorisation: the set width, or (0) 128 from N = 6144 up and 64 below.
+// At N = 12160 the 128-wide steps (two 64-column panel launches, K = 128 trailing products) took
+// the complex64 factorisation from 85.1 to 84.3 ms and the complex128 one from 144.9 to 135.6.
+static int block_width(const frico_ctx* c) { return c->nb ? c->nb : (c->N >= 6144 ? 128 : 64); }
 
 int frico_mesh_load(frico_ctx* c, const double* vertices, int64_t nv, const int64_t* triangles, const int32_t* tri_tags,
                     int64_t nt, const int64_t* iface_edges, const int32_t* iface_tags, int64_t nif, int degree)
@@ -862,7 +867,7 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     if (form == FRICO_SOLVE_C64_IR && (rc = c->LU32.ensure((size_t)(N * N))))
         return rc;
     c->factored = -1; // the factors are about to be overwritten
-    c->nb_factored = c->nb;
+    c->nb_factored = block_width(c);
     wk = lu_work(c);
     if (!lookahead) {
         wk.side = nullptr;
@@ -872,7 +877,7 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     // W products for the fused solves: one more N x N matrix of the factor's type, taken only
     // when it fits with room to spare
     c->wv_ok = false;
-    if (c->fused_solve && c->nb % 64 == 0 && N > 64) {
+    if (c->fused_solve && c->nb_factored % 64 == 0 && N > 64) {
         const size_t bytes = (size_t)(N * N) * (form == FRICO_SOLVE_C128 ? sizeof(c128) : sizeof(c64));
         size_t fr = 0, tot = 0;
         if (c->wv.n >= bytes || (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > bytes + c->wv.n + ((size_t)2 << 30))) {
@@ -891,14 +896,14 @@ int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
     int rc;
     phase_begin(c, c->t_factor);
     if (form == FRICO_SOLVE_C128) {
-        rc = lu_factor<double>(wk, c->Z64.p, N, N, c->ipiv.p, c->nb, c->st, c->profiling ? &c->t_gemm : nullptr);
+        rc = lu_factor<double>(wk, c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->st, c->profiling ? &c->t_gemm : nullptr);
     } else {
         if (c->zprec == FRICO_C128)
             rc = launch_convert_c128_c64(c->Z64.p, N, c->LU32.p, N, N, N, c->st);
         else
             rc = launch_copy_bytes(c->Z32.p, c->LU32.p, (size_t)(N * N) * sizeof(c64), c->st);
         if (!rc)
-            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb, c->st, c->profiling ? &c->t_gemm : nullptr);
+            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, c->profiling ? &c->t_gemm : nullptr);
     }
     phase_end(c, c->t_factor);
     return rc;

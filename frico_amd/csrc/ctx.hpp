@@ -176,7 +176,7 @@ struct frico_ctx {
     hipStream_t side = nullptr;      // look-ahead stream of the LU (default priority)
     std::vector<hipEvent_t> lu_events;
     bool lookahead = true; // FRICO_LU_LOOKAHEAD=0 disables the look-ahead stream
-    int nb = 64;
+    int nb = 0; // LU block width; 0: chosen from N at each factorisation
     int max_groups = 0;
     bool profiling = false;
     bool have_mesh = false;

@@ -435,6 +435,14 @@ __device__ inline void replay_panel_swaps(int k, int w, const int* piv, int* s_r
 // One workgroup per LASWP_COLS columns; all reads of a column precede its writes (through LDS).
 constexpr int LASWP_COLS = 8;
 
+// Interchange list of a panel from its pivots (absolute rows ipiv[k .. k + w)), for panels whose
+// pivots come from more than one panel launch.  One wavefront.
+__global__ __launch_bounds__(64) void build_perm_list(int k, int w, const int* __restrict__ ipiv, int* __restrict__ out)
+{
+    __shared__ int s_row[2 * LU_NB_MAX];
+    replay_panel_swaps(k, w, ipiv + k, s_row, out);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols,
                                              const int* __restrict__ perm)
@@ -600,7 +608,6 @@ __device__ inline unsigned cx_word(cx<double> x, int q)
     const long long b = __double_as_longlong(q < 2 ? x.re : x.im);
     return (q & 1) ? (unsigned)((unsigned long long)b >> 32) : (unsigned)b;
 }
-
 template <typename T>
 __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64_t lda, int64_t k, int64_t m, int w,
                                                     int R, int G, int* __restrict__ ipiv, PanelSync ps,
@@ -1288,17 +1295,23 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         return set_error("lu: block size out of range"), -1;
     const int64_t npan = (n + nb - 1) / nb;
     const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
-    int R = panel_rows<T>(nb, lookahead);
+    // Panels wider than 64 columns are factored as 64-column sub-panels (see panel() below): the
+    // panel kernel's per-column cost does not depend on its width, and the trailing products then
+    // run with K = nb.
+    const int pw = nb < 64 ? nb : 64;
+    int R = panel_rows<T>(pw, lookahead);
     if ((n + R - 1) / R > wk.max_groups)
-        R = panel_rows<T>(nb, false);
-    const size_t sh_max = ((size_t)nb * (R + 1) + 2 * nb) * sizeof(cx<T>);
+        R = panel_rows<T>(pw, false);
+    const size_t sh_max = ((size_t)pw * (R + 1) + 2 * pw) * sizeof(cx<T>);
     if (sh_max > 64 * 1024)
         FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&panel_factor<T>), sh_max));
     const int gmax = wk.max_groups;
     if (!wk.ll || wk.nll < ll_words(gmax))
         return set_error("lu: exchange workspace too small"), -1;
-    if (!wk.perm || wk.nperm < npan)
+    const bool composite = nb > pw && n > pw;
+    if (!wk.perm || wk.nperm < npan + (composite ? 2 : 0))
         return set_error("lu: interchange-list workspace too small"), -1;
+    int* sub_perm = wk.perm + npan * PERM_STRIDE; // lists of the two sub-panels of a composite panel
     const int64_t nz = 2 * (int64_t)ll_words(gmax);
     launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.ll, nz);
     launch_kernel(&zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
@@ -1308,16 +1321,43 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     ps.gmax = gmax;
     ps.abort_flag = wk.flags;
     ps.info = wk.flags + 1;
-    auto panel = [&](int64_t p) -> int {
-        const int64_t k = p * nb;
-        const int w = (int)((n - k) < nb ? (n - k) : nb);
+    // One panel kernel: columns [k, k + w) (w <= 64), rows [k, n); its interchange list to plist.
+    auto panel64 = [&](int64_t k, int w, int* plist) -> int {
         const int64_t m = n - k;
         const int G = (int)((m + R - 1) / R);
         if (G > gmax)
             return set_error("lu: panel needs %d workgroups, only %d can be co-resident", G, gmax), FRICO_ERR_LIMIT;
         const size_t sh = ((size_t)w * (R + 1) + 2 * w) * sizeof(cx<T>);
-        launch_kernel(&panel_factor<T>, dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps,
-                           wk.perm + p * PERM_STRIDE);
+        launch_kernel(&panel_factor<T>, dim3(G), dim3(256), sh, st, A, lda, k, m, w, R, G, ipiv, ps, plist);
+        FRICO_HIP_CHECK(hipGetLastError());
+        return 0;
+    };
+    // Panel p.  A composite panel (w > 64) is factored as [P1 | P2]: P1 by the panel kernel; P2
+    // given P1's interchanges, U12 = L11^{-1} A12 and A22 -= L21 U12 (the step update restricted to
+    // P2's columns); P2 by the panel kernel; then P2's interchanges applied to P1's columns, so that
+    // the w columns of L share one row order and the panel behaves as one of width w.  Its
+    // interchange list is rebuilt from the w pivots.  Partial pivoting picks the same rows as it
+    // would over the whole w columns: column by column, each among the fully updated rows.
+    auto panel = [&](int64_t p) -> int {
+        const int64_t k = p * nb;
+        const int w = (int)((n - k) < nb ? (n - k) : nb);
+        int* plist = wk.perm + p * PERM_STRIDE;
+        if (w <= pw)
+            return panel64(k, w, plist);
+        const int w1 = pw, w2 = w - pw;
+        int rc;
+        if ((rc = panel64(k, w1, sub_perm)))
+            return rc;
+        cx<T>* A12 = A + k + (k + w1) * lda;
+        launch_laswp<T>(A, lda, k + w1, w2, sub_perm, st);
+        launch_trsm<T, false>(w1, A + k + k * lda, lda, A12, lda, A12, lda, w2, st);
+        if ((rc = gemm_sub<T>(n - k - w1, w2, w1, A + (k + w1) + k * lda, lda, A12, lda, A + (k + w1) + (k + w1) * lda,
+                              lda, st, nullptr)))
+            return rc;
+        if ((rc = panel64(k + w1, w2, sub_perm + PERM_STRIDE)))
+            return rc;
+        launch_laswp<T>(A, lda, k, w1, sub_perm + PERM_STRIDE, st);
+        launch_kernel(&build_perm_list, dim3(1), dim3(64), 0, st, (int)k, w, (const int*)ipiv, plist);
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
     };

@@ -82,7 +82,9 @@ int frico_ctx_synchronize(frico_ctx* ctx);
 /* Enables HIP-event timing of phases and of every GEMM / fill launch (off by default). */
 int frico_ctx_set_profiling(frico_ctx* ctx, int enable);
 int frico_ctx_get_profile(frico_ctx* ctx, frico_profile* out);
-/* Block width of the LU (32, 64 or 128). */
+/* Block width of the LU: 32, 64 or 128, or 0 (the default) for 128 from N = 6144 up and 64 below.
+ * Panels of 128 columns are factored as two 64-column panel launches and give K = 128 trailing
+ * products. */
 int frico_ctx_set_block(frico_ctx* ctx, int nb);
 
 /*

@@ -34,6 +34,23 @@ def test_gradients_match_the_cpu_adjoint(gpu_ctx, degree):
     assert ez <= 1e-9 and eg <= 1e-9
 
 
+def test_gradients_with_128_column_panels():
+    """The transposed solves of the adjoint with factors from 128-column panels (each factored as
+    two 64-column panel launches)."""
+    import frico_amd
+    lam = meshes.wavelength(F0)
+    sm = meshes.bowtie(0.25 * lam, 60.0, 8, 5)
+    ref = oracle.antenna_gradients(sm, F0, 5, obs=OBS)
+    with frico_amd.Context(0) as c:
+        c.set_block(128)
+        c.load_mesh(sm, 5)
+        assert c.N > 128
+        g = c.antenna_gradients(F0, sm.feed, 1.0, OBS)
+    assert abs(g["zin"] - ref["zin"]) <= 1e-10 * abs(ref["zin"])
+    assert np.max(np.abs(g["dzin"] - ref["dzin"])) <= 1e-9 * np.max(np.abs(ref["dzin"]))
+    assert np.max(np.abs(g["dgain"] - ref["dgain"])) <= 1e-9 * np.max(np.abs(ref["dgain"]))
+
+
 def test_gradients_on_a_strip_with_a_multi_edge_gap(gpu_ctx):
     sm = meshes.strip_dipole(0.4, 0.03, 10, 3)
     ref = oracle.antenna_gradients(sm, F0, 5, obs=(0.0, 3.0, 4.0))

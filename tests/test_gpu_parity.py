@@ -146,17 +146,21 @@ def test_lu_needs_pivoting():
         assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
 
 
-def test_lu_singular_reports_column():
+@pytest.mark.parametrize("nb,col", [(64, 37), (128, 37), (128, 100), (128, 150)])
+def test_lu_singular_reports_column(nb, col):
+    """The first zero pivot is reported as its 1-based column, also from the second 64-column half
+    of a 128-column panel (factored by a second panel launch) and from the second panel."""
     import frico_amd
-    n = 90
+    n = 190
     rng = np.random.default_rng(3)
     A = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
-    A[:, 37] = 0.0
+    A[:, col] = 0.0
     with frico_amd.Context(0) as c:
+        c.set_block(nb)
         c.set_matrix(A)
         with pytest.raises(frico_amd.SingularMatrixError) as e:
             c.factor("c128")
-        assert e.value.code == 38
+        assert e.value.code == col + 1
 
 
 def test_solve_before_factor_is_an_error():
