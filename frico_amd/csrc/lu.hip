@@ -435,47 +435,79 @@ __device__ inline void replay_panel_swaps(int k, int w, const int* piv, int* s_r
 // One workgroup per LASWP_COLS columns; all reads of a column precede its writes (through LDS).
 constexpr int LASWP_COLS = 8;
 
-// Interchange list of a panel from its pivots (absolute rows ipiv[k .. k + w)), for panels whose
-// pivots come from more than one panel launch.  One wavefront.
-__global__ __launch_bounds__(64) void build_perm_list(int k, int w, const int* __restrict__ ipiv, int* __restrict__ out)
+// Interchange lists of the composite panels (see lu_factor) from their pivots, one wavefront per
+// panel: panel p covers columns [p nb, p nb + w), w = min(nb, n - p nb).
+__global__ __launch_bounds__(64) void build_perm_lists(int nb, int64_t n, const int* __restrict__ ipiv,
+                                                       int* __restrict__ out)
 {
     __shared__ int s_row[2 * LU_NB_MAX];
-    replay_panel_swaps(k, w, ipiv + k, s_row, out);
+    const int64_t k = (int64_t)blockIdx.x * nb;
+    const int w = (int)((n - k) < nb ? (n - k) : nb);
+    replay_panel_swaps((int)k, w, ipiv + k, s_row, out + (size_t)blockIdx.x * PERM_STRIDE);
 }
 
+// Row interchanges of one panel on columns [col0, col0 + ncols): one gather through LDS.  With a
+// second list (perm2, applied after perm) the two are composed first: row r of perm2 receives what
+// its source held before perm, i.e. the content of perm's source of that row; rows of perm that
+// perm2 also moves are dropped from perm's part.
 template <typename T>
 __global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda, int64_t col0, int64_t ncols,
-                                             const int* __restrict__ perm)
+                                             const int* __restrict__ perm, const int* __restrict__ perm2)
 {
     __shared__ int2 s_mv[2 * LU_NB_MAX];
     __shared__ cx<T> buf[2 * LU_NB_MAX * LASWP_COLS];
     const int tid = threadIdx.x;
-    const int L = perm[0];
-    for (int e = tid; e < L; e += 256)
+    const int L1 = perm[0];
+    int L = L1;
+    for (int e = tid; e < L1; e += 256)
         s_mv[e] = int2{perm[1 + 2 * e], perm[2 + 2 * e]};
+    if (perm2) {
+        const int L2 = perm2[0];
+        __syncthreads();
+        int2 a{0, 0}, b{0, 0}; // this thread's entries: a of perm2 (composed), b of perm (or dropped)
+        if (tid < L2) {
+            a = int2{perm2[1 + 2 * tid], perm2[2 + 2 * tid]};
+            for (int j = 0; j < L1; j++)
+                if (s_mv[j].x == a.y)
+                    a.y = s_mv[j].y;
+        }
+        if (tid < L1) {
+            b = s_mv[tid];
+            for (int j = 0; j < L2; j++)
+                if (perm2[1 + 2 * j] == b.x)
+                    b.x = -1;
+        }
+        __syncthreads();
+        if (tid < L1)
+            s_mv[tid] = b;
+        if (tid < L2)
+            s_mv[L1 + tid] = a;
+        L = L1 + L2;
+    }
     __syncthreads();
     const int64_t c0 = col0 + (int64_t)blockIdx.x * LASWP_COLS;
     const int nc = (int)((col0 + ncols - c0) < LASWP_COLS ? (col0 + ncols - c0) : LASWP_COLS);
     for (int e = tid; e < L * LASWP_COLS; e += 256) {
         const int m = e % L, cc = e / L;
-        if (cc < nc)
+        if (cc < nc && s_mv[m].x >= 0)
             buf[e] = A[(int64_t)s_mv[m].y + (c0 + cc) * lda];
     }
     __syncthreads();
     for (int e = tid; e < L * LASWP_COLS; e += 256) {
         const int m = e % L, cc = e / L;
-        if (cc < nc)
+        if (cc < nc && s_mv[m].x >= 0)
             A[(int64_t)s_mv[m].x + (c0 + cc) * lda] = buf[e];
     }
 }
 
 template <typename T>
-static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, const int* perm, hipStream_t st)
+static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, const int* perm, hipStream_t st,
+                         const int* perm2 = nullptr)
 {
     if (ncols <= 0)
         return;
     launch_kernel(&laswp<T>, dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
-                       col0, ncols, perm);
+                  col0, ncols, perm, perm2);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -1259,13 +1291,14 @@ template <typename T> int panel_rows(int w, bool shared_cu)
 
 // Update of columns [c0, c0 + nc) by step k (panel of width w): row interchanges, U12 = L11^{-1} A12
 // and A22 -= L21 U12.
+// perm2: a second interchange list applied after perm (the two halves of a composite panel).
 template <typename T>
-static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, int64_t k, int w, int64_t c0, int64_t nc,
-                       hipStream_t st, KernelTimer* kt)
+static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, const int* perm2, int64_t k, int w,
+                       int64_t c0, int64_t nc, hipStream_t st, KernelTimer* kt)
 {
     if (nc <= 0)
         return 0;
-    launch_laswp<T>(A, lda, c0, nc, perm, st);
+    launch_laswp<T>(A, lda, c0, nc, perm, st, perm2);
     for (int s = 0; s < w; s += 64) {
         const int ws = (w - s) < 64 ? (w - s) : 64;
         cx<T>* D = A + (k + s) + (k + s) * lda;
@@ -1309,9 +1342,21 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     if (!wk.ll || wk.nll < ll_words(gmax))
         return set_error("lu: exchange workspace too small"), -1;
     const bool composite = nb > pw && n > pw;
-    if (!wk.perm || wk.nperm < npan + (composite ? 2 : 0))
+    if (!wk.perm || wk.nperm < npan * (composite ? 3 : 1))
         return set_error("lu: interchange-list workspace too small"), -1;
-    int* sub_perm = wk.perm + npan * PERM_STRIDE; // lists of the two sub-panels of a composite panel
+    // lists of the two halves of composite panel p: sub_perm + (2 p, 2 p + 1) * PERM_STRIDE
+    int* sub_perm = wk.perm + npan * PERM_STRIDE;
+    // interchange lists the trailing updates of panel p apply
+    auto lists = [&](int64_t p, const int*& l1, const int*& l2) {
+        const int64_t k = p * nb;
+        if (composite && n - k > pw) {
+            l1 = sub_perm + 2 * p * PERM_STRIDE;
+            l2 = l1 + PERM_STRIDE;
+        } else {
+            l1 = wk.perm + p * PERM_STRIDE;
+            l2 = nullptr;
+        }
+    };
     const int64_t nz = 2 * (int64_t)ll_words(gmax);
     launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.ll, nz);
     launch_kernel(&zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
@@ -1336,8 +1381,9 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     // given P1's interchanges, U12 = L11^{-1} A12 and A22 -= L21 U12 (the step update restricted to
     // P2's columns); P2 by the panel kernel; then P2's interchanges applied to P1's columns, so that
     // the w columns of L share one row order and the panel behaves as one of width w.  Its
-    // interchange list is rebuilt from the w pivots.  Partial pivoting picks the same rows as it
-    // would over the whole w columns: column by column, each among the fully updated rows.
+    // two lists are applied by the trailing updates in one pass, and the panel's own list (for the
+    // solves) is built from the w pivots after the factorisation.  Partial pivoting picks the same
+    // rows as it would over the whole w columns: column by column, each among the fully updated rows.
     auto panel = [&](int64_t p) -> int {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
@@ -1345,19 +1391,19 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         if (w <= pw)
             return panel64(k, w, plist);
         const int w1 = pw, w2 = w - pw;
+        int* l1 = sub_perm + 2 * p * PERM_STRIDE;
         int rc;
-        if ((rc = panel64(k, w1, sub_perm)))
+        if ((rc = panel64(k, w1, l1)))
             return rc;
         cx<T>* A12 = A + k + (k + w1) * lda;
-        launch_laswp<T>(A, lda, k + w1, w2, sub_perm, st);
+        launch_laswp<T>(A, lda, k + w1, w2, l1, st);
         launch_trsm<T, false>(w1, A + k + k * lda, lda, A12, lda, A12, lda, w2, st);
         if ((rc = gemm_sub<T>(n - k - w1, w2, w1, A + (k + w1) + k * lda, lda, A12, lda, A + (k + w1) + (k + w1) * lda,
                               lda, st, nullptr)))
             return rc;
-        if ((rc = panel64(k + w1, w2, sub_perm + PERM_STRIDE)))
+        if ((rc = panel64(k + w1, w2, l1 + PERM_STRIDE)))
             return rc;
-        launch_laswp<T>(A, lda, k, w1, sub_perm + PERM_STRIDE, st);
-        launch_kernel(&build_perm_list, dim3(1), dim3(64), 0, st, (int)k, w, (const int*)ipiv, plist);
+        launch_laswp<T>(A, lda, k, w1, l1 + PERM_STRIDE, st);
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
     };
@@ -1366,7 +1412,9 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         for (int64_t p = 0; p < npan; p++) {
             const int64_t k = p * nb;
             const int w = (int)((n - k) < nb ? (n - k) : nb);
-            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, wk.perm + p * PERM_STRIDE, k, w, k + w, n - k - w, st, gemm_timer)))
+            const int *l1, *l2;
+            lists(p, l1, l2);
+            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, l1, l2, k, w, k + w, n - k - w, st, gemm_timer)))
                 return rc;
         }
     } else {
@@ -1387,21 +1435,28 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
             const int64_t nnext = nright < nb ? nright : nb;
             const int64_t nrest = nright - nnext;
             FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
+            const int *l1, *l2;
+            lists(p, l1, l2);
             if (nrest > 0) {
                 FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
-                if ((rc = step_update<T>(A, n, lda, wk.perm + p * PERM_STRIDE, k, w, k + w + nnext, nrest, wk.side, gemm_timer)))
+                if ((rc = step_update<T>(A, n, lda, l1, l2, k, w, k + w + nnext, nrest, wk.side, gemm_timer)))
                     return rc;
                 FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
             }
             if (prev_wide)
                 FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
-            if ((rc = step_update<T>(A, n, lda, wk.perm + p * PERM_STRIDE, k, w, k + w, nnext, st, gemm_timer)) || (rc = panel(p + 1)))
+            if ((rc = step_update<T>(A, n, lda, l1, l2, k, w, k + w, nnext, st, gemm_timer)) || (rc = panel(p + 1)))
                 return rc;
             prev_wide = nrest > 0;
         }
         // the main stream continues only after all side work
         FRICO_HIP_CHECK(hipEventRecord(ev0, wk.side));
         FRICO_HIP_CHECK(hipStreamWaitEvent(st, ev0, 0));
+    }
+    if (composite) {
+        // the composite panels' own lists, for the solves (a panel kernel wrote the others)
+        launch_kernel(&build_perm_lists, dim3((unsigned)npan), dim3(64), 0, st, nb, n, (const int*)ipiv, wk.perm);
+        FRICO_HIP_CHECK(hipGetLastError());
     }
     if (wk.dinv) {
         const int64_t nch = (n + 63) / 64;
