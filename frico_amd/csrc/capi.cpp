@@ -960,7 +960,7 @@ static int residual(frico_ctx* c, KernelTimer* kt)
     if (c->zprec == FRICO_C128) {
         if (c->z64_factored)
             return set_error("residual needs the unfactored Z"), FRICO_ERR_STATE;
-        return gemm_sub<double>(N, nr, N, c->Z64.p, N, c->X.p, N, c->R.p, N, c->st, kt);
+        return gemm_sub<double>(N, nr, N, c->Z64.p, N, c->X.p, N, c->R.p, N, c->st, kt, true);
     }
     return gemm_sub_mixed(N, nr, N, c->Z32.p, N, c->X.p, N, c->R.p, N, c->st, kt);
 }

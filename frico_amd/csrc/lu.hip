@@ -1195,11 +1195,13 @@ __global__ __launch_bounds__(256) void col_norm2(const c128* __restrict__ X, int
 // Host drivers
 // ----------------------------------------------------------------------------------------------
 // Complex products with three real MFMAs per complex multiply-add instead of four.  Normwise as
-// stable as the four-product form.  Used for complex64 (the LU of the refinement form) unless
-// FRICO_GEMM_3M=0: at N = 12160 the refinement still ends after two sweeps at the same residual
-// (9.8e-11) and the trailing update runs 1.13x faster.  Complex128 keeps four products: its
-// three-product tile needs 286 registers per lane (one wave per SIMD) and ran the K = 64 update
-// at 27 TFLOP/s against 40, and the K = 12160 residual product at the same speed.
+// stable as the four-product form, with a larger constant in the imaginary part.  Used unless
+// FRICO_GEMM_3M=0 or the caller asks for the exact form: always for complex64 (the LU of the
+// refinement form; at N = 12160 the refinement still ends after two sweeps and the update runs
+// 1.13x faster), and for complex128 from K = 128 (two waves per SIMD with 196 VGPRs: the K = 128
+// update at 64.4 TFLOP/s against 60.9; at K = 64 the four-product tile is faster, 49.7 against
+// 45.2).  The FP64 residuals stay exact: with three products (65.5 TFLOP/s against 54.9 at
+// K = 12160) a refinement to 1e-13 at n = 700 stalled at 1.2e-13.
 static bool gemm_three_products()
 {
     static const bool on = [] {
@@ -1211,7 +1213,7 @@ static bool gemm_three_products()
 
 template <typename T>
 int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb, cx<T>* C,
-             int64_t ldc, hipStream_t st, KernelTimer* kt)
+             int64_t ldc, hipStream_t st, KernelTimer* kt, bool exact)
 {
     if (M <= 0 || N <= 0 || K <= 0)
         return 0;
@@ -1222,14 +1224,10 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
     if (kt)
         kt->begin(st);
     const dim3 grid((unsigned)gx, (unsigned)gy);
-    if constexpr (sizeof(T) == 4) {
-        if (gemm_three_products())
-            launch_kernel(&cgemm_sub<T, T, false, true>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
-        else
-            launch_kernel(&cgemm_sub<T, T>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
-    } else {
+    if (!exact && gemm_three_products() && (sizeof(T) == 4 || K >= 128))
+        launch_kernel(&cgemm_sub<T, T, false, true>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
+    else
         launch_kernel(&cgemm_sub<T, T>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
-    }
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K);
     FRICO_HIP_CHECK(hipGetLastError());
@@ -1636,9 +1634,9 @@ int launch_col_norm2(const c128* X, int64_t ldx, int64_t rows, int64_t cols, dou
 }
 
 template int gemm_sub<double>(int64_t, int64_t, int64_t, const c128*, int64_t, const c128*, int64_t, c128*, int64_t,
-                              hipStream_t, KernelTimer*);
+                              hipStream_t, KernelTimer*, bool);
 template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, const c64*, int64_t, c64*, int64_t,
-                             hipStream_t, KernelTimer*);
+                             hipStream_t, KernelTimer*, bool);
 template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
 template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t,

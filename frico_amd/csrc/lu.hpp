@@ -29,9 +29,11 @@ struct LuWork {
     int nevents = 0;
 };
 
+// C -= A B.  exact: four real products per complex multiply-add even where three would be used
+// (residuals, whose rounding floor bounds the refinement).
 template <typename T>
 int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb, cx<T>* C,
-             int64_t ldc, hipStream_t st, KernelTimer* kt);
+             int64_t ldc, hipStream_t st, KernelTimer* kt, bool exact = false);
 // Residual GEMM with a complex64-stored A and FP64 arithmetic: C -= double(A) * B.
 int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const c128* B, int64_t ldb, c128* C,
                    int64_t ldc, hipStream_t st, KernelTimer* kt);
