@@ -168,6 +168,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->lookahead = std::atoi(la) != 0;
     if (const char* fs = std::getenv("FRICO_LU_FUSED_SOLVE"))
         c->fused_solve = std::atoi(fs) != 0;
+    if (const char* au = std::getenv("FRICO_LU_AUGMENT"))
+        c->augment = std::atoi(au) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -728,6 +730,7 @@ static int rhs_alloc(frico_ctx* c, int64_t nrhs)
         return rc;
     c->nrhs = nrhs;
     c->have_x = false;
+    c->rhs_version++;
     return 0;
 }
 
@@ -864,7 +867,11 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     if (rc || (rc = ensure_lu_work(c)) || (rc = c->ipiv.ensure((size_t)c->N)))
         return rc;
     const int64_t N = c->N;
-    if (form == FRICO_SOLVE_C64_IR && (rc = c->LU32.ensure((size_t)(N * N))))
+    // the complex64 factorisation carries the current right-hand sides along (forward elimination
+    // inside the trailing products); the first refinement sweep then only substitutes backward
+    c->aug_cols = (form == FRICO_SOLVE_C64_IR && c->augment && c->nrhs > 0) ? c->nrhs : 0;
+    c->aug_valid = false;
+    if (form == FRICO_SOLVE_C64_IR && (rc = c->LU32.ensure((size_t)(N * (N + c->aug_cols)))))
         return rc;
     c->factored = -1; // the factors are about to be overwritten
     c->nb_factored = block_width(c);
@@ -902,8 +909,11 @@ int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
             rc = launch_convert_c128_c64(c->Z64.p, N, c->LU32.p, N, N, N, c->st);
         else
             rc = launch_copy_bytes(c->Z32.p, c->LU32.p, (size_t)(N * N) * sizeof(c64), c->st);
+        if (!rc && c->aug_cols > 0)
+            rc = launch_convert_c128_c64(c->B.p, N, c->LU32.p + N * N, N, N, c->aug_cols, c->st);
         if (!rc)
-            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, c->profiling ? &c->t_gemm : nullptr);
+            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, c->profiling ? &c->t_gemm : nullptr,
+                                  c->aug_cols);
     }
     phase_end(c, c->t_factor);
     return rc;
@@ -914,6 +924,8 @@ int factor_finish(frico_ctx* c, int form, const LuWork& wk)
     if (form == FRICO_SOLVE_C128)
         c->z64_factored = true;
     c->factored = form;
+    c->aug_valid = c->aug_cols > 0;
+    c->aug_version = c->rhs_version;
     c->wv_ok = wk.wv != nullptr;
     return read_lu_flags(c);
 }
@@ -987,14 +999,22 @@ int refine_enqueue(frico_ctx* c, bool first)
     int rc;
     KernelTimer& tt = first ? c->t_solve : c->t_refine;
     phase_begin(c, tt);
-    if (first && ((rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)) ||
-                  (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st))))
+    if (first && (rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)))
         return rc;
-    if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
-        return rc;
-    if ((rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
-                              c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
-        return rc;
+    if (first && c->aug_valid && c->aug_version == c->rhs_version && c->aug_cols == nr) {
+        // the factorisation left L^{-1} P B in the appended columns
+        if ((rc = launch_copy_bytes(c->LU32.p + N * N, c->D32.p, (size_t)(N * nr) * sizeof(c64), c->st)) ||
+            (rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st)))
+            return rc;
+    } else {
+        if (first && (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st)))
+            return rc;
+        if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
+            return rc;
+        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
+                                  c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
+            return rc;
+    }
     if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
         return rc;
     phase_end(c, tt);

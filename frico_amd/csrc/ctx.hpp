@@ -206,6 +206,12 @@ struct frico_ctx {
     DevBuf<unsigned char> ysolve; // n x nrhs scratch of the fused solves
     bool wv_ok = false;           // wv holds the products of the current factors
     bool fused_solve = true;
+    // Right-hand sides carried through the complex64 factorisation (FRICO_LU_AUGMENT=0 disables):
+    // LU32 then holds aug_cols more columns, L^{-1} P B for the B of rhs_version aug_version.
+    bool augment = true;
+    int64_t aug_cols = 0;
+    bool aug_valid = false;
+    uint64_t rhs_version = 0, aug_version = 0;
     // rhs / solution
     int64_t nrhs = 0;
     bool have_x = false;

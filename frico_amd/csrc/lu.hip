@@ -1320,7 +1320,8 @@ static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, const 
 //   main: panel p -> [P_p] -> (wait W_{p-1}) -> narrow update p -> panel p+1
 //   side: (wait P_p) -> wide update p -> [W_p]
 template <typename T>
-int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer)
+int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer,
+              int64_t naug)
 {
     if (nb > LU_NB_MAX || nb < 1)
         return set_error("lu: block size out of range"), -1;
@@ -1412,7 +1413,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
             const int w = (int)((n - k) < nb ? (n - k) : nb);
             const int *l1, *l2;
             lists(p, l1, l2);
-            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, l1, l2, k, w, k + w, n - k - w, st, gemm_timer)))
+            if ((rc = panel(p)) || (rc = step_update<T>(A, n, lda, l1, l2, k, w, k + w, n + naug - k - w, st, gemm_timer)))
                 return rc;
         }
     } else {
@@ -1428,13 +1429,21 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
             const int64_t k = p * nb;
             const int w = (int)((n - k) < nb ? (n - k) : nb);
             const int64_t nright = n - k - w;
-            if (nright <= 0)
-                break;
-            const int64_t nnext = nright < nb ? nright : nb;
-            const int64_t nrest = nright - nnext;
-            FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
             const int *l1, *l2;
             lists(p, l1, l2);
+            if (nright <= 0) {
+                // the last panel's update of the appended columns
+                if (naug > 0) {
+                    if (prev_wide)
+                        FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
+                    if ((rc = step_update<T>(A, n, lda, l1, l2, k, w, n, naug, st, gemm_timer)))
+                        return rc;
+                }
+                break;
+            }
+            const int64_t nnext = nright < nb ? nright : nb;
+            const int64_t nrest = nright + naug - nnext; // the appended columns go with the wide part
+            FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
             if (nrest > 0) {
                 FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
                 if ((rc = step_update<T>(A, n, lda, l1, l2, k, w, k + w + nnext, nrest, wk.side, gemm_timer)))
@@ -1497,6 +1506,25 @@ static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* per
         launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, Y + kc, n, B + kc, ldb, nrhs, st);
         if (kc > 0) {
             int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, B + kc, ldb, Y, n, st, nullptr);
+            if (rc)
+                return rc;
+        }
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Backward half of the solve in place: per 64-row chunk j from the last, x_j = U_jj^{-1} b_j by
+// substitution, then b_above -= U(above, j) x_j.
+template <typename T>
+int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st)
+{
+    const int64_t nch = (n + 63) / 64;
+    for (int64_t ch = nch - 1; ch >= 0; ch--) {
+        const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
+        launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, B + kc, ldb, B + kc, ldb, nrhs, st);
+        if (kc > 0) {
+            int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, B + kc, ldb, B, ldb, st, nullptr);
             if (rc)
                 return rc;
         }
@@ -1637,8 +1665,10 @@ template int gemm_sub<double>(int64_t, int64_t, int64_t, const c128*, int64_t, c
                               hipStream_t, KernelTimer*, bool);
 template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, const c64*, int64_t, c64*, int64_t,
                              hipStream_t, KernelTimer*, bool);
-template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
-template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*);
+template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
+template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
+template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t);
+template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t);
 template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t,
                               const void*, const void*, void*);
 template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t,

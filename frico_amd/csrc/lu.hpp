@@ -38,8 +38,11 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
 int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const c128* B, int64_t ldb, c128* C,
                    int64_t ldc, hipStream_t st, KernelTimer* kt);
 template <typename T> int panel_rows(int w, bool shared_cu = false);
+// naug > 0: columns [n, n + naug) of A (same leading dimension) are right-hand sides carried
+// through the factorisation; they end as L^{-1} P B, ready for lu_solve_upper.
 template <typename T>
-int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer);
+int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer,
+              int64_t naug = 0);
 int launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 int launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st);
 // ints per panel of LuWork::perm
@@ -53,6 +56,9 @@ size_t lu_exchange_words(int gmax);
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
              hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr);
+// B = U^{-1} B in place (the backward half of the solve), U the upper factor of lu_factor.
+template <typename T>
+int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st);
 template <typename T>
 int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
                         int64_t nrhs, hipStream_t st);
