@@ -963,26 +963,52 @@ __global__ __launch_bounds__(256) void trsm_block(int w, const cx<T>* __restrict
 {
     __shared__ cx<T> sE[64][64];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int e = tid; e < 64 * 64; e += 256) {
-        const int r = e & 63, c = e >> 6;                 // D(r, c); r fastest, coalesced
-        const int i = TRANSD ? r : c, j = TRANSD ? c : r; // E_ji
-        const bool dep = UPPER ? j < i : j > i;
-        sE[i][j ^ i] = (dep && r < w && c < w) ? D[r + (int64_t)c * ldd] : cx<T>{0, 0};
-    }
-    cx<T> rd{1, 0};
-    if (!UNIT && lane < w)
-        rd = cdiv(cx<T>{1, 0}, D[lane + (int64_t)lane * ldd]);
     const int64_t cb = ((int64_t)blockIdx.x * TRSM_WAVES + wv) * NC; // first column of this wavefront
+    // every load in flight at once: the block (16 entries per thread, unconditional reads of an
+    // in-range entry, masked at the LDS store), the right-hand sides and the diagonal
+    cx<T> dv[16];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int e = tid + 256 * t, r = e & 63, c = e >> 6; // D(r, c); r fastest, coalesced
+        dv[t] = D[(r < w ? r : 0) + (int64_t)(c < w ? c : 0) * ldd];
+    }
     cx<T> b[NC];
 #pragma unroll
     for (int q = 0; q < NC; q++)
-        b[q] = (lane < w && cb + q < ncols) ? Bin[lane + (cb + q) * ldi] : cx<T>{0, 0};
+        b[q] = Bin[(lane < w ? lane : 0) + (cb + q < ncols ? cb + q : 0) * ldi];
+    const cx<T> dd = D[(lane < w ? lane : 0) * (ldd + 1)];
+#pragma unroll
+    for (int t = 0; t < 16; t++) {
+        const int e = tid + 256 * t, r = e & 63, c = e >> 6;
+        const int i = TRANSD ? r : c, j = TRANSD ? c : r; // E_ji
+        const bool dep = UPPER ? j < i : j > i;
+        sE[i][j ^ i] = (dep && r < w && c < w) ? dv[t] : cx<T>{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < NC; q++)
+        if (!(lane < w && cb + q < ncols))
+            b[q] = cx<T>{0, 0};
+    cx<T> rd{1, 0};
+    if (!UNIT && lane < w)
+        rd = cdiv(cx<T>{1, 0}, dd);
     __syncthreads();
+    // coefficients of the next PF steps in flight while a step's readlane chain runs
+    constexpr int PF = 4;
+    cx<T> ep[PF];
+#pragma unroll
+    for (int s = 0; s < PF; s++) {
+        const int i = UPPER ? 63 - s : s;
+        ep[s] = sE[i][lane ^ i];
+    }
 #pragma unroll
     for (int s = 0; s < 64; s++) {
         const int i = UPPER ? 63 - s : s;
+        const cx<T> e = ep[s % PF];
+        if (s + PF < 64) {
+            const int i2 = UPPER ? 63 - (s + PF) : s + PF;
+            ep[s % PF] = sE[i2][lane ^ i2];
+        }
         if (i < w) {
-            const cx<T> e = sE[i][lane ^ i];
 #pragma unroll
             for (int q = 0; q < NC; q++) {
                 const cx<T> x = read_lane(UNIT ? b[q] : b[q] * rd, i);
