@@ -455,28 +455,35 @@ __global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda,
                                              const int* __restrict__ perm, const int* __restrict__ perm2)
 {
     __shared__ int2 s_mv[2 * LU_NB_MAX];
-    __shared__ cx<T> buf[2 * LU_NB_MAX * LASWP_COLS];
+    __shared__ int s_x2[2 * LU_NB_MAX]; // rows of perm2
+    static_assert(2 * LU_NB_MAX <= 256, "one list entry per thread");
     const int tid = threadIdx.x;
+    // the count and every entry slot read together (a list occupies PERM_STRIDE ints)
     const int L1 = perm[0];
-    int L = L1;
-    for (int e = tid; e < L1; e += 256)
-        s_mv[e] = int2{perm[1 + 2 * e], perm[2 + 2 * e]};
+    const int2 p1{perm[1 + 2 * tid], perm[2 + 2 * tid]};
+    int2 a{0, 0};
+    int L2 = 0;
     if (perm2) {
-        const int L2 = perm2[0];
+        L2 = perm2[0];
+        a = int2{perm2[1 + 2 * tid], perm2[2 + 2 * tid]};
+    }
+    if (tid < L1)
+        s_mv[tid] = p1;
+    if (tid < L2)
+        s_x2[tid] = a.x;
+    int L = L1;
+    if (perm2) {
         __syncthreads();
-        int2 a{0, 0}, b{0, 0}; // this thread's entries: a of perm2 (composed), b of perm (or dropped)
-        if (tid < L2) {
-            a = int2{perm2[1 + 2 * tid], perm2[2 + 2 * tid]};
+        // this thread's entries: a of perm2 (source composed with perm), b of perm (or dropped)
+        int2 b = p1;
+        if (tid < L2)
             for (int j = 0; j < L1; j++)
                 if (s_mv[j].x == a.y)
                     a.y = s_mv[j].y;
-        }
-        if (tid < L1) {
-            b = s_mv[tid];
+        if (tid < L1)
             for (int j = 0; j < L2; j++)
-                if (perm2[1 + 2 * j] == b.x)
+                if (s_x2[j] == b.x)
                     b.x = -1;
-        }
         __syncthreads();
         if (tid < L1)
             s_mv[tid] = b;
@@ -487,17 +494,26 @@ __global__ __launch_bounds__(256) void laswp(cx<T>* __restrict__ A, int64_t lda,
     __syncthreads();
     const int64_t c0 = col0 + (int64_t)blockIdx.x * LASWP_COLS;
     const int nc = (int)((col0 + ncols - c0) < LASWP_COLS ? (col0 + ncols - c0) : LASWP_COLS);
-    for (int e = tid; e < L * LASWP_COLS; e += 256) {
+    // all of a thread's gathers in flight at once (unconditional reads of the column's first
+    // entry where an element does not move), then the scatters
+    constexpr int PER = 2 * LU_NB_MAX * LASWP_COLS / 256;
+    T vr[PER], vi[PER];
+    int64_t dst[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = tid + 256 * t;
         const int m = e % L, cc = e / L;
-        if (cc < nc && s_mv[m].x >= 0)
-            buf[e] = A[(int64_t)s_mv[m].y + (c0 + cc) * lda];
+        const bool mv = e < L * LASWP_COLS && cc < nc && s_mv[m].x >= 0;
+        const cx<T> x = A[(mv ? (int64_t)s_mv[m].y : 0) + (c0 + (mv ? cc : 0)) * lda];
+        vr[t] = x.re;
+        vi[t] = x.im;
+        dst[t] = mv ? (int64_t)s_mv[m].x + (c0 + cc) * lda : -1;
     }
     __syncthreads();
-    for (int e = tid; e < L * LASWP_COLS; e += 256) {
-        const int m = e % L, cc = e / L;
-        if (cc < nc && s_mv[m].x >= 0)
-            A[(int64_t)s_mv[m].x + (c0 + cc) * lda] = buf[e];
-    }
+#pragma unroll
+    for (int t = 0; t < PER; t++)
+        if (dst[t] >= 0)
+            A[dst[t]] = cx<T>{vr[t], vi[t]};
 }
 
 template <typename T>
