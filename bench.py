@@ -32,11 +32,11 @@ DEGREE = 5
 PEAK_FP32_MFMA = 157.3  # TFLOP/s, MI355X dense FP32 matrix
 PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0       # GB/s
-# HBM bytes per trailing-update launch of cgemm_sub<float> at N = 12160 (average over the 189
-# launches of one factorisation), from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by a copy
-# kernel of known size (profiles/plate_N12160_c64ir_hbm_traffic.md); algorithmic bytes per
-# launch: 792.8e6.
-GEMM_F32_TRAFFIC = 1069.5e6
+# HBM bytes per timed trailing-update launch of cgemm_sub<float> (K = 128) at N = 12160, nb = 128
+# (average over the 188 launches of one factorisation), from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs scaled by a copy kernel of known size (profiles/plate_N12160_c64ir_hbm_traffic.md);
+# algorithmic bytes per launch: 419.3e6.
+GEMM_F32_TRAFFIC = 700.1e6
 
 
 def lu_flops(n, nrhs):
@@ -206,7 +206,7 @@ def main():
                    "tol": args.tol, "block": args.nb},
         "roofline": {"bound": "mfma", "kernel": "cgemm_sub<float> (LU trailing update)",
                      "achieved": gemm_tf, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                     "frac": gemm_tf / PEAK_FP32_MFMA, "traffic": GEMM_F32_TRAFFIC if args.n == 64 else None,
+                     "frac": gemm_tf / PEAK_FP32_MFMA, "traffic": GEMM_F32_TRAFFIC if (args.n, args.nb) == (64, 128) else None,
                      "launches": prof_ir["gemm_launches"], "avg_launch_ms": gemm_avg_ms},
         "refinement": {"iterations": infos_ir[-1].iterations, "max_rel_residual": infos_ir[-1].max_rel_residual},
         "phases_ms": {k: prof_ir[k] / args.steps for k in ("fill_ms", "rhs_ms", "factor_ms", "solve_ms", "refine_ms")},
