@@ -170,6 +170,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->fused_solve = std::atoi(fs) != 0;
     if (const char* au = std::getenv("FRICO_LU_AUGMENT"))
         c->augment = std::atoi(au) != 0;
+    if (const char* sr = std::getenv("FRICO_STREAM_RESIDUAL"))
+        c->stream_residual = std::atoi(sr) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -993,35 +995,59 @@ int refine_prepare(frico_ctx* c)
     return 0;
 }
 
-int refine_enqueue(frico_ctx* c, bool first)
+int refine_enqueue(frico_ctx* c, bool first, bool overlap)
 {
     const int64_t N = c->N, nr = c->nrhs;
     int rc;
     KernelTimer& tt = first ? c->t_solve : c->t_refine;
+    const bool carried = first && c->aug_valid && c->aug_version == c->rhs_version && c->aug_cols == nr;
+    // X += D and R = B - Z X taken block by block beside the backward substitution: needs the FP64
+    // Z, the second stream, its events and the carried or fused solve
+    ResidStream rs;
+    const bool stream = overlap && c->stream_residual && c->side && c->zprec == FRICO_C128 && !c->z64_factored &&
+                        (carried || c->wv_ok) && (int64_t)c->lu_events.size() >= (N + 127) / 128 + 1;
+    if (stream) {
+        rs.side = c->side;
+        rs.ev = c->lu_events.data();
+        rs.nev = (int)c->lu_events.size();
+        rs.Z = c->Z64.p;
+        rs.ldz = N;
+        rs.S = c->X.p;
+        rs.lds = N;
+        rs.R = c->R.p;
+        rs.ldr = N;
+    }
     phase_begin(c, tt);
     if (first && (rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)))
         return rc;
-    if (first && c->aug_valid && c->aug_version == c->rhs_version && c->aug_cols == nr) {
-        // the factorisation left L^{-1} P B in the appended columns
-        if ((rc = launch_copy_bytes(c->LU32.p + N * N, c->D32.p, (size_t)(N * nr) * sizeof(c64), c->st)) ||
-            (rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st)))
-            return rc;
-    } else {
-        if (first && (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st)))
-            return rc;
-        if ((rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st)))
-            return rc;
-        if ((rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
-                                  c->wv_ok ? c->wv.p : nullptr, c->ysolve.p)))
-            return rc;
-    }
-    if ((rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
+    if (first && !carried && (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st)))
+        return rc;
+    // the correction's right-hand side: the carried L^{-1} P B, or the current residual
+    if (carried)
+        rc = launch_copy_bytes(c->LU32.p + N * N, c->D32.p, (size_t)(N * nr) * sizeof(c64), c->st);
+    else
+        rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st);
+    if (rc)
+        return rc;
+    // the residual of this sweep starts from B
+    if (stream && (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st)))
+        return rc;
+    if (carried)
+        rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st, stream ? &rs : nullptr);
+    else
+        rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
+                             c->wv_ok ? c->wv.p : nullptr, c->ysolve.p, stream ? &rs : nullptr);
+    if (rc)
+        return rc;
+    if (!stream && (rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))
         return rc;
     phase_end(c, tt);
-    phase_begin(c, c->t_refine);
-    if ((rc = residual(c, nullptr)))
-        return rc;
-    phase_end(c, c->t_refine);
+    if (!stream) {
+        phase_begin(c, c->t_refine);
+        if ((rc = residual(c, nullptr)))
+            return rc;
+        phase_end(c, c->t_refine);
+    }
     return launch_col_norm2(c->R.p, N, N, nr, c->norms.p, c->st);
 }
 
@@ -1084,7 +1110,7 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     inf.converged = 0;
     for (int it = 1; it <= maxit; it++) {
         double worst = 0;
-        if ((rc = refine_enqueue(c, it == 1)) || (rc = read_refine_norms(c, bn, &worst)))
+        if ((rc = refine_enqueue(c, it == 1, true)) || (rc = read_refine_norms(c, bn, &worst)))
             return rc;
         inf.iterations = it;
         inf.max_rel_residual = worst;

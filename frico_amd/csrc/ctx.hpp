@@ -209,6 +209,9 @@ struct frico_ctx {
     // Right-hand sides carried through the complex64 factorisation (FRICO_LU_AUGMENT=0 disables):
     // LU32 then holds aug_cols more columns, L^{-1} P B for the B of rhs_version aug_version.
     bool augment = true;
+    // Refinement residuals updated beside the backward substitution on the second stream
+    // (FRICO_STREAM_RESIDUAL=0: one full residual product after each solve).
+    bool stream_residual = true;
     int64_t aug_cols = 0;
     bool aug_valid = false;
     uint64_t rhs_version = 0, aug_version = 0;

@@ -18,6 +18,7 @@
 #include "lu.hpp"
 
 #include <cstdlib>
+#include <type_traits>
 
 namespace frico {
 
@@ -1520,15 +1521,60 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
     return 0;
 }
 
-// Fused form of the solve: per 64-row chunk j (k = 64 j, width w)
-//   forward:  y_j = Linv_j b_j  into Y,  b_below -= W(:, j) b_j      (after the panel's interchanges)
-//   backward: x_j = U_jj^{-1} y_j (substitution) into B,  y_above -= U(above, j) x_j
-// forward: one launch per chunk plus one laswp per panel; backward: a substitution and a product.
+constexpr int RESID_ROWS = 512; // rows of the solution per residual product beside the substitution
+
+// Backward half of the solve: per 64-row chunk j from the last, x_j = U_jj^{-1} y_j by substitution
+// (from Y into X, or in place when Y == X), then y_above -= U(above, j) x_j.  With rs (complex64
+// solves of a refinement), each RESID_ROWS-row block of X, once final, is added to the FP64 solution and
+// taken off the residual on a second stream while the substitution goes on:
+// S(block) += X(block), R -= Z(:, block) S(block); with R = B beforehand, R ends as B - Z S.
 template <typename T>
-static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb,
-                          int64_t nrhs, hipStream_t st, const cx<T>* inv, const cx<T>* WV, cx<T>* Y)
+static int lu_backward(const cx<T>* A, int64_t n, int64_t lda, cx<T>* Y, int64_t ldy, cx<T>* X, int64_t ldx,
+                       int64_t nrhs, hipStream_t st, const ResidStream* rs)
 {
-    const int64_t npan = (n + nb - 1) / nb, nch = (n + 63) / 64;
+    const int64_t nch = (n + 63) / 64;
+    const int64_t nblk = (n + RESID_ROWS - 1) / RESID_ROWS;
+    if (rs && (!std::is_same<T, float>::value || rs->nev < nblk + 1))
+        return set_error("solve: residual stream needs complex64 factors and %lld events", (long long)(nblk + 1)), -1;
+    for (int64_t ch = nch - 1; ch >= 0; ch--) {
+        const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
+        launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, Y + kc, ldy, X + kc, ldx, nrhs, st);
+        if (kc > 0) {
+            int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, X + kc, ldx, Y, ldy, st, nullptr);
+            if (rc)
+                return rc;
+        }
+        if constexpr (std::is_same<T, float>::value) {
+            if (rs && kc % RESID_ROWS == 0) {
+                const int64_t kb = kc, bw = (n - kb) < RESID_ROWS ? (n - kb) : RESID_ROWS, b = kb / RESID_ROWS;
+                FRICO_HIP_CHECK(hipEventRecord(rs->ev[b], st));
+                FRICO_HIP_CHECK(hipStreamWaitEvent(rs->side, rs->ev[b], 0));
+                int rc = launch_add_correction(rs->S + kb, rs->lds, X + kb, ldx, bw, nrhs, rs->side);
+                if (!rc)
+                    rc = gemm_sub<double>(n, nrhs, bw, rs->Z + kb * rs->ldz, rs->ldz, rs->S + kb, rs->lds, rs->R, rs->ldr,
+                                          rs->side, nullptr, true);
+                if (rc)
+                    return rc;
+            }
+        }
+    }
+    if (rs) {
+        // the caller's stream continues after the last residual product
+        FRICO_HIP_CHECK(hipEventRecord(rs->ev[nblk], rs->side));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(st, rs->ev[nblk], 0));
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// Forward half of the fused solve: per 64-row chunk j (k = 64 j, width w), after the panel's
+// interchanges, y_j = Linv_j b_j into Y and b_below -= W(:, j) b_j; one launch per chunk plus
+// one laswp per panel.
+template <typename T>
+static int lu_forward_fused(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb,
+                            int64_t nrhs, hipStream_t st, const cx<T>* inv, const cx<T>* WV, cx<T>* Y)
+{
+    const int64_t npan = (n + nb - 1) / nb;
     const unsigned gy = (unsigned)((nrhs + GBN - 1) / GBN);
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
@@ -1538,51 +1584,41 @@ static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* per
             const int64_t kc = k + s, ws = (n - kc) < 64 ? (n - kc) : 64, ch = kc / 64;
             const unsigned gx = (unsigned)((n - kc + GBM - 1) / GBM);
             launch_kernel(&solve_chunk<T>, dim3(gx, gy), dim3(256), 0, st, kc, n - kc, nrhs, ws, (int64_t)0,
-                               inv + (size_t)(2 * ch) * 4096, WV + kc * n, n, B + kc, ldb, B, ldb, Y, n);
-        }
-    }
-    const unsigned gcols = (unsigned)((nrhs + 63) / 64);
-    for (int64_t ch = nch - 1; ch >= 0; ch--) {
-        const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
-        // x_j = U_jj^{-1} y_j into B, then y_above -= U(above, j) x_j
-        launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, Y + kc, n, B + kc, ldb, nrhs, st);
-        if (kc > 0) {
-            int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, B + kc, ldb, Y, n, st, nullptr);
-            if (rc)
-                return rc;
+                          inv + (size_t)(2 * ch) * 4096, WV + kc * n, n, B + kc, ldb, B, ldb, Y, n);
         }
     }
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }
 
-// Backward half of the solve in place: per 64-row chunk j from the last, x_j = U_jj^{-1} b_j by
-// substitution, then b_above -= U(above, j) x_j.
+// Fused form of the solve: the forward half above, then the backward half from Y into B.
 template <typename T>
-int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st)
+static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb,
+                          int64_t nrhs, hipStream_t st, const cx<T>* inv, const cx<T>* WV, cx<T>* Y,
+                          const ResidStream* rs = nullptr)
 {
-    const int64_t nch = (n + 63) / 64;
-    for (int64_t ch = nch - 1; ch >= 0; ch--) {
-        const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
-        launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, B + kc, ldb, B + kc, ldb, nrhs, st);
-        if (kc > 0) {
-            int rc = gemm_sub<T>(kc, nrhs, ws, A + kc * lda, lda, B + kc, ldb, B, ldb, st, nullptr);
-            if (rc)
-                return rc;
-        }
-    }
-    FRICO_HIP_CHECK(hipGetLastError());
-    return 0;
+    int rc = lu_forward_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, WV, Y);
+    return rc ? rc : lu_backward<T>(A, n, lda, Y, n, B, ldb, nrhs, st, rs);
+}
+
+// Backward half of the solve in place (see lu_backward).
+template <typename T>
+int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st,
+                   const ResidStream* rs)
+{
+    return lu_backward<T>(A, n, lda, B, ldb, B, ldb, nrhs, st, rs);
 }
 
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st, const void* dinv, const void* wv, void* ybuf)
+             hipStream_t st, const void* dinv, const void* wv, void* ybuf, const ResidStream* rs)
 {
     const int64_t npan = (n + nb - 1) / nb;
     const cx<T>* inv = (const cx<T>*)dinv;
     if (inv && wv && ybuf && nb % 64 == 0 && n > 64)
-        return lu_solve_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf);
+        return lu_solve_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf, rs);
+    if (rs)
+        return set_error("solve: the residual stream needs the fused solve"), -1;
     const unsigned gcols = (unsigned)((nrhs + 63) / 64);
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
@@ -1709,12 +1745,13 @@ template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, con
                              hipStream_t, KernelTimer*, bool);
 template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
-template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t);
-template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t);
+template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t,
+                                    const ResidStream*);
+template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t, const ResidStream*);
 template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t,
-                              const void*, const void*, void*);
+                              const void*, const void*, void*, const ResidStream*);
 template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t,
-                             const void*, const void*, void*);
+                             const void*, const void*, void*, const ResidStream*);
 template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t,
                                          hipStream_t);
 template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,

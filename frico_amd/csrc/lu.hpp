@@ -53,12 +53,30 @@ size_t lu_exchange_words(int gmax);
 // dinv: the 64 x 64 diagonal-chunk inverses written by lu_factor (LuWork::dinv), or null to use
 // in-LDS triangular solves.  wv: the W/V products of lu_factor (LuWork::wv), or null; with wv the
 // solve runs one product per 64-row chunk and needs ybuf, an n x nrhs scratch (leading dimension n).
+// A second stream on which the backward half of a complex64 refinement solve adds each finished
+// 128-row block of the correction X to the FP64 solution S and takes it off the residual:
+// S(block) += X(block, :), R -= Z(:, block) S(block, :).  With R = B beforehand, R ends as
+// B - Z S, computed as the full product would (the K sum in blocks).  ev: at least
+// ceil(n / 128) + 1 events.
+struct ResidStream {
+    hipStream_t side = nullptr;
+    hipEvent_t* ev = nullptr;
+    int nev = 0;
+    const c128* Z = nullptr;
+    int64_t ldz = 0;
+    c128* S = nullptr;
+    int64_t lds = 0;
+    c128* R = nullptr;
+    int64_t ldr = 0;
+};
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr);
+             hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr,
+             const ResidStream* rs = nullptr);
 // B = U^{-1} B in place (the backward half of the solve), U the upper factor of lu_factor.
 template <typename T>
-int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st);
+int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st,
+                   const ResidStream* rs = nullptr);
 template <typename T>
 int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
                         int64_t nrhs, hipStream_t st);

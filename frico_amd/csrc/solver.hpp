@@ -17,9 +17,11 @@ int factor_finish(frico_ctx* c, int form, const LuWork& wk);
 
 // Complex64-LU refinement.  refine_prepare allocates; refine_enqueue(first) runs one sweep:
 //   first: X = 0, R = B;   then D = LU^{-1} R (complex64), X += D, R = B - Z X,
-// and leaves the squared column norms of R in c->norms.
+// and leaves the squared column norms of R in c->norms.  overlap: X += D and R = B - Z X run on
+// the context's second stream, 128 rows at a time as the backward substitution finishes them
+// (only outside graph recording).
 int refine_prepare(frico_ctx* c);
-int refine_enqueue(frico_ctx* c, bool first);
+int refine_enqueue(frico_ctx* c, bool first, bool overlap = false);
 // Column 2-norms of a device matrix (synchronises).
 int col_norms(frico_ctx* c, const c128* X, int64_t cols, std::vector<double>& out);
 // Reads c->norms (synchronises) and returns max_j ||R_j|| / ||B_j|| given the B norms.

@@ -184,6 +184,37 @@ def test_c64_ir_carried_rhs_matches_the_separate_forward_solve(monkeypatch):
     assert relmax(out[0][1], Xref) <= 1e-11 and relmax(out[0][1], out[1][1]) <= 1e-12
 
 
+@pytest.mark.parametrize("nb", [64, 128])
+def test_streamed_residual_matches_the_full_residual(monkeypatch, nb):
+    """With X += D and R = B - Z X taken 128 rows at a time beside the backward substitution, the
+    sweeps stop at the same count as with one full residual product after each solve, the reported
+    residual is a true residual (frico_residual's fresh product agrees to 5 %, plus 2e-14 of
+    rounding noise at the FP64 floor), and the solutions agree."""
+    import frico_amd
+    n = 650
+    rng = np.random.default_rng(8)
+    A = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)) + 0.5 * np.sqrt(n) * np.eye(n)
+    B = rng.uniform(-1, 1, (n, 37)) + 1j * rng.uniform(-1, 1, (n, 37))
+    out = []
+    for sr in ("1", "0"):
+        monkeypatch.setenv("FRICO_STREAM_RESIDUAL", sr)
+        with frico_amd.Context(0) as c:
+            c.set_block(nb)
+            c.set_matrix(A)
+            c.set_rhs(B)
+            c.factor("c64_ir")
+            info = c.solve(tol=1e-11, maxit=12)
+            assert info.converged
+            fresh = np.max(c.residual())
+            out.append((c.solution(), info.max_rel_residual, info.iterations, fresh))
+    for x, rep, its, fresh in out:
+        assert abs(rep - fresh) <= 0.05 * fresh + 2e-14, (rep, fresh)
+    (x1, r1, i1, f1), (x0, r0, i0, f0) = out
+    print(f"streamed: {i1} sweeps, residual {r1:.2e} (fresh {f1:.2e}); full: {i0} sweeps, {r0:.2e} (fresh {f0:.2e})")
+    assert i1 == i0
+    assert relmax(x1, x0) <= 1e-11 and relmax(x1, np.linalg.solve(A, B)) <= 1e-10
+
+
 def test_lu_needs_pivoting():
     import frico_amd
     n = 200
