@@ -323,29 +323,47 @@ __global__ __launch_bounds__(256) void tripair_f32(DevMesh m, double k, double i
 
 // ----------------------------------------------------------------------------------------------
 // Assembly: Z(j,i) = prodl * (C[i-][j-] - C[i-][j+] - C[i+][j-] + C[i+][j+]).
-// Grid: x over rows j (contiguous in the column-major Z), y over columns i.
+// One workgroup per column i, sweeping the rows j (contiguous in the column-major Z): the two
+// C rows of column i are then read by one CU, whose L2 serves both the j- and the j+ entries
+// (nearby in C for nearby j), instead of by workgroups on every XCD.  Two rows per thread per
+// pass keep eight gathers in flight.
 // ----------------------------------------------------------------------------------------------
 template <typename TC, typename TZ>
 __global__ __launch_bounds__(256) void assemble(DevMesh m, const cx<TC>* __restrict__ C, cx<TZ>* __restrict__ Z,
                                                 int64_t ldz, int64_t col0, int64_t ncols)
 {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t ic = blockIdx.y;
-    if (j >= m.N || ic >= ncols)
+    const int64_t ic = blockIdx.x;
+    if (ic >= ncols)
         return;
     const int64_t i = col0 + ic;
     const int im = m.rwg_t[2 * i], ip = m.rwg_t[2 * i + 1];
     const int lim = m.rwg_le[2 * i], lip = m.rwg_le[2 * i + 1];
-    const int jm = m.rwg_t[2 * j], jp = m.rwg_t[2 * j + 1];
-    const int ljm = m.rwg_le[2 * j], ljp = m.rwg_le[2 * j + 1];
     const cx<TC>* rowm = C + (size_t)(3 * im + lim) * m.T * 3;
     const cx<TC>* rowp = C + (size_t)(3 * ip + lip) * m.T * 3;
-    cx<TC> a = rowm[3 * jm + ljm], b = rowm[3 * jp + ljp], c = rowp[3 * jm + ljm], d = rowp[3 * jp + ljp];
-    const double prodl = 0.25 * m.rwg_len[i] * m.rwg_len[j] * INV_PI;
-    const TC pl = (TC)prodl;
-    TC re = ((a.re - b.re) - c.re) + d.re;
-    TC imv = ((a.im - b.im) - c.im) + d.im;
-    Z[j + i * ldz] = cx<TZ>{(TZ)(pl * re), (TZ)(pl * imv)};
+    const double li = 0.25 * m.rwg_len[i];
+    auto entry = [&](int64_t j, cx<TC> a, cx<TC> b, cx<TC> c, cx<TC> d) {
+        const double prodl = li * m.rwg_len[j] * INV_PI;
+        const TC pl = (TC)prodl;
+        TC re = ((a.re - b.re) - c.re) + d.re;
+        TC imv = ((a.im - b.im) - c.im) + d.im;
+        Z[j + i * ldz] = cx<TZ>{(TZ)(pl * re), (TZ)(pl * imv)};
+    };
+    for (int64_t j0 = threadIdx.x; j0 < m.N; j0 += 512) {
+        const int64_t j1 = j0 + 256;
+        const bool has1 = j1 < m.N;
+        const int64_t jj1 = has1 ? j1 : j0;
+        const int jm0 = m.rwg_t[2 * j0], jp0 = m.rwg_t[2 * j0 + 1];
+        const int ljm0 = m.rwg_le[2 * j0], ljp0 = m.rwg_le[2 * j0 + 1];
+        const int jm1 = m.rwg_t[2 * jj1], jp1 = m.rwg_t[2 * jj1 + 1];
+        const int ljm1 = m.rwg_le[2 * jj1], ljp1 = m.rwg_le[2 * jj1 + 1];
+        const cx<TC> a0 = rowm[3 * jm0 + ljm0], b0 = rowm[3 * jp0 + ljp0], c0 = rowp[3 * jm0 + ljm0],
+                     d0 = rowp[3 * jp0 + ljp0];
+        const cx<TC> a1 = rowm[3 * jm1 + ljm1], b1 = rowm[3 * jp1 + ljp1], c1 = rowp[3 * jm1 + ljm1],
+                     d1 = rowp[3 * jp1 + ljp1];
+        entry(j0, a0, b0, c0, d0);
+        if (has1)
+            entry(j1, a1, b1, c1, d1);
+    }
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -652,7 +670,7 @@ int launch_fill_quad(const DevMesh& m, double k, double inv_ksq, int prec, void*
     const int64_t slab = 65535;
     for (int64_t c0 = 0; c0 < m.N; c0 += slab) {
         int64_t nc = m.N - c0 < slab ? m.N - c0 : slab;
-        dim3 g2((m.N + 255) / 256, (unsigned)nc);
+        dim3 g2((unsigned)nc);
         if (prec == 0)
             hipLaunchKernelGGL((assemble<double, double>), g2, dim3(256), 0, st, m, (const c128*)Cbuf, (c128*)Z, ldz, c0, nc);
         else
