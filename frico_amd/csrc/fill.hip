@@ -76,6 +76,11 @@ __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double i
     const int q = QN<QC>::get(m.q);
     double4* sP = smem_tp;           // [TP_TILE][q]
     double4* sQ = smem_tp + TP_TILE * q; // [TQ_TILE][q]
+    // The moments are symmetric, C(P, m, Q, n) = C(Q, n, P, m) (swapping the triangles swaps the
+    // roles of Su and Sv): tiles below the diagonal are written by their mirror tile.
+    if (blockIdx.y > blockIdx.x)
+        return;
+    const bool mirror = blockIdx.y < blockIdx.x;
     const int P0 = blockIdx.y * TP_TILE, Q0 = blockIdx.x * TQ_TILE;
     for (int i = threadIdx.x; i < TP_TILE * q; i += blockDim.x) {
         int t = P0 + i / q;
@@ -189,6 +194,8 @@ __global__ __launch_bounds__(256) void tripair_f64(DevMesh m, double k, double i
             const double cr = (0.25 * (suvr + avr + bur + ab * s0r) - inv_ksq * s0r) * invAA;
             const double ci = (0.25 * (suvi + avi + bui + ab * s0i) - inv_ksq * s0i) * invAA;
             out[nn] = cx<OutT>{(OutT)cr, (OutT)ci};
+            if (mirror)
+                C[((size_t)(3 * Q + nn) * m.T + P) * 3 + mm] = cx<OutT>{(OutT)cr, (OutT)ci};
         }
     }
 }
