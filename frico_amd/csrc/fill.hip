@@ -212,6 +212,10 @@ __global__ __launch_bounds__(256) void tripair_f32(DevMesh m, double k, double i
     const int q = QN<QC>::get(m.q);
     float4* sP = smem_tp32;
     float4* sQ = smem_tp32 + TP_TILE * q;
+    // symmetric moments: tiles below the diagonal are written by their mirror tile
+    if (blockIdx.y > blockIdx.x)
+        return;
+    const bool mirror = blockIdx.y < blockIdx.x;
     const int P0 = blockIdx.y * TP_TILE, Q0 = blockIdx.x * TQ_TILE;
     for (int i = threadIdx.x; i < TP_TILE * q; i += blockDim.x) {
         int t = P0 + i / q;
@@ -322,8 +326,11 @@ __global__ __launch_bounds__(256) void tripair_f32(DevMesh m, double k, double i
             const float bur = bx * sur[0] + by * sur[1] + bz * sur[2];
             const float bui = bx * sui[0] + by * sui[1] + bz * sui[2];
             const float ab = ax * bx + ay * by + az * bz;
-            out[nn] = c64{(0.25f * (suvr + avr + bur + ab * s0r) - ikk * s0r) * invAA,
-                          (0.25f * (suvi + avi + bui + ab * s0i) - ikk * s0i) * invAA};
+            const c64 v{(0.25f * (suvr + avr + bur + ab * s0r) - ikk * s0r) * invAA,
+                        (0.25f * (suvi + avi + bui + ab * s0i) - ikk * s0i) * invAA};
+            out[nn] = v;
+            if (mirror)
+                C[((size_t)(3 * Q + nn) * m.T + P) * 3 + mm] = v;
         }
     }
 }
