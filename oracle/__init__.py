@@ -218,6 +218,27 @@ def lu_solve(A, B):
     return (B2[:, 0] if np.ndim(B) == 1 else B2), info
 
 
+def sy_solve(A, B):
+    """solve_symmetric (src/lapack_wrappers.cpp:135-177): LAPACK zsytrf('U') + zsytrs on a complex
+    symmetric A, through scipy's LAPACK (the reference links MKL or a system LAPACK; the routine is
+    the same published Bunch-Kaufman factorisation).  Returns (X, info)."""
+    from scipy.linalg import lapack
+    A = np.array(A, np.complex128, order="F", copy=True)
+    B2 = np.array(B, np.complex128, order="F", copy=True)
+    if B2.ndim == 1:
+        B2 = B2.reshape(-1, 1, order="F")
+    n = A.shape[0]
+    # workspace query, then the factorisation, as the reference does
+    _, _, info = lapack.zsytrf(A, lower=0, lwork=-1)
+    if info != 0:
+        return (B2[:, 0] if np.ndim(B) == 1 else B2), info
+    ldu, ipiv, info = lapack.zsytrf(A, lower=0, lwork=max(1, 64 * n))
+    if info != 0:
+        return (B2[:, 0] if np.ndim(B) == 1 else B2), info
+    X, info = lapack.zsytrs(ldu, ipiv, B2, lower=0)
+    return (X[:, 0] if np.ndim(B) == 1 else X), info
+
+
 def rcs_unit_Ei(Es, theta_s, phi_s, R):
     out = np.zeros(2)
     Es = np.ascontiguousarray(Es, np.complex128)
@@ -237,7 +258,9 @@ def antenna_analysis(sm, freq, degree=5, approx=False, voltage=1.0, R=10.0, Z0=5
     V = m.rhs_deltagap(freq, sm.feed, voltage, signs)
     if force_symmetry:
         Z = (Z + Z.T) / 2.0
-    I, info = lu_solve(Z, V)
+    # run_context's dispatch (src/emw_solver.h:118-132): the symmetric solve for the approximate
+    # matrix or a forced symmetry, the general one otherwise
+    I, info = (sy_solve if approx or force_symmetry else lu_solve)(Z, V)
     AJ, AdivJ, _ = m.scatter(I)
     port = m.port_values(I, sm.feed, voltage, signs, Z0)
     gains = m.gain(freq, AJ, AdivJ, port["power"].real, R)
@@ -249,7 +272,7 @@ def bistatic_analysis(sm, freq, theta, phi, eth=1.0, eph=1.0, degree=5, approx=F
     m = OracleMesh(sm)
     Z = m.fill(freq, degree, approx)
     V = m.rhs_planewave(freq, theta, phi, eth, eph, degree)
-    I, info = lu_solve(Z, V)
+    I, info = (sy_solve if approx else lu_solve)(Z, V)
     AJ, AdivJ, _ = m.scatter(I)
     tab = m.bistatic(freq, AJ, AdivJ, R)
     return {"mesh": m, "I": I, "V": V, "table": tab, "info": info}

@@ -30,6 +30,51 @@ def test_lu_solve_matches_lapack(n):
     assert r <= 1e-14
 
 
+@pytest.mark.parametrize("n", [1, 7, 64, 200])
+def test_symmetric_solve_matches_a_general_solve(n):
+    """sy_solve (zsytrf('U') + zsytrs, the reference's solve_symmetric) on complex symmetric
+    (not Hermitian) matrices agrees with a general solve."""
+    rng = np.random.default_rng(100 + n)
+    M = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    A = M + M.T + 2 * np.sqrt(n) * np.eye(n)
+    B = rng.uniform(-1, 1, (n, 3)) + 1j * rng.uniform(-1, 1, (n, 3))
+    X, info = oracle.sy_solve(A, B)
+    assert info == 0
+    assert np.max(np.abs(X - np.linalg.solve(A, B))) <= 1e-11 * np.max(np.abs(X))
+
+
+def test_symmetric_solve_takes_two_by_two_pivots():
+    """A zero diagonal leaves Bunch-Kaufman no 1x1 pivot at the first step; the factorisation takes
+    2x2 blocks (negative pivot entries) and the solution still matches."""
+    from scipy.linalg import lapack
+    n = 40
+    rng = np.random.default_rng(3)
+    M = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    A = M + M.T
+    A[np.arange(n), np.arange(n)] = 0.0
+    _, ipiv, info = lapack.zsytrf(A, lower=0)
+    assert info == 0 and np.any(ipiv < 0)
+    b = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    x, info = oracle.sy_solve(A, b)
+    assert info == 0
+    assert np.max(np.abs(A @ x - b)) <= 1e-11 * np.max(np.abs(b)) * np.linalg.cond(A)
+
+
+def test_symmetric_and_general_solves_agree_on_the_approximate_efie_matrix():
+    """The approximate fill's matrix is symmetric; run_context solves it with solve_symmetric.  The
+    symmetric and the general factorisations give the same currents to the conditioning."""
+    sm = meshes.strip_dipole(0.5, 0.005, 60, 1)
+    om = oracle.OracleMesh(sm)
+    Z = om.fill(300e6, approx=True)
+    assert np.max(np.abs(Z - Z.T)) <= 1e-13 * np.max(np.abs(Z))
+    signs = om.reorient(sm.feed)
+    V = om.rhs_deltagap(300e6, sm.feed, 1.0, signs)
+    Is, i1 = oracle.sy_solve(Z, V)
+    Ig, i2 = oracle.lu_solve(Z, V)
+    assert i1 == 0 and i2 == 0
+    assert np.max(np.abs(Is - Ig)) <= 1e-11 * np.max(np.abs(Ig))
+
+
 def test_lu_pivots_on_first_max_of_abs_re_plus_abs_im():
     """izamax uses |re|+|im|: with column [1+1j, 1.9, ...] the first row is chosen (2 > 1.9) even
     though |1.9| > |1+1j|; a modulus-based pivot would pick row 1.  Both pivots solve the system,
