@@ -14,10 +14,11 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib, meshes
-from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C128)
+from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_SYM_IR, SOLVE_C128)
 
 __all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
-           "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR"]
+           "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR",
+           "SOLVE_C64_SYM_IR"]
 
 
 class FricoError(RuntimeError):
@@ -53,7 +54,7 @@ def _modes(mode, prec=None):
 
 
 def _form(form):
-    return {"c128": SOLVE_C128, "c64_ir": SOLVE_C64_IR}.get(form, form)
+    return {"c128": SOLVE_C128, "c64_ir": SOLVE_C64_IR, "c64_sym_ir": SOLVE_C64_SYM_IR}.get(form, form)
 
 
 class Context:
@@ -183,6 +184,12 @@ class Context:
     # ---- factor / solve -----------------------------------------------------------------------
     def factor(self, form="c128"):
         return self._check(self._L.frico_factor(self._h, _form(form)))
+
+    @property
+    def factored_form(self) -> int:
+        """Form of the current factorisation (SOLVE_*), -1 if none: the symmetric form reports
+        SOLVE_C64_IR after falling back to the pivoted LU."""
+        return int(self._L.frico_factored_form(self._h))
 
     def solve(self, tol=1e-12, maxit=10) -> SolveInfo:
         info = _lib.frico_solve_info()

@@ -20,7 +20,7 @@ FRICO_ERR_SYNC = -1005
 FRICO_ERR_LIMIT = -1006
 FILL_QUAD, FILL_APPROX = 0, 1
 C128, C64 = 0, 1
-SOLVE_C128, SOLVE_C64_IR = 0, 1
+SOLVE_C128, SOLVE_C64_IR, SOLVE_C64_SYM_IR = 0, 1, 2
 BATCH_GRAPHS, BATCH_AUTOTUNE = 1, 2
 
 
@@ -71,6 +71,7 @@ SIGNATURES = {
     "frico_rhs_set": (C.c_int, [_vp, _d, C.c_int64, C.c_int64]),
     "frico_rhs_get": (C.c_int, [_vp, _d, C.c_int64]),
     "frico_factor": (C.c_int, [_vp, C.c_int]),
+    "frico_factored_form": (C.c_int, [_vp]),
     "frico_solve": (C.c_int, [_vp, C.c_double, C.c_int, C.POINTER(frico_solve_info)]),
     "frico_residual": (C.c_int, [_vp, _d]),
     "frico_solution_get": (C.c_int, [_vp, _d, C.c_int64]),

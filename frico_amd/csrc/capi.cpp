@@ -859,11 +859,11 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
 {
     if (c->zprec < 0)
         return set_error("no matrix to factor"), FRICO_ERR_STATE;
-    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR)
+    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR && form != FRICO_SOLVE_C64_SYM_IR)
         return set_error("unknown factorisation form %d", form), -2;
     if (form == FRICO_SOLVE_C128 && (c->zprec != FRICO_C128 || c->z64_factored))
         return set_error("complex128 factorisation needs an unfactored complex128 Z"), FRICO_ERR_STATE;
-    if (form == FRICO_SOLVE_C64_IR && c->zprec == FRICO_C128 && c->z64_factored)
+    if (form != FRICO_SOLVE_C128 && c->zprec == FRICO_C128 && c->z64_factored)
         return set_error("Z was overwritten by a complex128 factorisation"), FRICO_ERR_STATE;
     int rc = set_dev(c);
     if (rc || (rc = ensure_lu_work(c)) || (rc = c->ipiv.ensure((size_t)c->N)))
@@ -871,9 +871,9 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     const int64_t N = c->N;
     // the complex64 factorisation carries the current right-hand sides along (forward elimination
     // inside the trailing products); the first refinement sweep then only substitutes backward
-    c->aug_cols = (form == FRICO_SOLVE_C64_IR && c->augment && c->nrhs > 0) ? c->nrhs : 0;
+    c->aug_cols = (form != FRICO_SOLVE_C128 && c->augment && c->nrhs > 0) ? c->nrhs : 0;
     c->aug_valid = false;
-    if (form == FRICO_SOLVE_C64_IR && (rc = c->LU32.ensure((size_t)(N * (N + c->aug_cols)))))
+    if (form != FRICO_SOLVE_C128 && (rc = c->LU32.ensure((size_t)(N * (N + c->aug_cols)))))
         return rc;
     c->factored = -1; // the factors are about to be overwritten
     c->nb_factored = block_width(c);
@@ -913,9 +913,11 @@ int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
             rc = launch_copy_bytes(c->Z32.p, c->LU32.p, (size_t)(N * N) * sizeof(c64), c->st);
         if (!rc && c->aug_cols > 0)
             rc = launch_convert_c128_c64(c->B.p, N, c->LU32.p + N * N, N, N, c->aug_cols, c->st);
-        if (!rc)
-            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, c->profiling ? &c->t_gemm : nullptr,
-                                  c->aug_cols);
+        KernelTimer* kt = c->profiling ? &c->t_gemm : nullptr;
+        if (!rc && form == FRICO_SOLVE_C64_SYM_IR)
+            rc = ldlt_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
+        else if (!rc)
+            rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
     }
     phase_end(c, c->t_factor);
     return rc;
@@ -959,7 +961,16 @@ int frico_factor(frico_ctx* c, int form)
     int rc = factor_prepare(c, form, wk, true);
     if (rc || (rc = factor_enqueue(c, form, wk)))
         return rc;
-    return factor_finish(c, form, wk);
+    rc = factor_finish(c, form, wk);
+    // a zero pivot of the unpivoted symmetric factorisation: the pivoted LU instead
+    if (rc > 0 && form == FRICO_SOLVE_C64_SYM_IR)
+        return frico_factor(c, FRICO_SOLVE_C64_IR);
+    return rc;
+}
+
+int frico_factored_form(const frico_ctx* c)
+{
+    return c ? c->factored : -1;
 }
 
 // R = B - Z X with the kept (unfactored) Z.
@@ -1108,16 +1119,30 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     if ((rc = col_norms(c, c->B.p, nr, bn)))
         return rc;
     inf.converged = 0;
-    for (int it = 1; it <= maxit; it++) {
-        double worst = 0;
-        if ((rc = refine_enqueue(c, it == 1, true)) || (rc = read_refine_norms(c, bn, &worst)))
-            return rc;
-        inf.iterations = it;
-        inf.max_rel_residual = worst;
-        if (worst <= tol) {
-            inf.converged = 1;
-            break;
+    int sweeps = 0;
+    for (;;) {
+        // the unpivoted symmetric factors are left for the pivoted LU when a sweep fails to halve
+        // the residual before the tolerance is met, or when maxit sweeps do not meet it
+        const bool sym = c->factored == FRICO_SOLVE_C64_SYM_IR;
+        double prev = HUGE_VAL;
+        for (int it = 1; it <= maxit; it++) {
+            double worst = 0;
+            if ((rc = refine_enqueue(c, it == 1, true)) || (rc = read_refine_norms(c, bn, &worst)))
+                return rc;
+            inf.iterations = ++sweeps;
+            inf.max_rel_residual = worst;
+            if (worst <= tol) {
+                inf.converged = 1;
+                break;
+            }
+            if (sym && !(worst < 0.5 * prev))
+                break;
+            prev = worst;
         }
+        if (inf.converged || !sym)
+            break;
+        if ((rc = frico_factor(c, FRICO_SOLVE_C64_IR)))
+            return rc;
     }
     c->have_x = true;
     if (info)

@@ -269,6 +269,20 @@ __device__ inline void gemm_store(int64_t mr, int64_t nc, cx<T>* __restrict__ C,
         }
 }
 
+// One output tile (rows m0.., columns n0..) of C -= op(A) B.
+template <typename T, typename TA, bool TRANSA, bool M3, typename TL>
+__device__ inline void gemm_sub_tile(int64_t m0, int64_t n0, int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A,
+                                     int64_t lda, const cx<T>* __restrict__ B, int64_t ldb, cx<T>* __restrict__ C,
+                                     int64_t ldc, GemmSmem<T, TL>& sm)
+{
+    GemmAcc<T, TL> acc;
+    GemmC<T, TL> cv;
+    gemm_cload<T, TL>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, cv);
+    gemm_tile<T, TA, TRANSA, TL, false, M3>(M - m0, N - n0, K, TRANSA ? A + m0 * lda : A + m0, lda, B + n0 * ldb, ldb,
+                                             acc, sm);
+    gemm_store<T, TL, M3>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, &cv);
+}
+
 // C[M x N] -= op(A)[M x K] * B[K x N], complex, column-major; op(A) = A or A^T.
 template <typename T, typename TA, bool TRANSA = false, bool M3 = false>
 __global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* __restrict__ A, int64_t lda,
@@ -277,13 +291,34 @@ __global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub(int64_t M, int64_t
 {
     using TL = typename TileBig<T>::type;
     __shared__ GemmSmem<T, TL> sm;
-    const int64_t m0 = (int64_t)blockIdx.x * TL::BM, n0 = (int64_t)blockIdx.y * TL::BN;
-    GemmAcc<T, TL> acc;
-    GemmC<T, TL> cv;
-    gemm_cload<T, TL>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, cv);
-    gemm_tile<T, TA, TRANSA, TL, false, M3>(M - m0, N - n0, K, TRANSA ? A + m0 * lda : A + m0, lda, B + n0 * ldb, ldb,
-                                             acc, sm);
-    gemm_store<T, TL, M3>(M - m0, N - n0, C + m0 + n0 * ldc, ldc, acc, &cv);
+    gemm_sub_tile<T, TA, TRANSA, M3, TL>((int64_t)blockIdx.x * TL::BM, (int64_t)blockIdx.y * TL::BN, M, N, K, A, lda, B,
+                                          ldb, C, ldc, sm);
+}
+
+// The same product on the tiles (i, j) with i >= j only (row tile i, column tile j; square tiles,
+// so with C's first entry on the diagonal of a symmetric matrix these are the tiles that hold its
+// lower triangle).  One workgroup per such tile, numbered down the columns: column j holds the
+// gx - j tiles i = j .. gx - 1, so it starts at j gx - j (j - 1) / 2.
+template <typename T, bool M3 = false>
+__global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub_lower(int64_t M, int64_t N, int64_t K, int64_t gx, int64_t gy,
+                                                                    const cx<T>* __restrict__ A, int64_t lda,
+                                                                    const cx<T>* __restrict__ B, int64_t ldb,
+                                                                    cx<T>* __restrict__ C, int64_t ldc)
+{
+    using TL = typename TileBig<T>::type;
+    static_assert(TL::BM == TL::BN, "square tiles");
+    __shared__ GemmSmem<T, TL> sm;
+    const int64_t t = blockIdx.x;
+    auto start = [gx](int64_t j) { return j * gx - j * (j - 1) / 2; };
+    const double b = 2.0 * (double)gx + 1.0;
+    int64_t j = (int64_t)(0.5 * (b - sqrt(b * b - 8.0 * (double)t)));
+    j = j < 0 ? 0 : (j >= gy ? gy - 1 : j);
+    while (j > 0 && start(j) > t)
+        j--;
+    while (j + 1 < gy && start(j + 1) <= t)
+        j++;
+    const int64_t i = j + (t - start(j));
+    gemm_sub_tile<T, T, false, M3, TL>(i * TL::BM, j * TL::BN, M, N, K, A, lda, B, ldb, C, ldc, sm);
 }
 
 // W blocks for the forward solves: for row tile t > chunk j (64 x 64 each)
@@ -1277,6 +1312,34 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
     return 0;
 }
 
+// C -= A B on the tiles on and below C's diagonal (C square-tiled from a diagonal entry; N <= M).
+template <typename T>
+static int gemm_sub_lower(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb,
+                          cx<T>* C, int64_t ldc, hipStream_t st, KernelTimer* kt)
+{
+    if (M <= 0 || N <= 0 || K <= 0)
+        return 0;
+    if (N > M)
+        return set_error("lower product: more columns than rows"), -1;
+    using TL = typename TileBig<T>::type;
+    const int64_t gx = (M + TL::BM - 1) / TL::BM, gy = (N + TL::BN - 1) / TL::BN;
+    const int64_t tiles = gy * gx - gy * (gy - 1) / 2;
+    if (kt)
+        kt->begin(st);
+    if (gemm_three_products() && (sizeof(T) == 4 || K >= 128))
+        launch_kernel(&cgemm_sub_lower<T, true>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, gx, gy, A, lda, B, ldb,
+                      C, ldc);
+    else
+        launch_kernel(&cgemm_sub_lower<T>, dim3((unsigned)tiles), dim3(256), 0, st, M, N, K, gx, gy, A, lda, B, ldb, C,
+                      ldc);
+    // flops of the tiles computed (the diagonal tiles in full)
+    const double cols = (double)N, full = (double)M * cols - cols * (cols - TL::BN) / 2.0;
+    if (kt)
+        kt->end(st, 8.0 * full * (double)K);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const c128* B, int64_t ldb, c128* C,
                    int64_t ldc, hipStream_t st, KernelTimer* kt)
 {
@@ -1333,13 +1396,11 @@ template <typename T> int panel_rows(int w, bool shared_cu)
 // Update of columns [c0, c0 + nc) by step k (panel of width w): row interchanges, U12 = L11^{-1} A12
 // and A22 -= L21 U12.
 // perm2: a second interchange list applied after perm (the two halves of a composite panel).
+// U12 = L11^{-1} A12 on columns [c0, c0 + nc) of the block row [k, k + w) (L11 unit lower, in
+// 64-row chunks: substitution, then the chunk's product with the rows below it in the block).
 template <typename T>
-static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, const int* perm2, int64_t k, int w,
-                       int64_t c0, int64_t nc, hipStream_t st, KernelTimer* kt)
+static int row_block_solve(cx<T>* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, hipStream_t st)
 {
-    if (nc <= 0)
-        return 0;
-    launch_laswp<T>(A, lda, c0, nc, perm, st, perm2);
     for (int s = 0; s < w; s += 64) {
         const int ws = (w - s) < 64 ? (w - s) : 64;
         cx<T>* D = A + (k + s) + (k + s) * lda;
@@ -1353,9 +1414,24 @@ static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, const 
         }
     }
     FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int step_update(cx<T>* A, int64_t n, int64_t lda, const int* perm, const int* perm2, int64_t k, int w,
+                       int64_t c0, int64_t nc, hipStream_t st, KernelTimer* kt)
+{
+    if (nc <= 0)
+        return 0;
+    launch_laswp<T>(A, lda, c0, nc, perm, st, perm2);
+    if (int rc = row_block_solve<T>(A, lda, k, w, c0, nc, st))
+        return rc;
     return gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda, A + (k + w) + c0 * lda, lda,
                        st, kt);
 }
+
+template <typename T>
+static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int nb, hipStream_t st);
 
 // Right-looking blocked LU.  With a side stream the update of step p is split: the columns of panel
 // p+1 are updated on the main stream, which then factors panel p+1 while the side stream updates
@@ -1508,6 +1584,13 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         launch_kernel(&build_perm_lists, dim3((unsigned)npan), dim3(64), 0, st, nb, n, (const int*)ipiv, wk.perm);
         FRICO_HIP_CHECK(hipGetLastError());
     }
+    return solve_tables<T>(wk, A, n, lda, nb, st);
+}
+
+// The solves' tables of the factors: the diagonal-chunk inverses and the W products (LuWork).
+template <typename T>
+static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int nb, hipStream_t st)
+{
     if (wk.dinv) {
         const int64_t nch = (n + 63) / 64;
         launch_kernel(&diag_inverse<T>, dim3((unsigned)nch), dim3(256), 0, st, A, n, lda, (cx<T>*)wk.dinv);
@@ -1519,6 +1602,228 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
         }
     }
     return 0;
+}
+
+// ----------------------------------------------------------------------------------------------
+// Symmetric factorisation Z = L D L^T of a complex symmetric (not Hermitian) Z, without pivoting:
+// the factorisation behind solve_symmetric (zsytrf, src/lapack_wrappers.cpp:135-177) for the
+// refinement form.  Only the lower triangle of Z is read, and only the tiles holding the lower
+// triangle of each trailing matrix are updated: half the trailing products of the LU.  The factors
+// are stored as the LU of Z with no interchanges would be, L strictly below the diagonal and
+// U = D L^T on and above it, so the LU solves run on them unchanged with empty interchange lists.
+// Block column [k, k + w):
+//   ldlt_diag        L11 D1 L11^T of the w x w diagonal block, in LDS (one workgroup)
+//   sym_transpose    A12 = A21^T: the block row, taken from the block column
+//   row_block_solve  U12 = L11^{-1} A12 (= D1 L21^T)
+//   sym_transpose    L21 = U12^T D1^{-1}
+//   cgemm_sub_lower  A22 -= L21 U12 on the tiles on and below A22's diagonal
+// Without interchanges the factors can grow: a zero pivot is reported like zgetrf's info, and the
+// accuracy of the answer rests on the refinement against the kept Z (the caller returns to the
+// pivoted LU when the refinement stalls).
+// ----------------------------------------------------------------------------------------------
+constexpr int LDLT_THREADS = 1024;
+// block-row columns per transpose workgroup
+template <typename T> constexpr int kSymCols = sizeof(T) == 4 ? 32 : 16;
+
+// Right-looking L D L^T of the w x w (w <= 128) diagonal block at (k, k) in LDS: S[j ld + r] holds
+// entry (r, j), r >= j.  Step c: l_r = a(r, c) / d_c, a(r, j) -= l_r a(j, c) for c < j <= r; thread
+// (row r, group g) updates the columns j = c + 1 + g, c + 1 + g + 8, ... of row r.  Writes L below
+// the diagonal and U = D L^T on and above it.  A zero pivot leaves its column of L zero and sets
+// *info (1-based column) if unset.
+template <typename T>
+__global__ __launch_bounds__(LDLT_THREADS) void ldlt_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w,
+                                                         int* info)
+{
+    extern __shared__ __align__(16) unsigned char ldlt_smem[];
+    cx<T>* S = reinterpret_cast<cx<T>*>(ldlt_smem);
+    const int ld = w + 1;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < w * w; e += LDLT_THREADS) {
+        const int r = e % w, j = e / w;
+        if (r >= j)
+            S[j * ld + r] = A[(k + r) + (k + j) * lda];
+    }
+    __syncthreads();
+    constexpr int NG = LDLT_THREADS / LU_NB_MAX;
+    const int r = tid % LU_NB_MAX, g = tid / LU_NB_MAX;
+    for (int c = 0; c < w; c++) {
+        const cx<T> d = S[c * ld + c];
+        const bool zero = d.re == (T)0 && d.im == (T)0;
+        const cx<T> inv = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+        if (zero && tid == 0 && *info == 0)
+            *info = (int)(k + c + 1);
+        cx<T> l{0, 0};
+        if (r > c && r < w) {
+            l = S[c * ld + r] * inv;
+            for (int j = c + 1 + g; j <= r; j += NG) {
+                const cx<T> v = S[c * ld + j];
+                cx<T>& a = S[j * ld + r];
+                a.re -= l.re * v.re - l.im * v.im;
+                a.im -= l.re * v.im + l.im * v.re;
+            }
+        }
+        __syncthreads();
+        // column c is not read again: the next step reads column c + 1 and the columns right of it
+        if (g == 0 && r > c && r < w)
+            S[c * ld + r] = l;
+    }
+    __syncthreads();
+    for (int e = tid; e < w * w; e += LDLT_THREADS) {
+        const int r = e % w, j = e / w;
+        cx<T> v;
+        if (r >= j)
+            v = S[j * ld + r];
+        else
+            v = S[r * ld + r] * S[r * ld + j]; // U(r, j) = d_r L(j, r)
+        A[(k + r) + (k + j) * lda] = v;
+    }
+}
+
+// Block row <-> block column of step k (w columns, m = n - k - w rows below), kSymCols values of c a
+// workgroup:
+//   TO_LOWER false: A(k + i, k + w + c) = A(k + w + c, k + i)                        (A12 = A21^T)
+//   TO_LOWER true:  A(k + w + c, k + i) = A(k + i, k + w + c) / A(k + i, k + i)     (L21 = U12^T D^{-1})
+template <typename T, bool TO_LOWER>
+__global__ __launch_bounds__(256) void sym_transpose(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int64_t m)
+{
+    constexpr int TC = kSymCols<T>;
+    __shared__ cx<T> S[LU_NB_MAX][TC + 1];
+    __shared__ cx<T> sinv[LU_NB_MAX];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * TC;
+    const int nc = (int)((m - c0) < TC ? (m - c0) : TC);
+    cx<T>* A21 = A + (k + w + c0) + k * lda; // entry (c, i) at A21[c + i lda]
+    cx<T>* A12 = A + k + (k + w + c0) * lda; // entry (i, c) at A12[i + c lda]
+    if (TO_LOWER) {
+        for (int i = tid; i < w; i += 256) {
+            const cx<T> d = A[(k + i) * (lda + 1)];
+            sinv[i] = (d.re == (T)0 && d.im == (T)0) ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+        }
+        for (int e = tid; e < w * TC; e += 256) {
+            const int i = e % w, c = e / w;
+            if (c < nc)
+                S[i][c] = A12[i + (int64_t)c * lda];
+        }
+        __syncthreads();
+        for (int e = tid; e < w * TC; e += 256) {
+            const int c = e % TC, i = e / TC;
+            if (c < nc)
+                A21[c + (int64_t)i * lda] = S[i][c] * sinv[i];
+        }
+    } else {
+        for (int e = tid; e < w * TC; e += 256) {
+            const int c = e % TC, i = e / TC;
+            if (c < nc)
+                S[i][c] = A21[c + (int64_t)i * lda];
+        }
+        __syncthreads();
+        for (int e = tid; e < w * TC; e += 256) {
+            const int i = e % w, c = e / w;
+            if (c < nc)
+                A12[i + (int64_t)c * lda] = S[i][c];
+        }
+    }
+}
+
+__global__ void iota_pivots(int* ipiv, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n)
+        ipiv[i] = (int)(i + 1);
+}
+
+// Blocked L D L^T (see above).  With a side stream, as lu_factor: the update of step p is split
+// into the next block column (main stream, which then factors it) and the rest (side stream);
+// the appended right-hand-side columns are carried on the side stream.
+template <typename T>
+int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer,
+                int64_t naug)
+{
+    if (nb > LU_NB_MAX || nb < 1)
+        return set_error("ldlt: block size out of range"), -1;
+    const size_t sh_max = (size_t)nb * (nb + 1) * sizeof(cx<T>);
+    if (sh_max > 160 * 1024)
+        return set_error("ldlt: a %d-column diagonal block of this type does not fit in LDS", nb), -1;
+    const int64_t npan = (n + nb - 1) / nb;
+    if (!wk.perm || wk.nperm < npan)
+        return set_error("ldlt: interchange-list workspace too small"), -1;
+    if (sh_max > 64 * 1024)
+        FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&ldlt_diag<T>), sh_max));
+    const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
+    // no interchanges: empty lists for the solves, the identity in ipiv
+    const int64_t nz = npan * PERM_STRIDE;
+    launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.perm, nz);
+    launch_kernel(&zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
+    launch_kernel(&iota_pivots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ipiv, n);
+    FRICO_HIP_CHECK(hipGetLastError());
+    // diagonal block, block row and block column of step k
+    auto column = [&](int64_t k, int w) -> int {
+        const int64_t m = n - k - w;
+        launch_kernel(&ldlt_diag<T>, dim3(1), dim3(LDLT_THREADS), (size_t)w * (w + 1) * sizeof(cx<T>), st, A, lda, k, w,
+                      wk.flags + 1);
+        if (m > 0) {
+            const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
+            launch_kernel(&sym_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m);
+            if (int rc = row_block_solve<T>(A, lda, k, w, k + w, m, st))
+                return rc;
+            launch_kernel(&sym_transpose<T, true>, g, dim3(256), 0, st, A, lda, k, w, m);
+        }
+        FRICO_HIP_CHECK(hipGetLastError());
+        return 0;
+    };
+    // the appended columns: their block row solved with L11, then the rows below updated
+    auto appended = [&](int64_t k, int w, hipStream_t s) -> int {
+        if (naug <= 0)
+            return 0;
+        int rc = row_block_solve<T>(A, lda, k, w, n, naug, s);
+        return rc ? rc
+                  : gemm_sub<T>(n - k - w, naug, w, A + (k + w) + k * lda, lda, A + k + n * lda, lda,
+                                A + (k + w) + n * lda, lda, s, nullptr);
+    };
+    // lower triangle of the trailing matrix in columns [c0, c0 + nc) (rows [c0, n))
+    auto update = [&](int64_t k, int w, int64_t c0, int64_t nc, hipStream_t s) -> int {
+        return gemm_sub_lower<T>(n - c0, nc, w, A + c0 + k * lda, lda, A + k + c0 * lda, lda, A + c0 + c0 * lda, lda, s,
+                                 gemm_timer);
+    };
+    auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
+    int rc;
+    if (!lookahead) {
+        for (int64_t p = 0; p < npan; p++) {
+            const int64_t k = p * nb;
+            const int w = width(k);
+            if ((rc = column(k, w)) || (rc = appended(k, w, st)) || (rc = update(k, w, k + w, n - k - w, st)))
+                return rc;
+        }
+    } else {
+        hipEvent_t* evP = wk.events;
+        hipEvent_t* evW = wk.events + npan;
+        hipEvent_t ev0 = wk.events[2 * npan];
+        FRICO_HIP_CHECK(hipEventRecord(ev0, st));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, ev0, 0));
+        if ((rc = column(0, width(0))))
+            return rc;
+        for (int64_t p = 0; p < npan; p++) {
+            const int64_t k = p * nb;
+            const int w = width(k);
+            const int64_t nright = n - k - w;
+            const int64_t nnext = nright < nb ? nright : nb;
+            FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
+            FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
+            if ((rc = appended(k, w, wk.side)) || (rc = update(k, w, k + w + nnext, nright - nnext, wk.side)))
+                return rc;
+            FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
+            if (nright <= 0)
+                break;
+            // the next block column was last updated by the wide part of the previous step
+            if (p > 0)
+                FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
+            if ((rc = update(k, w, k + w, nnext, st)) || (rc = column(k + w, width(k + w))))
+                return rc;
+        }
+        FRICO_HIP_CHECK(hipEventRecord(ev0, wk.side));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(st, ev0, 0));
+    }
+    return solve_tables<T>(wk, A, n, lda, nb, st);
 }
 
 constexpr int RESID_ROWS = 512; // rows of the solution per residual product beside the substitution
@@ -1745,6 +2050,8 @@ template int gemm_sub<float>(int64_t, int64_t, int64_t, const c64*, int64_t, con
                              hipStream_t, KernelTimer*, bool);
 template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
+template int ldlt_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
+template int ldlt_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t,
                                     const ResidStream*);
 template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t, const ResidStream*);

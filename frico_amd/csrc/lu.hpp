@@ -43,6 +43,14 @@ template <typename T> int panel_rows(int w, bool shared_cu = false);
 template <typename T>
 int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st, KernelTimer* gemm_timer,
               int64_t naug = 0);
+// Z = L D L^T without pivoting, for complex symmetric Z (only its lower triangle is read).  The
+// factors are stored as lu_factor's would be with no interchanges (L below the diagonal, U = D L^T
+// on and above it; empty interchange lists in LuWork::perm, ipiv the identity), so lu_solve and
+// lu_solve_upper take them as they are.  naug as for lu_factor.  nb <= 128 for complex64, <= 64 for
+// complex128 (the diagonal block is factored in LDS).
+template <typename T>
+int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
+                KernelTimer* gemm_timer, int64_t naug = 0);
 int launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 int launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st);
 // ints per panel of LuWork::perm

@@ -50,6 +50,14 @@ struct port_values {
 static cd gamma_of(cd Z, double Z0) { return (Z - Z0) / (Z + Z0); }
 static double swr_of(cd g) { return (1.0 + std::abs(g)) / (1.0 - std::abs(g)); }
 
+// run_context solves approximate-mode and symmetrised systems with solve_symmetric
+// (src/emw_solver.h:124-128): the refinement form takes the symmetric factorisation there.
+static int solve_form(int form, int fill_mode, bool force_symmetry)
+{
+    return form == FRICO_SOLVE_C64_IR && (fill_mode == FRICO_FILL_APPROX || force_symmetry) ? FRICO_SOLVE_C64_SYM_IR
+                                                                                           : form;
+}
+
 // Delta-gap RHS, the solve in the configured form, and the solve status.
 static int run_context(simulation& sim, double freq, const delta_gap& dg, frico_solve_info* info)
 {
@@ -63,7 +71,7 @@ static int run_context(simulation& sim, double freq, const delta_gap& dg, frico_
                             sim.delta_gap_signs.empty() ? nullptr : sim.delta_gap_signs.data());
     if (rc)
         return rc;
-    if ((rc = frico_factor(sim.ctx, sim.cfg.form)))
+    if ((rc = frico_factor(sim.ctx, solve_form(sim.cfg.form, sim.cfg.fill_mode, sim.cfg.force_symmetry))))
         return rc;
     return frico_solve(sim.ctx, sim.cfg.tol, sim.cfg.maxit, info);
 }
@@ -233,7 +241,7 @@ int frico_bistatic_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mod
     std::vector<cd> E(3 * 718);
     for (int f = 0; f < nfreq; f++) {
         int rc = frico_fill(ctx, freqs[f], mode, FRICO_C128);
-        if (rc || (rc = frico_rhs_planewave(ctx, freqs[f], 1, th, ph, et, ep)) || (rc = frico_factor(ctx, form)) ||
+        if (rc || (rc = frico_rhs_planewave(ctx, freqs[f], 1, th, ph, et, ep)) || (rc = frico_factor(ctx, frico::maxwell::solve_form(form, mode, false))) ||
             (rc = frico_solve(ctx, tol, maxit, nullptr)))
             return rc;
         if ((rc = frico_fields(ctx, freqs[f], pts.data(), 718, nullptr, 1.0, reinterpret_cast<double*>(E.data()), nullptr)))
@@ -268,7 +276,8 @@ int frico_monostatic_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int m
         frico_solve_info info{};
         int rc = frico_fill(ctx, freqs[f], mode, FRICO_C128);
         if (rc || (rc = frico_rhs_planewave(ctx, freqs[f], nang, theta, phi, et.data(), ep.data())) ||
-            (rc = frico_factor(ctx, form)) || (rc = frico_solve(ctx, tol, maxit, &info)))
+            (rc = frico_factor(ctx, frico::maxwell::solve_form(form, mode, false))) ||
+            (rc = frico_solve(ctx, tol, maxit, &info)))
             return rc;
         if (info_out)
             info_out[f] = info;

@@ -41,9 +41,15 @@ extern "C" {
 /* Factorisation / solve forms (frico_factor):
  *   FRICO_SOLVE_C128      complex128 LU of Z, in place (zgetrf semantics)
  *   FRICO_SOLVE_C64_IR    complex64 LU of a copy of Z, FP64 residual b - Z x against the kept Z,
- *                         iterative refinement to the requested tolerance */
+ *                         iterative refinement to the requested tolerance
+ *   FRICO_SOLVE_C64_SYM_IR  for a symmetric Z (solve_symmetric, src/lapack_wrappers.cpp:135-177):
+ *                         complex64 L D L^T of the lower triangle of a copy of Z without pivoting,
+ *                         refined as FRICO_SOLVE_C64_IR.  A zero pivot, or a refinement that stops
+ *                         converging, falls back to FRICO_SOLVE_C64_IR by itself
+ *                         (frico_factored_form tells which factorisation the answer came from). */
 #define FRICO_SOLVE_C128 0
 #define FRICO_SOLVE_C64_IR 1
+#define FRICO_SOLVE_C64_SYM_IR 2
 
 typedef struct frico_ctx frico_ctx;
 
@@ -135,6 +141,8 @@ int frico_rhs_get(frico_ctx* ctx, double* B, int64_t ldb);
 /* LU of Z: zgetrf (src/lapack_wrappers.cpp:69) in the given form.  Returns > 0 for a zero
  * pivot (the 1-based column), like zgetrf's info. */
 int frico_factor(frico_ctx* ctx, int form);
+/* Form of the current factorisation (FRICO_SOLVE_*), or -1 if there is none. */
+int frico_factored_form(const frico_ctx* ctx);
 /* X = Z^{-1} B for every right-hand side column: zgetrs (src/lapack_wrappers.cpp:83,124),
  * followed by refinement for FRICO_SOLVE_C64_IR until max_c ||b_c - Z x_c||/||b_c|| <= tol or
  * maxit sweeps.  info may be NULL. */
@@ -169,7 +177,9 @@ int frico_fields(frico_ctx* ctx, double freq, const double* pts, int64_t npts, c
  * frico_monostatic_sweep: nang plane waves at (theta[a], phi[a]) solved as one block; the
  *   back-scattered sigma_VV, sigma_HH at (R, theta[a], phi[a]) (formula of
  *   src/emw_ana_bistatic_rcs.cpp:62-82); sigma_out[2*nang*f ..].
- * `form` selects FRICO_SOLVE_C128 or FRICO_SOLVE_C64_IR (then tol/maxit apply).
+ * `form` selects FRICO_SOLVE_C128 or FRICO_SOLVE_C64_IR (then tol/maxit apply).  As in run_context
+ *   (src/emw_solver.h:124-128), approximate-mode fills (and antenna sweeps with force_symmetry) are
+ *   solved as symmetric systems: FRICO_SOLVE_C64_IR becomes FRICO_SOLVE_C64_SYM_IR there.
  */
 int frico_antenna_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, int degree_unused,
                         const int32_t* ifaces, int nif, double v_re, double v_im, double R,

@@ -1,0 +1,181 @@
+"""GPU: the symmetric factorisation behind solve_symmetric (zsytrf/zsytrs,
+src/lapack_wrappers.cpp:135-177), which run_context uses for approximate-mode and symmetrised systems
+(src/emw_solver.h:124-128).  The library factors such systems as complex64 L D L^T without pivoting
+and refines against the FP64 Z; the answers are held to numpy's solve and to the oracle's zsytrf
+solve, and the fall-backs to the pivoted LU are exercised."""
+import numpy as np
+import pytest
+
+import oracle
+from frico_amd import meshes
+
+pytestmark = pytest.mark.gpu
+
+F0 = 300e6
+
+
+def relmax(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))) / max(np.max(np.abs(b)), 1e-300))
+
+
+def sym_system(n, seed, shift, nrhs=3):
+    rng = np.random.default_rng(seed)
+    M = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n))
+    A = M + M.T + shift * np.sqrt(n) * np.eye(n)
+    B = rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs))
+    return A, B
+
+
+@pytest.mark.parametrize("n,nb", [(1, 128), (63, 64), (200, 64), (517, 128), (1300, 128), (2000, 64)])
+def test_symmetric_form_solves_symmetric_systems(n, nb):
+    import frico_amd
+    A, B = sym_system(n, n, 4.0)
+    print(f"symmetric n={n} nb={nb}", flush=True)
+    with frico_amd.Context(0) as c:
+        c.set_block(nb)
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor("c64_sym_ir")
+        assert c.factored_form == frico_amd.SOLVE_C64_SYM_IR
+        info = c.solve(tol=1e-12, maxit=10)
+        print(f"  {info.iterations} sweeps, residual {info.max_rel_residual:.2e}")
+        assert info.converged and c.factored_form == frico_amd.SOLVE_C64_SYM_IR
+        assert info.iterations <= 4
+        assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
+
+
+def test_symmetric_first_iterate_is_as_accurate_as_the_lu_one():
+    """One complex64 sweep of each factorisation against the exact solution: L D L^T (no
+    interchanges, half the products) measured 4.3e-5 against the pivoted LU's 1.1e-5 at n = 900;
+    both stay below 1e-4 and within a factor of eight of each other."""
+    import frico_amd
+    A, B = sym_system(900, 11, 1.0, 8)
+    X = np.linalg.solve(A, B)
+    err = {}
+    for form in ("c64_sym_ir", "c64_ir"):
+        with frico_amd.Context(0) as c:
+            c.set_block(128)
+            c.set_matrix(A)
+            c.set_rhs(B)
+            c.factor(form)
+            c.solve(tol=1.0, maxit=1)
+            err[form] = relmax(c.solution(), X)
+    print("first-iterate errors", err)
+    assert err["c64_sym_ir"] <= 1e-4
+    assert err["c64_sym_ir"] <= 8 * err["c64_ir"] and err["c64_ir"] <= 8 * err["c64_sym_ir"]
+
+
+def test_symmetric_factors_do_not_depend_on_the_lookahead(monkeypatch):
+    """The look-ahead splits each trailing update over two streams; every tile is the same product
+    either way, so the first iterate is bit-identical with and without it."""
+    import frico_amd
+    A, B = sym_system(1100, 12, 2.0, 5)
+    out = []
+    for la in ("1", "0"):
+        monkeypatch.setenv("FRICO_LU_LOOKAHEAD", la)
+        with frico_amd.Context(0) as c:
+            c.set_block(128)
+            c.set_matrix(A)
+            c.set_rhs(B)
+            c.factor("c64_sym_ir")
+            c.solve(tol=1.0, maxit=1)
+            out.append(c.solution())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_symmetric_form_carried_and_separate_rhs_agree():
+    """Right-hand sides set before the factorisation ride through it (forward elimination in the
+    trailing products); set afterwards they are solved in full.  Same answer."""
+    import frico_amd
+    A, B = sym_system(700, 13, 1.0, 20)
+    sols = []
+    for before in (True, False):
+        with frico_amd.Context(0) as c:
+            c.set_block(128)
+            c.set_matrix(A)
+            if before:
+                c.set_rhs(B)
+            c.factor("c64_sym_ir")
+            if not before:
+                c.set_rhs(B)
+            info = c.solve(tol=1e-13, maxit=10)
+            assert info.converged and c.factored_form == frico_amd.SOLVE_C64_SYM_IR
+            sols.append(c.solution())
+    assert relmax(sols[0], sols[1]) <= 1e-12
+    assert relmax(sols[0], np.linalg.solve(A, B)) <= 1e-11
+
+
+def test_zero_pivot_falls_back_to_the_pivoted_lu():
+    """A zero diagonal entry stops L D L^T at its first step (zsytrf would take a 2 x 2 pivot): the
+    factorisation is redone as the pivoted LU, and the answer is the same."""
+    import frico_amd
+    A, B = sym_system(300, 14, 2.0)
+    A[0, 0] = 0.0
+    with frico_amd.Context(0) as c:
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor("c64_sym_ir")
+        assert c.factored_form == frico_amd.SOLVE_C64_IR
+        info = c.solve(tol=1e-12, maxit=10)
+        assert info.converged
+        assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
+
+
+def test_growth_without_pivoting_falls_back_after_the_refinement_stalls():
+    """A pivot of 1e-20 against entries of order one makes the unpivoted factors grow by 1e20: the
+    complex64 factors then carry no information on the rest of the matrix and the refinement cannot
+    converge.  The solve notices the stall, refactors with the pivoted LU and converges."""
+    import frico_amd
+    A, B = sym_system(400, 15, 2.0)
+    A[0, 0] = 1e-20
+    with frico_amd.Context(0) as c:
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor("c64_sym_ir")
+        assert c.factored_form == frico_amd.SOLVE_C64_SYM_IR
+        info = c.solve(tol=1e-12, maxit=10)
+        print(f"{info.iterations} sweeps in all, residual {info.max_rel_residual:.2e}")
+        assert info.converged and c.factored_form == frico_amd.SOLVE_C64_IR
+        assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
+
+
+def test_approximate_mode_plate_rcs_with_the_symmetric_form(gpu_ctx):
+    """Approximate-mode plate (20 x 20 squares, N = 1160), 7 incidences: the refinement form takes
+    the symmetric factorisation there, as run_context takes solve_symmetric, with no fall-back, and
+    the monostatic RCS matches the complex128 LU's to 1e-9 and the oracle's zsytrf solve to 1e-9."""
+    import frico_amd
+    lam = meshes.wavelength(F0)
+    sm = meshes.plate(1.5 * lam, 20)
+    th = np.radians(np.arange(0.0, 91.0, 15.0))
+    gpu_ctx.load_mesh(sm, 5)
+    s128, _ = gpu_ctx.monostatic_sweep([F0], th, 0.0, mode="approx", form="c128")
+    s64, info = gpu_ctx.monostatic_sweep([F0], th, 0.0, mode="approx", form="c64_ir", tol=1e-12)
+    print(f"N={gpu_ctx.N}: {info[0].iterations} sweeps, residual {info[0].max_rel_residual:.2e}")
+    assert gpu_ctx.factored_form == frico_amd.SOLVE_C64_SYM_IR
+    assert info[0].converged
+    assert relmax(s64, s128) <= 1e-9
+    # the currents against the oracle's symmetric solve of the same system
+    om = oracle.OracleMesh(sm)
+    Z = om.fill(F0, approx=True)
+    V = om.rhs_planewave(F0, th[2], 0.0, 1.0, 0.0)
+    I, info_o = oracle.sy_solve(Z, V)
+    assert info_o == 0
+    gpu_ctx.fill(F0, "approx", "c128")
+    gpu_ctx.rhs_planewave(F0, [th[2]], [0.0])
+    gpu_ctx.factor("c64_sym_ir")
+    assert gpu_ctx.solve(tol=1e-12).converged
+    assert relmax(gpu_ctx.solution()[:, 0], I) <= 1e-9
+
+
+def test_approximate_mode_dipole_antenna_uses_the_symmetric_form(gpu_ctx):
+    """The 399-edge strip dipole in approximate mode through frico_antenna_sweep in the refinement form:
+    symmetric factorisation, Zin and gain equal to the oracle's (zsytrf) to 1e-9."""
+    import frico_amd
+    sm = meshes.workload_dipole_c1()
+    ref = oracle.antenna_analysis(sm, F0, degree=5, approx=True)
+    gpu_ctx.load_mesh(sm, 5)
+    out = gpu_ctx.antenna_sweep([F0], sm.feed, mode="approx", form="c64_ir", tol=1e-12)[0]
+    assert gpu_ctx.factored_form == frico_amd.SOLVE_C64_SYM_IR
+    zo = ref["port"]["impedance"]
+    assert abs(out["impedance"] - zo) / abs(zo) <= 1e-9
+    assert relmax(out["gain"], ref["gain"]) <= 1e-9
