@@ -43,18 +43,18 @@ def lu_flops(n, nrhs):
     return 8.0 * n ** 3 / 3.0 + 8.0 * n ** 2 * nrhs
 
 
-def run_step(ctx, form, th, tol, maxit):
-    ctx.fill(FREQ, "quad", "c128")
+def run_step(ctx, form, th, tol, maxit, mode="quad"):
+    ctx.fill(FREQ, mode, "c128")
     ctx.rhs_planewave(FREQ, th, 0.0, 1.0, 0.0)
     ctx.factor(form)
     return ctx.solve(tol=tol, maxit=maxit)
 
 
-def timed(ctx, form, th, steps, tol, maxit):
+def timed(ctx, form, th, steps, tol, maxit, mode="quad"):
     ctx.set_profiling(True)
     ctx.synchronize()
     t0 = time.perf_counter()
-    infos = [run_step(ctx, form, th, tol, maxit) for _ in range(steps)]
+    infos = [run_step(ctx, form, th, tol, maxit, mode) for _ in range(steps)]
     ctx.synchronize()
     t1 = time.perf_counter()
     prof = ctx.profile()
@@ -123,6 +123,53 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def approx_bench(args):
+    """Approximate-mode plate (the same N = 12160 mesh and 361 incidences; centroid fill of
+    compute_matrix_approx), solved as run_context solves it, with solve_symmetric: the complex64
+    L D L^T + FP64 refinement, beside the pivoted complex64 LU + refinement and the complex128 LU of
+    the same system.  value = (8N^3/3 + 8N^2 nrhs) / t_step, the LU's flop count, so that the
+    forms compare by time."""
+    import frico_amd
+    from frico_amd import meshes
+    sm = meshes.workload_plate(FREQ, args.n)
+    th = np.radians(np.arange(NRHS) * 0.25)
+    ctx = frico_amd.Context(0)
+    ctx.set_block(args.nb)
+    ctx.load_mesh(sm, DEGREE)
+    N, T = ctx.N, ctx.T
+    F = lu_flops(N, NRHS)
+    res = {}
+    sols = {}
+    for form in ("c64_sym_ir", "c64_ir", "c128"):
+        for _ in range(args.warmup):
+            run_step(ctx, form, th, args.tol, 10, "approx")
+        t, infos, prof = timed(ctx, form, th, args.steps, args.tol, 10, "approx")
+        sols[form] = ctx.solution()
+        res[form] = {"ms_per_step": t * 1e3, "tflops": F / t / 1e12, "factored_form": ctx.factored_form,
+                     "iterations": infos[-1].iterations, "max_rel_residual": infos[-1].max_rel_residual,
+                     "phases_ms": {k: prof[k] / args.steps
+                                   for k in ("fill_ms", "rhs_ms", "factor_ms", "solve_ms", "refine_ms")},
+                     "gemm_tflops": prof["gemm_flops"] / max(prof["gemm_ms"], 1e-9) / 1e9}
+        print(f"# approx {form} {t * 1e3:.2f} ms/step {infos[-1]} form {ctx.factored_form}", file=sys.stderr,
+              flush=True)
+    ref = sols["c128"]
+    for form in ("c64_sym_ir", "c64_ir"):
+        res[form]["solution_rel_err_vs_c128"] = float(np.max(np.abs(sols[form] - ref)) / np.max(np.abs(ref)))
+    sym = res["c64_sym_ir"]
+    out = {"metric": "N=12k approximate-mode fill+LDL^T+361-RHS time, TFLOP/s (LU flop count) at 1e-10 residual",
+           "value": sym["tflops"], "unit": "TFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": sym["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64 LDL^T + c128 fill/residual", "data": "synthetic (generated plate mesh, analytic plane waves)",
+           "config": {"workload": f"PEC plate 3x3 lambda @300 MHz, {args.n}x{args.n} squares, N={N}, T={T}, "
+                                  f"{NRHS} plane waves, approximate-mode fill", "N": N, "nrhs": NRHS,
+                      "tol": args.tol, "block": args.nb},
+           "forms": res,
+           "speedup_sym_vs_lu": res["c64_ir"]["ms_per_step"] / sym["ms_per_step"],
+           "speedup_sym_vs_c128": res["c128"]["ms_per_step"] / sym["ms_per_step"]}
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -133,8 +180,9 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=["plate", "batch"], default="plate",
-                    help="plate: the N=12160 headline; batch: the uneven 32-mesh x 16-frequency sweep")
+    ap.add_argument("--workload", choices=["plate", "batch", "approx"], default="plate",
+                    help="plate: the N=12160 headline; batch: the uneven 32-mesh x 16-frequency sweep; "
+                         "approx: the same plate in approximate mode with the symmetric solve")
     ap.add_argument("--workers", type=int, default=16, help="batch workload: concurrent contexts")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -144,6 +192,9 @@ def main():
 
     if args.workload == "batch":
         batch_bench(args)
+        return
+    if args.workload == "approx":
+        approx_bench(args)
         return
 
     import frico_amd

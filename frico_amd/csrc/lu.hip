@@ -1612,7 +1612,7 @@ static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int 
 // are stored as the LU of Z with no interchanges would be, L strictly below the diagonal and
 // U = D L^T on and above it, so the LU solves run on them unchanged with empty interchange lists.
 // Block column [k, k + w):
-//   ldlt_diag        L11 D1 L11^T of the w x w diagonal block, in LDS (one workgroup)
+//   ldlt_diag        L11 D1 L11^T of the w x w diagonal block, in registers (one workgroup)
 //   sym_transpose    A12 = A21^T: the block row, taken from the block column
 //   row_block_solve  U12 = L11^{-1} A12 (= D1 L21^T)
 //   sym_transpose    L21 = U12^T D1^{-1}
@@ -1621,72 +1621,124 @@ static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int 
 // accuracy of the answer rests on the refinement against the kept Z (the caller returns to the
 // pivoted LU when the refinement stalls).
 // ----------------------------------------------------------------------------------------------
-constexpr int LDLT_THREADS = 1024;
+// Workgroup of the diagonal-block kernel: one thread per (row, column group), 16 entries a thread.
+constexpr int LDLT_GROUPS = 8;
+constexpr int LDLT_THREADS = LU_NB_MAX * LDLT_GROUPS;
+constexpr int LDLT_OWN = LU_NB_MAX / LDLT_GROUPS;
 // block-row columns per transpose workgroup
 template <typename T> constexpr int kSymCols = sizeof(T) == 4 ? 32 : 16;
 
-// Right-looking L D L^T of the w x w (w <= 128) diagonal block at (k, k) in LDS: S[j ld + r] holds
-// entry (r, j), r >= j.  Step c: l_r = a(r, c) / d_c, a(r, j) -= l_r a(j, c) for c < j <= r; thread
-// (row r, group g) updates the columns j = c + 1 + g, c + 1 + g + 8, ... of row r.  Writes L below
-// the diagonal and U = D L^T on and above it.  A zero pivot leaves its column of L zero and sets
-// *info (1-based column) if unset.
+// Right-looking L D L^T of the w x w (w <= 128) diagonal block at (k, k), held in registers: thread
+// (row r, group g) keeps the lower-triangle entries (r, j), j = g, g + 8, ..., j <= r.  Step c reads
+// pivot column c from LDS (every thread's l_r = a(r, c) / d_c and its a(j, c)), updates
+// a(r, j) -= l_r a(j, c) for c < j <= r, and the owners of column c + 1 publish it for the next step
+// (two buffers, one barrier a step).  Writes L below the diagonal and U = D L^T on and above it.  A
+// zero pivot leaves its column of L zero and sets *info (1-based column) if unset.
 template <typename T>
 __global__ __launch_bounds__(LDLT_THREADS) void ldlt_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w,
                                                          int* info)
 {
-    extern __shared__ __align__(16) unsigned char ldlt_smem[];
-    cx<T>* S = reinterpret_cast<cx<T>*>(ldlt_smem);
-    const int ld = w + 1;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < w * w; e += LDLT_THREADS) {
-        const int r = e % w, j = e / w;
-        if (r >= j)
-            S[j * ld + r] = A[(k + r) + (k + j) * lda];
+    __shared__ cx<T> col[2][LU_NB_MAX];
+    __shared__ cx<T> pinv[2]; // 1 / d_c of the current step (0 for a zero pivot)
+    __shared__ cx<T> piv[LU_NB_MAX];
+    const int tid = threadIdx.x, r = tid % LU_NB_MAX, g = tid / LU_NB_MAX;
+    const bool row_ok = r < w;
+    cx<T> a[LDLT_OWN];
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++) {
+        const int j = g + LDLT_GROUPS * q;
+        const bool own = row_ok && j <= r;
+        a[q] = A[(own ? k + r : k) + (own ? k + j : k) * lda];
     }
-    __syncthreads();
-    constexpr int NG = LDLT_THREADS / LU_NB_MAX;
-    const int r = tid % LU_NB_MAX, g = tid / LU_NB_MAX;
-    for (int c = 0; c < w; c++) {
-        const cx<T> d = S[c * ld + c];
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++)
+        if (!(row_ok && g + LDLT_GROUPS * q <= r))
+            a[q] = cx<T>{0, 0};
+    // the pivot's owner publishes its inverse with the column
+    auto publish_inverse = [&](cx<T> d, int slot) {
         const bool zero = d.re == (T)0 && d.im == (T)0;
-        const cx<T> inv = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
-        if (zero && tid == 0 && *info == 0)
-            *info = (int)(k + c + 1);
-        cx<T> l{0, 0};
-        if (r > c && r < w) {
-            l = S[c * ld + r] * inv;
-            for (int j = c + 1 + g; j <= r; j += NG) {
-                const cx<T> v = S[c * ld + j];
-                cx<T>& a = S[j * ld + r];
-                a.re -= l.re * v.re - l.im * v.im;
-                a.im -= l.re * v.im + l.im * v.re;
+        pinv[slot] = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+        if (zero && *info == 0)
+            *info = (int)(k + r + 1);
+    };
+    if (g == 0 && row_ok)
+        col[0][r] = a[0];
+    if (tid == 0)
+        publish_inverse(a[0], 0);
+    __syncthreads();
+    const int rmax = r | 63; // last row of this wave
+    for (int c = 0; c < w; c++) {
+        const cx<T>* cb = col[c & 1];
+        // waves whose 64 rows all lie at or above row c, or past the block, have nothing left to do
+        if (rmax > c && rmax - 63 < w) {
+            cx<T> v[LDLT_OWN];
+#pragma unroll
+            for (int q = 0; q < LDLT_OWN; q++)
+                v[q] = cb[g + LDLT_GROUPS * q];
+            const cx<T> vr = cb[r];
+            const cx<T> inv = pinv[c & 1];
+            const bool below = row_ok && r > c;
+            const cx<T> l = below ? vr * inv : cx<T>{0, 0};
+            // columns c < j <= rmax (a wave-uniform test).  A lane also updates its entries with
+            // j > r, outside the triangle: they are never read, and skipping them would cost more.
+#pragma unroll
+            for (int q = 0; q < LDLT_OWN; q++) {
+                const int j = g + LDLT_GROUPS * q;
+                if (j > c && j <= rmax) {
+                    a[q].re -= l.re * v[q].re - l.im * v[q].im;
+                    a[q].im -= l.re * v[q].im + l.im * v[q].re;
+                }
+            }
+            const int qc = c / LDLT_GROUPS, gc = c % LDLT_GROUPS;
+            const int qn = (c + 1) / LDLT_GROUPS, gn = (c + 1) % LDLT_GROUPS;
+            if (g == gc) {
+#pragma unroll
+                for (int q = 0; q < LDLT_OWN; q++)
+                    if (q == qc)
+                        a[q] = below ? l : a[q];
+            }
+            if (g == gn && c + 1 < w) {
+                cx<T> nx = a[0];
+#pragma unroll
+                for (int q = 1; q < LDLT_OWN; q++)
+                    nx = q == qn ? a[q] : nx;
+                if (below)
+                    col[(c + 1) & 1][r] = nx;
+                if (r == c + 1)
+                    publish_inverse(nx, (c + 1) & 1);
             }
         }
         __syncthreads();
-        // column c is not read again: the next step reads column c + 1 and the columns right of it
-        if (g == 0 && r > c && r < w)
-            S[c * ld + r] = l;
     }
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++)
+        if (row_ok && g + LDLT_GROUPS * q == r)
+            piv[r] = a[q];
     __syncthreads();
-    for (int e = tid; e < w * w; e += LDLT_THREADS) {
-        const int r = e % w, j = e / w;
-        cx<T> v;
-        if (r >= j)
-            v = S[j * ld + r];
-        else
-            v = S[r * ld + r] * S[r * ld + j]; // U(r, j) = d_r L(j, r)
-        A[(k + r) + (k + j) * lda] = v;
+    cx<T> dj[LDLT_OWN];
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++)
+        dj[q] = piv[g + LDLT_GROUPS * q];
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++) {
+        const int j = g + LDLT_GROUPS * q;
+        if (row_ok && j <= r) {
+            A[(k + r) + (k + j) * lda] = a[q];            // L(r, j); d_r on the diagonal
+            if (j < r)
+                A[(k + j) + (k + r) * lda] = dj[q] * a[q]; // U(j, r) = d_j L(r, j)
+        }
     }
 }
 
 // Block row <-> block column of step k (w columns, m = n - k - w rows below), kSymCols values of c a
-// workgroup:
+// workgroup, every load of a thread in flight before its LDS stores:
 //   TO_LOWER false: A(k + i, k + w + c) = A(k + w + c, k + i)                        (A12 = A21^T)
 //   TO_LOWER true:  A(k + w + c, k + i) = A(k + i, k + w + c) / A(k + i, k + i)     (L21 = U12^T D^{-1})
 template <typename T, bool TO_LOWER>
 __global__ __launch_bounds__(256) void sym_transpose(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int64_t m)
 {
     constexpr int TC = kSymCols<T>;
+    constexpr int NE = LU_NB_MAX * TC / 256;
     __shared__ cx<T> S[LU_NB_MAX][TC + 1];
     __shared__ cx<T> sinv[LU_NB_MAX];
     const int tid = threadIdx.x;
@@ -1694,32 +1746,49 @@ __global__ __launch_bounds__(256) void sym_transpose(cx<T>* __restrict__ A, int6
     const int nc = (int)((m - c0) < TC ? (m - c0) : TC);
     cx<T>* A21 = A + (k + w + c0) + k * lda; // entry (c, i) at A21[c + i lda]
     cx<T>* A12 = A + k + (k + w + c0) * lda; // entry (i, c) at A12[i + c lda]
+    cx<T> v[NE];
     if (TO_LOWER) {
-        for (int i = tid; i < w; i += 256) {
-            const cx<T> d = A[(k + i) * (lda + 1)];
-            sinv[i] = (d.re == (T)0 && d.im == (T)0) ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+        // block row, down each column (i fastest)
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, i = e % LU_NB_MAX, c = e / LU_NB_MAX;
+            const bool ok = i < w && c < nc;
+            v[s] = A12[ok ? i + (int64_t)c * lda : 0];
         }
-        for (int e = tid; e < w * TC; e += 256) {
-            const int i = e % w, c = e / w;
-            if (c < nc)
-                S[i][c] = A12[i + (int64_t)c * lda];
+        if (tid < w) {
+            const cx<T> d = A[(k + tid) * (lda + 1)];
+            sinv[tid] = (d.re == (T)0 && d.im == (T)0) ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+        }
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, i = e % LU_NB_MAX, c = e / LU_NB_MAX;
+            S[i][c] = v[s];
         }
         __syncthreads();
-        for (int e = tid; e < w * TC; e += 256) {
-            const int c = e % TC, i = e / TC;
-            if (c < nc)
+        // block column, down each of its columns (c fastest)
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, c = e % TC, i = e / TC;
+            if (i < w && c < nc)
                 A21[c + (int64_t)i * lda] = S[i][c] * sinv[i];
         }
     } else {
-        for (int e = tid; e < w * TC; e += 256) {
-            const int c = e % TC, i = e / TC;
-            if (c < nc)
-                S[i][c] = A21[c + (int64_t)i * lda];
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, c = e % TC, i = e / TC;
+            const bool ok = i < w && c < nc;
+            v[s] = A21[ok ? c + (int64_t)i * lda : 0];
+        }
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, c = e % TC, i = e / TC;
+            S[i][c] = v[s];
         }
         __syncthreads();
-        for (int e = tid; e < w * TC; e += 256) {
-            const int i = e % w, c = e / w;
-            if (c < nc)
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, i = e % LU_NB_MAX, c = e / LU_NB_MAX;
+            if (i < w && c < nc)
                 A12[i + (int64_t)c * lda] = S[i][c];
         }
     }
@@ -1741,14 +1810,9 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
 {
     if (nb > LU_NB_MAX || nb < 1)
         return set_error("ldlt: block size out of range"), -1;
-    const size_t sh_max = (size_t)nb * (nb + 1) * sizeof(cx<T>);
-    if (sh_max > 160 * 1024)
-        return set_error("ldlt: a %d-column diagonal block of this type does not fit in LDS", nb), -1;
     const int64_t npan = (n + nb - 1) / nb;
     if (!wk.perm || wk.nperm < npan)
         return set_error("ldlt: interchange-list workspace too small"), -1;
-    if (sh_max > 64 * 1024)
-        FRICO_HIP_CHECK(allow_lds(reinterpret_cast<const void*>(&ldlt_diag<T>), sh_max));
     const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
     // no interchanges: empty lists for the solves, the identity in ipiv
     const int64_t nz = npan * PERM_STRIDE;
@@ -1759,8 +1823,7 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
     // diagonal block, block row and block column of step k
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
-        launch_kernel(&ldlt_diag<T>, dim3(1), dim3(LDLT_THREADS), (size_t)w * (w + 1) * sizeof(cx<T>), st, A, lda, k, w,
-                      wk.flags + 1);
+        launch_kernel(&ldlt_diag<T>, dim3(1), dim3(LDLT_THREADS), 0, st, A, lda, k, w, wk.flags + 1);
         if (m > 0) {
             const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
             launch_kernel(&sym_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m);

@@ -46,8 +46,7 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
 // Z = L D L^T without pivoting, for complex symmetric Z (only its lower triangle is read).  The
 // factors are stored as lu_factor's would be with no interchanges (L below the diagonal, U = D L^T
 // on and above it; empty interchange lists in LuWork::perm, ipiv the identity), so lu_solve and
-// lu_solve_upper take them as they are.  naug as for lu_factor.  nb <= 128 for complex64, <= 64 for
-// complex128 (the diagonal block is factored in LDS).
+// lu_solve_upper take them as they are.  naug as for lu_factor.  nb <= 128.
 template <typename T>
 int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
                 KernelTimer* gemm_timer, int64_t naug = 0);
