@@ -170,6 +170,57 @@ def approx_bench(args):
     print(json.dumps(out))
 
 
+def nearfield_bench(args):
+    """Near-field sampling plane on the N = 12160 plate: eval_fields(mesh) (one current vector, E
+    and H at every vertex of a -P plane) after one normal-incidence solve.  The plane is
+    "xz:3l:3l:h" (y = 0, across the plate), h = lambda/100: 301 x 301 = 90601 points, 7.4e8
+    point-triangle evaluations.  value = evaluations per second over frico_fields calls (host
+    arrays in and out, as the reference's caller gets them)."""
+    import frico_amd
+    from frico_amd import meshes
+    sm = meshes.workload_plate(FREQ, args.n)
+    lam = meshes.wavelength(FREQ)
+    ctx = frico_amd.Context(0)
+    ctx.load_mesh(sm, DEGREE)
+    N, T = ctx.N, ctx.T
+    ctx.fill(FREQ, "quad", "c128")
+    ctx.rhs_planewave(FREQ, [0.0], [0.0], 1.0, 0.0)
+    ctx.factor("c64_ir")
+    ctx.solve(tol=args.tol)
+    spec = f"xz:{3 * lam}:{3 * lam}:{lam / 100}"
+    sp = frico_amd.parse_sampling_plane(spec)
+    V, _ = frico_amd.sampling_plane_mesh(sp)
+    P = V.shape[0]
+    for _ in range(args.warmup):
+        ctx.fields(FREQ, V)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.fields(FREQ, V)
+    ctx.synchronize()
+    t = (time.perf_counter() - t0) / args.steps
+    out = {"metric": "near-field plane point-triangle evaluations per second (eval_fields on a -P plane)",
+           "value": P * T / t, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": t * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (generated plate mesh, its normal-incidence currents)",
+           "config": {"workload": f"PEC plate 3x3 lambda, N={N}, T={T}; plane {spec}", "points": P, "triangles": T}}
+    if not args.no_cpu:
+        import oracle
+        om = oracle.OracleMesh(sm)
+        I = ctx.solution()[:, 0]
+        AJ, AdivJ, _ = om.scatter(I)
+        sample = V[:: max(1, P // 4000)]
+        t0 = time.perf_counter()
+        om.fields(FREQ, AJ, AdivJ, sample)
+        tc = time.perf_counter() - t0
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        out["cpu_baseline"] = {"value": sample.shape[0] * T / tc, "unit": "evals/s", "cores": threads, "kind": "port",
+                               "sample": f"{sample.shape[0]} of the plane's points, oracle eval_fields (C++ OpenMP), "
+                                         f"{tc:.2f} s"}
+    ctx.close()
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -180,9 +231,10 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--workload", choices=["plate", "batch", "approx"], default="plate",
+    ap.add_argument("--workload", choices=["plate", "batch", "approx", "nearfield"], default="plate",
                     help="plate: the N=12160 headline; batch: the uneven 32-mesh x 16-frequency sweep; "
-                         "approx: the same plate in approximate mode with the symmetric solve")
+                         "approx: the same plate in approximate mode with the symmetric solve; "
+                         "nearfield: E and H on a 90601-point sampling plane over that plate")
     ap.add_argument("--workers", type=int, default=16, help="batch workload: concurrent contexts")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -195,6 +247,9 @@ def main():
         return
     if args.workload == "approx":
         approx_bench(args)
+        return
+    if args.workload == "nearfield":
+        nearfield_bench(args)
         return
 
     import frico_amd

@@ -18,7 +18,7 @@ from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_SY
 
 __all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR",
-           "SOLVE_C64_SYM_IR"]
+           "SOLVE_C64_SYM_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh"]
 
 
 class FricoError(RuntimeError):
@@ -40,6 +40,46 @@ class SolveInfo:
     iterations: int
     converged: bool
     max_rel_residual: float
+
+
+@dataclass
+class SamplingPlane:
+    """A -P plane (src/sampling_planes.h:33-38): `axis` is the reference's `normal`, the axis
+    about which the rectangle width x height (in z = 0) is turned by 90 degrees; h = element size."""
+    axis: tuple
+    width: float
+    height: float
+    h: float
+
+
+def parse_sampling_plane(spec: str) -> SamplingPlane:
+    """parse_sampling_plane (src/sampling_planes.cpp:84-106): "xy|yz|xz:w:h:size" or
+    "nx:ny:nz:w:h:size".  Raises FricoError for an invalid specification."""
+    L = _lib.load()
+    ax = np.zeros(3)
+    w, hh, sz = C.c_double(), C.c_double(), C.c_double()
+    rc = L.frico_sampling_plane_parse(spec.encode(), _p(ax), C.byref(w), C.byref(hh), C.byref(sz))
+    if rc:
+        raise FricoError(rc, L.frico_last_error().decode())
+    return SamplingPlane(tuple(float(a) for a in ax), w.value, hh.value, sz.value)
+
+
+def sampling_plane_mesh(sp: SamplingPlane):
+    """The plane's mesh (make_sampling_plane_mesh, src/sampling_planes.cpp:108-156) as a structured
+    mesh of the same rectangle: returns (vertices nv x 3, triangles nt x 3)."""
+    L = _lib.load()
+    ax = np.ascontiguousarray(sp.axis, np.float64)
+    nv, nt = C.c_int64(), C.c_int64()
+    rc = L.frico_sampling_plane_mesh(_p(ax), sp.width, sp.height, sp.h, None, 0, None, 0, C.byref(nv), C.byref(nt))
+    if rc:
+        raise FricoError(rc, L.frico_last_error().decode())
+    V = np.zeros((nv.value, 3))
+    T = np.zeros((nt.value, 3), np.int64)
+    rc = L.frico_sampling_plane_mesh(_p(ax), sp.width, sp.height, sp.h, _p(V), nv.value, _p(T, C.c_int64),
+                                     nt.value, C.byref(nv), C.byref(nt))
+    if rc:
+        raise FricoError(rc, L.frico_last_error().decode())
+    return V, T
 
 
 def _p(a, t=C.c_double):
@@ -229,6 +269,15 @@ class Context:
         self._check(self._L.frico_fields(self._h, float(freq), _p(pts), n,
                                          None if cc is None else _p(cc, C.c_int64), float(h_sign), _p(E), _p(H)))
         return E, H
+
+    def fields_on_plane(self, freq, plane, col=0):
+        """eval_fields(sim, ctx, smpmsh, E, H) (src/emw_postpro_common.cpp:77-90) on a -P plane
+        (a spec string or a SamplingPlane): E and H at every vertex of the plane's mesh from
+        solution column `col`, as write_fields evaluates them (:180-189).  Returns (V, T, E, H)."""
+        sp = parse_sampling_plane(plane) if isinstance(plane, str) else plane
+        V, T = sampling_plane_mesh(sp)
+        E, H = self.fields(freq, V, cols=np.full(V.shape[0], col, np.int64))
+        return V, T, E, H
 
     # ---- whole analyses (do_sweep mirrors) ----------------------------------------------------
     def antenna_sweep(self, freqs, ifaces, voltage=1.0, mode="quad", R=10.0, Z0=50.0,

@@ -165,6 +165,24 @@ int frico_fields(frico_ctx* ctx, double freq, const double* pts, int64_t npts, c
                  double h_sign, double* E, double* H);
 
 /*
+ * Near-field sampling planes (the -P argument, src/main.cpp:164-181).
+ * frico_sampling_plane_parse: parse_sampling_plane (src/sampling_planes.cpp:30-106):
+ *   "xy|yz|xz:width:height:h" or "nx:ny:nz:width:height:h".  axis[3] is the reference's `normal`:
+ *   the axis about which its mesh turns the rectangle by 90 degrees ("xy" -> z, "yz" -> y,
+ *   "xz" -> x), normalised in the long form.  Returns -1 for an invalid specification.
+ * frico_sampling_plane_mesh: the plane's mesh (make_sampling_plane_mesh, :108-156): the rectangle
+ *   [-w/2, w/2] x [-h/2, h/2] in z = 0 turned by pi/2 about axis, as a structured mesh of
+ *   ceil(width/h) x ceil(height/h) cells in right triangles (the reference meshes it with gmsh).
+ *   vertices: nv x 3 row-major, triangles: nt x 3; pass NULL for both to get nv and nt only.
+ * eval_fields(mesh) (src/emw_postpro_common.cpp:77-90) is frico_fields at the mesh's vertices.
+ */
+int frico_sampling_plane_parse(const char* spec, double* axis, double* width, double* height,
+                               double* h);
+int frico_sampling_plane_mesh(const double* axis, double width, double height, double h,
+                              double* vertices, int64_t max_vertices, int64_t* triangles,
+                              int64_t max_triangles, int64_t* nv, int64_t* nt);
+
+/*
  * Whole analyses, mirroring the reference entry points for one frequency each.
  *
  * frico_antenna_sweep: do_sweep(simulation&, antenna_analysis&) (src/emw_ana_antenna.cpp:236-295)
