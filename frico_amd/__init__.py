@@ -18,7 +18,9 @@ from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_SY
 
 __all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR",
-           "SOLVE_C64_SYM_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh"]
+           "SOLVE_C64_SYM_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh",
+           "format_double", "write_port_sweep", "write_radiation_diagram", "write_bistatic_rcs",
+           "write_antenna_text_outputs"]
 
 
 class FricoError(RuntimeError):
@@ -80,6 +82,53 @@ def sampling_plane_mesh(sp: SamplingPlane):
     if rc:
         raise FricoError(rc, L.frico_last_error().decode())
     return V, T
+
+
+def format_double(v: float) -> str:
+    """std::format("{}", v) (std::to_chars shortest form), as the reference's text outputs print."""
+    L = _lib.load()
+    buf = C.create_string_buffer(64)
+    rc = L.frico_format_double(float(v), buf, 64)
+    if rc:
+        raise FricoError(rc, L.frico_last_error().decode())
+    return buf.value.decode()
+
+
+def _write(fn, *args):
+    L = _lib.load()
+    rc = getattr(L, fn)(*args)
+    if rc:
+        raise FricoError(rc, L.frico_last_error().decode())
+
+
+def write_port_sweep(path, Z0, freqs, zin, swr):
+    """port_sweep.txt (write_sweep_data, src/emw_ana_antenna.cpp:150-169)."""
+    f = np.ascontiguousarray(freqs, np.float64)
+    z = np.ascontiguousarray(np.asarray(zin, np.complex128).view(np.float64))
+    w = np.ascontiguousarray(swr, np.float64)
+    _write("frico_write_port_sweep", str(path).encode(), float(Z0), f.size, _p(f), _p(z), _p(w))
+
+
+def write_radiation_diagram(path, freq, gain):
+    """polar_<n>.txt (write_radiation_diagrams, src/emw_ana_antenna.cpp:171-203); gain 3 x 360."""
+    g = np.ascontiguousarray(gain, np.float64).reshape(3, 360)
+    _write("frico_write_radiation_diagram", str(path).encode(), float(freq), _p(g))
+
+
+def write_bistatic_rcs(path, table):
+    """bistatic_rcs_<n>.txt (src/emw_ana_bistatic_rcs.cpp:97-117); table 359 x 4."""
+    t = np.ascontiguousarray(table, np.float64).reshape(359, 4)
+    _write("frico_write_bistatic_rcs", str(path).encode(), _p(t))
+
+
+def write_antenna_text_outputs(directory, freqs, results, Z0=50.0):
+    """The text-output block of do_sweep (src/emw_ana_antenna.cpp:276-285): polar_<n>.txt for each
+    sweep step and port_sweep.txt, from antenna_sweep's results."""
+    import os
+    for n, r in enumerate(results):
+        write_radiation_diagram(os.path.join(directory, f"polar_{n}.txt"), freqs[n], r["gain"])
+    write_port_sweep(os.path.join(directory, "port_sweep.txt"), Z0, freqs, [r["impedance"] for r in results],
+                     [r["swr"] for r in results])
 
 
 def _p(a, t=C.c_double):

@@ -92,6 +92,10 @@ SIGNATURES = {
     "frico_sampling_plane_parse": (C.c_int, [C.c_char_p, _d, _d, _d, _d]),
     "frico_sampling_plane_mesh": (C.c_int, [_d, C.c_double, C.c_double, C.c_double, _d, C.c_int64, _i64, C.c_int64,
                                             _i64, _i64]),
+    "frico_write_port_sweep": (C.c_int, [C.c_char_p, C.c_double, C.c_int64, _d, _d, _d]),
+    "frico_write_radiation_diagram": (C.c_int, [C.c_char_p, C.c_double, _d]),
+    "frico_write_bistatic_rcs": (C.c_int, [C.c_char_p, _d]),
+    "frico_format_double": (C.c_int, [C.c_double, C.c_char_p, C.c_int64]),
     "frico_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "frico_batch_destroy": (C.c_int, [_vp]),
     "frico_batch_add_mesh": (C.c_int, [_vp, _d, C.c_int64, _i64, _i32, C.c_int64, _i64, _i32, C.c_int64, C.c_int,

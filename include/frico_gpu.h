@@ -183,6 +183,23 @@ int frico_sampling_plane_mesh(const double* axis, double width, double height, d
                               int64_t max_triangles, int64_t* nv, int64_t* nt);
 
 /*
+ * Text outputs, byte for byte as the reference's std::println writes them ("{}" of a double is
+ * the shortest round-trip std::to_chars form):
+ * frico_write_port_sweep: port_sweep.txt, write_sweep_data (src/emw_ana_antenna.cpp:150-169);
+ *   zin[2*i], zin[2*i+1] = Re, Im of Zin at step i.
+ * frico_write_radiation_diagram: polar_<n>.txt, write_radiation_diagrams (:171-203); gain = 3 x 360
+ *   (planes xy, yz, xz), as frico_antenna_sweep's gain_out for one frequency.
+ * frico_write_bistatic_rcs: bistatic_rcs_<n>.txt (src/emw_ana_bistatic_rcs.cpp:97-117); table =
+ *   359 x 4, as frico_bistatic_sweep's table_out for one frequency.
+ * frico_format_double: the "{}" form of one double (for callers writing their own lines).
+ */
+int frico_write_port_sweep(const char* path, double Z0, int64_t nsteps, const double* freqs,
+                           const double* zin, const double* swr);
+int frico_write_radiation_diagram(const char* path, double freq, const double* gain);
+int frico_write_bistatic_rcs(const char* path, const double* table);
+int frico_format_double(double v, char* buf, int64_t size);
+
+/*
  * Whole analyses, mirroring the reference entry points for one frequency each.
  *
  * frico_antenna_sweep: do_sweep(simulation&, antenna_analysis&) (src/emw_ana_antenna.cpp:236-295)
