@@ -925,7 +925,42 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 __syncthreads(); // published rows are read before the update below rewrites them
             }
         }
-        if (mine) {
+        if (R <= 128) {
+            // two threads a row (the slab has at most 128 rows): the row's owner and thread
+            // owner + 128 take alternate groups of eight columns, the group after the last pair
+            // and the remainder going to the side with fewer groups.  The multiplier is read from
+            // the slab, where the owner wrote it before the candidate's barrier.
+            const int rr = tid & 127, side = tid >> 7;
+            if (pval > 0.0 && rr < nr && r0 + rr > c) {
+                const cx<T> lr = slab[(size_t)c * RS + rr];
+                const int j0 = c + 2, ng = (w - j0) > 0 ? (w - j0) / 8 : 0;
+                for (int gi = side; gi < ng; gi += 2) {
+                    const int j = j0 + 8 * gi;
+                    cx<T> a[8], u[8];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        a[q] = slab[(size_t)(j + q) * RS + rr];
+                        u[q] = prow[j + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        a[q].re -= lr.re * u[q].re - lr.im * u[q].im;
+                        a[q].im -= lr.re * u[q].im + lr.im * u[q].re;
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        slab[(size_t)(j + q) * RS + rr] = a[q];
+                }
+                if (side == (ng & 1)) {
+                    for (int j = j0 + 8 * ng; j < w; j++) {
+                        const cx<T> u = prow[j];
+                        cx<T>& a = slab[(size_t)j * RS + rr];
+                        a.re -= lr.re * u.re - lr.im * u.im;
+                        a.im -= lr.re * u.im + lr.im * u.re;
+                    }
+                }
+            }
+        } else if (mine) {
             int j = c + 2;
             // eight columns at a time, all LDS reads of a group in flight together
             for (; j + 8 <= w; j += 8) {
