@@ -19,7 +19,7 @@ from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_SY
 __all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR",
            "SOLVE_C64_SYM_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh",
-           "format_double", "write_port_sweep", "write_radiation_diagram", "write_bistatic_rcs",
+           "read_msh", "format_double", "write_port_sweep", "write_radiation_diagram", "write_bistatic_rcs",
            "write_antenna_text_outputs"]
 
 
@@ -82,6 +82,44 @@ def sampling_plane_mesh(sp: SamplingPlane):
     if rc:
         raise FricoError(rc, L.frico_last_error().decode())
     return V, T
+
+
+def read_msh(path, feed_group=None):
+    """A gmsh 4.1 ASCII .msh file read as load_from_gmsh reads it through gmsh (see frico_msh_read):
+    returns (SurfaceMesh, groups) with groups = {name: (dim, tag, [entity tags])}.  feed_group
+    names the physical group of the delta-gap curves, as FRICO's source option does; its entity
+    tags become the mesh's feed."""
+    L = _lib.load()
+    h = C.c_void_p()
+    rc = L.frico_msh_read(str(path).encode(), C.byref(h))
+    if rc:
+        raise FricoError(rc, L.frico_last_error().decode())
+    try:
+        nv, nt, nb, ng = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int32()
+        L.frico_msh_sizes(h, C.byref(nv), C.byref(nt), C.byref(nb), C.byref(ng))
+        V = np.zeros((nv.value, 3))
+        T = np.zeros((nt.value, 3), np.int64)
+        tg = np.zeros(nt.value, np.int32)
+        B = np.zeros((nb.value, 2), np.int64)
+        bt = np.zeros(nb.value, np.int32)
+        L.frico_msh_arrays(h, _p(V), _p(T, C.c_int64), _p(tg, C.c_int32), _p(B, C.c_int64), _p(bt, C.c_int32))
+        groups = {}
+        for i in range(ng.value):
+            name = C.create_string_buffer(256)
+            dim, tag, ne = C.c_int32(), C.c_int32(), C.c_int32()
+            L.frico_msh_group(h, i, name, 256, C.byref(dim), C.byref(tag), None, 0, C.byref(ne))
+            ents = np.zeros(ne.value, np.int32)
+            L.frico_msh_group(h, i, None, 0, None, None, _p(ents, C.c_int32), ne.value, C.byref(ne))
+            groups[name.value.decode()] = (dim.value, tag.value, [int(e) for e in ents])
+    finally:
+        L.frico_msh_free(h)
+    feed = []
+    if feed_group is not None:
+        if feed_group not in groups:
+            raise FricoError(-2, f'Unknown physical group "{feed_group}": cannot enable source')
+        feed = groups[feed_group][2]
+    sm = meshes.SurfaceMesh(V, T, tg, B, bt, name=str(path), feed=feed)
+    return sm, groups
 
 
 def format_double(v: float) -> str:

@@ -96,6 +96,11 @@ SIGNATURES = {
     "frico_write_radiation_diagram": (C.c_int, [C.c_char_p, C.c_double, _d]),
     "frico_write_bistatic_rcs": (C.c_int, [C.c_char_p, _d]),
     "frico_format_double": (C.c_int, [C.c_double, C.c_char_p, C.c_int64]),
+    "frico_msh_read": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
+    "frico_msh_free": (C.c_int, [_vp]),
+    "frico_msh_sizes": (C.c_int, [_vp, _i64, _i64, _i64, _i32]),
+    "frico_msh_arrays": (C.c_int, [_vp, _d, _i64, _i32, _i64, _i32]),
+    "frico_msh_group": (C.c_int, [_vp, C.c_int32, C.c_char_p, C.c_int64, _i32, _i32, _i32, C.c_int32, _i32]),
     "frico_batch_create": (C.c_int, [C.c_int, C.c_int, C.POINTER(_vp)]),
     "frico_batch_destroy": (C.c_int, [_vp]),
     "frico_batch_add_mesh": (C.c_int, [_vp, _d, C.c_int64, _i64, _i32, C.c_int64, _i64, _i32, C.c_int64, C.c_int,

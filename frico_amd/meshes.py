@@ -260,3 +260,72 @@ def workload_plate(freq: float = 300e6, n: int = 64) -> SurfaceMesh:
 def workload_bowtie(freq: float = 300e6, nx: int = 20, ny: int = 17) -> SurfaceMesh:
     """Bowtie, arm 0.25 wavelength, flare 60 degrees: N = 6 nx ny - 2 nx - ny (1983)."""
     return bowtie(0.25 * wavelength(freq), 60.0, nx, ny, name="bowtie")
+
+
+def write_msh(sm: SurfaceMesh, path, groups=None, node_blocks=None, tag_offset: int = 0) -> None:
+    """Writes a SurfaceMesh as a gmsh 4.1 ASCII .msh file (for meshes made elsewhere and for tests).
+
+    Surfaces are the distinct triangle tags, curves the distinct edge tags; groups maps a physical
+    name to (dim, [entity tags]).  node_blocks: optional list of (dim, entity tag, vertex indices)
+    giving the $Nodes blocks in file order (default: every vertex in one block on the first
+    surface).  Node tags are vertex index + 1 + tag_offset.
+    """
+    groups = groups or {}
+    stags = sorted(set(int(t) for t in sm.tags))
+    ctags = sorted(set(int(t) for t in sm.btags))
+    pnum = {}  # (dim, name) -> physical tag
+    for i, (name, (dim, _)) in enumerate(sorted(groups.items()), start=1):
+        pnum[(dim, name)] = i
+
+    def phys(dim, etag):
+        return [pnum[(d, n)] for n, (d, ents) in sorted(groups.items()) if d == dim and etag in ents]
+
+    V = sm.vertices
+    lines = ["$MeshFormat", "4.1 0 8", "$EndMeshFormat"]
+    if groups:
+        lines.append("$PhysicalNames")
+        lines.append(str(len(groups)))
+        for (dim, name), t in sorted(pnum.items(), key=lambda kv: kv[1]):
+            lines.append(f'{dim} {t} "{name}"')
+        lines.append("$EndPhysicalNames")
+    lines.append("$Entities")
+    lines.append(f"0 {len(ctags)} {len(stags)} 0")
+    lo, hi = V.min(axis=0), V.max(axis=0)
+    box = " ".join(repr(float(x)) for x in (*lo, *hi))
+    for t in ctags:
+        p = phys(1, t)
+        lines.append(f"{t} {box} {len(p)} {' '.join(map(str, p))} 0".replace("  ", " "))
+    for t in stags:
+        p = phys(2, t)
+        lines.append(f"{t} {box} {len(p)} {' '.join(map(str, p))} 0".replace("  ", " "))
+    lines.append("$EndEntities")
+    if node_blocks is None:
+        node_blocks = [(2, stags[0], list(range(V.shape[0])))]
+    nn = sum(len(b[2]) for b in node_blocks)
+    alltags = [i + 1 + tag_offset for b in node_blocks for i in b[2]]
+    lines.append("$Nodes")
+    lines.append(f"{len(node_blocks)} {nn} {min(alltags)} {max(alltags)}")
+    for dim, etag, idx in node_blocks:
+        lines.append(f"{dim} {etag} 0 {len(idx)}")
+        lines += [str(i + 1 + tag_offset) for i in idx]
+        lines += [" ".join(repr(float(x)) for x in V[i]) for i in idx]
+    lines.append("$EndNodes")
+    eblocks = []
+    for t in ctags:
+        e = sm.bedges[sm.btags == t]
+        eblocks.append((1, t, 1, e))
+    for t in stags:
+        e = sm.triangles[sm.tags == t]
+        eblocks.append((2, t, 2, e))
+    ne = sum(len(b[3]) for b in eblocks)
+    lines.append("$Elements")
+    lines.append(f"{len(eblocks)} {ne} 1 {ne}")
+    et = 1
+    for dim, t, typ, e in eblocks:
+        lines.append(f"{dim} {t} {typ} {len(e)}")
+        for row in e:
+            lines.append(f"{et} " + " ".join(str(int(v) + 1 + tag_offset) for v in row))
+            et += 1
+    lines.append("$EndElements")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")

@@ -52,6 +52,7 @@ extern "C" {
 #define FRICO_SOLVE_C64_SYM_IR 2
 
 typedef struct frico_ctx frico_ctx;
+typedef struct frico_msh frico_msh;
 
 /* One RWG basis function (src/rwg_basis.h:41-62), in matrix order. */
 typedef struct frico_rwg {
@@ -163,6 +164,23 @@ int frico_triangle_currents(frico_ctx* ctx, int64_t col, double* tri_AJ, double*
  * row-major.  h_sign = +1 keeps the reference's exp(+jkR) in H (:60-61), -1 uses exp(-jkR). */
 int frico_fields(frico_ctx* ctx, double freq, const double* pts, int64_t npts, const int64_t* cols,
                  double h_sign, double* E, double* H);
+
+/*
+ * gmsh .msh ingestion (format 4.1 ASCII): load_from_gmsh (src/input_gmsh.cpp:36-406) without the
+ * gmsh library.  The arrays are what the reference reads back through gmsh's API, in its order:
+ * nodes by entity (dim, tag) and file order within an entity; triangles of the surfaces by tag,
+ * then file order; line elements of the curves likewise; vertices no triangle uses dropped (the
+ * others keep their order).  Pass them to frico_mesh_load.  Physical groups are listed by name
+ * (the reference's std::map), each with its dimension, tag and entity tags.
+ * Returns FRICO_ERR_MESH with a message for a file it cannot read.
+ */
+int frico_msh_read(const char* path, frico_msh** out);
+int frico_msh_free(frico_msh* msh);
+int frico_msh_sizes(const frico_msh* msh, int64_t* nv, int64_t* nt, int64_t* nb, int32_t* ngroups);
+int frico_msh_arrays(const frico_msh* msh, double* vertices, int64_t* triangles, int32_t* tags,
+                     int64_t* bedges, int32_t* btags);
+int frico_msh_group(const frico_msh* msh, int32_t i, char* name, int64_t name_size, int32_t* dim,
+                    int32_t* tag, int32_t* entities, int32_t max_entities, int32_t* nentities);
 
 /*
  * Near-field sampling planes (the -P argument, src/main.cpp:164-181).
