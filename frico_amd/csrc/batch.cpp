@@ -93,13 +93,17 @@ struct RunOpts {
 struct Graph {
     hipGraph_t g = nullptr;
     hipGraphExec_t x = nullptr;
-    ~Graph()
+    bool exact = true; // residual form the graph was recorded with
+    void reset()
     {
         if (x)
             (void)hipGraphExecDestroy(x);
         if (g)
             (void)hipGraphDestroy(g);
+        x = nullptr;
+        g = nullptr;
     }
+    ~Graph() { reset(); }
 };
 
 // Builds an executable graph from the kernels fn launches on the context's stream (recorded, not
@@ -140,6 +144,13 @@ int solve_graphs(frico_ctx* c, const RunOpts& o, Graph& g1, Graph& g2, frico_sol
     std::vector<double> bn;
     if ((rc = col_norms(c, c->B.p, c->nrhs, bn)))
         return rc;
+    // the residual form follows the tolerance (as in frico_solve) and is part of the recording
+    c->resid_exact = !(o.tol >= 1e-10);
+    if (g1.x && g1.exact != c->resid_exact) {
+        g1.reset();
+        g2.reset();
+    }
+    g1.exact = g2.exact = c->resid_exact;
     if (!g1.x && ((rc = record(c, g1, [&] {
                       int r = factor_enqueue(c, o.form, wk);
                       return r ? r : refine_enqueue(c, true);

@@ -985,7 +985,7 @@ static int residual(frico_ctx* c, KernelTimer* kt)
     if (c->zprec == FRICO_C128) {
         if (c->z64_factored)
             return set_error("residual needs the unfactored Z"), FRICO_ERR_STATE;
-        return gemm_sub<double>(N, nr, N, c->Z64.p, N, c->X.p, N, c->R.p, N, c->st, kt, true);
+        return gemm_sub<double>(N, nr, N, c->Z64.p, N, c->X.p, N, c->R.p, N, c->st, kt, c->resid_exact);
     }
     return gemm_sub_mixed(N, nr, N, c->Z32.p, N, c->X.p, N, c->R.p, N, c->st, kt);
 }
@@ -1027,6 +1027,7 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
         rs.lds = N;
         rs.R = c->R.p;
         rs.ldr = N;
+        rs.exact = c->resid_exact;
     }
     phase_begin(c, tt);
     if (first && (rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)))
@@ -1113,6 +1114,7 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     }
     if (maxit < 1)
         return set_error("maxit must be >= 1"), -3;
+    c->resid_exact = !(tol >= 1e-10);
     if ((rc = refine_prepare(c)))
         return rc;
     std::vector<double> bn;
@@ -1163,6 +1165,7 @@ int frico_residual(frico_ctx* c, double* rel)
     if (!rel)
         return set_error("null output"), -2;
     int rc = set_dev(c);
+    c->resid_exact = true; // a check of the answer: always the exact form
     if (rc || (rc = residual(c, nullptr)))
         return rc;
     std::vector<double> bn, rn;

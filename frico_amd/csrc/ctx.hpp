@@ -212,6 +212,10 @@ struct frico_ctx {
     // Refinement residuals updated beside the backward substitution on the second stream
     // (FRICO_STREAM_RESIDUAL=0: one full residual product after each solve).
     bool stream_residual = true;
+    // Residual products of the current solve in the exact four-product form: taken when the
+    // tolerance is within 1e3 of the three-product form's rounding floor (~1e-13 relative, where a
+    // refinement to 1e-13 stalled), i.e. below 1e-10.
+    bool resid_exact = true;
     int64_t aug_cols = 0;
     bool aug_valid = false;
     uint64_t rhs_version = 0, aug_version = 0;

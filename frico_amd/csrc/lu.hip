@@ -1955,7 +1955,7 @@ static int lu_backward(const cx<T>* A, int64_t n, int64_t lda, cx<T>* Y, int64_t
                 int rc = launch_add_correction(rs->S + kb, rs->lds, X + kb, ldx, bw, nrhs, rs->side);
                 if (!rc)
                     rc = gemm_sub<double>(n, nrhs, bw, rs->Z + kb * rs->ldz, rs->ldz, rs->S + kb, rs->lds, rs->R, rs->ldr,
-                                          rs->side, nullptr, true);
+                                          rs->side, nullptr, rs->exact);
                 if (rc)
                     return rc;
             }

@@ -75,6 +75,7 @@ struct ResidStream {
     int64_t lds = 0;
     c128* R = nullptr;
     int64_t ldr = 0;
+    bool exact = true; // four real products per complex multiply-add (see gemm_sub)
 };
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,

@@ -215,6 +215,28 @@ def test_streamed_residual_matches_the_full_residual(monkeypatch, nb):
     assert relmax(x1, x0) <= 1e-11 and relmax(x1, np.linalg.solve(A, B)) <= 1e-10
 
 
+def test_residual_form_follows_the_tolerance():
+    """At tol >= 1e-10 the refinement's FP64 residual products take three real products per complex
+    multiply-add (rounding floor ~1e-13 relative, 1e3 below the tolerance); the residual it reports
+    agrees with frico_residual's exact four-product one, and the answer meets the tolerance in the
+    exact residual too."""
+    import frico_amd
+    n = 900
+    rng = np.random.default_rng(9)
+    A = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)) + 0.5 * np.sqrt(n) * np.eye(n)
+    B = rng.uniform(-1, 1, (n, 30)) + 1j * rng.uniform(-1, 1, (n, 30))
+    with frico_amd.Context(0) as c:
+        c.set_block(128)
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor("c64_ir")
+        info = c.solve(tol=1e-10, maxit=10)
+        exact = float(np.max(c.residual()))
+    print(f"reported {info.max_rel_residual:.3e}, exact {exact:.3e}")
+    assert info.converged and exact <= 1e-10
+    assert abs(info.max_rel_residual - exact) <= 1e-13 + 0.05 * exact
+
+
 def test_lu_needs_pivoting():
     import frico_amd
     n = 200
