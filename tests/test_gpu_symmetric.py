@@ -213,3 +213,32 @@ def test_approximate_mode_dipole_antenna_uses_the_symmetric_form(gpu_ctx):
     zo = ref["port"]["impedance"]
     assert abs(out["impedance"] - zo) / abs(zo) <= 1e-9
     assert relmax(out["gain"], ref["gain"]) <= 1e-9
+
+
+def test_approximate_mode_bistatic_sweep_uses_the_symmetric_form(gpu_ctx):
+    """frico_bistatic_sweep in approximate mode and the refinement form takes the symmetric
+    factorisation, as run_context does for do_sweep(bistatic); RCS table equal to the oracle's
+    (zsytrf) to 1e-9 of its largest entry."""
+    import frico_amd
+    lam = meshes.wavelength(F0)
+    sm = meshes.plate(0.8 * lam, 8)
+    ref = oracle.bistatic_analysis(sm, F0, 0.3, 0.2, 1.0, 1.0, degree=5, approx=True)
+    gpu_ctx.load_mesh(sm, 5)
+    tab = gpu_ctx.bistatic_sweep([F0], 0.3, 0.2, mode="approx", form="c64_ir", tol=1e-12)[0]
+    assert gpu_ctx.factored_form == frico_amd.SOLVE_C64_SYM_IR
+    assert np.max(np.abs(tab - ref["table"])) <= 1e-9 * np.max(np.abs(ref["table"]))
+
+
+def test_forced_symmetry_antenna_uses_the_symmetric_form(gpu_ctx):
+    """Quadrature mode with force_symmetry: Z = (Z + Z^T)/2 (src/emw_solver.h:118-120), then
+    solve_symmetric; the refinement form factors it as L D L^T and matches the oracle's zsytrf on the
+    symmetrised matrix to 1e-9 in Zin and gain."""
+    import frico_amd
+    sm = meshes.bowtie(0.25 * meshes.wavelength(F0), 60.0, 6, 4)
+    ref = oracle.antenna_analysis(sm, F0, degree=5, force_symmetry=True)
+    gpu_ctx.load_mesh(sm, 5)
+    out = gpu_ctx.antenna_sweep([F0], sm.feed, force_symmetry=True, form="c64_ir", tol=1e-12)[0]
+    assert gpu_ctx.factored_form == frico_amd.SOLVE_C64_SYM_IR
+    zo = ref["port"]["impedance"]
+    assert abs(out["impedance"] - zo) / abs(zo) <= 1e-9
+    assert np.max(np.abs(out["gain"] - ref["gain"])) <= 1e-9 * np.max(ref["gain"])
