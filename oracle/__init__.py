@@ -2,6 +2,16 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() as the checker and by
 bench.py's cpu_baseline leg.  The product package (frico_amd) never imports it.
+
+How each function here is pinned (tests/test_oracle_*.py):
+  OracleMesh (fill, rhs, scatter, fields, port values, gain, bistatic): see cpu_ref.cpp's header;
+  lu_solve: scipy's zgetrf/zgetrs on random matrices, normwise backward error, the izamax pivot
+      rule and the first-zero-pivot info;
+  sy_solve: scipy's LAPACK zsytrf/zsytrs themselves (the routines solve_symmetric calls), held to a
+      general solve on complex symmetric matrices, on a zero-diagonal matrix that forces 2 x 2
+      pivots, and on the approximate-mode EFIE matrix;
+  antenna_analysis / bistatic_analysis: the reference's dipole impedance table and physics pins.
+  The vendor LAPACK's own rounding, which the reference inherits, stays unpinned (absent here).
 """
 from __future__ import annotations
 
