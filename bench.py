@@ -229,6 +229,9 @@ def main():
     ap.add_argument("--nb", type=int, default=128)
     ap.add_argument("--n", type=int, default=64, help="plate cells per side (64 -> N=12160)")
     ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--form", default="c64_ir", choices=["c64_ir", "c64_np_ir"],
+                    help="complex64 factorisation of the headline form: pivoted LU, or LU without "
+                         "pivoting (falls back to the pivoted LU if the refinement stalls)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", choices=["plate", "batch", "approx", "nearfield"], default="plate",
@@ -266,11 +269,13 @@ def main():
 
     # ---- headline form: complex64 LU + FP64 refinement -------------------------------------
     for _ in range(args.warmup):
-        run_step(ctx, "c64_ir", th, args.tol, 10)
-    print("# warmup done (c64_ir)", file=sys.stderr, flush=True)
-    t_ir, infos_ir, prof_ir = timed(ctx, "c64_ir", th, args.steps, args.tol, 10)
+        run_step(ctx, args.form, th, args.tol, 10)
+    print(f"# warmup done ({args.form})", file=sys.stderr, flush=True)
+    t_ir, infos_ir, prof_ir = timed(ctx, args.form, th, args.steps, args.tol, 10)
+    form_used = ctx.factored_form
     X_ir = ctx.solution()
-    print(f"# c64_ir {t_ir * 1e3:.2f} ms/step {infos_ir[-1]}", file=sys.stderr, flush=True)
+    print(f"# {args.form} {t_ir * 1e3:.2f} ms/step {infos_ir[-1]} (factored form {form_used})", file=sys.stderr,
+          flush=True)
 
     # ---- plain complex128 form ---------------------------------------------------------------
     fp64 = None

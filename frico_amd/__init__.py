@@ -14,11 +14,12 @@ from dataclasses import dataclass
 import numpy as np
 
 from . import _lib, meshes
-from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_SYM_IR, SOLVE_C128)
+from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_NP_IR, SOLVE_C64_SYM_IR,
+                   SOLVE_C128)
 
 __all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR",
-           "SOLVE_C64_SYM_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh",
+           "SOLVE_C64_SYM_IR", "SOLVE_C64_NP_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh",
            "read_msh", "format_double", "write_port_sweep", "write_radiation_diagram", "write_bistatic_rcs",
            "write_antenna_text_outputs"]
 
@@ -181,7 +182,8 @@ def _modes(mode, prec=None):
 
 
 def _form(form):
-    return {"c128": SOLVE_C128, "c64_ir": SOLVE_C64_IR, "c64_sym_ir": SOLVE_C64_SYM_IR}.get(form, form)
+    return {"c128": SOLVE_C128, "c64_ir": SOLVE_C64_IR, "c64_sym_ir": SOLVE_C64_SYM_IR,
+            "c64_np_ir": SOLVE_C64_NP_IR}.get(form, form)
 
 
 class Context:

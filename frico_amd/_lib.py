@@ -20,7 +20,7 @@ FRICO_ERR_SYNC = -1005
 FRICO_ERR_LIMIT = -1006
 FILL_QUAD, FILL_APPROX = 0, 1
 C128, C64 = 0, 1
-SOLVE_C128, SOLVE_C64_IR, SOLVE_C64_SYM_IR = 0, 1, 2
+SOLVE_C128, SOLVE_C64_IR, SOLVE_C64_SYM_IR, SOLVE_C64_NP_IR = 0, 1, 2, 3
 BATCH_GRAPHS, BATCH_AUTOTUNE = 1, 2
 
 

@@ -859,7 +859,8 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
 {
     if (c->zprec < 0)
         return set_error("no matrix to factor"), FRICO_ERR_STATE;
-    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR && form != FRICO_SOLVE_C64_SYM_IR)
+    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR && form != FRICO_SOLVE_C64_SYM_IR &&
+        form != FRICO_SOLVE_C64_NP_IR)
         return set_error("unknown factorisation form %d", form), -2;
     if (form == FRICO_SOLVE_C128 && (c->zprec != FRICO_C128 || c->z64_factored))
         return set_error("complex128 factorisation needs an unfactored complex128 Z"), FRICO_ERR_STATE;
@@ -877,7 +878,13 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
         return rc;
     c->factored = -1; // the factors are about to be overwritten
     c->nb_factored = block_width(c);
+    if (form == FRICO_SOLVE_C64_NP_IR && (rc = c->tbuf.ensure((size_t)(LU_NB_MAX * N))))
+        return rc;
     wk = lu_work(c);
+    if (form == FRICO_SOLVE_C64_NP_IR) {
+        wk.tbuf = c->tbuf.p;
+        wk.ntbuf = (int64_t)c->tbuf.n;
+    }
     if (!lookahead) {
         wk.side = nullptr;
         wk.events = nullptr;
@@ -916,6 +923,8 @@ int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
         KernelTimer* kt = c->profiling ? &c->t_gemm : nullptr;
         if (!rc && form == FRICO_SOLVE_C64_SYM_IR)
             rc = ldlt_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
+        else if (!rc && form == FRICO_SOLVE_C64_NP_IR)
+            rc = lu_nopiv_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
         else if (!rc)
             rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
     }
@@ -962,8 +971,8 @@ int frico_factor(frico_ctx* c, int form)
     if (rc || (rc = factor_enqueue(c, form, wk)))
         return rc;
     rc = factor_finish(c, form, wk);
-    // a zero pivot of the unpivoted symmetric factorisation: the pivoted LU instead
-    if (rc > 0 && form == FRICO_SOLVE_C64_SYM_IR)
+    // a zero pivot of an unpivoted factorisation: the pivoted LU instead
+    if (rc > 0 && (form == FRICO_SOLVE_C64_SYM_IR || form == FRICO_SOLVE_C64_NP_IR))
         return frico_factor(c, FRICO_SOLVE_C64_IR);
     return rc;
 }
@@ -1123,9 +1132,9 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
     inf.converged = 0;
     int sweeps = 0;
     for (;;) {
-        // the unpivoted symmetric factors are left for the pivoted LU when a sweep fails to halve
-        // the residual before the tolerance is met, or when maxit sweeps do not meet it
-        const bool sym = c->factored == FRICO_SOLVE_C64_SYM_IR;
+        // unpivoted factors are left for the pivoted LU when a sweep fails to halve the residual
+        // before the tolerance is met, or when maxit sweeps do not meet it
+        const bool sym = c->factored == FRICO_SOLVE_C64_SYM_IR || c->factored == FRICO_SOLVE_C64_NP_IR;
         double prev = HUGE_VAL;
         for (int it = 1; it <= maxit; it++) {
             double worst = 0;

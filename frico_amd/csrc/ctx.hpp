@@ -204,6 +204,7 @@ struct frico_ctx {
     DevBuf<int> perm;             // row-interchange lists of the last factorisation
     DevBuf<unsigned char> wv;     // W products of the last factorisation (fused solves)
     DevBuf<unsigned char> ysolve; // n x nrhs scratch of the fused solves
+    DevBuf<c64> tbuf;             // transposed block column of the unpivoted LU
     bool wv_ok = false;           // wv holds the products of the current factors
     bool fused_solve = true;
     // Right-hand sides carried through the complex64 factorisation (FRICO_LU_AUGMENT=0 disables):

@@ -1924,6 +1924,296 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
     return solve_tables<T>(wk, A, n, lda, nb, st);
 }
 
+// ----------------------------------------------------------------------------------------------
+// LU without pivoting, for the refinement form: the refinement against the kept FP64 Z corrects
+// what the complex64 factors lose, and the caller returns to the pivoted LU when a pivot is zero
+// or the refinement stalls.  Without interchanges a block column needs no grid-wide exchange per
+// column: one workgroup factors the diagonal block and the rows below are triangular solves.
+// Block column [k, k + w):
+//   lu_diag           L11 U11 of the w x w diagonal block, in registers (one workgroup)
+//   col_transpose     T = A21^T into a workspace
+//   trsm (U11^T)      T = U11^{-T} T, i.e. L21 = A21 U11^{-1}
+//   col_transpose     A21 = T^T
+// and the trailing updates are the LU's (U12 = L11^{-1} A12, A22 -= L21 U12) with no interchanges.
+// ----------------------------------------------------------------------------------------------
+
+// Right-looking LU of the w x w (w <= 128) diagonal block at (k, k), held in registers: thread
+// (row r, group g) keeps the entries (r, g + 8q) of its row.  Step c reads pivot column c and pivot
+// row c from LDS, updates a(r, j) -= l_r u_cj (j > c), scales column c, and the owners of column
+// c + 1 and row c + 1 publish them (two buffers each, one barrier a step).  Rows r <= c take l = 0.
+// A zero pivot leaves its column of L zero and sets *info (1-based column) if unset.
+template <typename T>
+__global__ __launch_bounds__(LDLT_THREADS) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info)
+{
+    __shared__ cx<T> col[2][LU_NB_MAX];
+    __shared__ cx<T> rowb[2][LU_NB_MAX];
+    __shared__ cx<T> pinv[2];
+    const int tid = threadIdx.x, r = tid % LU_NB_MAX, g = tid / LU_NB_MAX;
+    const bool row_ok = r < w;
+    cx<T> a[LDLT_OWN];
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++) {
+        const int j = g + LDLT_GROUPS * q;
+        const bool own = row_ok && j < w;
+        a[q] = A[(own ? k + r : k) + (own ? k + j : k) * lda];
+    }
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++)
+        if (!(row_ok && g + LDLT_GROUPS * q < w))
+            a[q] = cx<T>{0, 0};
+    auto publish_inverse = [&](cx<T> d, int slot) {
+        const bool zero = d.re == (T)0 && d.im == (T)0;
+        pinv[slot] = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
+        if (zero && *info == 0)
+            *info = (int)(k + r + 1);
+    };
+    // column 0 and row 0
+    if (g == 0 && row_ok)
+        col[0][r] = a[0];
+    if (r == 0) {
+#pragma unroll
+        for (int q = 0; q < LDLT_OWN; q++)
+            if (g + LDLT_GROUPS * q < w)
+                rowb[0][g + LDLT_GROUPS * q] = a[q];
+    }
+    if (tid == 0)
+        publish_inverse(a[0], 0);
+    __syncthreads();
+    const int rmax = r | 63; // last row of this wave
+    for (int c = 0; c < w; c++) {
+        const cx<T>* cb = col[c & 1];
+        const cx<T>* rb = rowb[c & 1];
+        if (rmax > c && rmax - 63 < w) {
+            cx<T> u[LDLT_OWN];
+#pragma unroll
+            for (int q = 0; q < LDLT_OWN; q++)
+                u[q] = rb[g + LDLT_GROUPS * q];
+            const cx<T> vr = cb[r];
+            const cx<T> inv = pinv[c & 1];
+            const bool below = row_ok && r > c;
+            const cx<T> l = below ? vr * inv : cx<T>{0, 0};
+            // columns c < j < w (a wave-uniform test); rows at or above c have l = 0
+#pragma unroll
+            for (int q = 0; q < LDLT_OWN; q++) {
+                const int j = g + LDLT_GROUPS * q;
+                if (j > c && j < w) {
+                    a[q].re -= l.re * u[q].re - l.im * u[q].im;
+                    a[q].im -= l.re * u[q].im + l.im * u[q].re;
+                }
+            }
+            const int qc = c / LDLT_GROUPS, gc = c % LDLT_GROUPS;
+            const int qn = (c + 1) / LDLT_GROUPS, gn = (c + 1) % LDLT_GROUPS;
+            if (g == gc) {
+#pragma unroll
+                for (int q = 0; q < LDLT_OWN; q++)
+                    if (q == qc)
+                        a[q] = below ? l : a[q];
+            }
+            if (c + 1 < w) {
+                if (g == gn) {
+                    cx<T> nx = a[0];
+#pragma unroll
+                    for (int q = 1; q < LDLT_OWN; q++)
+                        nx = q == qn ? a[q] : nx;
+                    if (below)
+                        col[(c + 1) & 1][r] = nx;
+                    if (r == c + 1)
+                        publish_inverse(nx, (c + 1) & 1);
+                }
+                if (r == c + 1) {
+#pragma unroll
+                    for (int q = 0; q < LDLT_OWN; q++) {
+                        const int j = g + LDLT_GROUPS * q;
+                        if (j > c && j < w)
+                            rowb[(c + 1) & 1][j] = a[q];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < LDLT_OWN; q++) {
+        const int j = g + LDLT_GROUPS * q;
+        if (row_ok && j < w)
+            A[(k + r) + (k + j) * lda] = a[q]; // L below the diagonal, U on and above it
+    }
+}
+
+// T (w x m, leading dimension w) <-> the block column A21 = A(k + w + c, k + i), 32 values of c a
+// workgroup through LDS, all loads of a thread in flight before its stores:
+//   BACK false: T(i, c) = A21(c, i);   BACK true: A21(c, i) = T(i, c).
+template <typename T, bool BACK>
+__global__ __launch_bounds__(256) void col_transpose(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int64_t m,
+                                                     cx<T>* __restrict__ Tb)
+{
+    constexpr int TC = kSymCols<T>;
+    constexpr int NE = LU_NB_MAX * TC / 256;
+    __shared__ cx<T> S[LU_NB_MAX][TC + 1];
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * TC;
+    const int nc = (int)((m - c0) < TC ? (m - c0) : TC);
+    cx<T>* A21 = A + (k + w + c0) + k * lda; // entry (c, i) at A21[c + i lda]
+    cx<T>* Tc = Tb + c0 * w;                 // entry (i, c) at Tc[i + c w]
+    cx<T> v[NE];
+    if (!BACK) {
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, c = e % TC, i = e / TC;
+            v[s] = A21[(i < w && c < nc) ? c + (int64_t)i * lda : 0];
+        }
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, c = e % TC, i = e / TC;
+            S[i][c] = v[s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, i = e % LU_NB_MAX, c = e / LU_NB_MAX;
+            if (i < w && c < nc)
+                Tc[i + (int64_t)c * w] = S[i][c];
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, i = e % LU_NB_MAX, c = e / LU_NB_MAX;
+            v[s] = Tc[(i < w && c < nc) ? i + (int64_t)c * w : 0];
+        }
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, i = e % LU_NB_MAX, c = e / LU_NB_MAX;
+            S[i][c] = v[s];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < NE; s++) {
+            const int e = tid + 256 * s, c = e % TC, i = e / TC;
+            if (i < w && c < nc)
+                A21[c + (int64_t)i * lda] = S[i][c];
+        }
+    }
+}
+
+// C[M x N] -= A^T B: the transposed-operand product (A^T(r, kk) = A[kk + r lda]).
+template <typename T>
+static int gemm_sub_ta(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb,
+                       cx<T>* C, int64_t ldc, hipStream_t st)
+{
+    if (M <= 0 || N <= 0 || K <= 0)
+        return 0;
+    using TL = typename TileBig<T>::type;
+    const int64_t gx = (M + TL::BM - 1) / TL::BM, gy = (N + TL::BN - 1) / TL::BN;
+    if (gy > 65535)
+        return set_error("gemm: too many column tiles"), -1;
+    launch_kernel(&cgemm_sub<T, T, true>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B, ldb, C,
+                  ldc);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
+                    KernelTimer* gemm_timer, int64_t naug)
+{
+    if (nb > LU_NB_MAX || nb < 1)
+        return set_error("lu (no pivoting): block size out of range"), -1;
+    const int64_t npan = (n + nb - 1) / nb;
+    if (!wk.perm || wk.nperm < npan)
+        return set_error("lu (no pivoting): interchange-list workspace too small"), -1;
+    if (!wk.tbuf || wk.ntbuf < (int64_t)nb * n)
+        return set_error("lu (no pivoting): transpose workspace too small"), -1;
+    cx<T>* Tb = (cx<T>*)wk.tbuf;
+    const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
+    const int64_t nz = npan * PERM_STRIDE;
+    launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.perm, nz);
+    launch_kernel(&zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
+    launch_kernel(&iota_pivots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ipiv, n);
+    FRICO_HIP_CHECK(hipGetLastError());
+    // diagonal block and block column of step k
+    auto column = [&](int64_t k, int w) -> int {
+        const int64_t m = n - k - w;
+        launch_kernel(&lu_diag<T>, dim3(1), dim3(LDLT_THREADS), 0, st, A, lda, k, w, wk.flags + 1);
+        if (m > 0) {
+            const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
+            launch_kernel(&col_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
+            for (int s = 0; s < w; s += 64) {
+                const int ws = (w - s) < 64 ? (w - s) : 64;
+                launch_trsm<T, false, true, false>(ws, A + (k + s) + (k + s) * lda, lda, Tb + s, w, Tb + s, w, m, st);
+                if (s + ws < w) {
+                    int rc = gemm_sub_ta<T>(w - s - ws, m, ws, A + (k + s) + (k + s + ws) * lda, lda, Tb + s, w,
+                                            Tb + s + ws, w, st);
+                    if (rc)
+                        return rc;
+                }
+            }
+            launch_kernel(&col_transpose<T, true>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
+        }
+        FRICO_HIP_CHECK(hipGetLastError());
+        return 0;
+    };
+    // update of columns [c0, c0 + nc) by step k: U12 = L11^{-1} A12, A22 -= L21 U12
+    auto update = [&](int64_t k, int w, int64_t c0, int64_t nc, hipStream_t s) -> int {
+        if (nc <= 0)
+            return 0;
+        int rc = row_block_solve<T>(A, lda, k, w, c0, nc, s);
+        return rc ? rc
+                  : gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda,
+                                A + (k + w) + c0 * lda, lda, s, gemm_timer);
+    };
+    auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
+    int rc;
+    if (!lookahead) {
+        for (int64_t p = 0; p < npan; p++) {
+            const int64_t k = p * nb;
+            const int w = width(k);
+            if ((rc = column(k, w)) || (rc = update(k, w, k + w, n + naug - k - w, st)))
+                return rc;
+        }
+    } else {
+        hipEvent_t* evP = wk.events;
+        hipEvent_t* evW = wk.events + npan;
+        hipEvent_t ev0 = wk.events[2 * npan];
+        FRICO_HIP_CHECK(hipEventRecord(ev0, st));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, ev0, 0));
+        if ((rc = column(0, width(0))))
+            return rc;
+        bool prev_wide = false;
+        for (int64_t p = 0; p < npan; p++) {
+            const int64_t k = p * nb;
+            const int w = width(k);
+            const int64_t nright = n - k - w;
+            if (nright <= 0) {
+                // the last block's update of the appended columns
+                if (naug > 0) {
+                    if (prev_wide)
+                        FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
+                    if ((rc = update(k, w, n, naug, st)))
+                        return rc;
+                }
+                break;
+            }
+            const int64_t nnext = nright < nb ? nright : nb;
+            const int64_t nrest = nright + naug - nnext; // the appended columns go with the wide part
+            FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
+            if (nrest > 0) {
+                FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
+                if ((rc = update(k, w, k + w + nnext, nrest, wk.side)))
+                    return rc;
+                FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
+            }
+            if (prev_wide)
+                FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
+            if ((rc = update(k, w, k + w, nnext, st)) || (rc = column(k + w, width(k + w))))
+                return rc;
+            prev_wide = nrest > 0;
+        }
+        FRICO_HIP_CHECK(hipEventRecord(ev0, wk.side));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(st, ev0, 0));
+    }
+    return solve_tables<T>(wk, A, n, lda, nb, st);
+}
+
 constexpr int RESID_ROWS = 512; // rows of the solution per residual product beside the substitution
 
 // Backward half of the solve: per 64-row chunk j from the last, x_j = U_jj^{-1} y_j by substitution
@@ -2150,6 +2440,8 @@ template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipS
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int ldlt_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int ldlt_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
+template int lu_nopiv_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
+template int lu_nopiv_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t,
                                     const ResidStream*);
 template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t, const ResidStream*);

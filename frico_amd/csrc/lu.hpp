@@ -27,6 +27,9 @@ struct LuWork {
     hipStream_t side = nullptr;    // second stream (default priority) for the look-ahead, or null
     hipEvent_t* events = nullptr;  // >= 2 * npanels + 1 events (timing disabled)
     int nevents = 0;
+    // lu_nopiv_factor: nb x n complex values (the transposed block column), or null
+    void* tbuf = nullptr;
+    int64_t ntbuf = 0;
 };
 
 // C -= A B.  exact: four real products per complex multiply-add even where three would be used
@@ -50,6 +53,13 @@ int lu_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, h
 template <typename T>
 int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
                 KernelTimer* gemm_timer, int64_t naug = 0);
+// LU without pivoting (for the refinement form, which corrects its rounding): the same layout as
+// lu_factor's factors with no interchanges (empty lists in LuWork::perm, ipiv the identity), so the
+// solves take them as they are.  Needs LuWork::tbuf.  naug as for lu_factor.  A zero pivot is
+// reported like zgetrf's info.
+template <typename T>
+int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
+                    KernelTimer* gemm_timer, int64_t naug = 0);
 int launch_zero_bytes(void* p, size_t bytes, hipStream_t st);
 int launch_copy_bytes(const void* src, void* dst, size_t bytes, hipStream_t st);
 // ints per panel of LuWork::perm

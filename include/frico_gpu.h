@@ -46,10 +46,16 @@ extern "C" {
  *                         complex64 L D L^T of the lower triangle of a copy of Z without pivoting,
  *                         refined as FRICO_SOLVE_C64_IR.  A zero pivot, or a refinement that stops
  *                         converging, falls back to FRICO_SOLVE_C64_IR by itself
- *                         (frico_factored_form tells which factorisation the answer came from). */
+ *                         (frico_factored_form tells which factorisation the answer came from).
+ *   FRICO_SOLVE_C64_NP_IR complex64 LU of a copy of Z without pivoting, refined as
+ *                         FRICO_SOLVE_C64_IR; no grid-wide pivot search per column, so the
+ *                         factorisation is bound by its trailing products.  Falls back to
+ *                         FRICO_SOLVE_C64_IR like the symmetric form.  For matrices whose
+ *                         unpivoted factors stay bounded, as EFIE matrices' do. */
 #define FRICO_SOLVE_C128 0
 #define FRICO_SOLVE_C64_IR 1
 #define FRICO_SOLVE_C64_SYM_IR 2
+#define FRICO_SOLVE_C64_NP_IR 3
 
 typedef struct frico_ctx frico_ctx;
 typedef struct frico_msh frico_msh;
