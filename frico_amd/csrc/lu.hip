@@ -1937,107 +1937,133 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
 // and the trailing updates are the LU's (U12 = L11^{-1} A12, A22 -= L21 U12) with no interchanges.
 // ----------------------------------------------------------------------------------------------
 
-// Right-looking LU of the w x w (w <= 128) diagonal block at (k, k), held in registers: thread
-// (row r, group g) keeps the entries (r, g + 8q) of its row.  Step c reads pivot column c and pivot
-// row c from LDS, updates a(r, j) -= l_r u_cj (j > c), scales column c, and the owners of column
-// c + 1 and row c + 1 publish them (two buffers each, one barrier a step).  Rows r <= c take l = 0.
-// A zero pivot leaves its column of L zero and sets *info (1-based column) if unset.
+// Right-looking LU of the w x w (w <= 128) diagonal block at (k, k), held in registers as 4 x 4
+// tiles: thread t keeps rows 4 (t / 32) .. + 3 and columns 4 (t % 32) .. + 3, so a wavefront holds
+// 8 whole rows and drops out once the factorisation has passed them.  Step c reads the 4
+// multipliers' column entries and the 4 pivot-row entries it needs from LDS (masked to zero
+// outside rows > c and columns > c, which makes the update exact elsewhere), updates its 16
+// entries, scales column c, and the owners of column c + 1 and row c + 1 publish them (two buffers
+// each, one barrier a step).  A zero pivot leaves its column of L zero and sets *info (1-based
+// column) if unset.
+constexpr int LUD_TILE = 4;
 template <typename T>
-__global__ __launch_bounds__(LDLT_THREADS) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info)
+__global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info)
 {
-    __shared__ cx<T> col[2][LU_NB_MAX];
-    __shared__ cx<T> rowb[2][LU_NB_MAX];
+    __shared__ __align__(16) cx<T> col[2][LU_NB_MAX];
+    __shared__ __align__(16) cx<T> rowb[2][LU_NB_MAX];
     __shared__ cx<T> pinv[2];
-    const int tid = threadIdx.x, r = tid % LU_NB_MAX, g = tid / LU_NB_MAX;
-    const bool row_ok = r < w;
-    cx<T> a[LDLT_OWN];
+    const int tid = threadIdx.x, bi = tid >> 5, bj = tid & 31;
+    const int r0 = LUD_TILE * bi, j0 = LUD_TILE * bj;
+    cx<T> a[LUD_TILE][LUD_TILE];
 #pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++) {
-        const int j = g + LDLT_GROUPS * q;
-        const bool own = row_ok && j < w;
-        a[q] = A[(own ? k + r : k) + (own ? k + j : k) * lda];
-    }
+    for (int i = 0; i < LUD_TILE; i++)
 #pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++)
-        if (!(row_ok && g + LDLT_GROUPS * q < w))
-            a[q] = cx<T>{0, 0};
-    auto publish_inverse = [&](cx<T> d, int slot) {
+        for (int jj = 0; jj < LUD_TILE; jj++) {
+            const bool own = r0 + i < w && j0 + jj < w;
+            a[i][jj] = A[(own ? k + r0 + i : k) + (own ? k + j0 + jj : k) * lda];
+        }
+#pragma unroll
+    for (int i = 0; i < LUD_TILE; i++)
+#pragma unroll
+        for (int jj = 0; jj < LUD_TILE; jj++)
+            if (!(r0 + i < w && j0 + jj < w))
+                a[i][jj] = cx<T>{0, 0};
+    auto publish_inverse = [&](cx<T> d, int slot, int cglob) {
         const bool zero = d.re == (T)0 && d.im == (T)0;
         pinv[slot] = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
         if (zero && *info == 0)
-            *info = (int)(k + r + 1);
+            *info = (int)(k + cglob + 1);
     };
-    // column 0 and row 0
-    if (g == 0 && row_ok)
-        col[0][r] = a[0];
-    if (r == 0) {
+    // column 0, row 0 and the first pivot
+    if (bj == 0)
 #pragma unroll
-        for (int q = 0; q < LDLT_OWN; q++)
-            if (g + LDLT_GROUPS * q < w)
-                rowb[0][g + LDLT_GROUPS * q] = a[q];
-    }
+        for (int i = 0; i < LUD_TILE; i++)
+            if (r0 + i < w)
+                col[0][r0 + i] = a[i][0];
+    if (bi == 0)
+#pragma unroll
+        for (int jj = 0; jj < LUD_TILE; jj++)
+            if (j0 + jj < w)
+                rowb[0][j0 + jj] = a[0][jj];
     if (tid == 0)
-        publish_inverse(a[0], 0);
+        publish_inverse(a[0][0], 0, 0);
     __syncthreads();
-    const int rmax = r | 63; // last row of this wave
+    const int wr0 = 8 * (tid >> 6); // first row of this wavefront
     for (int c = 0; c < w; c++) {
-        const cx<T>* cb = col[c & 1];
-        const cx<T>* rb = rowb[c & 1];
-        if (rmax > c && rmax - 63 < w) {
-            cx<T> u[LDLT_OWN];
-#pragma unroll
-            for (int q = 0; q < LDLT_OWN; q++)
-                u[q] = rb[g + LDLT_GROUPS * q];
-            const cx<T> vr = cb[r];
+        if (wr0 + 7 > c && wr0 < w) {
+            const cx<T>* cb = col[c & 1];
+            const cx<T>* rb = rowb[c & 1];
             const cx<T> inv = pinv[c & 1];
-            const bool below = row_ok && r > c;
-            const cx<T> l = below ? vr * inv : cx<T>{0, 0};
-            // columns c < j < w (a wave-uniform test); rows at or above c have l = 0
+            cx<T> l[LUD_TILE], u[LUD_TILE];
 #pragma unroll
-            for (int q = 0; q < LDLT_OWN; q++) {
-                const int j = g + LDLT_GROUPS * q;
-                if (j > c && j < w) {
-                    a[q].re -= l.re * u[q].re - l.im * u[q].im;
-                    a[q].im -= l.re * u[q].im + l.im * u[q].re;
-                }
+            for (int i = 0; i < LUD_TILE; i++) {
+                const int r = r0 + i;
+                const cx<T> v = cb[r];
+                l[i] = (r > c && r < w) ? v * inv : cx<T>{0, 0};
             }
-            const int qc = c / LDLT_GROUPS, gc = c % LDLT_GROUPS;
-            const int qn = (c + 1) / LDLT_GROUPS, gn = (c + 1) % LDLT_GROUPS;
-            if (g == gc) {
 #pragma unroll
-                for (int q = 0; q < LDLT_OWN; q++)
-                    if (q == qc)
-                        a[q] = below ? l : a[q];
+            for (int jj = 0; jj < LUD_TILE; jj++) {
+                const int j = j0 + jj;
+                const cx<T> v = rb[j];
+                u[jj] = (j > c && j < w) ? v : cx<T>{0, 0};
+            }
+#pragma unroll
+            for (int i = 0; i < LUD_TILE; i++)
+#pragma unroll
+                for (int jj = 0; jj < LUD_TILE; jj++) {
+                    a[i][jj].re -= l[i].re * u[jj].re - l[i].im * u[jj].im;
+                    a[i][jj].im -= l[i].re * u[jj].im + l[i].im * u[jj].re;
+                }
+            // column c of L
+            if (bj == (c >> 2)) {
+                const int jc = c & 3;
+#pragma unroll
+                for (int jj = 0; jj < LUD_TILE; jj++)
+                    if (jj == jc)
+#pragma unroll
+                        for (int i = 0; i < LUD_TILE; i++)
+                            if (r0 + i > c && r0 + i < w)
+                                a[i][jj] = l[i];
             }
             if (c + 1 < w) {
-                if (g == gn) {
-                    cx<T> nx = a[0];
+                const int cn = c + 1, tn = cn >> 2, sn = cn & 3;
+                // column c + 1 (rows > c) and row c + 1 (columns > c), updated above
+                if (bj == tn) {
 #pragma unroll
-                    for (int q = 1; q < LDLT_OWN; q++)
-                        nx = q == qn ? a[q] : nx;
-                    if (below)
-                        col[(c + 1) & 1][r] = nx;
-                    if (r == c + 1)
-                        publish_inverse(nx, (c + 1) & 1);
+                    for (int jj = 0; jj < LUD_TILE; jj++)
+                        if (jj == sn)
+#pragma unroll
+                            for (int i = 0; i < LUD_TILE; i++)
+                                if (r0 + i > c && r0 + i < w)
+                                    col[cn & 1][r0 + i] = a[i][jj];
                 }
-                if (r == c + 1) {
+                if (bi == tn) {
 #pragma unroll
-                    for (int q = 0; q < LDLT_OWN; q++) {
-                        const int j = g + LDLT_GROUPS * q;
-                        if (j > c && j < w)
-                            rowb[(c + 1) & 1][j] = a[q];
-                    }
+                    for (int i = 0; i < LUD_TILE; i++)
+                        if (i == sn)
+#pragma unroll
+                            for (int jj = 0; jj < LUD_TILE; jj++)
+                                if (j0 + jj > c && j0 + jj < w)
+                                    rowb[cn & 1][j0 + jj] = a[i][jj];
+                    if (bj == tn)
+#pragma unroll
+                        for (int i = 0; i < LUD_TILE; i++)
+                            if (i == sn)
+#pragma unroll
+                                for (int jj = 0; jj < LUD_TILE; jj++)
+                                    if (jj == sn)
+                                        publish_inverse(a[i][jj], cn & 1, cn);
                 }
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++) {
-        const int j = g + LDLT_GROUPS * q;
-        if (row_ok && j < w)
-            A[(k + r) + (k + j) * lda] = a[q]; // L below the diagonal, U on and above it
-    }
+    for (int i = 0; i < LUD_TILE; i++)
+#pragma unroll
+        for (int jj = 0; jj < LUD_TILE; jj++)
+            if (r0 + i < w && j0 + jj < w)
+                A[(k + r0 + i) + (k + j0 + jj) * lda] = a[i][jj]; // L below the diagonal, U on and above
 }
 
 // T (w x m, leading dimension w) <-> the block column A21 = A(k + w + c, k + i), 32 values of c a
@@ -2133,7 +2159,7 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
     // diagonal block and block column of step k
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
-        launch_kernel(&lu_diag<T>, dim3(1), dim3(LDLT_THREADS), 0, st, A, lda, k, w, wk.flags + 1);
+        launch_kernel(&lu_diag<T>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
         if (m > 0) {
             const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
             launch_kernel(&col_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
