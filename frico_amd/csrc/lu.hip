@@ -1938,8 +1938,7 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
 // ----------------------------------------------------------------------------------------------
 
 // Right-looking LU of the w x w (w <= 128) diagonal block at (k, k), held in registers as 4 x 4
-// tiles: thread t keeps rows 4 (t / 32) .. + 3 and columns 4 (t % 32) .. + 3, so a wavefront holds
-// 8 whole rows and drops out once the factorisation has passed them.  Step c reads the 4
+// tiles, one a thread (wavefronts of 16 rows by 64 columns, see below).  Step c reads the 4
 // multipliers' column entries and the 4 pivot-row entries it needs from LDS (masked to zero
 // outside rows > c and columns > c, which makes the update exact elsewhere), updates its 16
 // entries, scales column c, and the owners of column c + 1 and row c + 1 publish them (two buffers
@@ -1952,8 +1951,12 @@ __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t l
     __shared__ __align__(16) cx<T> col[2][LU_NB_MAX];
     __shared__ __align__(16) cx<T> rowb[2][LU_NB_MAX];
     __shared__ cx<T> pinv[2];
-    const int tid = threadIdx.x, bi = tid >> 5, bj = tid & 31;
+    // wavefront v holds tile rows 4 (v / 2) .. + 3 and tile columns 16 (v % 2) .. + 15: 16 rows by
+    // 64 columns, so it drops out once the factorisation has passed either its rows or its columns
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int bi = 4 * (wv >> 1) + (lane >> 4), bj = 16 * (wv & 1) + (lane & 15);
     const int r0 = LUD_TILE * bi, j0 = LUD_TILE * bj;
+    const int wr0 = 16 * (wv >> 1), wc0 = 64 * (wv & 1); // first row and column of this wavefront
     cx<T> a[LUD_TILE][LUD_TILE];
 #pragma unroll
     for (int i = 0; i < LUD_TILE; i++)
@@ -1988,9 +1991,9 @@ __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t l
     if (tid == 0)
         publish_inverse(a[0][0], 0, 0);
     __syncthreads();
-    const int wr0 = 8 * (tid >> 6); // first row of this wavefront
     for (int c = 0; c < w; c++) {
-        if (wr0 + 7 > c && wr0 < w) {
+        // the pivot column's owners (columns around c) stay active to scale and publish it
+        if (wr0 + 15 > c && wr0 < w && wc0 + 63 >= c && wc0 < w) {
             const cx<T>* cb = col[c & 1];
             const cx<T>* rb = rowb[c & 1];
             const cx<T> inv = pinv[c & 1];
