@@ -6,8 +6,9 @@ Workload: PEC plate 3 x 3 wavelengths at 300 MHz, 64 x 64 squares split into rig
 (theta = 0..90 deg in 0.25 deg steps, phi = 0, E_theta = 1) solved as one block.
 
 A step is one pass of the path: impedance fill (complex128, quadrature mode), plane-wave RHS
-block, LU, triangular solves, and -- in the headline form -- complex64 LU with FP64-residual
-iterative refinement to max_c ||b_c - Z x_c|| / ||b_c|| <= 1e-10.  The plain complex128 form is
+block, LU, triangular solves, and -- in the headline form -- complex64 LU (without pivoting; the
+pivoted LU takes over if a pivot is zero or the refinement stalls) with FP64-residual iterative
+refinement to max_c ||b_c - Z x_c|| / ||b_c|| <= 1e-10.  The plain complex128 form is
 timed the same way.  value = F / t_step with F = 8N^3/3 + 8N^2 nrhs (LAPACK flop convention).
 
 Prints one JSON line.  Multi-GPU runs are not part of this build: --gpus > 1 exits.
@@ -34,9 +35,10 @@ PEAK_FP64_MFMA = 78.6   # TFLOP/s, MI355X dense FP64 matrix
 PEAK_HBM = 8000.0       # GB/s
 # HBM bytes per timed trailing-update launch of cgemm_sub<float> (K = 128) at N = 12160, nb = 128
 # (average over the 188 launches of one factorisation), from rocprofv3 FETCH_SIZE / WRITE_SIZE
-# runs scaled by a copy kernel of known size (profiles/plate_N12160_c64ir_hbm_traffic.md);
+# runs scaled by a copy kernel of known size (profiles/plate_N12160_c64np_hbm_traffic.md for the
+# default form, 699.9e6; plate_N12160_c64ir_hbm_traffic.md for the pivoted one, 700.1e6);
 # algorithmic bytes per launch: 419.3e6.
-GEMM_F32_TRAFFIC = 700.1e6
+GEMM_F32_TRAFFIC = {"c64_np_ir": 699.9e6, "c64_ir": 700.1e6}
 
 
 def lu_flops(n, nrhs):
@@ -229,9 +231,10 @@ def main():
     ap.add_argument("--nb", type=int, default=128)
     ap.add_argument("--n", type=int, default=64, help="plate cells per side (64 -> N=12160)")
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--form", default="c64_ir", choices=["c64_ir", "c64_np_ir"],
-                    help="complex64 factorisation of the headline form: pivoted LU, or LU without "
-                         "pivoting (falls back to the pivoted LU if the refinement stalls)")
+    ap.add_argument("--form", default="c64_np_ir", choices=["c64_ir", "c64_np_ir"],
+                    help="complex64 factorisation of the headline form: LU without pivoting (the "
+                         "default; falls back to the pivoted LU on a zero pivot or a stalled "
+                         "refinement), or the pivoted LU")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--workload", choices=["plate", "batch", "approx", "nearfield"], default="plate",
@@ -313,11 +316,15 @@ def main():
         "data": "synthetic (generated plate mesh, analytic plane-wave excitations)",
         "config": {"workload": f"PEC plate 3x3 lambda @300 MHz, {args.n}x{args.n} squares, N={N}, T={T}, "
                                f"{NRHS} plane waves theta 0..90 deg step 0.25, Dunavant degree {DEGREE}",
-                   "N": N, "nrhs": NRHS, "form": "complex64 LU + FP64 iterative refinement",
+                   "N": N, "nrhs": NRHS,
+                   "form": ("complex64 LU without pivoting + FP64 iterative refinement"
+                            if args.form == "c64_np_ir" else "complex64 LU (partial pivoting) + FP64 iterative refinement"),
+                   "factored_form": {0: "c128", 1: "c64 LU, partial pivoting", 2: "c64 LDL^T",
+                                     3: "c64 LU, no pivoting"}.get(form_used, str(form_used)),
                    "tol": args.tol, "block": args.nb},
         "roofline": {"bound": "mfma", "kernel": "cgemm_sub<float> (LU trailing update)",
                      "achieved": gemm_tf, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                     "frac": gemm_tf / PEAK_FP32_MFMA, "traffic": GEMM_F32_TRAFFIC if (args.n, args.nb) == (64, 128) else None,
+                     "frac": gemm_tf / PEAK_FP32_MFMA, "traffic": GEMM_F32_TRAFFIC[args.form] if (args.n, args.nb) == (64, 128) else None,
                      "launches": prof_ir["gemm_launches"], "avg_launch_ms": gemm_avg_ms},
         "refinement": {"iterations": infos_ir[-1].iterations, "max_rel_residual": infos_ir[-1].max_rel_residual},
         "phases_ms": {k: prof_ir[k] / args.steps for k in ("fill_ms", "rhs_ms", "factor_ms", "solve_ms", "refine_ms")},
