@@ -1930,143 +1930,201 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
 // or the refinement stalls.  Without interchanges a block column needs no grid-wide exchange per
 // column: one workgroup factors the diagonal block and the rows below are triangular solves.
 // Block column [k, k + w):
-//   lu_diag           L11 U11 of the w x w diagonal block, in registers (one workgroup)
+//   lu_diag           L11 U11 of the w x w diagonal block (one workgroup, 16-column sub-panels)
 //   col_transpose     T = A21^T into a workspace
 //   trsm (U11^T)      T = U11^{-T} T, i.e. L21 = A21 U11^{-1}
 //   col_transpose     A21 = T^T
 // and the trailing updates are the LU's (U12 = L11^{-1} A12, A22 -= L21 U12) with no interchanges.
 // ----------------------------------------------------------------------------------------------
 
-// Right-looking LU of the w x w (w <= 128) diagonal block at (k, k), held in registers as 4 x 4
-// tiles, one a thread (wavefronts of 16 rows by 64 columns, see below).  Step c reads the 4
-// multipliers' column entries and the 4 pivot-row entries it needs from LDS (masked to zero
-// outside rows > c and columns > c, which makes the update exact elsewhere), updates its 16
-// entries, scales column c, and the owners of column c + 1 and row c + 1 publish them (two buffers
-// each, one barrier a step).  A zero pivot leaves its column of L zero and sets *info (1-based
-// column) if unset.
-constexpr int LUD_TILE = 4;
+// Blocked right-looking LU of the w x w (w <= 128) diagonal block at (k, k), in 16-column
+// sub-panels, one workgroup.  Each thread keeps a 4 x 4 tile of the block in registers for the
+// trailing updates (wavefronts of 16 rows by 64 columns).  Sub-panel j, with D the dj x dj block
+// at s0 = 16 j and e = s0 + dj:
+//   * the panel column (rows >= s0) and panel row (columns >= e) are staged in LDS;
+//   * row wavefronts hold D's rows in lanes 0-15 and up to 48 rows below it in lanes 16-63, one
+//     row a lane, and run D's column steps with the pivot row read by readlane (no barrier):
+//     lanes 0-15 end with D's L\U, the others with their rows of L21 = A21 U_D^-1;
+//   * column wavefronts do the same on the transposed strip (D's columns in lanes 0-15, up to 48
+//     columns of A12 in lanes 16-63) and end with U12 = L_D^-1 A12;
+//   * both write their results to A and to LDS, and every thread takes the rank-16 product
+//     L21 U12 off its tile of the trailing block.
+// Two barriers a sub-panel; the staging buffers alternate.  A zero pivot leaves its column of L
+// zero and sets *info (1-based column) if unset.
+constexpr int LUD_SUB = 16;   // sub-panel width
+constexpr int LUD_CHAIN = 48; // rows (columns) past D per chain wavefront
+
+__device__ inline float lane_value(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ inline double lane_value(double v, int l)
+{
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+template <typename T> __device__ inline cx<T> lane_value(cx<T> v, int l) { return {lane_value(v.re, l), lane_value(v.im, l)}; }
+
+// 1 / d by Smith's scaling (no overflow for any finite nonzero d), with hardware reciprocals for
+// complex64: the pivot inverse is on the dependent chain of every column step.
+__device__ inline cx<float> pivot_inverse(cx<float> d)
+{
+    const bool big_re = fabsf(d.re) >= fabsf(d.im);
+    const float p = big_re ? d.re : d.im, q = big_re ? d.im : d.re;
+    const float r = q * __builtin_amdgcn_rcpf(p);
+    const float rd = __builtin_amdgcn_rcpf(p + q * r);
+    return big_re ? cx<float>{rd, -r * rd} : cx<float>{r * rd, -rd};
+}
+__device__ inline cx<double> pivot_inverse(cx<double> d) { return cdiv(cx<double>{1, 0}, d); }
+
 template <typename T>
 __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info)
 {
-    __shared__ __align__(16) cx<T> col[2][LU_NB_MAX];
-    __shared__ __align__(16) cx<T> rowb[2][LU_NB_MAX];
-    __shared__ cx<T> pinv[2];
-    // wavefront v holds tile rows 4 (v / 2) .. + 3 and tile columns 16 (v % 2) .. + 15: 16 rows by
-    // 64 columns, so it drops out once the factorisation has passed either its rows or its columns
+    constexpr int SB = LUD_SUB, CR = LUD_CHAIN;
+    __shared__ __align__(16) cx<T> Pc[2][SB][LU_NB_MAX];      // panel column: Pc[b][t][r - s0], r >= s0
+    __shared__ __align__(16) cx<T> Pr[2][SB][LU_NB_MAX - SB]; // panel row: Pr[b][i][c - e], c >= e
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int bi = 4 * (wv >> 1) + (lane >> 4), bj = 16 * (wv & 1) + (lane & 15);
-    const int r0 = LUD_TILE * bi, j0 = LUD_TILE * bj;
-    const int wr0 = 16 * (wv >> 1), wc0 = 64 * (wv & 1); // first row and column of this wavefront
-    cx<T> a[LUD_TILE][LUD_TILE];
+    const int r0 = 4 * bi, j0 = 4 * bj;
+    cx<T> a[4][4];
 #pragma unroll
-    for (int i = 0; i < LUD_TILE; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int jj = 0; jj < LUD_TILE; jj++) {
+        for (int jj = 0; jj < 4; jj++) {
             const bool own = r0 + i < w && j0 + jj < w;
             a[i][jj] = A[(own ? k + r0 + i : k) + (own ? k + j0 + jj : k) * lda];
         }
 #pragma unroll
-    for (int i = 0; i < LUD_TILE; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int jj = 0; jj < LUD_TILE; jj++)
+        for (int jj = 0; jj < 4; jj++)
             if (!(r0 + i < w && j0 + jj < w))
                 a[i][jj] = cx<T>{0, 0};
-    auto publish_inverse = [&](cx<T> d, int slot, int cglob) {
-        const bool zero = d.re == (T)0 && d.im == (T)0;
-        pinv[slot] = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
-        if (zero && *info == 0)
-            *info = (int)(k + cglob + 1);
+    // the tiles of sub-panel j's panel column and panel row into buffer b
+    auto stage = [&](int j, int b) {
+        const int s0 = SB * j, e = s0 + (w - s0 < SB ? w - s0 : SB);
+        if (j0 >= s0 && j0 < e && r0 >= s0 && r0 < w)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++)
+                    Pc[b][j0 + jj - s0][r0 + i - s0] = a[i][jj];
+        if (r0 >= s0 && r0 < e && j0 >= e && j0 < w)
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int jj = 0; jj < 4; jj++)
+                    Pr[b][r0 + i - s0][j0 + jj - e] = a[i][jj];
     };
-    // column 0, row 0 and the first pivot
-    if (bj == 0)
-#pragma unroll
-        for (int i = 0; i < LUD_TILE; i++)
-            if (r0 + i < w)
-                col[0][r0 + i] = a[i][0];
-    if (bi == 0)
-#pragma unroll
-        for (int jj = 0; jj < LUD_TILE; jj++)
-            if (j0 + jj < w)
-                rowb[0][j0 + jj] = a[0][jj];
-    if (tid == 0)
-        publish_inverse(a[0][0], 0, 0);
+    stage(0, 0);
     __syncthreads();
-    for (int c = 0; c < w; c++) {
-        // the pivot column's owners (columns around c) stay active to scale and publish it
-        if (wr0 + 15 > c && wr0 < w && wc0 + 63 >= c && wc0 < w) {
-            const cx<T>* cb = col[c & 1];
-            const cx<T>* rb = rowb[c & 1];
-            const cx<T> inv = pinv[c & 1];
-            cx<T> l[LUD_TILE], u[LUD_TILE];
+    const int nsp = (w + SB - 1) / SB;
+    for (int j = 0; j < nsp; j++) {
+        const int b = j & 1, s0 = SB * j, dj = w - s0 < SB ? w - s0 : SB, e = s0 + dj, R = w - e;
+        const int q = wv >> 1;
+        const int nrw = R > 0 ? (R + CR - 1) / CR : 1, ncw = (R + CR - 1) / CR;
+        if ((wv & 1) == 0 && q < nrw) {
+            // row wavefront q: lane -> row s0 + idx
+            const int idx = lane < SB ? lane : dj + CR * q + (lane - SB);
+            const bool valid = lane < SB ? lane < dj : idx < w - s0;
+            const int ci = valid ? idx : 0;
+            cx<T> x[SB];
 #pragma unroll
-            for (int i = 0; i < LUD_TILE; i++) {
-                const int r = r0 + i;
-                const cx<T> v = cb[r];
-                l[i] = (r > c && r < w) ? v * inv : cx<T>{0, 0};
-            }
+            for (int t = 0; t < SB; t++)
+                x[t] = (valid && t < dj) ? Pc[b][t][ci] : cx<T>{0, 0};
 #pragma unroll
-            for (int jj = 0; jj < LUD_TILE; jj++) {
-                const int j = j0 + jj;
-                const cx<T> v = rb[j];
-                u[jj] = (j > c && j < w) ? v : cx<T>{0, 0};
-            }
+            for (int c = 0; c < SB; c++) {
+                if (c < dj) {
+                    const cx<T> d = lane_value(x[c], c);
+                    const bool zero = d.re == (T)0 && d.im == (T)0;
+                    const cx<T> inv = zero ? cx<T>{0, 0} : pivot_inverse(d);
+                    if (zero && q == 0 && lane == 0 && *info == 0)
+                        *info = (int)(k + s0 + c + 1);
+                    const bool below = lane > c;
+                    const cx<T> l = below ? x[c] * inv : cx<T>{0, 0};
+                    if (below)
+                        x[c] = l;
 #pragma unroll
-            for (int i = 0; i < LUD_TILE; i++)
-#pragma unroll
-                for (int jj = 0; jj < LUD_TILE; jj++) {
-                    a[i][jj].re -= l[i].re * u[jj].re - l[i].im * u[jj].im;
-                    a[i][jj].im -= l[i].re * u[jj].im + l[i].im * u[jj].re;
+                    for (int jj = c + 1; jj < SB; jj++) {
+                        const cx<T> u = lane_value(x[jj], c);
+                        x[jj].re -= l.re * u.re - l.im * u.im;
+                        x[jj].im -= l.re * u.im + l.im * u.re;
+                    }
                 }
-            // column c of L
-            if (bj == (c >> 2)) {
-                const int jc = c & 3;
-#pragma unroll
-                for (int jj = 0; jj < LUD_TILE; jj++)
-                    if (jj == jc)
-#pragma unroll
-                        for (int i = 0; i < LUD_TILE; i++)
-                            if (r0 + i > c && r0 + i < w)
-                                a[i][jj] = l[i];
             }
-            if (c + 1 < w) {
-                const int cn = c + 1, tn = cn >> 2, sn = cn & 3;
-                // column c + 1 (rows > c) and row c + 1 (columns > c), updated above
-                if (bj == tn) {
+            if (valid && (q == 0 || lane >= SB)) {
+                cx<T>* dst = A + (k + s0 + idx) + (k + s0) * lda;
 #pragma unroll
-                    for (int jj = 0; jj < LUD_TILE; jj++)
-                        if (jj == sn)
+                for (int t = 0; t < SB; t++)
+                    if (t < dj)
+                        dst[t * lda] = x[t];
+                if (lane >= SB)
 #pragma unroll
-                            for (int i = 0; i < LUD_TILE; i++)
-                                if (r0 + i > c && r0 + i < w)
-                                    col[cn & 1][r0 + i] = a[i][jj];
+                    for (int t = 0; t < SB; t++)
+                        Pc[b][t][idx] = x[t];
+            }
+        } else if ((wv & 1) == 1 && q < ncw) {
+            // column wavefront q: lanes 0-15 -> D's column s0 + lane, the others -> column e + idx
+            const int idx = lane < SB ? lane : CR * q + (lane - SB);
+            const bool valid = lane < SB ? lane < dj : idx < R;
+            const int pi = (lane >= SB && valid) ? idx : 0;
+            cx<T> y[SB];
+#pragma unroll
+            for (int i = 0; i < SB; i++)
+                y[i] = (valid && i < dj) ? (lane < SB ? Pc[b][lane & (SB - 1)][i] : Pr[b][i][pi]) : cx<T>{0, 0};
+#pragma unroll
+            for (int c = 0; c < SB; c++) {
+                if (c < dj) {
+                    const cx<T> d = lane_value(y[c], c);
+                    const bool zero = d.re == (T)0 && d.im == (T)0;
+                    const cx<T> inv = zero ? cx<T>{0, 0} : pivot_inverse(d);
+                    const cx<T> u = lane > c ? y[c] * inv : cx<T>{0, 0};
+#pragma unroll
+                    for (int i = c + 1; i < SB; i++) {
+                        const cx<T> l = lane_value(y[i], c);
+                        y[i].re -= l.re * u.re - l.im * u.im;
+                        y[i].im -= l.re * u.im + l.im * u.re;
+                    }
                 }
-                if (bi == tn) {
+            }
+            if (valid && lane >= SB) {
+                cx<T>* dst = A + (k + s0) + (k + e + idx) * lda;
 #pragma unroll
-                    for (int i = 0; i < LUD_TILE; i++)
-                        if (i == sn)
-#pragma unroll
-                            for (int jj = 0; jj < LUD_TILE; jj++)
-                                if (j0 + jj > c && j0 + jj < w)
-                                    rowb[cn & 1][j0 + jj] = a[i][jj];
-                    if (bj == tn)
-#pragma unroll
-                        for (int i = 0; i < LUD_TILE; i++)
-                            if (i == sn)
-#pragma unroll
-                                for (int jj = 0; jj < LUD_TILE; jj++)
-                                    if (jj == sn)
-                                        publish_inverse(a[i][jj], cn & 1, cn);
-                }
+                for (int i = 0; i < SB; i++)
+                    if (i < dj) {
+                        dst[i] = y[i];
+                        Pr[b][i][idx] = y[i];
+                    }
             }
         }
         __syncthreads();
+        if (R > 0) {
+            // dj == SB here: a partial sub-panel is the last one
+            if (r0 >= e && j0 >= e && r0 < w && j0 < w) {
+#pragma unroll 4
+                for (int t = 0; t < SB; t++) {
+                    cx<T> l[4], u[4];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        l[i] = Pc[b][t][r0 + i - s0];
+#pragma unroll
+                    for (int jj = 0; jj < 4; jj++)
+                        u[jj] = Pr[b][t][j0 + jj - e];
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+#pragma unroll
+                        for (int jj = 0; jj < 4; jj++) {
+                            a[i][jj].re -= l[i].re * u[jj].re - l[i].im * u[jj].im;
+                            a[i][jj].im -= l[i].re * u[jj].im + l[i].im * u[jj].re;
+                        }
+                }
+            }
+            stage(j + 1, b ^ 1);
+        }
+        __syncthreads();
     }
-#pragma unroll
-    for (int i = 0; i < LUD_TILE; i++)
-#pragma unroll
-        for (int jj = 0; jj < LUD_TILE; jj++)
-            if (r0 + i < w && j0 + jj < w)
-                A[(k + r0 + i) + (k + j0 + jj) * lda] = a[i][jj]; // L below the diagonal, U on and above
 }
 
 // T (w x m, leading dimension w) <-> the block column A21 = A(k + w + c, k + i), 32 values of c a
@@ -2182,13 +2240,14 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         return 0;
     };
     // update of columns [c0, c0 + nc) by step k: U12 = L11^{-1} A12, A22 -= L21 U12
-    auto update = [&](int64_t k, int w, int64_t c0, int64_t nc, hipStream_t s) -> int {
+    // parts: 1 = the block-row solve, 2 = the product, 3 = both
+    auto update = [&](int64_t k, int w, int64_t c0, int64_t nc, hipStream_t s, int parts) -> int {
         if (nc <= 0)
             return 0;
-        int rc = row_block_solve<T>(A, lda, k, w, c0, nc, s);
-        return rc ? rc
-                  : gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda,
-                                A + (k + w) + c0 * lda, lda, s, gemm_timer);
+        int rc = (parts & 1) ? row_block_solve<T>(A, lda, k, w, c0, nc, s) : 0;
+        return rc || !(parts & 2) ? rc
+                                  : gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda,
+                                                A + (k + w) + c0 * lda, lda, s, gemm_timer);
     };
     auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
     int rc;
@@ -2196,7 +2255,7 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         for (int64_t p = 0; p < npan; p++) {
             const int64_t k = p * nb;
             const int w = width(k);
-            if ((rc = column(k, w)) || (rc = update(k, w, k + w, n + naug - k - w, st)))
+            if ((rc = column(k, w)) || (rc = update(k, w, k + w, n + naug - k - w, st, 3)))
                 return rc;
         }
     } else {
@@ -2217,23 +2276,35 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
                 if (naug > 0) {
                     if (prev_wide)
                         FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
-                    if ((rc = update(k, w, n, naug, st)))
+                    if ((rc = update(k, w, n, naug, st, 3)))
                         return rc;
                 }
                 break;
             }
             const int64_t nnext = nright < nb ? nright : nb;
             const int64_t nrest = nright + naug - nnext; // the appended columns go with the wide part
+            // The wide part's block-row solve starts with the step; its product waits for the
+            // narrow update, so that the next diagonal block (one 1024-thread workgroup, which
+            // does not fit beside a CU's three product workgroups) is dispatched before the product
+            // fills the CUs, and factors beside it instead of after it.
             FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
             if (nrest > 0) {
                 FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
-                if ((rc = update(k, w, k + w + nnext, nrest, wk.side)))
+                if ((rc = update(k, w, k + w + nnext, nrest, wk.side, 1)))
                     return rc;
-                FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
             }
             if (prev_wide)
                 FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
-            if ((rc = update(k, w, k + w, nnext, st)) || (rc = column(k + w, width(k + w))))
+            if ((rc = update(k, w, k + w, nnext, st, 3)))
+                return rc;
+            if (nrest > 0) {
+                FRICO_HIP_CHECK(hipEventRecord(ev0, st));
+                FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, ev0, 0));
+                if ((rc = update(k, w, k + w + nnext, nrest, wk.side, 2)))
+                    return rc;
+                FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
+            }
+            if ((rc = column(k + w, width(k + w))))
                 return rc;
             prev_wide = nrest > 0;
         }
@@ -2469,7 +2540,6 @@ template int lu_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipS
 template int lu_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int ldlt_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int ldlt_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
-template int lu_nopiv_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_nopiv_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t,
                                     const ResidStream*);
