@@ -1056,8 +1056,10 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
     if (carried)
         rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st, stream ? &rs : nullptr);
     else
-        rc = lu_solve<float>(c->LU32.p, N, N, c->perm.p, c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p,
-                             c->wv_ok ? c->wv.p : nullptr, c->ysolve.p, stream ? &rs : nullptr);
+        // the factors without pivoting have no interchanges to apply
+        rc = lu_solve<float>(c->LU32.p, N, N, c->factored == FRICO_SOLVE_C64_NP_IR ? nullptr : c->perm.p,
+                             c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p, c->wv_ok ? c->wv.p : nullptr,
+                             c->ysolve.p, stream ? &rs : nullptr);
     if (rc)
         return rc;
     if (!stream && (rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))

@@ -556,7 +556,7 @@ template <typename T>
 static void launch_laswp(cx<T>* A, int64_t lda, int64_t col0, int64_t ncols, const int* perm, hipStream_t st,
                          const int* perm2 = nullptr)
 {
-    if (ncols <= 0)
+    if (ncols <= 0 || !perm) // no list: no interchanges
         return;
     launch_kernel(&laswp<T>, dim3((unsigned)((ncols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, st, A, lda,
                   col0, ncols, perm, perm2);
@@ -2372,7 +2372,7 @@ static int lu_forward_fused(const cx<T>* A, int64_t n, int64_t lda, const int* p
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
-        launch_laswp<T>(B, ldb, 0, nrhs, perm + p * PERM_STRIDE, st);
+        launch_laswp<T>(B, ldb, 0, nrhs, perm ? perm + p * PERM_STRIDE : nullptr, st);
         for (int s = 0; s < w; s += 64) {
             const int64_t kc = k + s, ws = (n - kc) < 64 ? (n - kc) : 64, ch = kc / 64;
             const unsigned gx = (unsigned)((n - kc + GBM - 1) / GBM);
@@ -2416,7 +2416,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx
     for (int64_t p = 0; p < npan; p++) {
         const int64_t k = p * nb;
         const int w = (int)((n - k) < nb ? (n - k) : nb);
-        launch_laswp<T>(B, ldb, 0, nrhs, perm + p * PERM_STRIDE, st);
+        launch_laswp<T>(B, ldb, 0, nrhs, perm ? perm + p * PERM_STRIDE : nullptr, st);
         for (int s = 0; s < w; s += 64) {
             const int ws = (w - s) < 64 ? (w - s) : 64;
             // the diagonal sub-blocks of a chunk inverse are the inverses of the chunk's diagonal sub-blocks

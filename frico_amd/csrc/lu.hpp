@@ -87,6 +87,7 @@ struct ResidStream {
     int64_t ldr = 0;
     bool exact = true; // four real products per complex multiply-add (see gemm_sub)
 };
+// perm: the factorisation's interchange lists, or null for factors without interchanges.
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
              hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr,
