@@ -1647,7 +1647,8 @@ static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int 
 // are stored as the LU of Z with no interchanges would be, L strictly below the diagonal and
 // U = D L^T on and above it, so the LU solves run on them unchanged with empty interchange lists.
 // Block column [k, k + w):
-//   ldlt_diag        L11 D1 L11^T of the w x w diagonal block, in registers (one workgroup)
+//   lu_diag<T, true> L11 and U11 = D1 L11^T of the w x w diagonal block from its lower triangle
+//                    (the blocked diagonal-block LU below, one workgroup)
 //   sym_transpose    A12 = A21^T: the block row, taken from the block column
 //   row_block_solve  U12 = L11^{-1} A12 (= D1 L21^T)
 //   sym_transpose    L21 = U12^T D1^{-1}
@@ -1656,114 +1657,10 @@ static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int 
 // accuracy of the answer rests on the refinement against the kept Z (the caller returns to the
 // pivoted LU when the refinement stalls).
 // ----------------------------------------------------------------------------------------------
-// Workgroup of the diagonal-block kernel: one thread per (row, column group), 16 entries a thread.
-constexpr int LDLT_GROUPS = 8;
-constexpr int LDLT_THREADS = LU_NB_MAX * LDLT_GROUPS;
-constexpr int LDLT_OWN = LU_NB_MAX / LDLT_GROUPS;
+template <typename T, bool SYM>
+__global__ void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info);
 // block-row columns per transpose workgroup
 template <typename T> constexpr int kSymCols = sizeof(T) == 4 ? 32 : 16;
-
-// Right-looking L D L^T of the w x w (w <= 128) diagonal block at (k, k), held in registers: thread
-// (row r, group g) keeps the lower-triangle entries (r, j), j = g, g + 8, ..., j <= r.  Step c reads
-// pivot column c from LDS (every thread's l_r = a(r, c) / d_c and its a(j, c)), updates
-// a(r, j) -= l_r a(j, c) for c < j <= r, and the owners of column c + 1 publish it for the next step
-// (two buffers, one barrier a step).  Writes L below the diagonal and U = D L^T on and above it.  A
-// zero pivot leaves its column of L zero and sets *info (1-based column) if unset.
-template <typename T>
-__global__ __launch_bounds__(LDLT_THREADS) void ldlt_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w,
-                                                         int* info)
-{
-    __shared__ cx<T> col[2][LU_NB_MAX];
-    __shared__ cx<T> pinv[2]; // 1 / d_c of the current step (0 for a zero pivot)
-    __shared__ cx<T> piv[LU_NB_MAX];
-    const int tid = threadIdx.x, r = tid % LU_NB_MAX, g = tid / LU_NB_MAX;
-    const bool row_ok = r < w;
-    cx<T> a[LDLT_OWN];
-#pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++) {
-        const int j = g + LDLT_GROUPS * q;
-        const bool own = row_ok && j <= r;
-        a[q] = A[(own ? k + r : k) + (own ? k + j : k) * lda];
-    }
-#pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++)
-        if (!(row_ok && g + LDLT_GROUPS * q <= r))
-            a[q] = cx<T>{0, 0};
-    // the pivot's owner publishes its inverse with the column
-    auto publish_inverse = [&](cx<T> d, int slot) {
-        const bool zero = d.re == (T)0 && d.im == (T)0;
-        pinv[slot] = zero ? cx<T>{0, 0} : cdiv(cx<T>{1, 0}, d);
-        if (zero && *info == 0)
-            *info = (int)(k + r + 1);
-    };
-    if (g == 0 && row_ok)
-        col[0][r] = a[0];
-    if (tid == 0)
-        publish_inverse(a[0], 0);
-    __syncthreads();
-    const int rmax = r | 63; // last row of this wave
-    for (int c = 0; c < w; c++) {
-        const cx<T>* cb = col[c & 1];
-        // waves whose 64 rows all lie at or above row c, or past the block, have nothing left to do
-        if (rmax > c && rmax - 63 < w) {
-            cx<T> v[LDLT_OWN];
-#pragma unroll
-            for (int q = 0; q < LDLT_OWN; q++)
-                v[q] = cb[g + LDLT_GROUPS * q];
-            const cx<T> vr = cb[r];
-            const cx<T> inv = pinv[c & 1];
-            const bool below = row_ok && r > c;
-            const cx<T> l = below ? vr * inv : cx<T>{0, 0};
-            // columns c < j <= rmax (a wave-uniform test).  A lane also updates its entries with
-            // j > r, outside the triangle: they are never read, and skipping them would cost more.
-#pragma unroll
-            for (int q = 0; q < LDLT_OWN; q++) {
-                const int j = g + LDLT_GROUPS * q;
-                if (j > c && j <= rmax) {
-                    a[q].re -= l.re * v[q].re - l.im * v[q].im;
-                    a[q].im -= l.re * v[q].im + l.im * v[q].re;
-                }
-            }
-            const int qc = c / LDLT_GROUPS, gc = c % LDLT_GROUPS;
-            const int qn = (c + 1) / LDLT_GROUPS, gn = (c + 1) % LDLT_GROUPS;
-            if (g == gc) {
-#pragma unroll
-                for (int q = 0; q < LDLT_OWN; q++)
-                    if (q == qc)
-                        a[q] = below ? l : a[q];
-            }
-            if (g == gn && c + 1 < w) {
-                cx<T> nx = a[0];
-#pragma unroll
-                for (int q = 1; q < LDLT_OWN; q++)
-                    nx = q == qn ? a[q] : nx;
-                if (below)
-                    col[(c + 1) & 1][r] = nx;
-                if (r == c + 1)
-                    publish_inverse(nx, (c + 1) & 1);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++)
-        if (row_ok && g + LDLT_GROUPS * q == r)
-            piv[r] = a[q];
-    __syncthreads();
-    cx<T> dj[LDLT_OWN];
-#pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++)
-        dj[q] = piv[g + LDLT_GROUPS * q];
-#pragma unroll
-    for (int q = 0; q < LDLT_OWN; q++) {
-        const int j = g + LDLT_GROUPS * q;
-        if (row_ok && j <= r) {
-            A[(k + r) + (k + j) * lda] = a[q];            // L(r, j); d_r on the diagonal
-            if (j < r)
-                A[(k + j) + (k + r) * lda] = dj[q] * a[q]; // U(j, r) = d_j L(r, j)
-        }
-    }
-}
 
 // Block row <-> block column of step k (w columns, m = n - k - w rows below), kSymCols values of c a
 // workgroup, every load of a thread in flight before its LDS stores:
@@ -1858,7 +1755,7 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
     // diagonal block, block row and block column of step k
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
-        launch_kernel(&ldlt_diag<T>, dim3(1), dim3(LDLT_THREADS), 0, st, A, lda, k, w, wk.flags + 1);
+        launch_kernel(&lu_diag<T, true>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
         if (m > 0) {
             const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
             launch_kernel(&sym_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m);
@@ -1905,6 +1802,15 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
             const int w = width(k);
             const int64_t nright = n - k - w;
             const int64_t nnext = nright < nb ? nright : nb;
+            // The side stream's products start after the narrow update, so that the next diagonal
+            // block (one 1024-thread workgroup) is dispatched before they fill the CUs.
+            if (nright > 0) {
+                // the next block column was last updated by the wide part of the previous step
+                if (p > 0)
+                    FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
+                if ((rc = update(k, w, k + w, nnext, st)))
+                    return rc;
+            }
             FRICO_HIP_CHECK(hipEventRecord(evP[p], st));
             FRICO_HIP_CHECK(hipStreamWaitEvent(wk.side, evP[p], 0));
             if ((rc = appended(k, w, wk.side)) || (rc = update(k, w, k + w + nnext, nright - nnext, wk.side)))
@@ -1912,10 +1818,7 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
             FRICO_HIP_CHECK(hipEventRecord(evW[p], wk.side));
             if (nright <= 0)
                 break;
-            // the next block column was last updated by the wide part of the previous step
-            if (p > 0)
-                FRICO_HIP_CHECK(hipStreamWaitEvent(st, evW[p - 1], 0));
-            if ((rc = update(k, w, k + w, nnext, st)) || (rc = column(k + w, width(k + w))))
+            if ((rc = column(k + w, width(k + w))))
                 return rc;
         }
         FRICO_HIP_CHECK(hipEventRecord(ev0, wk.side));
@@ -1950,7 +1853,8 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
 //   * both write their results to A and to LDS, and every thread takes the rank-16 product
 //     L21 U12 off its tile of the trailing block.
 // Two barriers a sub-panel; the staging buffers alternate.  A zero pivot leaves its column of L
-// zero and sets *info (1-based column) if unset.
+// zero and sets *info (1-based column) if unset.  SYM: the block is symmetric and only its lower
+// triangle is read (entry (r, c), r < c, is taken from (c, r)); the factors are then L and U = D L^T.
 constexpr int LUD_SUB = 16;   // sub-panel width
 constexpr int LUD_CHAIN = 48; // rows (columns) past D per chain wavefront
 
@@ -1979,7 +1883,7 @@ __device__ inline cx<float> pivot_inverse(cx<float> d)
 }
 __device__ inline cx<double> pivot_inverse(cx<double> d) { return cdiv(cx<double>{1, 0}, d); }
 
-template <typename T>
+template <typename T, bool SYM = false>
 __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info)
 {
     constexpr int SB = LUD_SUB, CR = LUD_CHAIN;
@@ -1993,8 +1897,9 @@ __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t l
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int jj = 0; jj < 4; jj++) {
-            const bool own = r0 + i < w && j0 + jj < w;
-            a[i][jj] = A[(own ? k + r0 + i : k) + (own ? k + j0 + jj : k) * lda];
+            const int gr = r0 + i, gc = j0 + jj;
+            const bool own = gr < w && gc < w, up = SYM && gc > gr;
+            a[i][jj] = A[(own ? k + (up ? gc : gr) : k) + (own ? k + (up ? gr : gc) : k) * lda];
         }
 #pragma unroll
     for (int i = 0; i < 4; i++)
