@@ -36,9 +36,9 @@ PEAK_HBM = 8000.0       # GB/s
 # HBM bytes per timed trailing-update launch of cgemm_sub<float> (K = 128) at N = 12160, nb = 128
 # (average over the 188 launches of one factorisation), from rocprofv3 FETCH_SIZE / WRITE_SIZE
 # runs scaled by a copy kernel of known size (profiles/plate_N12160_c64np_hbm_traffic.md for the
-# default form, 699.9e6; plate_N12160_c64ir_hbm_traffic.md for the pivoted one, 700.1e6);
+# default form, 699.7e6; plate_N12160_c64ir_hbm_traffic.md for the pivoted one, 700.1e6);
 # algorithmic bytes per launch: 419.3e6.
-GEMM_F32_TRAFFIC = {"c64_np_ir": 699.9e6, "c64_ir": 700.1e6}
+GEMM_F32_TRAFFIC = {"c64_np_ir": 699.7e6, "c64_ir": 700.1e6}
 
 
 def lu_flops(n, nrhs):
