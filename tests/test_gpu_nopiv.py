@@ -105,6 +105,32 @@ def test_zero_pivot_falls_back_to_the_pivoted_lu():
         assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
 
 
+@pytest.mark.parametrize("z", [40, 150, 255, 383])
+def test_zero_pivot_inside_a_diagonal_block_falls_back(z):
+    """Block-diagonal A = diag(A1, [[0, 1], [1, 0]], A2) with the 2 x 2 swap at column z: the
+    unpivoted elimination of A1 never reaches the swap block, so its pivot at z is exactly zero.
+    z = 40, 150, 255, 383 put the zero in the first and second 128-column diagonal blocks, in a
+    middle sub-panel, at a block's last column and at the last column of the matrix's third block
+    (the swap then straddles two diagonal blocks)."""
+    import frico_amd
+    rng = np.random.default_rng(z)
+    n = 520
+    A = np.zeros((n, n), dtype=np.complex128)
+    A[:z, :z] = rng.uniform(-1, 1, (z, z)) + 1j * rng.uniform(-1, 1, (z, z)) + 2.0 * np.sqrt(z) * np.eye(z)
+    A[z, z + 1] = A[z + 1, z] = 1.0
+    m = n - z - 2
+    A[z + 2:, z + 2:] = rng.uniform(-1, 1, (m, m)) + 1j * rng.uniform(-1, 1, (m, m)) + 2.0 * np.sqrt(m) * np.eye(m)
+    B = rng.uniform(-1, 1, (n, 3)) + 1j * rng.uniform(-1, 1, (n, 3))
+    with frico_amd.Context(0) as c:
+        c.set_block(128)
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor("c64_np_ir")
+        assert c.factored_form == frico_amd.SOLVE_C64_IR
+        assert c.solve(tol=1e-12).converged
+        assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
+
+
 def test_growth_falls_back_after_the_refinement_stalls():
     import frico_amd
     rng = np.random.default_rng(5)
