@@ -56,7 +56,7 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
 // LU without pivoting (for the refinement form, which corrects its rounding): the same layout as
 // lu_factor's factors with no interchanges (empty lists in LuWork::perm, ipiv the identity), so the
 // solves take them as they are.  Needs LuWork::tbuf.  naug as for lu_factor.  A zero pivot is
-// reported like zgetrf's info.
+// reported like zgetrf's info.  Instantiated for complex64 only.
 template <typename T>
 int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
                     KernelTimer* gemm_timer, int64_t naug = 0);
