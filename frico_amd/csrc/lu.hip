@@ -1939,26 +1939,37 @@ __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t l
 #pragma unroll
             for (int t = 0; t < SB; t++)
                 x[t] = (valid && t < dj) ? Pc[b][t][ci] : cx<T>{0, 0};
+            // D's column steps; the pivot is wave-uniform (read by readlane), and so is the first
+            // zero pivot's column zc, recorded after the steps.  A full sub-panel (dj == SB, all
+            // but possibly the last) runs as one straight-line sequence.
+            int zc = -1;
+            auto steps = [&](auto full) {
 #pragma unroll
-            for (int c = 0; c < SB; c++) {
-                if (c < dj) {
-                    const cx<T> d = lane_value(x[c], c);
-                    const bool zero = d.re == (T)0 && d.im == (T)0;
-                    const cx<T> inv = zero ? cx<T>{0, 0} : pivot_inverse(d);
-                    if (zero && q == 0 && lane == 0 && *info == 0)
-                        *info = (int)(k + s0 + c + 1);
-                    const bool below = lane > c;
-                    const cx<T> l = below ? x[c] * inv : cx<T>{0, 0};
-                    if (below)
-                        x[c] = l;
+                for (int c = 0; c < SB; c++) {
+                    if (full || c < dj) {
+                        const cx<T> d = lane_value(x[c], c);
+                        const bool zero = d.re == (T)0 && d.im == (T)0;
+                        const cx<T> inv = zero ? cx<T>{0, 0} : pivot_inverse(d);
+                        zc = (zero && zc < 0) ? c : zc;
+                        const bool below = lane > c;
+                        const cx<T> l = below ? x[c] * inv : cx<T>{0, 0};
+                        if (below)
+                            x[c] = l;
 #pragma unroll
-                    for (int jj = c + 1; jj < SB; jj++) {
-                        const cx<T> u = lane_value(x[jj], c);
-                        x[jj].re -= l.re * u.re - l.im * u.im;
-                        x[jj].im -= l.re * u.im + l.im * u.re;
+                        for (int jj = c + 1; jj < SB; jj++) {
+                            const cx<T> u = lane_value(x[jj], c);
+                            x[jj].re -= l.re * u.re - l.im * u.im;
+                            x[jj].im -= l.re * u.im + l.im * u.re;
+                        }
                     }
                 }
-            }
+            };
+            if (dj == SB)
+                steps(std::true_type{});
+            else
+                steps(std::false_type{});
+            if (zc >= 0 && q == 0 && lane == 0 && *info == 0)
+                *info = (int)(k + s0 + zc + 1);
             if (valid && (q == 0 || lane >= SB)) {
                 cx<T>* dst = A + (k + s0 + idx) + (k + s0) * lda;
 #pragma unroll
@@ -1979,21 +1990,27 @@ __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t l
 #pragma unroll
             for (int i = 0; i < SB; i++)
                 y[i] = (valid && i < dj) ? (lane < SB ? Pc[b][lane & (SB - 1)][i] : Pr[b][i][pi]) : cx<T>{0, 0};
+            auto steps = [&](auto full) {
 #pragma unroll
-            for (int c = 0; c < SB; c++) {
-                if (c < dj) {
-                    const cx<T> d = lane_value(y[c], c);
-                    const bool zero = d.re == (T)0 && d.im == (T)0;
-                    const cx<T> inv = zero ? cx<T>{0, 0} : pivot_inverse(d);
-                    const cx<T> u = lane > c ? y[c] * inv : cx<T>{0, 0};
+                for (int c = 0; c < SB; c++) {
+                    if (full || c < dj) {
+                        const cx<T> d = lane_value(y[c], c);
+                        const bool zero = d.re == (T)0 && d.im == (T)0;
+                        const cx<T> inv = zero ? cx<T>{0, 0} : pivot_inverse(d);
+                        const cx<T> u = lane > c ? y[c] * inv : cx<T>{0, 0};
 #pragma unroll
-                    for (int i = c + 1; i < SB; i++) {
-                        const cx<T> l = lane_value(y[i], c);
-                        y[i].re -= l.re * u.re - l.im * u.im;
-                        y[i].im -= l.re * u.im + l.im * u.re;
+                        for (int i = c + 1; i < SB; i++) {
+                            const cx<T> l = lane_value(y[i], c);
+                            y[i].re -= l.re * u.re - l.im * u.im;
+                            y[i].im -= l.re * u.im + l.im * u.re;
+                        }
                     }
                 }
-            }
+            };
+            if (dj == SB)
+                steps(std::true_type{});
+            else
+                steps(std::false_type{});
             if (valid && lane >= SB) {
                 cx<T>* dst = A + (k + s0) + (k + e + idx) * lda;
 #pragma unroll
