@@ -358,7 +358,8 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
         return set_error("no frequencies"), -2;
     if (fill_prec != FRICO_C128 && fill_prec != FRICO_C64)
         return set_error("unknown fill precision %d", fill_prec), -4;
-    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR)
+    if (form != FRICO_SOLVE_C128 && form != FRICO_SOLVE_C64_IR && form != FRICO_SOLVE_C64_SYM_IR &&
+        form != FRICO_SOLVE_C64_NP_IR)
         return set_error("unknown factorisation form %d", form), -5;
     if (form == FRICO_SOLVE_C128 && fill_prec != FRICO_C128)
         return set_error("the complex128 form needs a complex128 fill"), -4;

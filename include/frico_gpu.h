@@ -280,8 +280,10 @@ int frico_adjoint_backward(frico_ctx* ctx, double g_zin_re, double g_zin_im, dou
  *   zin[2*(m*nfreq + f) ..] = V / I_port (compute_port_values, src/emw_ana_antenna.cpp:76-105);
  *   info[m*nfreq + f] is the solve status.  Meshes run largest first on up to `workers` concurrent
  *   contexts (fewer when ceil(Nmax / R) panel workgroups per context would exceed the CUs).
- *   FRICO_BATCH_GRAPHS replays a captured HIP graph per mesh for the complex64 LU and each
- *   refinement sweep; FRICO_BATCH_AUTOTUNE picks the LU block width per size class (N / 256) by
+ *   FRICO_BATCH_GRAPHS replays a captured HIP graph per mesh for the complex64 LU
+ *   (FRICO_SOLVE_C64_IR) and each refinement sweep; the other complex64 forms run eagerly, with
+ *   their fall-back to the pivoted LU (the worker count is sized for that fall-back's panel).
+ *   FRICO_BATCH_AUTOTUNE picks the LU block width per size class (N / 256) by
  *   timing the candidates on the first system of that class.
  * frico_batch_block_sizes: nb[m] = block width used for mesh m by the last frico_batch_antenna;
  *   returns the number of meshes whose frequencies after the first ran as graph replays.

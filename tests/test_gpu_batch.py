@@ -75,6 +75,26 @@ def test_autotuned_block_width_keeps_the_answer():
     assert np.max(np.abs(z1 - z2) / np.abs(z1)) <= 1e-8
 
 
+def test_unpivoted_form_in_a_batch():
+    """The complex64 LU without pivoting (eager, no graphs) in a batch of contexts: every unit
+    converges and matches the complex128 form of a single context."""
+    import frico_amd
+    ms = small_batch()[:3]
+    fr = freqs3()
+    with frico_amd.Batch(0, workers=3) as b:
+        for sm in ms:
+            b.add_mesh(sm)
+        z, info = b.antenna(fr, fill="c128", form="c64_np_ir", tol=1e-12, autotune=False)
+    assert all(i.converged and i.max_rel_residual <= 1e-12 for row in info for i in row)
+    with frico_amd.Context(0) as ctx:
+        for m, sm in enumerate(ms):
+            ctx.load_mesh(sm, 5)
+            single = ctx.antenna_sweep(fr, sm.feed)
+            for f in range(len(fr)):
+                zs = single[f]["impedance"]
+                assert abs(z[m, f] - zs) <= 1e-9 * abs(zs), (m, f, z[m, f], zs)
+
+
 def test_complex128_form_in_a_batch():
     import frico_amd
     ms = small_batch()[:3]
