@@ -176,6 +176,13 @@ int frico_ctx_create(int device, frico_ctx** out)
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
     }
+    // the device address of frico_fields_noalloc's scratch, resolved once per device
+    if (int rc = noalloc_init(device)) {
+        (void)hipStreamDestroy(c->st);
+        (void)hipStreamDestroy(c->side);
+        delete c;
+        return rc;
+    }
     *out = c;
     return 0;
 }
@@ -1276,35 +1283,31 @@ int frico_fields(frico_ctx* c, double freq, const double* pts, int64_t npts, con
     int rc = set_dev(c);
     if (rc || (rc = scatter_all(c)))
         return rc;
-    std::vector<int2> rq((size_t)npts);
-    for (int64_t p = 0; p < npts; p++) {
-        const int64_t col = cols ? cols[p] : 0;
-        if (col < 0 || col >= c->nrhs)
-            return set_error("point %lld uses column %lld out of range", (long long)p, (long long)col), -5;
-        rq[p] = int2{(int)p, (int)col};
+    std::vector<int> cl;
+    if (cols) {
+        cl.resize((size_t)npts);
+        for (int64_t p = 0; p < npts; p++) {
+            if (cols[p] < 0 || cols[p] >= c->nrhs)
+                return set_error("point %lld uses column %lld out of range", (long long)p, (long long)cols[p]), -5;
+            cl[p] = (int)cols[p];
+        }
+        if ((rc = c->pcols.upload(cl.data(), cl.size(), c->st)))
+            return rc;
     }
-    if ((rc = c->pts.upload(pts, (size_t)(3 * npts), c->st)) || (rc = c->req.upload(rq.data(), rq.size(), c->st)) ||
-        (rc = c->EH.ensure((size_t)(6 * npts))))
+    if ((rc = c->pts.upload(pts, (size_t)(3 * npts), c->st)) || (rc = c->EH.ensure((size_t)(6 * npts))))
         return rc;
     double omega;
     const double k = kwave(freq, nullptr, &omega);
-    if ((rc = launch_fields(c->dm, k, omega * MU0, c->TS.p, c->pts.p, c->req.p, (int)npts, h_sign >= 0 ? 1.0 : -1.0,
-                            c->EH.p, c->st)))
+    if ((rc = launch_fields(c->dm, k, omega * MU0, c->TS.p, (int)c->nrhs, c->pts.p, cols ? c->pcols.p : nullptr,
+                            (int)npts, h_sign >= 0 ? 1.0 : -1.0, c->EH.p, c->EH.p + 3 * npts, c->st)))
         return rc;
-    std::vector<c128> eh((size_t)(6 * npts));
-    FRICO_HIP_CHECK(hipMemcpyAsync(eh.data(), c->EH.p, eh.size() * sizeof(c128), hipMemcpyDeviceToHost, c->st));
+    // E and H are npts x 3 complex each: the staging buffer holds E, then H
+    if (E)
+        FRICO_HIP_CHECK(hipMemcpyAsync(E, c->EH.p, (size_t)(3 * npts) * sizeof(c128), hipMemcpyDeviceToHost, c->st));
+    if (H)
+        FRICO_HIP_CHECK(hipMemcpyAsync(H, c->EH.p + 3 * npts, (size_t)(3 * npts) * sizeof(c128), hipMemcpyDeviceToHost,
+                                       c->st));
     FRICO_HIP_CHECK(hipStreamSynchronize(c->st));
-    for (int64_t p = 0; p < npts; p++)
-        for (int d = 0; d < 3; d++) {
-            if (E) {
-                E[2 * (3 * p + d)] = eh[6 * p + d].re;
-                E[2 * (3 * p + d) + 1] = eh[6 * p + d].im;
-            }
-            if (H) {
-                H[2 * (3 * p + d)] = eh[6 * p + 3 + d].re;
-                H[2 * (3 * p + d) + 1] = eh[6 * p + 3 + d].im;
-            }
-        }
     return 0;
 }
 

@@ -166,6 +166,10 @@ struct HostMesh {
     std::vector<double> qbeta, qsub_beta;
 };
 
+// Resolves the device address of frico_fields_noalloc's scratch on `device` (current), once per
+// device (noalloc.hip).
+int noalloc_init(int device);
+
 } // namespace frico
 
 using namespace frico;
@@ -229,7 +233,7 @@ struct frico_ctx {
     // fields
     DevBuf<c128> TS, EH;
     DevBuf<double> pts;
-    DevBuf<int2> req;
+    DevBuf<int> pcols;
     DevBuf<PlaneWaveDev> waves;
     // profiling
     KernelTimer t_fill, t_rhs, t_factor, t_solve, t_refine, t_gemm, t_tripair;

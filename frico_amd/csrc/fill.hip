@@ -581,18 +581,31 @@ __global__ __launch_bounds__(256) void scatter_currents(DevMesh m, const c128* _
 }
 
 // ----------------------------------------------------------------------------------------------
-// Field sums E, H at a point (src/emw_postpro_common.cpp:31-66): one workgroup per request
-// (point, current vector); threads stride over triangles, block reduction of 12 doubles.
-// H keeps the reference's exp(+jkR) factor unless h_sign = -1.
+// Field sums E, H at a point (src/emw_postpro_common.cpp:31-66): one workgroup per point, which
+// uses current vector cols[point] (column 0 when cols is null); threads stride over triangles,
+// block reduction of 12 doubles.  H keeps the reference's exp(+jkR) factor unless h_sign = -1.
+// A point whose column is outside [0, ncols) gets NaN fields (the callers that take device-side
+// column lists cannot check them on the host).  E and H: npts x 3 complex, either may be null.
 // ----------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void eval_fields(DevMesh m, double k, double omega_mu0, const c128* __restrict__ TS,
-                                                   const double* __restrict__ pts, const int2* __restrict__ req,
-                                                   int nreq, double h_sign, c128* __restrict__ EH)
+                                                   int ncols, const double* __restrict__ pts,
+                                                   const int* __restrict__ cols, int npts, double h_sign,
+                                                   c128* __restrict__ E, c128* __restrict__ H)
 {
-    const int rq = blockIdx.x;
-    if (rq >= nreq)
+    const int ip = blockIdx.x;
+    if (ip >= npts)
         return;
-    const int ip = req[rq].x, r = req[rq].y;
+    const int r = cols ? cols[ip] : 0;
+    if (r < 0 || r >= ncols) {
+        if (threadIdx.x < 3) {
+            const double nan = __builtin_nan("");
+            if (E)
+                E[3 * (size_t)ip + threadIdx.x] = c128{nan, nan};
+            if (H)
+                H[3 * (size_t)ip + threadIdx.x] = c128{nan, nan};
+        }
+        return;
+    }
     const double px = pts[3 * ip], py = pts[3 * ip + 1], pz = pts[3 * ip + 2];
     double acc[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     const c128* ts = TS + (size_t)r * m.T * 4;
@@ -643,7 +656,9 @@ __global__ __launch_bounds__(256) void eval_fields(DevMesh m, double k, double o
         double s = 0;
         for (int w = 0; w < (int)(blockDim.x / 64); w++)
             s += red[threadIdx.x][w];
-        reinterpret_cast<double*>(EH + (size_t)rq * 6)[threadIdx.x] = s;
+        c128* out = threadIdx.x < 6 ? E : H;
+        if (out)
+            reinterpret_cast<double*>(out + (size_t)ip * 3)[threadIdx.x % 6] = s;
     }
 }
 
@@ -735,10 +750,11 @@ int launch_scatter(const DevMesh& m, const c128* I, int64_t ldi, int nrhs, c128*
     return 0;
 }
 
-int launch_fields(const DevMesh& m, double k, double omega_mu0, const c128* TS, const double* pts, const int2* req,
-                  int nreq, double h_sign, c128* EH, hipStream_t st)
+int launch_fields(const DevMesh& m, double k, double omega_mu0, const c128* TS, int ncols, const double* pts,
+                  const int* cols, int npts, double h_sign, c128* E, c128* H, hipStream_t st)
 {
-    hipLaunchKernelGGL(eval_fields, dim3(nreq), dim3(256), 0, st, m, k, omega_mu0, TS, pts, req, nreq, h_sign, EH);
+    hipLaunchKernelGGL(eval_fields, dim3(npts), dim3(256), 0, st, m, k, omega_mu0, TS, ncols, pts, cols, npts, h_sign,
+                       E, H);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -47,7 +47,9 @@ int launch_fill_approx(const DevMesh& m, double k, double inv_ksq, int prec, voi
 int launch_rhs_planewave(const DevMesh& m, double k, double omega_mu0, const PlaneWaveDev* waves, int nang, c128* B,
                          int64_t ldb, hipStream_t st);
 int launch_scatter(const DevMesh& m, const c128* I, int64_t ldi, int nrhs, c128* TS, hipStream_t st);
-int launch_fields(const DevMesh& m, double k, double omega_mu0, const c128* TS, const double* pts, const int2* req,
-                  int nreq, double h_sign, c128* EH, hipStream_t st);
+// E and H at npts points from the triangle currents TS of ncols current vectors; point p uses
+// column cols[p] (cols null: column 0), NaN fields for a column outside [0, ncols).
+int launch_fields(const DevMesh& m, double k, double omega_mu0, const c128* TS, int ncols, const double* pts,
+                  const int* cols, int npts, double h_sign, c128* E, c128* H, hipStream_t st);
 
 } // namespace frico

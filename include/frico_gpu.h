@@ -4,7 +4,8 @@
  * Plain C: pointers, sizes and status codes; no exceptions cross it and no C++ or HIP types
  * appear in the signatures.  Complex numbers are interleaved (re, im) doubles, the layout of
  * std::complex<double>; matrices are column-major with an explicit leading dimension, like the
- * reference's Eigen matrices (src/eigen.h:36-40).  All device memory is owned by the context.
+ * reference's Eigen matrices (src/eigen.h:36-40).  Device memory is owned by the context, except
+ * the caller's device buffers of frico_fields_noalloc.
  *
  * Each entry point names the reference interface it replaces (paths relative to the reference
  * source root).  Status convention: 0 = OK; -k = argument k invalid (LAPACK style); > 0 =
@@ -170,6 +171,24 @@ int frico_triangle_currents(frico_ctx* ctx, int64_t col, double* tri_AJ, double*
  * row-major.  h_sign = +1 keeps the reference's exp(+jkR) in H (:60-61), -1 uses exp(-jkR). */
 int frico_fields(frico_ctx* ctx, double freq, const double* pts, int64_t npts, const int64_t* cols,
                  double h_sign, double* E, double* H);
+
+/*
+ * frico_fields_noalloc: frico_fields on device buffers, allocating nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Every pointer is DEVICE memory of the
+ * context's device: currents = ncols current vectors (N x ncols complex128, leading dimension ldi,
+ * e.g. the caller's own solve), pts = npts x 3 doubles, cols = npts column indices (NULL: column 0
+ * for every point), E and H = npts x 3 complex128 outputs, row-major [point][xyz] (either may be
+ * NULL).  A point whose column is outside [0, ncols) gets NaN fields.  The triangle currents go to
+ * a fixed per-device scratch of FRICO_FIELDS_NOALLOC_MAX_BYTES: 64 * T * ncols bytes must fit in it
+ * (T = triangles), otherwise FRICO_ERR_LIMIT.  The work is only enqueued on the context's stream
+ * (frico_ctx_synchronize waits for it).  Calls that overlap on one GPU (contexts on the same
+ * device whose streams run concurrently) race on the shared scratch.
+ */
+#define FRICO_FIELDS_NOALLOC_MAX_BYTES (64LL * 1024 * 1024)
+int frico_fields_noalloc(frico_ctx* ctx, double freq, const double* currents, int64_t ldi, int64_t ncols,
+                         const double* pts, int64_t npts, const int32_t* cols, double h_sign, double* E,
+                         double* H);
 
 /*
  * gmsh .msh ingestion (format 4.1 ASCII): load_from_gmsh (src/input_gmsh.cpp:36-406) without the
