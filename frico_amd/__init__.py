@@ -515,6 +515,15 @@ class Batch:
         """Concurrent contexts of the last run (capped by GPU_MAX_HW_QUEUES and co-residency)."""
         return int(self._L.frico_batch_workers_used(self._h))
 
+    def factored_forms(self, nfreq):
+        """The factorisation (SOLVE_*) each (mesh, frequency) of the last run was solved with, after
+        any fall-back: an array [mesh, frequency]."""
+        forms = np.zeros(len(self.meshes) * nfreq, np.int32)
+        n = self._L.frico_batch_factored_forms(self._h, _p(forms, C.c_int32))
+        if n < 0:
+            self._check(n)
+        return forms.reshape(len(self.meshes), nfreq)
+
     def block_sizes(self):
         """(block width per mesh, number of meshes that replayed graphs) of the last run."""
         nb = np.zeros(len(self.meshes), np.int32)

@@ -112,6 +112,7 @@ SIGNATURES = {
     "frico_batch_antenna": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
                                       C.c_double, _d, C.POINTER(frico_solve_info)]),
     "frico_batch_block_sizes": (C.c_int, [_vp, _i32]),
+    "frico_batch_factored_forms": (C.c_int, [_vp, _i32]),
 }
 
 _lib = None

@@ -46,6 +46,7 @@ struct frico_batch {
     std::vector<frico_ctx*> ctx;
     std::vector<Mesh> meshes;
     std::vector<int32_t> mesh_nb; // block width used for each mesh by the last run
+    std::vector<int32_t> unit_form; // factorisation each (mesh, frequency) was solved with (FRICO_SOLVE_*)
     std::atomic<int> graph_units{0}; // units of the last run that replayed graphs
     int workers_used = 0;            // concurrent contexts of the last run
     std::mutex mu;
@@ -184,9 +185,11 @@ int solve_graphs(frico_ctx* c, const RunOpts& o, Graph& g1, Graph& g2, frico_sol
 }
 
 // Block width for this size class: the table entry, or (autotune) the faster of the candidates
-// on the current system, timed as factor + solve.
-int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, int* nb_out)
+// on the current system, timed as factor + solve.  The complex128 form factors Z in place, so the
+// system at `freq` is filled again before every pass and once more at the end.
+int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, double freq, int* nb_out)
 {
+    const bool refill = o.form == FRICO_SOLVE_C128;
     const int64_t cls = c->N / 256;
     {
         std::lock_guard<std::mutex> lock(b->mu);
@@ -199,13 +202,16 @@ int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, int* nb_out)
     const int cand[2] = {64, 32};
     double best = 1e300;
     int best_nb = 64;
+    int rc = 0;
     for (int nb : cand) {
-        int rc = frico_ctx_set_block(c, nb);
+        rc = frico_ctx_set_block(c, nb);
         if (rc)
             return rc;
         frico_solve_info inf{};
         // one untimed pass (workspace allocation, first launches), then the timed one
         for (int pass = 0; pass < 2; pass++) {
+            if (refill && (rc = frico_fill(c, freq, FRICO_FILL_QUAD, o.fill_prec)))
+                return rc;
             const auto t0 = std::chrono::steady_clock::now();
             if ((rc = solve_eager(c, o, &inf)))
                 return rc;
@@ -216,6 +222,8 @@ int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, int* nb_out)
             }
         }
     }
+    if (refill && (rc = frico_fill(c, freq, FRICO_FILL_QUAD, o.fill_prec)))
+        return rc;
     std::lock_guard<std::mutex> lock(b->mu);
     b->tuned.emplace(cls, best_nb);
     *nb_out = b->tuned[cls];
@@ -250,7 +258,7 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
             return rc;
         frico_solve_info& inf = infos[(size_t)mi * o.nfreq + f];
         if (f == 0) {
-            if ((rc = pick_block(b, c, o, &nb)) || (rc = frico_ctx_set_block(c, nb)))
+            if ((rc = pick_block(b, c, o, freq, &nb)) || (rc = frico_ctx_set_block(c, nb)))
                 return rc;
             b->mesh_nb[mi] = nb;
             rc = solve_eager(c, o, &inf);
@@ -261,6 +269,7 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
         }
         if (rc)
             return rc;
+        b->unit_form[(size_t)mi * o.nfreq + f] = c->factored;
         if (f == o.nfreq - 1 && g1.x)
             b->graph_units++;
         // compute_port_values (src/emw_ana_antenna.cpp:76-105): I_port = sum l I sign over the feed
@@ -363,6 +372,11 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
         return set_error("unknown factorisation form %d", form), -5;
     if (form == FRICO_SOLVE_C128 && fill_prec != FRICO_C128)
         return set_error("the complex128 form needs a complex128 fill"), -4;
+    // batches fill in quadrature mode, whose Z is not symmetric (the sub-triangle rule on the row
+    // basis, src/emw_solver.cpp:67-78); run_context takes solve_symmetric only for approximate-mode
+    // or symmetrised matrices (src/emw_solver.h:124-128)
+    if (form == FRICO_SOLVE_C64_SYM_IR)
+        return set_error("the symmetric form needs a symmetric Z; batched quadrature-mode fills are not"), -5;
     if (maxit < 1)
         return set_error("maxit must be >= 1"), -7;
     if (!zin || !info)
@@ -389,6 +403,7 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
         1, std::min<int64_t>(std::min<int64_t>((int64_t)b->ctx.size(), hwq), b->cus / gmax));
     b->workers_used = W;
     b->mesh_nb.assign((size_t)nm, 0);
+    b->unit_form.assign((size_t)nm * nfreq, -1);
     b->graph_units = 0;
     for (int i = 0; i < nm * nfreq; i++)
         info[i] = frico_solve_info{0, 0, -1.0};
@@ -432,6 +447,15 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
     if (failed)
         return set_error("a batch worker could not select the device"), FRICO_ERR_HIP;
     return 0;
+}
+
+int frico_batch_factored_forms(const frico_batch* b, int32_t* forms)
+{
+    if (!b || !forms)
+        return -1;
+    for (size_t i = 0; i < b->unit_form.size(); i++)
+        forms[i] = b->unit_form[i];
+    return (int)b->unit_form.size();
 }
 
 int frico_batch_block_sizes(const frico_batch* b, int32_t* nb)

@@ -1090,7 +1090,10 @@ int read_refine_norms(frico_ctx* c, const std::vector<double>& bn, double* worst
     double w = 0;
     for (int64_t j = 0; j < nr; j++) {
         const double r = std::sqrt(rn[j]);
-        w = std::max(w, bn[j] > 0 ? r / bn[j] : r);
+        const double rel = bn[j] > 0 ? r / bn[j] : r;
+        // a NaN or infinite residual (factors broken down, a non-finite Z or B) is the worst
+        // possible one: std::max would drop a NaN and report the column as converged
+        w = std::isfinite(rel) ? std::max(w, rel) : HUGE_VAL;
     }
     *worst = w;
     return 0;

@@ -633,6 +633,15 @@ __device__ inline bool ll_spin(long& spins, int* abort_flag)
     return true;
 }
 
+// Pivot-search key of an entry: |re| + |im| (izamax's measure), with NaN taken as +inf so that a
+// NaN is chosen and published as a pivot like any other entry and spreads through the factors
+// (the refinement then reports it), instead of leaving a column with no published candidate.
+template <typename T> __device__ inline double pivot_key(cx<T> a)
+{
+    const double v = (double)cabs1(a);
+    return v != v ? __builtin_huge_val() : v;
+}
+
 __device__ inline void argmax_merge(double& v, int& i, double ov, int oi)
 {
     if (ov > v || (ov == v && oi < i)) {
@@ -786,7 +795,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                 ll_put(ps.diag(pb) + t, sq, cx_word(current(t / CW, cn - r0), t % CW));
     };
 
-    candidate(own ? (double)cabs1(slab[tid]) : -1.0, own ? grow : 0x7fffffff);
+    candidate(own ? pivot_key(slab[tid]) : -1.0, own ? grow : 0x7fffffff);
     if (G > 1)
         publish(0);
 
@@ -915,7 +924,7 @@ __global__ __launch_bounds__(256) void panel_factor(cx<T>* __restrict__ A, int64
                     a1.im -= l.re * u.im + l.im * u.re;
                     slab[(size_t)(c + 1) * RS + tid] = a1;
                 }
-                v = (double)cabs1(a1);
+                v = pivot_key(a1);
             }
             candidate(v, v >= 0.0 ? grow : 0x7fffffff);
             if (G > 1) {

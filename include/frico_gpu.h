@@ -300,12 +300,18 @@ int frico_adjoint_backward(frico_ctx* ctx, double g_zin_re, double g_zin_im, dou
  *   info[m*nfreq + f] is the solve status.  Meshes run largest first on up to `workers` concurrent
  *   contexts (fewer when ceil(Nmax / R) panel workgroups per context would exceed the CUs).
  *   FRICO_BATCH_GRAPHS replays a captured HIP graph per mesh for the complex64 LU
- *   (FRICO_SOLVE_C64_IR) and each refinement sweep; the other complex64 forms run eagerly, with
- *   their fall-back to the pivoted LU (the worker count is sized for that fall-back's panel).
+ *   (FRICO_SOLVE_C64_IR) and each refinement sweep; FRICO_SOLVE_C64_NP_IR runs eagerly, with its
+ *   fall-back to the pivoted LU (the worker count is sized for that fall-back's panel).  The
+ *   symmetric form is rejected (-5): the batch's quadrature-mode Z is not symmetric, and
+ *   run_context solves symmetrically only approximate-mode or symmetrised matrices
+ *   (src/emw_solver.h:124-128).
  *   FRICO_BATCH_AUTOTUNE picks the LU block width per size class (N / 256) by
  *   timing the candidates on the first system of that class.
  * frico_batch_block_sizes: nb[m] = block width used for mesh m by the last frico_batch_antenna;
  *   returns the number of meshes whose frequencies after the first ran as graph replays.
+ * frico_batch_factored_forms: forms[m*nfreq + f] = the factorisation (FRICO_SOLVE_*) the last
+ *   frico_batch_antenna solved unit (m, f) with, after any fall-back (frico_factored_form of the
+ *   unit's context); returns the number of units.
  */
 #define FRICO_BATCH_GRAPHS 1
 #define FRICO_BATCH_AUTOTUNE 2
@@ -324,6 +330,7 @@ int frico_batch_antenna(frico_batch* batch, const double* freqs, int nfreq, int 
                         double tol, int maxit, int flags, double v_re, double v_im, double* zin,
                         frico_solve_info* info);
 int frico_batch_block_sizes(const frico_batch* batch, int32_t* nb);
+int frico_batch_factored_forms(const frico_batch* batch, int32_t* forms);
 
 #ifdef __cplusplus
 }

@@ -85,7 +85,10 @@ def test_unpivoted_form_in_a_batch():
         for sm in ms:
             b.add_mesh(sm)
         z, info = b.antenna(fr, fill="c128", form="c64_np_ir", tol=1e-12, autotune=False)
+        forms = b.factored_forms(len(fr))
     assert all(i.converged and i.max_rel_residual <= 1e-12 for row in info for i in row)
+    # the unpivoted factors themselves gave the answers (no unit fell back to the pivoted LU)
+    assert np.all(forms == frico_amd.SOLVE_C64_NP_IR), forms
     with frico_amd.Context(0) as ctx:
         for m, sm in enumerate(ms):
             ctx.load_mesh(sm, 5)
@@ -95,20 +98,58 @@ def test_unpivoted_form_in_a_batch():
                 assert abs(z[m, f] - zs) <= 1e-9 * abs(zs), (m, f, z[m, f], zs)
 
 
-def test_complex128_form_in_a_batch():
+@pytest.mark.parametrize("autotune", [False, True])
+def test_complex128_form_in_a_batch(autotune):
+    """The complex128 LU factors Z in place; with autotuning the tuner refills the system before
+    every timed pass and after it, so the answer is the single-context one."""
     import frico_amd
     ms = small_batch()[:3]
     fr = freqs3()[:2]
     with frico_amd.Batch(0, workers=3) as b:
         for sm in ms:
             b.add_mesh(sm)
-        z, info = b.antenna(fr, fill="c128", form="c128", autotune=False)
+        z, info = b.antenna(fr, fill="c128", form="c128", autotune=autotune)
+        assert np.all(b.factored_forms(len(fr)) == frico_amd.SOLVE_C128)
     with frico_amd.Context(0) as ctx:
         for m, sm in enumerate(ms):
             ctx.load_mesh(sm, 5)
             single = ctx.antenna_sweep(fr, sm.feed)
             for f in range(len(fr)):
                 assert abs(z[m, f] - single[f]["impedance"]) <= 1e-12 * abs(single[f]["impedance"])
+
+
+def test_unpivoted_form_in_an_autotuned_batch():
+    """nb = 32 and 64 are both tried on the unpivoted form; the answer does not depend on the pick."""
+    import frico_amd
+    ms = small_batch()[:3]
+    fr = freqs3()[:2]
+    with frico_amd.Batch(0, workers=2) as b:
+        for sm in ms:
+            b.add_mesh(sm)
+        z, info = b.antenna(fr, fill="c128", form="c64_np_ir", tol=1e-12, autotune=True)
+        nb, _ = b.block_sizes()
+        forms = b.factored_forms(len(fr))
+    assert set(nb.tolist()) <= {32, 64}
+    assert all(i.converged and i.max_rel_residual <= 1e-12 for row in info for i in row)
+    assert np.all(forms == frico_amd.SOLVE_C64_NP_IR), forms
+    with frico_amd.Context(0) as ctx:
+        for m, sm in enumerate(ms):
+            ctx.load_mesh(sm, 5)
+            single = ctx.antenna_sweep(fr, sm.feed)
+            for f in range(len(fr)):
+                zs = single[f]["impedance"]
+                assert abs(z[m, f] - zs) <= 1e-9 * abs(zs), (m, f, z[m, f], zs)
+
+
+def test_symmetric_form_is_rejected_in_a_batch():
+    """Batches fill in quadrature mode, whose Z is not symmetric; the L D L^T form would factor a
+    different matrix (run_context solves symmetrically only approximate-mode or symmetrised Z)."""
+    import frico_amd
+    with frico_amd.Batch(0, workers=1) as b:
+        b.add_mesh(small_batch()[0])
+        with pytest.raises(frico_amd.FricoError) as e:
+            b.antenna(freqs3()[:1], fill="c128", form="c64_sym_ir")
+    assert e.value.code == -5
 
 
 def test_uneven_workload_32_meshes_16_frequencies():

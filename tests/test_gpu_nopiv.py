@@ -34,7 +34,7 @@ def exact_lu_system(n, seed, per_row=2, nrhs=4):
     return A, A @ X, X
 
 
-@pytest.mark.parametrize("nb", [64, 128])
+@pytest.mark.parametrize("nb", [32, 64, 128])
 @pytest.mark.parametrize("carried", [True, False])
 def test_unpivoted_lu_is_exact_on_integer_factors(nb, carried):
     import frico_amd
@@ -53,7 +53,8 @@ def test_unpivoted_lu_is_exact_on_integer_factors(nb, carried):
         assert np.array_equal(c.solution(), X)
 
 
-@pytest.mark.parametrize("n,nb", [(1, 128), (63, 64), (200, 64), (517, 128), (1300, 128), (2000, 64)])
+@pytest.mark.parametrize("n,nb", [(1, 128), (63, 64), (200, 64), (517, 128), (1300, 128), (2000, 64),
+                                  (45, 32), (700, 32)])
 def test_unpivoted_lu_solves_dominant_systems(n, nb):
     import frico_amd
     rng = np.random.default_rng(n + 1)
@@ -161,3 +162,26 @@ def test_quadrature_plate_monostatic_rcs(gpu_ctx):
     assert gpu_ctx.factored_form == frico_amd.SOLVE_C64_NP_IR
     assert info[0].converged and info[0].iterations <= 3
     assert relmax(s64, s128) <= 1e-9
+
+
+@pytest.mark.parametrize("form", ["c64_np_ir", "c64_sym_ir", "c64_ir"])
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_non_finite_matrix_is_never_reported_converged(form, bad):
+    """A NaN or infinite entry of Z spreads NaN through the factors and the residual.  The
+    refinement must count such a residual as the worst possible one: the unpivoted forms fall back
+    to the pivoted LU (whose pivot search takes a NaN as an ordinary, largest pivot), and the solve
+    ends in FRICO_ERR_NOCONV instead of reporting convergence."""
+    import frico_amd
+    n = 300
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    A = A + A.T + n * np.eye(n)
+    A[17, 42] = A[42, 17] = bad
+    B = rng.standard_normal((n, 3)) + 1j * rng.standard_normal((n, 3))
+    with frico_amd.Context(0) as c:
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor(form)
+        with pytest.raises(frico_amd.NotConvergedError):
+            c.solve(tol=1e-10, maxit=3)
+        assert c.factored_form == frico_amd.SOLVE_C64_IR

@@ -26,7 +26,8 @@ def sym_system(n, seed, shift, nrhs=3):
     return A, B
 
 
-@pytest.mark.parametrize("n,nb", [(1, 128), (63, 64), (200, 64), (517, 128), (1300, 128), (2000, 64)])
+@pytest.mark.parametrize("n,nb", [(1, 128), (63, 64), (200, 64), (517, 128), (1300, 128), (2000, 64),
+                                  (45, 32), (700, 32)])
 def test_symmetric_form_solves_symmetric_systems(n, nb):
     import frico_amd
     A, B = sym_system(n, n, 4.0)
@@ -60,7 +61,7 @@ def exact_system(n, seed, per_row=2, nrhs=4):
     return A, A @ X, X
 
 
-@pytest.mark.parametrize("nb", [64, 128])
+@pytest.mark.parametrize("nb", [32, 64, 128])
 @pytest.mark.parametrize("carried", [True, False])
 def test_symmetric_factorisation_is_exact_on_integer_factors(nb, carried):
     import frico_amd
