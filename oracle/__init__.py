@@ -72,6 +72,8 @@ def lib():
         L.orc_rcs_unit_Ei.argtypes = [_d, C.c_double, C.c_double, C.c_double, _d]
         L.orc_bistatic.argtypes = [C.c_void_p, C.c_double, _d, _d, C.c_double, _d]
         L.orc_sph2rect.argtypes = [C.c_double, C.c_double, C.c_double, _d]
+        L.orc_plane_wave_field.argtypes = [C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                           C.c_double, C.c_double, _d, C.c_int64, _d]
         _lib = L
     return _lib
 
@@ -247,6 +249,21 @@ def sy_solve(A, B):
         return (B2[:, 0] if np.ndim(B) == 1 else B2), info
     X, info = lapack.zsytrs(ldu, ipiv, B2, lower=0)
     return (X[:, 0] if np.ndim(B) == 1 else X), info
+
+
+def plane_wave_field(freq, theta, phi, eth, eph, pts):
+    """plane_wave's incident field (src/sources.h:34-75) at pts (n x 3): returns E (n x 3 complex)."""
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    E = np.zeros((pts.shape[0], 3), np.complex128)
+    eth, eph = complex(eth), complex(eph)
+    lib().orc_plane_wave_field(freq, theta, phi, eth.real, eth.imag, eph.real, eph.imag, _p(pts), pts.shape[0], _p(E))
+    return E
+
+
+def sph2rect(R, theta, phi):
+    p = np.zeros(3)
+    lib().orc_sph2rect(R, theta, phi, _p(p))
+    return p
 
 
 def rcs_unit_Ei(Es, theta_s, phi_s, R):

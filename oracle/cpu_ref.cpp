@@ -628,9 +628,46 @@ int orc_rhs_deltagap(const orc_mesh* m, double freq, const int32_t* ifaces, int 
 }
 
 /*
- * update_rhs(plane_wave): src/emw_solver.cpp:380-416 with plane_wave src/sources.h:34-75.
- * E(r) = (Eth theta_hat + Eph phi_hat) exp(+j k dir.r), dir pointing from the origin to the
- * source.  V accumulates (+=).
+ * plane_wave (src/sources.h:34-75): E(r) = (Eth theta_hat + Eph phi_hat) exp(+j k dir.r), with dir
+ * the unit vector from the origin toward the source at (theta, phi) (:47-58, 67-74).
+ */
+struct plane_wave_field {
+    vec3 dir;
+    cvec3 E0;
+    cd jk;
+    plane_wave_field(double k, double theta, double phi, cd Eth, cd Eph)
+    {
+        dir = vec3{std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+        vec3 th{std::cos(theta) * std::cos(phi), std::cos(theta) * std::sin(phi), -std::sin(theta)};
+        vec3 ph{-std::sin(phi), std::cos(phi), 0.0};
+        E0 = cvec3{Eth * th.x + Eph * ph.x, Eth * th.y + Eph * ph.y, Eth * th.z + Eph * ph.z};
+        jk = cd{0.0, k};
+    }
+    cvec3 operator()(const vec3& p) const
+    {
+        cd ph_ = std::exp(jk * dot(dir, p));
+        return {E0.x * ph_, E0.y * ph_, E0.z * ph_};
+    }
+};
+
+/* The incident field of plane_wave at npts points (pts npts x 3, E npts x 3 complex). */
+void orc_plane_wave_field(double freq, double theta, double phi, double eth_re, double eth_im, double eph_re,
+                          double eph_im, const double* pts, int64_t npts, double* Eout)
+{
+    const double k = 2.0 * M_PI * freq * std::sqrt(MU0 * EPS0);
+    const plane_wave_field field(k, theta, phi, cd{eth_re, eth_im}, cd{eph_re, eph_im});
+    cd* E = reinterpret_cast<cd*>(Eout);
+    for (int64_t i = 0; i < npts; i++) {
+        const cvec3 e = field(vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]});
+        E[3 * i] = e.x;
+        E[3 * i + 1] = e.y;
+        E[3 * i + 2] = e.z;
+    }
+}
+
+/*
+ * update_rhs(plane_wave): src/emw_solver.cpp:380-416 with the plane_wave field above.
+ * V accumulates (+=).
  */
 int orc_rhs_planewave(orc_mesh* m, double freq, int degree, double theta, double phi, double eth_re,
                       double eth_im, double eph_re, double eph_im, double* Vout)
@@ -641,16 +678,7 @@ int orc_rhs_planewave(orc_mesh* m, double freq, int degree, double theta, double
     cd* V = reinterpret_cast<cd*>(Vout);
     double omega = 2.0 * M_PI * freq;
     double k = omega * std::sqrt(MU0 * EPS0);
-    vec3 dir{std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
-    vec3 th{std::cos(theta) * std::cos(phi), std::cos(theta) * std::sin(phi), -std::sin(theta)};
-    vec3 ph{-std::sin(phi), std::cos(phi), 0.0};
-    cd Eth{eth_re, eth_im}, Eph{eph_re, eph_im};
-    cvec3 E0{Eth * th.x + Eph * ph.x, Eth * th.y + Eph * ph.y, Eth * th.z + Eph * ph.z};
-    cd jk{0.0, k};
-    auto field = [&](const vec3& p) -> cvec3 {
-        cd ph_ = std::exp(jk * dot(dir, p));
-        return {E0.x * ph_, E0.y * ph_, E0.z * ph_};
-    };
+    const plane_wave_field field(k, theta, phi, cd{eth_re, eth_im}, cd{eph_re, eph_im});
     for (const auto& b : m->bf) {
         cd v{0.0, 0.0};
         double wminus = 0.5 * b.length / b.Aminus;
