@@ -172,6 +172,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->augment = std::atoi(au) != 0;
     if (const char* sr = std::getenv("FRICO_STREAM_RESIDUAL"))
         c->stream_residual = std::atoi(sr) != 0;
+    if (const char* gf = std::getenv("FRICO_GEMM_F16"))
+        c->gemm_f16 = std::atoi(gf) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -891,6 +893,24 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     if (form == FRICO_SOLVE_C64_NP_IR) {
         wk.tbuf = c->tbuf.p;
         wk.ntbuf = (int64_t)c->tbuf.n;
+        if (c->gemm_f16) {
+            // L21 of two steps and U12 of two streams' products, padded to 128-row tiles
+            const int64_t acap = (N + 127) / 128 * 128, bcap = (N + c->aug_cols + 127) / 128 * 128;
+            const size_t abytes = split_f16_bytes(acap), bbytes = split_f16_bytes(bcap);
+            if ((rc = c->hsplit.ensure(2 * (abytes + bbytes))))
+                return rc;
+            unsigned char* p = c->hsplit.p;
+            for (int i = 0; i < 2; i++) {
+                wk.ha[i] = reinterpret_cast<_Float16*>(p);
+                wk.hea[i] = reinterpret_cast<int*>(p + (size_t)acap * 4 * HSPLIT_K * 2);
+                p += abytes;
+                wk.hb[i] = reinterpret_cast<_Float16*>(p);
+                wk.heb[i] = reinterpret_cast<int*>(p + (size_t)bcap * 4 * HSPLIT_K * 2);
+                p += bbytes;
+            }
+            wk.hacap = acap;
+            wk.hbcap = bcap;
+        }
     }
     if (!lookahead) {
         wk.side = nullptr;

@@ -209,6 +209,8 @@ struct frico_ctx {
     DevBuf<unsigned char> wv;     // W products of the last factorisation (fused solves)
     DevBuf<unsigned char> ysolve; // n x nrhs scratch of the fused solves
     DevBuf<c64> tbuf;             // transposed block column of the unpivoted LU
+    DevBuf<unsigned char> hsplit; // fp16 split operands of the unpivoted LU's trailing products
+    bool gemm_f16 = true;         // FRICO_GEMM_F16=0: fp32 trailing products in the unpivoted LU
     bool wv_ok = false;           // wv holds the products of the current factors
     bool fused_solve = true;
     // Right-hand sides carried through the complex64 factorisation (FRICO_LU_AUGMENT=0 disables):

@@ -2143,6 +2143,11 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         return set_error("lu (no pivoting): transpose workspace too small"), -1;
     cx<T>* Tb = (cx<T>*)wk.tbuf;
     const bool lookahead = wk.side != nullptr && wk.events != nullptr && wk.nevents >= 2 * npan + 1;
+    // complex64 trailing products on fp16 MFMA with split operands (gemm_f16.hip)
+    constexpr bool c64f = std::is_same<T, float>::value;
+    const bool f16 = c64f && wk.ha[0] && wk.ha[1] && wk.hb[0] && wk.hb[1];
+    if (f16 && (nb > HSPLIT_K || (n + 127) / 128 * 128 > wk.hacap || (n + naug + 127) / 128 * 128 > wk.hbcap))
+        return set_error("lu (no pivoting): fp16 split buffers too small"), -1;
     const int64_t nz = npan * PERM_STRIDE;
     launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)wk.perm, nz);
     launch_kernel(&zero_u32, dim3(1), dim3(256), 0, st, (unsigned*)wk.flags, (int64_t)2);
@@ -2166,6 +2171,16 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
                 }
             }
             launch_kernel(&col_transpose<T, true>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
+            // L21 (held transposed in Tb, k contiguous per row) split once for the step's products
+            if constexpr (c64f) {
+                if (f16) {
+                    const int par = (int)((k / nb) & 1);
+                    int rc = launch_split_f16(reinterpret_cast<const c64*>(Tb), w, m, w, wk.ha[par], wk.hea[par],
+                                              wk.hacap, st);
+                    if (rc)
+                        return rc;
+                }
+            }
         }
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
@@ -2176,9 +2191,24 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         if (nc <= 0)
             return 0;
         int rc = (parts & 1) ? row_block_solve<T>(A, lda, k, w, c0, nc, s) : 0;
-        return rc || !(parts & 2) ? rc
-                                  : gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda,
-                                                A + (k + w) + c0 * lda, lda, s, gemm_timer);
+        if (rc || !(parts & 2))
+            return rc;
+        if constexpr (c64f) {
+            if (f16) {
+                const int64_t m = n - k - w;
+                if (m <= 0)
+                    return 0;
+                // U12 of these columns split on this stream's buffer, then the product with the step's L21
+                const int par = (int)((k / nb) & 1), sb = s == st ? 0 : 1;
+                rc = launch_split_f16(reinterpret_cast<const c64*>(A + k + c0 * lda), lda, nc, w, wk.hb[sb],
+                                      wk.heb[sb], wk.hbcap, s);
+                return rc ? rc
+                          : gemm_sub_f16(m, nc, w, wk.ha[par], wk.hea[par], wk.hacap, wk.hb[sb], wk.heb[sb], wk.hbcap,
+                                         reinterpret_cast<c64*>(A + (k + w) + c0 * lda), lda, s, gemm_timer);
+            }
+        }
+        return gemm_sub<T>(n - k - w, nc, w, A + (k + w) + k * lda, lda, A + k + c0 * lda, lda, A + (k + w) + c0 * lda,
+                           lda, s, gemm_timer);
     };
     auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
     int rc;

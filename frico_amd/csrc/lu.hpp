@@ -30,7 +30,29 @@ struct LuWork {
     // lu_nopiv_factor: nb x n complex values (the transposed block column), or null
     void* tbuf = nullptr;
     int64_t ntbuf = 0;
+    // lu_nopiv_factor (complex64): fp16 split operands of the trailing products (see gemm_f16.hip),
+    // or null for the fp32 products.  ha[parity]: L21 of the steps of that parity (hacap rows);
+    // hb[0] / hb[1]: U12 of the products on the main / side stream (hbcap columns).  Each holds
+    // four planes of cap x HSPLIT_K halves; hea / heb their row / column exponents.
+    _Float16* ha[2] = {nullptr, nullptr};
+    int* hea[2] = {nullptr, nullptr};
+    int64_t hacap = 0;
+    _Float16* hb[2] = {nullptr, nullptr};
+    int* heb[2] = {nullptr, nullptr};
+    int64_t hbcap = 0;
 };
+
+// fp16 split products (gemm_f16.hip).  Rows of a split operand: HSPLIT_K halves each.
+constexpr int HSPLIT_K = 128;
+// Device bytes of a split operand of cap rows (cap a multiple of 128): planes and exponents.
+inline size_t split_f16_bytes(int64_t cap) { return (size_t)cap * (4 * HSPLIT_K * 2 + sizeof(int)); }
+// S[k + r ld], r < R, k < K <= HSPLIT_K -> planes (cap rows each) and exponents of the R rows, padded
+// with zero rows to a multiple of 128.
+int launch_split_f16(const c64* S, int64_t ld, int64_t R, int K, _Float16* planes, int* e, int64_t cap,
+                     hipStream_t st);
+// C[M x N] -= L U, L as split rows (La, ea; acap rows), U as split columns (Ub, eb; bcap columns).
+int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
+                 const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);
 
 // C -= A B.  exact: four real products per complex multiply-add even where three would be used
 // (residuals, whose rounding floor bounds the refinement).
