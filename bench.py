@@ -39,6 +39,10 @@ PEAK_HBM = 8000.0       # GB/s
 # default form, 699.7e6; plate_N12160_c64ir_hbm_traffic.md for the pivoted one, 700.1e6);
 # algorithmic bytes per launch: 419.3e6.
 GEMM_F32_TRAFFIC = {"c64_np_ir": 699.7e6, "c64_ir": 700.1e6}
+# HBM bytes per timed launch of cgemm_sub_f16 at N = 12160, nb = 128 (rocprofv3 FETCH_SIZE / WRITE_SIZE,
+# scaled by a copy kernel of known size); None until measured for the current kernel.
+GEMM_F16_TRAFFIC = None
+PEAK_FP16_MFMA = 2500.0  # TFLOP/s, MI355X dense FP16 matrix
 
 
 def lu_flops(n, nrhs):
@@ -64,27 +68,53 @@ def timed(ctx, form, th, steps, tol, maxit, mode="quad"):
     return (t1 - t0) / steps, infos, prof
 
 
-def cpu_baseline(threads):
-    """Oracle fill (OpenMP restatement of compute_matrix) + LAPACK zgetrf/zgetrs through scipy, on
-    a bounded sample of the same workload family: plate 2 x 2 wavelengths, 32 x 32 squares
-    (N = 3008), 361 plane waves."""
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(threads, n_target):
+    """The reference's CPU path restated: the oracle's OpenMP fill (compute_matrix's loop structure,
+    src/emw_solver.cpp:45-142), its plane-wave RHS block, and LAPACK zgetrf/zgetrs through scipy
+    (solve_general, src/lapack_wrappers.cpp:53-133).  Timed on plates of the headline's mesh density
+    (64 cells per 3 wavelengths) with N ~ 1000, 2000 and 4000 and 361 incidences; t_fill = a N^2,
+    t_factor = b N^3 and t_solve = c N^2 fitted by least squares and extrapolated to the headline's
+    N, whose algorithmic flops divided by that time give the value."""
     import oracle
     import scipy.linalg as sla
     from frico_amd import meshes
     os.environ.setdefault("OMP_NUM_THREADS", str(threads))
-    sm = meshes.plate(2 * meshes.wavelength(FREQ), 32)
+    lam = meshes.wavelength(FREQ)
     th = np.radians(np.arange(NRHS) * 0.25)
-    t0 = time.perf_counter()
-    om = oracle.OracleMesh(sm)
-    Z = om.fill(FREQ, DEGREE)
-    B = np.stack([om.rhs_planewave(FREQ, t, 0.0, 1.0, 0.0, DEGREE) for t in th], axis=1)
-    lu = sla.lu_factor(Z, overwrite_a=True, check_finite=False)
-    sla.lu_solve(lu, B, check_finite=False)
-    t = time.perf_counter() - t0
-    n = om.N
-    return {"value": lu_flops(n, NRHS) / t / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"plate 2x2 lambda 32x32 squares N={n}, {NRHS} RHS: oracle fill (C++ OpenMP) + "
-                      f"scipy LAPACK zgetrf/zgetrs, {t:.2f} s"}
+    rows = []
+    for n in (18, 26, 37):
+        sm = meshes.plate(3.0 * lam * n / 64.0, n)
+        t0 = time.perf_counter()
+        om = oracle.OracleMesh(sm)
+        Z = om.fill(FREQ, DEGREE)
+        B = np.stack([om.rhs_planewave(FREQ, t, 0.0, 1.0, 0.0, DEGREE) for t in th], axis=1)
+        t1 = time.perf_counter()
+        lu = sla.lu_factor(Z, overwrite_a=True, check_finite=False)
+        t2 = time.perf_counter()
+        sla.lu_solve(lu, B, check_finite=False)
+        t3 = time.perf_counter()
+        rows.append((om.N, t1 - t0, t2 - t1, t3 - t2))
+    Ns = np.array([r[0] for r in rows], float)
+    fit = lambda t, p: float(np.dot(Ns ** p, t) / np.dot(Ns ** p, Ns ** p))  # least squares t = c N^p
+    a, b, c = fit([r[1] for r in rows], 2), fit([r[2] for r in rows], 3), fit([r[3] for r in rows], 2)
+    t_target = a * n_target ** 2 + b * n_target ** 3 + c * n_target ** 2
+    meas = ", ".join(f"N={r[0]}: fill {r[1]:.2f} s, zgetrf {r[2]:.2f} s, zgetrs {r[3]:.2f} s" for r in rows)
+    return {"value": lu_flops(n_target, NRHS) / t_target / 1e12, "unit": "TFLOP/s", "cores": threads, "kind": "port",
+            "extrapolated": True, "seconds_at_target": t_target, "cpu": _cpu_model(), "nproc": os.cpu_count(),
+            "lapack": "scipy (bundled OpenBLAS) zgetrf/zgetrs",
+            "sample": f"plates of the headline's mesh density, {NRHS} RHS: oracle fill (C++ OpenMP) + scipy LAPACK; "
+                      f"{meas}; fitted a N^2 + b N^3 + c N^2 extrapolated to N={n_target}: {t_target:.1f} s"}
 
 
 def batch_bench(args):
@@ -280,6 +310,22 @@ def main():
     print(f"# {args.form} {t_ir * 1e3:.2f} ms/step {infos_ir[-1]} (factored form {form_used})", file=sys.stderr,
           flush=True)
 
+    # ---- error against the FP64 oracle's answer (tests/golden fixture), untimed ---------------
+    def oracle_error(form):
+        fx = os.path.join(ROOT, "tests", "golden", "plate64_c3_monostatic.npz")
+        if (args.n, args.nb) != (64, 128) or not os.path.exists(fx):
+            return None
+        g = np.load(fx)
+        sig, inf = ctx.monostatic_sweep([FREQ], g["theta"], 0.0, form=form, tol=args.tol, R=float(g["R"]))
+        X = ctx.solution()
+        return {"sigma_max_rel": float(np.max(np.abs(sig[0] - g["sigma"])) / np.max(np.abs(g["sigma"]))),
+                "solution_sample_max_rel": float(np.max(np.abs(X[g["x_rows"], :] - g["x_vals"])) /
+                                                 np.max(np.abs(g["x_vals"]))),
+                "max_rel_residual": inf[0].max_rel_residual,
+                "fixture": "tests/golden/plate64_c3_monostatic.npz (oracle FP64 fill + LAPACK zgetrf/zgetrs)"}
+    err_ir = oracle_error(args.form)
+    print(f"# {args.form} vs oracle: {err_ir}", file=sys.stderr, flush=True)
+
     # ---- plain complex128 form ---------------------------------------------------------------
     fp64 = None
     if not args.no_fp64:
@@ -291,7 +337,9 @@ def main():
         ctx.fill(FREQ, "quad", "c128")
         res64 = float(np.max(ctx.residual()))
         err = float(np.max(np.abs(X_ir - X_64)) / np.max(np.abs(X_64)))
+        err_64 = oracle_error("c128")
         fp64 = {"ms_per_step": t_64 * 1e3, "tflops": F / t_64 / 1e12, "max_rel_residual": res64,
+                "error_vs_oracle": err_64,
                 "phases_ms": {k: prof_64[k] / args.steps for k in ("fill_ms", "rhs_ms", "factor_ms", "solve_ms")},
                 "gemm_tflops": prof_64["gemm_flops"] / max(prof_64["gemm_ms"], 1e-9) / 1e9,
                 "c64_ir_solution_rel_err_vs_c128": err}
@@ -299,7 +347,23 @@ def main():
               file=sys.stderr, flush=True)
 
     gemm_tf = prof_ir["gemm_flops"] / max(prof_ir["gemm_ms"], 1e-9) / 1e9
+    gemm_gbs = prof_ir["gemm_bytes"] / max(prof_ir["gemm_ms"], 1e-9) / 1e6
     gemm_avg_ms = prof_ir["gemm_ms"] / max(prof_ir["gemm_launches"], 1)
+    f16 = args.form == "c64_np_ir" and os.environ.get("FRICO_GEMM_F16", "1") != "0"
+    if f16:
+        # the trailing products on fp16 MFMA are bound by their C read-modify-write in HBM
+        roofline = {"bound": "hbm", "kernel": "cgemm_sub_f16 (LU trailing update, fp16 MFMA on split operands)",
+                    "achieved": gemm_gbs, "peak": PEAK_HBM, "unit": "GB/s", "frac": gemm_gbs / PEAK_HBM,
+                    "traffic": GEMM_F16_TRAFFIC if (args.n, args.nb) == (64, 128) else None,
+                    "algorithmic_bytes_per_launch": prof_ir["gemm_bytes"] / max(prof_ir["gemm_launches"], 1),
+                    "launches": prof_ir["gemm_launches"], "avg_launch_ms": gemm_avg_ms,
+                    "complex_tflops": gemm_tf, "fp16_mfma_peak_tflops": PEAK_FP16_MFMA}
+    else:
+        roofline = {"bound": "mfma", "kernel": "cgemm_sub<float> (LU trailing update)",
+                    "achieved": gemm_tf, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
+                    "frac": gemm_tf / PEAK_FP32_MFMA,
+                    "traffic": GEMM_F32_TRAFFIC.get(args.form) if (args.n, args.nb) == (64, 128) else None,
+                    "launches": prof_ir["gemm_launches"], "avg_launch_ms": gemm_avg_ms}
     fill_ms = prof_ir["fill_ms"] / args.steps
     out = {
         "metric": "N=12k fill+LU+361-RHS time, TFLOP/s: FP32+IR vs FP64 GPU at 1e-10 residual",
@@ -312,7 +376,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 LU + c128 fill/residual",
+        "dtype": ("c64 LU (trailing products on fp16 MFMA, split operands) + c128 fill/residual" if f16
+                  else "c64 LU + c128 fill/residual"),
         "data": "synthetic (generated plate mesh, analytic plane-wave excitations)",
         "config": {"workload": f"PEC plate 3x3 lambda @300 MHz, {args.n}x{args.n} squares, N={N}, T={T}, "
                                f"{NRHS} plane waves theta 0..90 deg step 0.25, Dunavant degree {DEGREE}",
@@ -322,11 +387,9 @@ def main():
                    "factored_form": {0: "c128", 1: "c64 LU, partial pivoting", 2: "c64 LDL^T",
                                      3: "c64 LU, no pivoting"}.get(form_used, str(form_used)),
                    "tol": args.tol, "block": args.nb},
-        "roofline": {"bound": "mfma", "kernel": "cgemm_sub<float> (LU trailing update)",
-                     "achieved": gemm_tf, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
-                     "frac": gemm_tf / PEAK_FP32_MFMA, "traffic": GEMM_F32_TRAFFIC[args.form] if (args.n, args.nb) == (64, 128) else None,
-                     "launches": prof_ir["gemm_launches"], "avg_launch_ms": gemm_avg_ms},
+        "roofline": roofline,
         "refinement": {"iterations": infos_ir[-1].iterations, "max_rel_residual": infos_ir[-1].max_rel_residual},
+        "error_vs_oracle": err_ir,
         "phases_ms": {k: prof_ir[k] / args.steps for k in ("fill_ms", "rhs_ms", "factor_ms", "solve_ms", "refine_ms")},
         "fill": {"ms": fill_ms, "z_write_GBps": 16.0 * N * N / (fill_ms * 1e-3) / 1e9,
                  "tripair_ms": prof_ir["tripair_ms"] / args.steps,
@@ -336,7 +399,7 @@ def main():
     }
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = cpu_baseline(threads)
+        out["cpu_baseline"] = cpu_baseline(threads, N)
     ctx.close()
     print(json.dumps(out))
 

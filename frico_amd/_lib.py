@@ -38,7 +38,8 @@ class frico_solve_info(C.Structure):
 class frico_profile(C.Structure):
     _fields_ = [("fill_ms", C.c_double), ("rhs_ms", C.c_double), ("factor_ms", C.c_double),
                 ("solve_ms", C.c_double), ("refine_ms", C.c_double), ("gemm_ms", C.c_double),
-                ("gemm_flops", C.c_double), ("gemm_launches", C.c_int64), ("tripair_ms", C.c_double),
+                ("gemm_flops", C.c_double), ("gemm_launches", C.c_int64), ("gemm_bytes", C.c_double),
+                ("tripair_ms", C.c_double),
                 ("tripair_pairs", C.c_double), ("tripair_launches", C.c_int64)]
 
 

@@ -174,6 +174,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->stream_residual = std::atoi(sr) != 0;
     if (const char* gf = std::getenv("FRICO_GEMM_F16"))
         c->gemm_f16 = std::atoi(gf) != 0;
+    if (const char* ds = std::getenv("FRICO_LU_DIRECT_SOLVES"))
+        c->direct_solves = std::atoi(ds) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -241,6 +243,7 @@ int frico_ctx_get_profile(frico_ctx* c, frico_profile* out)
     out->gemm_ms = c->t_gemm.total_ms;
     out->gemm_flops = c->t_gemm.total_work;
     out->gemm_launches = c->t_gemm.launches;
+    out->gemm_bytes = c->t_gemm.total_bytes;
     out->tripair_ms = c->t_tripair.total_ms;
     out->tripair_pairs = c->t_tripair.total_work;
     out->tripair_launches = c->t_tripair.launches;
@@ -893,6 +896,7 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     if (form == FRICO_SOLVE_C64_NP_IR) {
         wk.tbuf = c->tbuf.p;
         wk.ntbuf = (int64_t)c->tbuf.n;
+        wk.direct_solves = c->direct_solves;
         if (c->gemm_f16) {
             // L21 of two steps and U12 of two streams' products, padded to 128-row tiles
             const int64_t acap = (N + 127) / 128 * 128, bcap = (N + c->aug_cols + 127) / 128 * 128;

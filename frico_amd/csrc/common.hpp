@@ -126,9 +126,9 @@ namespace frico {
 struct KernelTimer {
     bool enabled = false;
     std::vector<hipEvent_t> ev;
-    std::vector<double> work;
+    std::vector<double> work, bytes;
     size_t used = 0;
-    double total_ms = 0, total_work = 0;
+    double total_ms = 0, total_work = 0, total_bytes = 0;
     long launches = 0;
 
     void begin(hipStream_t st)
@@ -144,13 +144,16 @@ struct KernelTimer {
         }
         (void)hipEventRecord(ev[used], st);
     }
-    void end(hipStream_t st, double w)
+    // w: algorithmic work of the bracketed launches (flops); b: their algorithmic device bytes
+    void end(hipStream_t st, double w, double b = 0.0)
     {
         if (!enabled || used + 2 > ev.size())
             return;
         (void)hipEventRecord(ev[used + 1], st);
         work.resize(used / 2 + 1);
+        bytes.resize(used / 2 + 1);
         work[used / 2] = w;
+        bytes[used / 2] = b;
         used += 2;
     }
     // Waits for the recorded events and folds them into the totals.
@@ -162,6 +165,7 @@ struct KernelTimer {
             (void)hipEventElapsedTime(&ms, ev[i], ev[i + 1]);
             total_ms += ms;
             total_work += work[i / 2];
+            total_bytes += bytes[i / 2];
             launches++;
         }
         used = 0;
@@ -169,7 +173,7 @@ struct KernelTimer {
     void reset()
     {
         collect();
-        total_ms = total_work = 0;
+        total_ms = total_work = total_bytes = 0;
         launches = 0;
     }
     ~KernelTimer()

@@ -19,6 +19,8 @@
 #include "common.hpp"
 #include "lu.hpp"
 
+#include <cstdlib>
+
 namespace frico {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -79,145 +81,202 @@ __global__ __launch_bounds__(256) void split_f16(const c64* __restrict__ S, int6
 
 // ----------------------------------------------------------------------------------------------
 // C[M x N] -= L U with L given as split rows (La, ea: M rows) and U as split columns (Ub, eb:
-// N columns), K <= 128 (kc = K rounded up to 32).  128 x 128 tile per 256-thread workgroup, 64 x 64
-// per wave (4 x 4 blocks of 16 x 16), 32-deep k chunks through LDS.  The split planes are padded
-// to whole tiles, so the loads need no bounds; only C is masked.
+// N columns), K <= 128 (kc = K rounded up to 32).  Tile BM x BN = 32 RB x 32 CB per 256-thread
+// workgroup, 2 x 2 waves of RB x CB blocks of 16 x 16, 32-deep k chunks through LDS.  The split
+// planes are padded to whole 128-row tiles, so the loads need no bounds; only C is masked.  Small
+// tiles (RB CB <= 4) would load their part of C before the products; the shapes used read it in
+// the epilogue.
 // ----------------------------------------------------------------------------------------------
-constexpr int HB = 128;   // tile rows / columns
-constexpr int HKC = 32;   // k chunk
+constexpr int HKC = 32; // k chunk
 // LDS row of one plane: 32 halves = four 16-byte segments, stored at segment s ^ ((row >> 1) & 3)
 // so that eight consecutive rows read by one quarter-wave hit distinct banks.
 __device__ inline int hseg(int row, int s) { return s ^ ((row >> 1) & 3); }
 
-__global__ __launch_bounds__(256, 2) void cgemm_sub_f16(int64_t M, int64_t N, int kc, const _Float16* __restrict__ La,
-                                                       const int* __restrict__ ea, int64_t acap,
-                                                       const _Float16* __restrict__ Ub, const int* __restrict__ eb,
-                                                       int64_t bcap, c64* __restrict__ C, int64_t ldc)
+// Tile order.  Workgroups go to the eight XCDs in turn (workgroup b to XCD b % 8), each XCD with its
+// own L2.  Workgroup b takes tile s = (b % 8) ceil(T / 8) + b / 8 of a sequence that runs through
+// groups of HG column tiles, all row tiles of a group (HG tiles a row) before the next group: each
+// XCD works through a contiguous stretch of groups, keeping its columns of U in its L2 and reusing
+// each row block of L across the group.
+constexpr int HG = 4;
+__device__ inline bool tile_of(int64_t b, int64_t gx, int64_t gy, int64_t& tm, int64_t& tn)
 {
-    __shared__ half8 sA[4][HB][4]; // [plane][row][segment]
-    __shared__ half8 sB[4][HB][4]; // [plane][column][segment]
+    const int64_t T = gx * gy, per = (T + 7) / 8;
+    const int64_t s = (b % 8) * per + b / 8;
+    if (s >= T)
+        return false;
+    const int64_t full = gy / HG, infull = full * gx * HG;
+    int64_t g, si, width;
+    if (s < infull) {
+        g = s / (gx * HG);
+        si = s % (gx * HG);
+        width = HG;
+    } else {
+        g = full;
+        si = s - infull;
+        width = gy - full * HG;
+    }
+    tm = si / width;
+    tn = g * HG + si % width;
+    return true;
+}
+
+template <int RB, int CB, int OCC>
+__global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, int kc, const _Float16* __restrict__ La,
+                                                         const int* __restrict__ ea, int64_t acap,
+                                                         const _Float16* __restrict__ Ub, const int* __restrict__ eb,
+                                                         int64_t bcap, c64* __restrict__ C, int64_t ldc, int64_t gx,
+                                                         int64_t gy)
+{
+    constexpr int BM = 32 * RB, BN = 32 * CB;
+    constexpr bool PREC = RB * CB <= 4;
+    __shared__ half8 sA[4][BM][4]; // [plane][row][segment]
+    __shared__ half8 sB[4][BN][4]; // [plane][column][segment]
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
-    const int64_t m0 = (int64_t)blockIdx.x * HB, n0 = (int64_t)blockIdx.y * HB;
+    int64_t tm, tn;
+    if (!tile_of(blockIdx.x, gx, gy, tm, tn))
+        return;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
     const int64_t aps = acap * HSPLIT_K, bps = bcap * HSPLIT_K;
+    // lane's rows: m0 + wm 16 RB + rb 16 + (lane & 15); columns: n0 + wn 16 CB + cb 16 + 4 (lane >> 4) + r
+    const int64_t rbase = m0 + wm * 16 * RB + (lane & 15), cbase = n0 + wn * 16 * CB + 4 * (lane >> 4);
 
-    f32x4 are[4][4], aim[4][4]; // [row block][column block]
+    c64 cv[PREC ? RB : 1][PREC ? CB : 1][4];
+    if constexpr (PREC) {
 #pragma unroll
-    for (int a = 0; a < 4; a++)
+        for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
+            for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t row = rbase + rb * 16, col = cbase + cb * 16 + r;
+                    const int64_t ok = row < M && col < N;
+                    cv[rb][cb][r] = C[ok * (row + col * ldc)];
+                }
+    }
+
+    f32x4 are[RB][CB], aim[RB][CB];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < CB; b++) {
             are[a][b] = f32x4{0, 0, 0, 0};
             aim[a][b] = f32x4{0, 0, 0, 0};
         }
 
+    constexpr int SA = BM / 16, SB = BN / 16; // 16-byte segments per thread per chunk
     for (int k0 = 0; k0 < kc; k0 += HKC) {
-        // global -> registers: 2048 16-byte segments of each operand, eight per thread; thread e
-        // takes segment e % 4 of row (e / 4) % 128 of plane e / 512
-        half8 ra[8], rb[8];
+        // global -> registers: thread e takes segment e % 4 of row (e / 4) % BM of plane e / (4 BM)
+        half8 ra[SA], rb_[SB];
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int e = tid + 256 * s, p = e >> 9, row = (e >> 2) & 127, sg = e & 3;
+        for (int s = 0; s < SA; s++) {
+            const int e = tid + 256 * s, p = e / (4 * BM), row = (e >> 2) % BM, sg = e & 3;
             ra[s] = *reinterpret_cast<const half8*>(La + p * aps + (m0 + row) * HSPLIT_K + k0 + 8 * sg);
-            rb[s] = *reinterpret_cast<const half8*>(Ub + p * bps + (n0 + row) * HSPLIT_K + k0 + 8 * sg);
+        }
+#pragma unroll
+        for (int s = 0; s < SB; s++) {
+            const int e = tid + 256 * s, p = e / (4 * BN), row = (e >> 2) % BN, sg = e & 3;
+            rb_[s] = *reinterpret_cast<const half8*>(Ub + p * bps + (n0 + row) * HSPLIT_K + k0 + 8 * sg);
         }
         __syncthreads(); // the previous chunk's fragments have been read
 #pragma unroll
-        for (int s = 0; s < 8; s++) {
-            const int e = tid + 256 * s, p = e >> 9, row = (e >> 2) & 127, sg = e & 3;
+        for (int s = 0; s < SA; s++) {
+            const int e = tid + 256 * s, p = e / (4 * BM), row = (e >> 2) % BM, sg = e & 3;
             sA[p][row][hseg(row, sg)] = ra[s];
-            sB[p][row][hseg(row, sg)] = rb[s];
+        }
+#pragma unroll
+        for (int s = 0; s < SB; s++) {
+            const int e = tid + 256 * s, p = e / (4 * BN), row = (e >> 2) % BN, sg = e & 3;
+            sB[p][row][hseg(row, sg)] = rb_[s];
         }
         __syncthreads();
-        // this wave's four column blocks of U (the MFMA's A operand: U^T, 16 columns x 32 k)
+        // this wave's column blocks of U (the MFMA's A operand: U^T, 16 columns x 32 k)
         const int sgl = lane >> 4;
-        half8 bf[4][4];
+        half8 bf[CB][4];
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++) {
-            const int col = wn * 64 + cb * 16 + (lane & 15);
+        for (int cb = 0; cb < CB; cb++) {
+            const int col = wn * 16 * CB + cb * 16 + (lane & 15);
 #pragma unroll
             for (int p = 0; p < 4; p++)
                 bf[cb][p] = sB[p][col][hseg(col, sgl)];
         }
 #pragma unroll
-        for (int rb = 0; rb < 4; rb++) {
-            const int row = wm * 64 + rb * 16 + (lane & 15);
+        for (int rb = 0; rb < RB; rb++) {
+            const int row = wm * 16 * RB + rb * 16 + (lane & 15);
             half8 af[4];
 #pragma unroll
             for (int p = 0; p < 4; p++)
                 af[p] = sA[p][row][hseg(row, sgl)];
             const half8 nih = -af[2], nil = -af[3];
-            // re += Ur Lr - Ui Li
+            // re += Ur Lr - Ui Li ; im += Ur Li + Ui Lr (hi hi + lo hi + hi lo of each)
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
+            for (int cb = 0; cb < CB; cb++) {
                 are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[0], are[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][1], af[0], are[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[1], are[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], nih, are[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][3], nih, are[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], nil, are[rb][cb], 0, 0, 0);
-            // im += Ur Li + Ui Lr
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
                 aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[2], aim[rb][cb], 0, 0, 0);
+            }
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
+            for (int cb = 0; cb < CB; cb++) {
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][1], af[0], are[rb][cb], 0, 0, 0);
                 aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][1], af[2], aim[rb][cb], 0, 0, 0);
+            }
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
+            for (int cb = 0; cb < CB; cb++) {
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[1], are[rb][cb], 0, 0, 0);
                 aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[3], aim[rb][cb], 0, 0, 0);
+            }
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
+            for (int cb = 0; cb < CB; cb++) {
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], nih, are[rb][cb], 0, 0, 0);
                 aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], af[0], aim[rb][cb], 0, 0, 0);
+            }
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
+            for (int cb = 0; cb < CB; cb++) {
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][3], nih, are[rb][cb], 0, 0, 0);
                 aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][3], af[0], aim[rb][cb], 0, 0, 0);
+            }
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++)
+            for (int cb = 0; cb < CB; cb++) {
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], nil, are[rb][cb], 0, 0, 0);
                 aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], af[1], aim[rb][cb], 0, 0, 0);
+            }
         }
     }
 
     // C -= 2^-(ea + eb) D^T: lane holds row (lane & 15) of each row block and columns
     // 4 (lane >> 4) + r of each column block
-    int ecol[4][4];
+    int ecol[CB][4];
 #pragma unroll
-    for (int cb = 0; cb < 4; cb++)
+    for (int cb = 0; cb < CB; cb++)
 #pragma unroll
         for (int r = 0; r < 4; r++)
-            ecol[cb][r] = eb[n0 + wn * 64 + cb * 16 + 4 * (lane >> 4) + r];
+            ecol[cb][r] = eb[cbase + cb * 16 + r];
 #pragma unroll
-    for (int rb = 0; rb < 4; rb++) {
-        const int64_t row = m0 + wm * 64 + rb * 16 + (lane & 15);
+    for (int rb = 0; rb < RB; rb++) {
+        const int64_t row = rbase + rb * 16;
         if (row >= M)
             continue;
         const int erow = ea[row];
-        c64 cv[4][4];
+        c64 cr[CB][4];
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++)
+        for (int cb = 0; cb < CB; cb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t col = n0 + wn * 64 + cb * 16 + 4 * (lane >> 4) + r;
-                const int64_t ok = col < N;
-                cv[cb][r] = C[ok * (row + col * ldc)];
+                if constexpr (PREC) {
+                    cr[cb][r] = cv[rb][cb][r];
+                } else {
+                    const int64_t col = cbase + cb * 16 + r;
+                    const int64_t ok = col < N;
+                    cr[cb][r] = C[ok * (row + col * ldc)];
+                }
             }
 #pragma unroll
-        for (int cb = 0; cb < 4; cb++)
+        for (int cb = 0; cb < CB; cb++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int64_t col = n0 + wn * 64 + cb * 16 + 4 * (lane >> 4) + r;
+                const int64_t col = cbase + cb * 16 + r;
                 if (col < N) {
                     const int sc = -(erow + ecol[cb][r]);
-                    c64 v = cv[cb][r];
+                    c64 v = cr[cb][r];
                     v.re -= ldexpf(are[rb][cb][r], sc);
                     v.im -= ldexpf(aim[rb][cb][r], sc);
                     C[row + col * ldc] = v;
@@ -233,12 +292,22 @@ int launch_split_f16(const c64* S, int64_t ld, int64_t R, int K, _Float16* plane
         return 0;
     if (K < 1 || K > HSPLIT_K)
         return set_error("fp16 split: K = %d outside 1..%d", K, HSPLIT_K), -1;
-    const int64_t Rp = (R + HB - 1) / HB * HB;
+    const int64_t Rp = (R + 127) / 128 * 128;
     if (Rp > cap)
         return set_error("fp16 split: %lld rows exceed the buffer's %lld", (long long)Rp, (long long)cap), -1;
     launch_kernel(&split_f16, dim3((unsigned)(Rp * 16 / 256)), dim3(256), 0, st, S, ld, R, K, Rp, planes, cap, e);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
+}
+
+// Tile shape of the products: 128 x 128, or 128 x 64 (FRICO_GEMM_F16_TILE=12864 for every product).
+static int f16_tile()
+{
+    static const int t = [] {
+        const char* e = std::getenv("FRICO_GEMM_F16_TILE");
+        return e && std::atoi(e) == 12864 ? 12864 : 128;
+    }();
+    return t;
 }
 
 int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
@@ -248,18 +317,31 @@ int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea,
         return 0;
     if (K > HSPLIT_K)
         return set_error("fp16 product: K = %d above %d", K, HSPLIT_K), -1;
-    const int64_t gx = (M + HB - 1) / HB, gy = (N + HB - 1) / HB;
-    if (gx * HB > acap || gy * HB > bcap)
+    // 128 x 128 tiles for products that fill the GPU several times over, 128 x 64 below
+    int tile = f16_tile();
+    if (tile == 128 && ((M + 127) / 128) * ((N + 127) / 128) < 1024)
+        tile = 12864;
+    const int BM = 128, BN = tile == 128 ? 128 : 64;
+    const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
+    // the split planes are padded to 128-row tiles, which cover any smaller tile
+    if ((M + 127) / 128 * 128 > acap || (N + 127) / 128 * 128 > bcap)
         return set_error("fp16 product: operands not padded to whole tiles"), -1;
-    if (gy > 65535)
-        return set_error("fp16 product: too many column tiles"), -1;
+    const int64_t nblk = (gx * gy + 7) / 8 * 8;
+    if (nblk > 0x7fffffff)
+        return set_error("fp16 product: too many tiles"), -1;
     const int kc = (K + HKC - 1) / HKC * HKC;
     if (kt)
         kt->begin(st);
-    launch_kernel(&cgemm_sub_f16, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb,
-                  bcap, C, ldc);
+    if (tile == 128)
+        launch_kernel(&cgemm_sub_f16<4, 4, 2>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap,
+                      C, ldc, gx, gy);
+    else
+        launch_kernel(&cgemm_sub_f16<4, 2, 3>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap,
+                      C, ldc, gx, gy);
+    // algorithmic bytes: C read and written once (complex64), L and U read once as complex64
     if (kt)
-        kt->end(st, 8.0 * (double)M * (double)N * (double)K);
+        kt->end(st, 8.0 * (double)M * (double)N * (double)K,
+                16.0 * (double)M * (double)N + 8.0 * ((double)M + (double)N) * (double)K);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -1351,7 +1351,8 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
     else
         launch_kernel(&cgemm_sub<T, T>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
     if (kt)
-        kt->end(st, 8.0 * (double)M * (double)N * (double)K);
+        kt->end(st, 8.0 * (double)M * (double)N * (double)K,
+                (double)sizeof(cx<T>) * (2.0 * (double)M * (double)N + ((double)M + (double)N) * (double)K));
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2154,9 +2155,25 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
     launch_kernel(&iota_pivots, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, ipiv, n);
     FRICO_HIP_CHECK(hipGetLastError());
     // diagonal block and block column of step k
+    const bool direct = c64f && wk.direct_solves;
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
         launch_kernel(&lu_diag<T>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
+        if constexpr (c64f) {
+            if (direct) {
+                // L21 = A21 U11^{-1} a row per lane, then its fp16 split straight from A
+                int rc = launch_solve_l21(reinterpret_cast<c64*>(A), lda, k, w, m, st);
+                if (!rc && f16 && m > 0) {
+                    const int par = (int)((k / nb) & 1);
+                    rc = launch_split_f16_cm(reinterpret_cast<const c64*>(A + (k + w) + k * lda), lda, m, w, wk.ha[par],
+                                             wk.hea[par], wk.hacap, st);
+                }
+                if (rc)
+                    return rc;
+                FRICO_HIP_CHECK(hipGetLastError());
+                return 0;
+            }
+        }
         if (m > 0) {
             const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
             launch_kernel(&col_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
@@ -2190,7 +2207,15 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
     auto update = [&](int64_t k, int w, int64_t c0, int64_t nc, hipStream_t s, int parts) -> int {
         if (nc <= 0)
             return 0;
-        int rc = (parts & 1) ? row_block_solve<T>(A, lda, k, w, c0, nc, s) : 0;
+        int rc = 0;
+        if (parts & 1) {
+            if constexpr (c64f) {
+                rc = direct ? launch_solve_u12(reinterpret_cast<c64*>(A), lda, k, w, c0, nc, s)
+                            : row_block_solve<T>(A, lda, k, w, c0, nc, s);
+            } else {
+                rc = row_block_solve<T>(A, lda, k, w, c0, nc, s);
+            }
+        }
         if (rc || !(parts & 2))
             return rc;
         if constexpr (c64f) {

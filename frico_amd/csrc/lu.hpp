@@ -40,6 +40,9 @@ struct LuWork {
     _Float16* hb[2] = {nullptr, nullptr};
     int* heb[2] = {nullptr, nullptr};
     int64_t hbcap = 0;
+    // lu_nopiv_factor (complex64): block-column and block-row solves by the lane-per-row kernels
+    // (false: transposed block column through tbuf and the 64-row triangular-solve kernels)
+    bool direct_solves = true;
 };
 
 // fp16 split products (gemm_f16.hip).  Rows of a split operand: HSPLIT_K halves each.
@@ -50,6 +53,13 @@ inline size_t split_f16_bytes(int64_t cap) { return (size_t)cap * (4 * HSPLIT_K 
 // with zero rows to a multiple of 128.
 int launch_split_f16(const c64* S, int64_t ld, int64_t R, int K, _Float16* planes, int* e, int64_t cap,
                      hipStream_t st);
+// The same split of S[r + k ld] (rows contiguous, as a block column of a matrix lies in memory).
+int launch_split_f16_cm(const c64* S, int64_t ld, int64_t R, int K, _Float16* planes, int* e, int64_t cap,
+                        hipStream_t st);
+// Block solves of the unpivoted complex64 LU after A11 = L11 U11 (w <= 128 columns from k):
+// L21 = A21 U11^{-1} for the m rows below the block, and U12 = L11^{-1} A12 for columns c0 .. c0 + nc.
+int launch_solve_l21(c64* A, int64_t lda, int64_t k, int w, int64_t m, hipStream_t st);
+int launch_solve_u12(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, hipStream_t st);
 // C[M x N] -= L U, L as split rows (La, ea; acap rows), U as split columns (Ub, eb; bcap columns).
 int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
                  const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);

@@ -82,6 +82,7 @@ typedef struct frico_profile {
     double fill_ms, rhs_ms, factor_ms, solve_ms, refine_ms; /* per phase, HIP events */
     double gemm_ms, gemm_flops;  /* trailing-update GEMM launches of the last factorisations */
     int64_t gemm_launches;
+    double gemm_bytes;           /* their algorithmic bytes: C read and written once, operands once */
     double tripair_ms, tripair_pairs; /* triangle-pair kernel of the quadrature fill */
     int64_t tripair_launches;
 } frico_profile;
