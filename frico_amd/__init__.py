@@ -20,7 +20,7 @@ from ._lib import (C64, C128, FILL_APPROX, FILL_QUAD, SOLVE_C64_IR, SOLVE_C64_NP
 __all__ = ["Context", "Batch", "AntennaResponse", "FricoError", "SingularMatrixError", "NotConvergedError", "SolveInfo",
            "meshes", "FILL_QUAD", "FILL_APPROX", "C128", "C64", "SOLVE_C128", "SOLVE_C64_IR",
            "SOLVE_C64_SYM_IR", "SOLVE_C64_NP_IR", "SamplingPlane", "parse_sampling_plane", "sampling_plane_mesh",
-           "read_msh", "format_double", "write_port_sweep", "write_radiation_diagram", "write_bistatic_rcs",
+           "read_msh", "sweep_frequencies", "format_double", "write_port_sweep", "write_radiation_diagram", "write_bistatic_rcs",
            "write_antenna_text_outputs"]
 
 
@@ -121,6 +121,21 @@ def read_msh(path, feed_group=None):
         feed = groups[feed_group][2]
     sm = meshes.SurfaceMesh(V, T, tg, B, bt, name=str(path), feed=feed)
     return sm, groups
+
+
+def sweep_frequencies(start: float, end: float, step: float) -> np.ndarray:
+    """The frequencies of a sweep as init_sweep builds them (src/emw_solver.cpp:460-471): start, then
+    freq += step while freq <= end; the accumulation can drop the end point."""
+    L = _lib.load()
+    n = C.c_int64()
+    rc = L.frico_sweep_frequencies(float(start), float(end), float(step), None, 0, C.byref(n))
+    if rc:
+        raise FricoError(rc, "bad sweep range")
+    f = np.zeros(n.value)
+    rc = L.frico_sweep_frequencies(float(start), float(end), float(step), _p(f), f.size, C.byref(n))
+    if rc:
+        raise FricoError(rc, "bad sweep range")
+    return f
 
 
 def format_double(v: float) -> str:
@@ -376,7 +391,7 @@ class Context:
         port = np.zeros((f.size, 10))
         gain = np.zeros((f.size, 3, 360))
         v = complex(voltage)
-        self._check(self._L.frico_antenna_sweep(self._h, _p(f), f.size, _modes(mode), 0, _p(ifa, C.c_int32),
+        self._check(self._L.frico_antenna_sweep(self._h, _p(f), f.size, _modes(mode), _p(ifa, C.c_int32),
                                                 len(ifa), v.real, v.imag, R, Z0, int(force_symmetry),
                                                 int(force_reorient), _form(form), tol, maxit, _p(port), _p(gain)))
         out = []

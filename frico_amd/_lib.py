@@ -82,7 +82,8 @@ SIGNATURES = {
     "frico_fields": (C.c_int, [_vp, C.c_double, _d, C.c_int64, _i64, C.c_double, _d, _d]),
     "frico_fields_noalloc": (C.c_int, [_vp, C.c_double, _vp, C.c_int64, C.c_int64, _vp, C.c_int64, _vp,
                                        C.c_double, _vp, _vp]),
-    "frico_antenna_sweep": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_int, _i32, C.c_int, C.c_double,
+    "frico_sweep_frequencies": (C.c_int, [C.c_double, C.c_double, C.c_double, _d, C.c_int64, _i64]),
+    "frico_antenna_sweep": (C.c_int, [_vp, _d, C.c_int, C.c_int, _i32, C.c_int, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                       C.c_double, C.c_int, _d, _d]),
     "frico_bistatic_sweep": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,

@@ -17,7 +17,7 @@ ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 # Packed FP32 instructions are disabled on purpose (see the build notes).
-FLAGS = ["-O3", "-std=c++20", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
+FLAGS = ["-O3", "-std=c++23", "-fPIC", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include"),
          "-I", CSRC, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-result", "-Wno-unused-command-line-argument"]
 
@@ -30,6 +30,24 @@ def _compile(src: str) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError(f"hipcc failed on {src}")
     return out
+
+
+CAPI_SRC = os.path.join(ROOT, "tests", "capi", "run_context.cpp")
+CAPI_BIN = os.path.join(ROOT, "tests", "capi", "run_context")
+
+
+def build_capi_caller(verbose: bool = False) -> str:
+    """The compiled C++ caller of the C ABI (tests/capi/run_context.cpp), with g++ against
+    include/frico_gpu.h and the in-tree libfrico_hip.so."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", os.path.join(ROOT, "include"), CAPI_SRC, "-L", HERE,
+           "-lfrico_hip", "-Wl,-rpath,$ORIGIN/../../frico_amd", "-o", CAPI_BIN]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed on tests/capi/run_context.cpp")
+    if verbose:
+        print("built", CAPI_BIN)
+    return CAPI_BIN
 
 
 def build(verbose: bool = False) -> str:
@@ -45,6 +63,7 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_capi_caller(verbose)
     return LIB
 
 

@@ -166,7 +166,27 @@ using namespace frico::maxwell;
 
 extern "C" {
 
-int frico_antenna_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, int, const int32_t* ifaces, int nif,
+int frico_sweep_frequencies(double start, double end, double step, double* freqs, int64_t max_freqs, int64_t* count)
+{
+    if (!count)
+        return -6;
+    if (!(step > 0))
+        return -3;
+    // init_sweep (src/emw_solver.cpp:460-471): the frequency is accumulated, freq += step, and the
+    // loop runs while freq <= end, so rounding can drop the last point of a sweep
+    int64_t n = 0;
+    for (double freq = start; freq <= end; freq += step) {
+        if (freqs && n < max_freqs)
+            freqs[n] = freq;
+        n++;
+        if (n > (int64_t)1 << 30)
+            return FRICO_ERR_LIMIT;
+    }
+    *count = n;
+    return (freqs && n > max_freqs) ? -5 : 0;
+}
+
+int frico_antenna_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, const int32_t* ifaces, int nif,
                         double v_re, double v_im, double R, double Z0, int force_symmetry, int force_reorient, int form,
                         double tol, int maxit, double* port_out, double* gain_out)
 {

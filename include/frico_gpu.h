@@ -244,6 +244,15 @@ int frico_write_bistatic_rcs(const char* path, const double* table);
 int frico_format_double(double v, char* buf, int64_t size);
 
 /*
+ * Frequency points of a sweep as init_sweep builds them (src/emw_solver.cpp:460-471): start, then
+ * freq += step while freq <= end (accumulated, so rounding can drop the end point).  *count
+ * receives the number of points; freqs (max_freqs entries, may be NULL) receives them.  Returns -5
+ * if freqs is too short, -3 for a non-positive step.  The sweeps below take such lists.
+ */
+int frico_sweep_frequencies(double start, double end, double step, double* freqs, int64_t max_freqs,
+                            int64_t* count);
+
+/*
  * Whole analyses, mirroring the reference entry points for one frequency each.
  *
  * frico_antenna_sweep: do_sweep(simulation&, antenna_analysis&) (src/emw_ana_antenna.cpp:236-295)
@@ -260,7 +269,7 @@ int frico_format_double(double v, char* buf, int64_t size);
  *   (src/emw_solver.h:124-128), approximate-mode fills (and antenna sweeps with force_symmetry) are
  *   solved as symmetric systems: FRICO_SOLVE_C64_IR becomes FRICO_SOLVE_C64_SYM_IR there.
  */
-int frico_antenna_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode, int degree_unused,
+int frico_antenna_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int mode,
                         const int32_t* ifaces, int nif, double v_re, double v_im, double R,
                         double Z0, int force_symmetry, int force_reorient, int form, double tol,
                         int maxit, double* port_out, double* gain_out);

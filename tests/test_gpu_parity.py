@@ -62,6 +62,19 @@ def test_fill_quad_parity(gpu_ctx, name, degree):
     assert relmax(Zg, Zo) <= 1e-10
 
 
+@pytest.mark.parametrize("degree", list(range(0, 21)))
+def test_fill_quad_parity_every_dunavant_degree(gpu_ctx, degree):
+    """Every rule of src/quadratures.cpp:69-317 through the GPU fill on a tiny plate (3 x 3 cells,
+    N = 21): rule 3's negative weight, the points outside the triangle of rules 11, 15, 16, 18 and
+    20, and the sub-triangle rule with each of them."""
+    sm = meshes.plate(0.7 * meshes.wavelength(F0), 3)
+    print(f"fill quad plate3 degree {degree}", flush=True)
+    Zo = oracle.OracleMesh(sm).fill(F0, degree)
+    gpu_ctx.load_mesh(sm, degree)
+    gpu_ctx.fill(F0, "quad", "c128")
+    assert relmax(gpu_ctx.matrix(), Zo) <= 1e-10
+
+
 @pytest.mark.parametrize("name", list(small_meshes()))
 def test_fill_approx_parity(gpu_ctx, name):
     sm = small_meshes()[name]

@@ -118,3 +118,21 @@ def test_bistatic_rcs_file(tmp_path):
 def test_write_errors(tmp_path):
     with pytest.raises(fa.FricoError):
         fa.write_bistatic_rcs(tmp_path / "missing_dir" / "x.txt", np.zeros((359, 4)))
+
+
+def test_sweep_frequencies_accumulate_like_init_sweep():
+    """init_sweep (src/emw_solver.cpp:460-471) accumulates freq += step while freq <= end.  With a
+    step of 0.1 from 1 to 2 the eleventh value is 2.0000000000000004 > 2, so the sweep has ten
+    points, not eleven; with a step that is a power of two the end point stays."""
+    import frico_amd
+    f = frico_amd.sweep_frequencies(1.0, 2.0, 0.1)
+    assert f.size == 10
+    assert f[0] == 1.0 and f[1] == 1.1 and f[2] == 1.2000000000000002
+    assert 1.0 + 10 * 0.1 == 2.0 and f[-1] < 2.0 - 0.05  # a multiplied sweep would keep 2.0
+    g = frico_amd.sweep_frequencies(100e6, 500e6, 100e6)
+    assert g.tolist() == [100e6, 200e6, 300e6, 400e6, 500e6]
+    h = frico_amd.sweep_frequencies(1.0, 2.0, 0.25)
+    assert h.tolist() == [1.0, 1.25, 1.5, 1.75, 2.0]
+    assert frico_amd.sweep_frequencies(3.0, 2.0, 0.5).size == 0
+    with pytest.raises(frico_amd.FricoError):
+        frico_amd.sweep_frequencies(1.0, 2.0, 0.0)
