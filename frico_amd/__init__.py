@@ -438,6 +438,32 @@ class Context:
         self._check(self._L.frico_adjoint_backward(self._h, float(g_zin_re), float(g_zin_im), float(g_gain), _p(g)))
         return g.reshape(-1, 3)
 
+    def rcs_forward(self, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, R=10.0):
+        """Quadrature-mode RCS kept for a backward pass: (sigma_VV, sigma_HH) at (R, theta_s, phi_s)
+        for the plane wave from (theta_i, phi_i); monostatic when theta_s, phi_s are omitted."""
+        th = complex(eth)
+        ph_ = complex(eph)
+        a = np.array([th.real, th.imag])
+        b = np.array([ph_.real, ph_.imag])
+        out = np.zeros(2)
+        ts = theta_i if theta_s is None else theta_s
+        ps = phi_i if phi_s is None else phi_s
+        self._check(self._L.frico_adjoint_rcs_forward(self._h, float(freq), float(theta_i), float(phi_i), _p(a), _p(b),
+                                                      float(ts), float(ps), float(R), _p(out)))
+        return float(out[0]), float(out[1])
+
+    def rcs_backward(self, g_vv=0.0, g_hh=0.0):
+        """Vertex-coordinate gradient of g_vv sigma_VV + g_hh sigma_HH, shape (nv, 3)."""
+        g = np.zeros(self.mesh.nv * 3)
+        self._check(self._L.frico_adjoint_rcs_backward(self._h, float(g_vv), float(g_hh), _p(g)))
+        return g.reshape(-1, 3)
+
+    def rcs_gradients(self, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, R=10.0):
+        """sigma_VV, sigma_HH and their vertex gradients (each (nv, 3))."""
+        s = self.rcs_forward(freq, theta_i, phi_i, eth, eph, theta_s, phi_s, R)
+        return {"sigma": np.array(s), "dsigma_vv": self.rcs_backward(1.0, 0.0),
+                "dsigma_hh": self.rcs_backward(0.0, 1.0)}
+
     def antenna_gradients(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0)):
         """Zin, G and their gradients: dZin (complex, (nv,3)) and dG (real, (nv,3))."""
         zin, gain = self.adjoint_forward(freq, ifaces, voltage, obs)

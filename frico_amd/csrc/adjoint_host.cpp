@@ -1,4 +1,5 @@
-// Adjoint forward/backward pair of the antenna analysis (no counterpart in the reference).
+// Adjoint forward/backward pairs of the antenna analysis and of the RCS (no counterpart in the
+// reference).
 //
 // forward:  Z(x) I = V(x) with the delta-gap source, Zin = V / I_port, I_port = sum_gap l s I,
 //           G = 4 pi |p|^2 |S| / P at the observation point p, S = E x conj(H) / 2,
@@ -101,6 +102,106 @@ void field_vjp(const HostMesh& hm, double freq, const std::vector<cd>& I, const 
         for (int kk = 0; kk < 3; kk++)
             for (int d = 0; d < 3; d++)
                 gx[3 * hm.T[t][kk] + d] += gc[3 * t + d] / 3.0;
+}
+
+// Reference-triangle Dunavant rule: (beta_0, beta_1, beta_2, reference weight) of each point, in the
+// expansion order of dunavant() (src/quadratures.cpp:321-372); the point is sum_k beta_k v_k and its
+// weight the reference weight times the triangle's area.
+void rule_ref(int degree, std::vector<std::array<double, 4>>& out)
+{
+    out.clear();
+    std::vector<std::array<double, 3>> bary;
+    dunavant_bary(degree, bary);
+    const int rule = degree == 0 ? 0 : degree - 1;
+    size_t idx = 0;
+    for (int o = frico_dunavant_orbit_begin[rule]; o < frico_dunavant_orbit_begin[rule + 1]; o++)
+        for (int s = 0; s < frico_dunavant_orbits[o].orbit; s++, idx++)
+            out.push_back({bary[idx][0], bary[idx][1], bary[idx][2], frico_dunavant_orbits[o].w});
+}
+
+// Gradient of the area |(b - a) x (c - a)| / 2 with respect to a, b and c.
+void area_grad(vec3 a, vec3 b, vec3 c, vec3 g[3])
+{
+    const vec3 n = cross(b - a, c - a);
+    const double nn = norm(n);
+    if (nn == 0) {
+        g[0] = g[1] = g[2] = vec3{0, 0, 0};
+        return;
+    }
+    const vec3 u = n / nn;
+    g[0] = cross(u, c - b) * 0.5;
+    g[1] = cross(u, a - c) * 0.5;
+    g[2] = cross(u, b - a) * 0.5;
+}
+
+// g (3 nv) += sum_n W_n dV_n/dx for the plane-wave moments (update_rhs(plane_wave),
+// src/emw_solver.cpp:380-416): V_n = sum over T-, T+ of -+ (l / 2A) sum_p w_p (r_p - p_T) . E(r_p)
+// / (-j omega mu0), E(r) = E0 exp(+j k dir . r).  Every vertex dependence is differentiated: l,
+// A, the weights w_p = w_ref area, the points r_p = sum_k beta_pk v_k, the free vertex p_T and the
+// field's phase (dE/dr = j k dir E^T).
+void planewave_vjp(const HostMesh& hm, double freq, double theta, double phi, cd eth, cd eph, const std::vector<cd>& W,
+                   std::vector<cd>& g)
+{
+    const double omega = 2.0 * PI * freq, k = omega * std::sqrt(MU0 * EPS0);
+    const vec3 dir{std::sin(theta) * std::cos(phi), std::sin(theta) * std::sin(phi), std::cos(theta)};
+    const vec3 th{std::cos(theta) * std::cos(phi), std::cos(theta) * std::sin(phi), -std::sin(theta)};
+    const vec3 ph{-std::sin(phi), std::cos(phi), 0.0};
+    const cd E0[3] = {eth * th.x + eph * ph.x, eth * th.y + eph * ph.y, eth * th.z + eph * ph.z};
+    const cd jk{0.0, k}, c0 = 1.0 / cd{0.0, -omega * MU0};
+    const int evmap[3] = {2, 0, 1};
+    std::vector<std::array<double, 4>> rule;
+    rule_ref(hm.degree, rule);
+    auto addv = [&](int64_t v, vec3 d, cd s) {
+        g[3 * v] += d.x * s;
+        g[3 * v + 1] += d.y * s;
+        g[3 * v + 2] += d.z * s;
+    };
+    for (size_t n = 0; n < hm.bf.size(); n++) {
+        const frico_rwg& b = hm.bf[n];
+        if (W[n] == cd(0))
+            continue;
+        for (int side = 0; side < 2; side++) {
+            const int64_t t = side == 0 ? b.itminus : b.itplus;
+            const double sgn = side == 0 ? -1.0 : 1.0, A = side == 0 ? b.Aminus : b.Aplus;
+            const int le = side == 0 ? b.loc_eminus : b.loc_eplus;
+            const int64_t fv = hm.T[t][evmap[le]];
+            const vec3 P[3] = {hm.V[hm.T[t][0]], hm.V[hm.T[t][1]], hm.V[hm.T[t][2]]};
+            const double area = kahan_area(P[0], P[1], P[2]);
+            vec3 ga[3];
+            area_grad(P[0], P[1], P[2], ga);
+            const vec3 pf = P[evmap[le]];
+            cd F = 0;
+            cd dF[3][3] = {}, dFree[3] = {};
+            for (const auto& r : rule) {
+                const vec3 rp = P[0] * r[0] + P[1] * r[1] + P[2] * r[2];
+                const double w = r[3] * area;
+                const cd e = std::exp(jk * dot(dir, rp));
+                const cd E[3] = {E0[0] * e, E0[1] * e, E0[2] * e};
+                const vec3 rho = rp - pf;
+                const cd rE = rho.x * E[0] + rho.y * E[1] + rho.z * E[2];
+                F += w * rE;
+                const double dd[3] = {dir.x, dir.y, dir.z};
+                for (int kk = 0; kk < 3; kk++) {
+                    const double gw[3] = {ga[kk].x * r[3], ga[kk].y * r[3], ga[kk].z * r[3]};
+                    for (int d = 0; d < 3; d++)
+                        dF[kk][d] += gw[d] * rE + w * r[kk] * (E[d] + rE * jk * dd[d]);
+                }
+                for (int d = 0; d < 3; d++)
+                    dFree[d] -= w * E[d];
+            }
+            const cd pre = W[n] * sgn * c0;
+            const double lA = b.length / (2.0 * A);
+            // d(l / 2A) F = (dl / 2A - l dA / 2A^2) F (A = |e0 x e1| / 2, as the moments use)
+            add_length_grad(hm, b.edge_index, b.length, pre * F / (2.0 * A), g);
+            for (int kk = 0; kk < 3; kk++)
+                addv(hm.T[t][kk], ga[kk], -pre * F * b.length / (2.0 * A * A));
+            for (int kk = 0; kk < 3; kk++)
+                for (int d = 0; d < 3; d++)
+                    g[3 * hm.T[t][kk] + d] += pre * lA * dF[kk][d];
+            for (int d = 0; d < 3; d++)
+                g[3 * fv + d] += pre * lA * dFree[d];
+        }
+    }
 }
 
 int compute_gradients(frico_ctx* c)
@@ -261,6 +362,122 @@ int frico_adjoint_forward(frico_ctx* c, double freq, const int32_t* ifaces, int 
     out[1] = A.zin.imag();
     out[2] = A.gain;
     A.forward_done = true;
+    return 0;
+}
+
+int frico_adjoint_rcs_forward(frico_ctx* c, double freq, double theta_i, double phi_i, const double* eth,
+                              const double* eph, double theta_s, double phi_s, double R, double* out)
+{
+    if (!c)
+        return set_error("null context"), -1;
+    if (!c->have_mesh)
+        return set_error("no mesh loaded"), FRICO_ERR_STATE;
+    if (!eth || !eph)
+        return set_error("null polarisation amplitudes"), -5;
+    if (!(R > 0))
+        return set_error("observation distance must be positive"), -9;
+    if (!out)
+        return set_error("null output"), -10;
+    const int q = c->hm.q;
+    if (q != 1 && q != 3 && q != 6 && q != 7)
+        return set_error("adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
+    auto& A = c->adj;
+    A = frico_ctx::AdjointState{};
+    const int64_t N = c->N;
+    int rc;
+    if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, FRICO_C128)) ||
+        (rc = frico_rhs_planewave(c, freq, 1, &theta_i, &phi_i, eth, eph)))
+        return rc;
+    A.V.resize((size_t)N);
+    if ((rc = frico_rhs_get(c, reinterpret_cast<double*>(A.V.data()), N)))
+        return rc;
+    if ((rc = frico_factor(c, FRICO_SOLVE_C128)) || (rc = frico_solve(c, 0.0, 1, nullptr)))
+        return rc;
+    A.I.resize((size_t)N);
+    if ((rc = frico_solution_get(c, reinterpret_cast<double*>(A.I.data()), N)))
+        return rc;
+    A.freq = freq;
+    A.theta_i = theta_i;
+    A.phi_i = phi_i;
+    A.theta_s = theta_s;
+    A.phi_s = phi_s;
+    A.R = R;
+    A.eth = cd(eth[0], eth[1]);
+    A.eph = cd(eph[0], eph[1]);
+    // the observation point sph2rect(R, theta_s, phi_s) (src/utils.h:53-60)
+    A.obs[0] = R * std::cos(phi_s) * std::sin(theta_s);
+    A.obs[1] = R * std::sin(phi_s) * std::sin(theta_s);
+    A.obs[2] = R * std::cos(theta_s);
+    A.E.resize(3);
+    A.H.resize(3);
+    if ((rc = frico_fields(c, freq, A.obs, 1, nullptr, 1.0, reinterpret_cast<double*>(A.E.data()),
+                           reinterpret_cast<double*>(A.H.data()))))
+        return rc;
+    // bistatic_RCS_unit_Ei (src/emw_ana_bistatic_rcs.cpp:62-82)
+    const double th[3] = {std::cos(theta_s) * std::cos(phi_s), std::cos(theta_s) * std::sin(phi_s), -std::sin(theta_s)};
+    const double ph[3] = {-std::sin(phi_s), std::cos(phi_s), 0.0};
+    A.a_vv = th[0] * A.E[0] + th[1] * A.E[1] + th[2] * A.E[2];
+    A.a_hh = ph[0] * A.E[0] + ph[1] * A.E[1] + ph[2] * A.E[2];
+    out[0] = 4.0 * PI * R * R * std::norm(A.a_vv);
+    out[1] = 4.0 * PI * R * R * std::norm(A.a_hh);
+    A.rcs_done = true;
+    return 0;
+}
+
+int frico_adjoint_rcs_backward(frico_ctx* c, double g_vv, double g_hh, double* grad)
+{
+    if (!c)
+        return set_error("null context"), -1;
+    if (!grad)
+        return set_error("null output"), -4;
+    auto& A = c->adj;
+    if (!A.rcs_done)
+        return set_error("backward before forward"), FRICO_ERR_STATE;
+    if (!c->have_x || c->factored != FRICO_SOLVE_C128)
+        return set_error("the forward factors were replaced: run forward again"), FRICO_ERR_STATE;
+    const HostMesh& hm = c->hm;
+    const int64_t N = c->N, nv = hm.nv;
+    hipStream_t st = c->st;
+    int rc;
+    // d f / d E (Wirtinger) of f = g_vv sigma_VV + g_hh sigma_HH, sigma = 4 pi R^2 |component|^2
+    const double th[3] = {std::cos(A.theta_s) * std::cos(A.phi_s), std::cos(A.theta_s) * std::sin(A.phi_s),
+                          -std::sin(A.theta_s)};
+    const double ph[3] = {-std::sin(A.phi_s), std::cos(A.phi_s), 0.0};
+    const double s4 = 4.0 * PI * A.R * A.R;
+    cd eE[3], eH[3] = {0, 0, 0};
+    for (int d = 0; d < 3; d++)
+        eE[d] = s4 * (g_vv * std::conj(A.a_vv) * th[d] + g_hh * std::conj(A.a_hh) * ph[d]);
+    std::vector<cd> gx, gI;
+    field_vjp(hm, A.freq, A.I, A.obs, eE, eH, gx, gI);
+    // Z^T mu = gI on the complex128 factors
+    if ((rc = c->adjW.ensure((size_t)N)) || (rc = c->aw.ensure((size_t)(3 * hm.nt))) ||
+        (rc = c->bw.ensure((size_t)(3 * hm.nt))) || (rc = c->d_gvert.ensure((size_t)(6 * nv))))
+        return rc;
+    FRICO_HIP_CHECK(hipMemcpyAsync(c->adjW.p, gI.data(), (size_t)N * sizeof(c128), hipMemcpyHostToDevice, st));
+    if ((rc = lu_solve_transposed<double>(c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->adjW.p, N, 1, st)))
+        return rc;
+    std::vector<cd> mu((size_t)N);
+    FRICO_HIP_CHECK(hipMemcpyAsync(mu.data(), c->adjW.p, (size_t)N * sizeof(c128), hipMemcpyDeviceToHost, st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(st));
+    // mu^T dZ I: the device contraction at fixed edge lengths, then the l_i l_j prefactor
+    const double omega = 2.0 * PI * A.freq;
+    const double k = omega * std::sqrt(MU0 * EPS0), inv_ksq = 1. / (omega * omega * MU0 * EPS0);
+    if ((rc = launch_adjoint_contract(c->dm, c->am, k, inv_ksq, c->d_near.p, (int64_t)hm.near.size(), c->X.p, c->adjW.p,
+                                      c->aw.p, c->bw.p, c->d_gvert.p, nv, st)))
+        return rc;
+    std::vector<cd> C((size_t)(3 * nv));
+    FRICO_HIP_CHECK(hipMemcpyAsync(C.data(), c->d_gvert.p, (size_t)(6 * nv) * sizeof(double), hipMemcpyDeviceToHost, st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(st));
+    for (int64_t n = 0; n < N; n++) {
+        const double l = hm.bf[n].length;
+        if (l != 0)
+            add_length_grad(hm, hm.bf[n].edge_index, l, (A.I[n] * gI[n] + mu[n] * A.V[n]) / l, C);
+    }
+    // mu^T dV of the plane-wave moments
+    std::vector<cd> dV((size_t)(3 * nv), cd(0));
+    planewave_vjp(hm, A.freq, A.theta_i, A.phi_i, A.eth, A.eph, mu, dV);
+    for (int64_t x = 0; x < 3 * nv; x++)
+        grad[x] = 2.0 * (gx[x] + dV[x] - C[x]).real();
     return 0;
 }
 

@@ -256,6 +256,10 @@ struct frico_ctx {
         double absS = 0, P = 0, gain = 0;
         std::vector<std::complex<double>> dzin; // 3 nv
         std::vector<double> dgain;              // 3 nv
+        // RCS pair (frico_adjoint_rcs_*): the plane wave, the observation and the two components
+        bool rcs_done = false;
+        double theta_i = 0, phi_i = 0, theta_s = 0, phi_s = 0, R = 0;
+        std::complex<double> eth = 0, eph = 0, a_vv = 0, a_hh = 0;
     } adj;
 };
 

@@ -297,6 +297,23 @@ int frico_adjoint_backward(frico_ctx* ctx, double g_zin_re, double g_zin_im, dou
                            double* grad);
 
 /*
+ * RCS adjoint pair (no counterpart in the reference; it differentiates bistatic_RCS_unit_Ei,
+ * src/emw_ana_bistatic_rcs.cpp:62-82, of the currents of update_rhs(plane_wave),
+ * src/emw_solver.cpp:380-416, through compute_matrix, src/emw_solver.cpp:45-142).
+ * frico_adjoint_rcs_forward: quadrature-mode fill at `freq` (Dunavant degree 1, 2, 4 or 5), one plane
+ *   wave from (theta_i, phi_i) with complex amplitudes eth[2], eph[2], complex128 LU and solve;
+ *   out[0] = sigma_VV, out[1] = sigma_HH at sph2rect(R, theta_s, phi_s) (monostatic: theta_s =
+ *   theta_i, phi_s = phi_i).
+ * frico_adjoint_rcs_backward: grad[3*v + c] = g_vv dsigma_VV/dx + g_hh dsigma_HH/dx for every vertex
+ *   coordinate, from one solve with Z^T on the forward factors, the device contraction of dZ/dx
+ *   with the forward and adjoint currents, and the vertex derivatives of the plane-wave moments and
+ *   of the field sums.
+ */
+int frico_adjoint_rcs_forward(frico_ctx* ctx, double freq, double theta_i, double phi_i, const double* eth,
+                              const double* eph, double theta_s, double phi_s, double R, double* out);
+int frico_adjoint_rcs_backward(frico_ctx* ctx, double g_vv, double g_hh, double* grad);
+
+/*
  * Batched uneven antenna workloads (the reference runs one mesh and one frequency at a time:
  * do_sweep, src/emw_ana_antenna.cpp:236-295, over init_sweep's frequencies,
  * src/emw_solver.cpp:460-471).  A batch owns `workers` contexts on one device; meshes are added

@@ -310,6 +310,8 @@ def bistatic_analysis(sm, freq, theta, phi, eth=1.0, eph=1.0, degree=5, approx=F
 def _adj_sigs():
     L = lib()
     L.orc_adjoint_contract.argtypes = [C.c_void_p, C.c_double, C.c_int, _d, _d, _d]
+    L.orc_rhs_planewave_vjp.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, _d, _d]
     L.orc_fields_vjp.argtypes = [C.c_void_p, C.c_double, _d, _d, _d, _d, _d, _d]
     L.orc_length_grad.argtypes = [C.c_void_p, C.c_int64, _d]
     return L
@@ -386,3 +388,54 @@ def antenna_gradients(sm, freq, degree=5, obs=(0.0, 0.0, 10.0), voltage=1.0):
     dG = 4 * math.pi * float(obs @ obs) / P * dabsS - G / P * dP
     return {"zin": zin, "gain": G, "dzin": dzin.reshape(-1, 3), "dgain": dG.reshape(-1, 3), "I": I,
             "lam": lam, "mu": mu, "N": m.N}
+
+
+def rhs_planewave_vjp(om, freq, degree, theta, phi, eth, eph, W):
+    """g[v, c] = sum_n W_n dV_n/dx_{v,c} (complex) for the plane-wave moments of rhs_planewave."""
+    L = _adj_sigs()
+    W = np.ascontiguousarray(W, np.complex128)
+    g = np.zeros(om.sm.nv * 3, np.complex128)
+    eth, eph = complex(eth), complex(eph)
+    rc = L.orc_rhs_planewave_vjp(om.h, freq, degree, theta, phi, eth.real, eth.imag, eph.real, eph.imag, _p(W), _p(g))
+    if rc:
+        raise ValueError("bad degree")
+    return g.reshape(-1, 3)
+
+
+def rcs_gradients(sm, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, degree=5, R=10.0):
+    """FP64 CPU reference of the RCS adjoint: sigma_VV and sigma_HH at (R, theta_s, phi_s) for the
+    plane wave from (theta_i, phi_i) (bistatic_RCS_unit_Ei, src/emw_ana_bistatic_rcs.cpp:62-82, of
+    the currents of update_rhs(plane_wave), src/emw_solver.cpp:380-416, and compute_matrix), with
+    their gradients with respect to every vertex coordinate.  Monostatic when theta_s, phi_s are
+    omitted.
+
+    sigma = 4 pi R^2 |a|^2 with a the theta (or phi) component of E_s; with Z^T mu = dE-cotangent
+    pulled back to the currents, d sigma = 2 Re(g_x + mu^T dV - mu^T dZ I), g_x the explicit
+    vertex dependence of the field sums."""
+    L = _adj_sigs()
+    theta_s = theta_i if theta_s is None else theta_s
+    phi_s = phi_i if phi_s is None else phi_s
+    m = OracleMesh(sm)
+    Z = m.fill(freq, degree)
+    V = m.rhs_planewave(freq, theta_i, phi_i, eth, eph, degree)
+    I, info = lu_solve(Z, V)
+    AJ, AdivJ, _ = m.scatter(I)
+    obs = sph2rect(R, theta_s, phi_s)
+    E, _ = m.fields(freq, AJ, AdivJ, obs)
+    E = E[0]
+    th = np.array([math.cos(theta_s) * math.cos(phi_s), math.cos(theta_s) * math.sin(phi_s), -math.sin(theta_s)])
+    ph = np.array([-math.sin(phi_s), math.cos(phi_s), 0.0])
+    a, b = th @ E, ph @ E
+    sig = 4 * math.pi * R * R * np.array([abs(a) ** 2, abs(b) ** 2])
+    out = {"sigma": sig, "I": I, "N": m.N}
+    for key, vec, amp in (("dsigma_vv", th, a), ("dsigma_hh", ph, b)):
+        eE = 4 * math.pi * R * R * np.conj(amp) * vec.astype(np.complex128)
+        gx = np.zeros(sm.nv * 3, np.complex128)
+        gI = np.zeros(m.N, np.complex128)
+        L.orc_fields_vjp(m.h, freq, _p(np.ascontiguousarray(I)), _p(np.ascontiguousarray(obs)),
+                         _p(np.ascontiguousarray(eE)), _p(np.zeros(3, np.complex128)), _p(gx), _p(gI))
+        mu = np.linalg.solve(Z.T, gI)
+        dV = rhs_planewave_vjp(m, freq, degree, theta_i, phi_i, eth, eph, mu).reshape(-1)
+        Cmu = adjoint_contract(m, freq, degree, I, mu).reshape(-1)
+        out[key] = (2 * (gx + dV - Cmu).real).reshape(-1, 3)
+    return out

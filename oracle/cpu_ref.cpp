@@ -1343,6 +1343,96 @@ int orc_fields_vjp(const orc_mesh* m, double freq, const double* Iin, const doub
 }
 
 /* d length(edge of RWG n) / d x: writes gl[3*nv] real for one RWG (zero elsewhere). */
+/*
+ * g[3*v + c] (complex) = sum_n W_n dV_n/dx_{v,c} for the plane-wave moments of orc_rhs_planewave
+ * (update_rhs(plane_wave), src/emw_solver.cpp:380-416): V_n = sum over T+, T- of
+ * +-(l / 2A) sum_p w_p (r_p - p_T) . E(r_p) / (-j omega mu0), with every vertex dependence
+ * differentiated: the edge length l = hypot(dx, dy), the area A = |e0 x e1| / 2, the rule weights
+ * (reference weight times the triangle's area), the rule points r_p = sum_k beta_pk v_k, the free
+ * vertex p_T, and the field E(r) = E0 exp(+j k dir.r), dE/dr_p = j k dir E^T.  g is nv*3 complex.
+ */
+int orc_rhs_planewave_vjp(orc_mesh* m, double freq, int degree, double theta, double phi, double eth_re,
+                          double eth_im, double eph_re, double eph_im, const double* Win, double* gout)
+{
+    if (degree < 0 || degree > FRICO_DUNAVANT_MAX_DEGREE)
+        return -2;
+    const cd* W = reinterpret_cast<const cd*>(Win);
+    cd* G = reinterpret_cast<cd*>(gout);
+    const size_t nv = m->V.size();
+    std::fill(G, G + 3 * nv, cd(0));
+    const double omega = 2.0 * M_PI * freq;
+    const double k = omega * std::sqrt(MU0 * EPS0);
+    const plane_wave_field field(k, theta, phi, cd{eth_re, eth_im}, cd{eph_re, eph_im});
+    const cd c0 = 1.0 / cd{0, -omega * MU0};
+    const int evmap[3] = {2, 0, 1};
+    auto addv = [&](int64_t v, const vec3& g, cd s) {
+        G[3 * v] += g.x * s;
+        G[3 * v + 1] += g.y * s;
+        G[3 * v + 2] += g.z * s;
+    };
+    std::vector<dqpoint> q;
+    for (const auto& b : m->bf) {
+        const cd wn = W[b.matrix_index];
+        if (wn == cd(0))
+            continue;
+        for (int a = 0; a < 2; a++) {
+            const int64_t t = a == 0 ? b.itminus : b.itplus;
+            const double sgn = a == 0 ? -1.0 : 1.0;
+            const double A = a == 0 ? b.Aminus : b.Aplus;
+            const vec3 pf = a == 0 ? b.pminus : b.pplus;
+            const int64_t fv = m->T[t][evmap[a == 0 ? b.loc_eminus : b.loc_eplus]];
+            rules_d(*m, t, degree, false, q);
+            // F = sum_p w_p rho_p . E_p and its gradient per vertex of the triangle
+            cd F = 0;
+            cvec3acc dFv[3], dFfree;
+            for (const auto& p : q) {
+                const cvec3 E = field(p.p);
+                const vec3 rho = p.p - pf;
+                const cd rE = rho.x * E.x + rho.y * E.y + rho.z * E.z;
+                F += p.w * rE;
+                for (int kk = 0; kk < 3; kk++) {
+                    // weight: dw_p/dv_k (rho . E); point: w_p beta_pk (E + (rho . E) j k dir)
+                    dFv[kk].add(p.dw[kk], rE);
+                    const cd s = p.w * p.beta[kk];
+                    dFv[kk].x += s * (E.x + rE * field.jk * field.dir.x);
+                    dFv[kk].y += s * (E.y + rE * field.jk * field.dir.y);
+                    dFv[kk].z += s * (E.z + rE * field.jk * field.dir.z);
+                }
+                // free vertex: d rho / d p_T = -1
+                dFfree.x -= p.w * E.x;
+                dFfree.y -= p.w * E.y;
+                dFfree.z -= p.w * E.z;
+            }
+            const cd pre = wn * sgn * c0;               // V_n += sgn c0 (l / 2A) F
+            const double lA = b.length / (2.0 * A);
+            // d(l / 2A) = dl / 2A - l dA / 2A^2
+            {
+                const edge_t& e = m->edges[b.edge_index];
+                if (b.length > 0) {
+                    const vec3 d = m->V[e.iv1] - m->V[e.iv0];
+                    const vec3 gl{d.x / b.length, d.y / b.length, 0.0};
+                    addv(e.iv1, gl, pre * F / (2.0 * A));
+                    addv(e.iv0, gl, -pre * F / (2.0 * A));
+                }
+                const auto& tr = m->T[t];
+                vec3 ga[3];
+                area_grad(m->V[tr[0]], m->V[tr[1]], m->V[tr[2]], ga);
+                for (int kk = 0; kk < 3; kk++)
+                    addv(tr[kk], ga[kk], -pre * F * b.length / (2.0 * A * A));
+                for (int kk = 0; kk < 3; kk++) {
+                    G[3 * tr[kk]] += pre * lA * dFv[kk].x;
+                    G[3 * tr[kk] + 1] += pre * lA * dFv[kk].y;
+                    G[3 * tr[kk] + 2] += pre * lA * dFv[kk].z;
+                }
+                G[3 * fv] += pre * lA * dFfree.x;
+                G[3 * fv + 1] += pre * lA * dFfree.y;
+                G[3 * fv + 2] += pre * lA * dFfree.z;
+            }
+        }
+    }
+    return 0;
+}
+
 int orc_length_grad(const orc_mesh* m, int64_t n, double* gl)
 {
     std::fill(gl, gl + 3 * m->V.size(), 0.0);

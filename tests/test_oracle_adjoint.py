@@ -64,3 +64,59 @@ def test_impedance_and_gain_gradients_match_central_differences():
             zm, gm = fwd(Vm)
             assert abs((zp - zm) / (2 * h) - r["dzin"][v, c]) <= 1e-6 * np.max(np.abs(r["dzin"]))
             assert abs((gp - gm) / (2 * h) - r["dgain"][v, c]) <= 1e-6 * np.max(np.abs(r["dgain"]))
+
+
+@pytest.mark.parametrize("th,ph", [(0.0, 0.0), (0.6, 0.4), (2.2, -1.3)])
+def test_planewave_rhs_vjp_matches_central_differences(th, ph):
+    """sum_n W_n dV_n/dx of the plane-wave moments (rule points, weights, areas, edge lengths, free
+    vertices and the field's phase all move with the vertices) against central differences of the
+    oracle's own update_rhs(plane_wave)."""
+    lam = meshes.wavelength(F0)
+    sm = meshes.bowtie(0.25 * lam, 60.0, 4, 3)
+    om = oracle.OracleMesh(sm)
+    rng = np.random.default_rng(3)
+    W = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    eth, eph = 1.0 - 0.3j, 0.4 + 0.2j
+    g = oracle.rhs_planewave_vjp(om, F0, 5, th, ph, eth, eph, W)
+    h = 1e-6 * lam
+    for v in range(0, sm.nv, 2):
+        for c in range(3):
+            Vp, Vm = sm.vertices.copy(), sm.vertices.copy()
+            Vp[v, c] += h
+            Vm[v, c] -= h
+            fp = W @ oracle.OracleMesh(_perturbed(sm, Vp)).rhs_planewave(F0, th, ph, eth, eph, 5)
+            fm = W @ oracle.OracleMesh(_perturbed(sm, Vm)).rhs_planewave(F0, th, ph, eth, eph, 5)
+            assert abs((fp - fm) / (2 * h) - g[v, c]) <= 1e-7 * np.max(np.abs(g))
+
+
+@pytest.mark.parametrize("bistatic", [False, True])
+def test_rcs_gradients_match_central_differences(bistatic):
+    """sigma_VV and sigma_HH of a plate lit obliquely (theta-phi mixed polarisation), monostatic and
+    bistatic, and their vertex gradients against central differences (step 1e-6 wavelengths) of the
+    oracle's forward RCS."""
+    lam = meshes.wavelength(F0)
+    sm = meshes.plate(0.6 * lam, 3)
+    thi, phi_ = 0.5, 0.3
+    ths, phs = (1.1, -0.7) if bistatic else (thi, phi_)
+    eth, eph = 1.0, 0.5j
+    r = oracle.rcs_gradients(sm, F0, thi, phi_, eth, eph, ths, phs, degree=5, R=10.0)
+
+    def fwd(V):
+        s2 = _perturbed(sm, V)
+        m = oracle.OracleMesh(s2)
+        I, _ = oracle.lu_solve(m.fill(F0, 5), m.rhs_planewave(F0, thi, phi_, eth, eph, 5))
+        AJ, AdivJ, _ = m.scatter(I)
+        E, _ = m.fields(F0, AJ, AdivJ, oracle.sph2rect(10.0, ths, phs))
+        return oracle.rcs_unit_Ei(E[0], ths, phs, 10.0)
+
+    s0 = fwd(sm.vertices)
+    assert np.allclose(s0, r["sigma"], rtol=1e-12)
+    h = 1e-6 * lam
+    for v in range(0, sm.nv, 3):
+        for c in range(3):
+            Vp, Vm = sm.vertices.copy(), sm.vertices.copy()
+            Vp[v, c] += h
+            Vm[v, c] -= h
+            d = (fwd(Vp) - fwd(Vm)) / (2 * h)
+            assert abs(d[0] - r["dsigma_vv"][v, c]) <= 1e-6 * np.max(np.abs(r["dsigma_vv"]))
+            assert abs(d[1] - r["dsigma_hh"][v, c]) <= 1e-6 * np.max(np.abs(r["dsigma_hh"]))
