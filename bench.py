@@ -40,8 +40,8 @@ PEAK_HBM = 8000.0       # GB/s
 # algorithmic bytes per launch: 419.3e6.
 GEMM_F32_TRAFFIC = {"c64_np_ir": 699.7e6, "c64_ir": 700.1e6}
 # HBM bytes per timed launch of cgemm_sub_f16 at N = 12160, nb = 128 (rocprofv3 FETCH_SIZE / WRITE_SIZE,
-# scaled by a copy kernel of known size); None until measured for the current kernel.
-GEMM_F16_TRAFFIC = None
+# scaled by a copy kernel of known size: profiles/plate_N12160_c64np_hbm_traffic.md).
+GEMM_F16_TRAFFIC = 482.5e6
 PEAK_FP16_MFMA = 2500.0  # TFLOP/s, MI355X dense FP16 matrix
 
 

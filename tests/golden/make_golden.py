@@ -2,6 +2,7 @@
 
     python tests/golden/make_golden.py          # the small fixtures
     python tests/golden/make_golden.py --c3     # the N = 12160 headline fixture (minutes)
+    python tests/golden/make_golden.py --c4     # the N = 1983 bowtie gradient fixture (a minute)
 
 The reference cannot be built or run here (C++23 <print>, Eigen, gmsh, SILO and HDF5 are absent),
 so the fixtures are produced by the oracle, whose functions are pinned separately by the tests in
@@ -88,10 +89,28 @@ def c3_fixture():
           f"max oracle residual {res.max():.2e}")
 
 
+def c4_gradient_fixture():
+    """The adjoint workload: planar bowtie (arm 0.25 wavelength, flare 60 deg, the C4 mesh of
+    meshes.workload_bowtie, N = 1983) at 300 MHz, Dunavant degree 5, 1 V delta gap: the oracle's
+    FP64 input impedance, gain at (0, 0, 10 m) and their gradients with respect to every vertex
+    coordinate (oracle.antenna_gradients).  About a minute on 8 cores."""
+    import time
+    t0 = time.perf_counter()
+    sm = meshes.workload_bowtie(F0)
+    r = oracle.antenna_gradients(sm, F0, 5, obs=(0.0, 0.0, 10.0))
+    np.savez_compressed(os.path.join(HERE, "bowtie_c4_gradients.npz"), freq=F0, degree=5, N=r["N"],
+                        obs=np.array([0.0, 0.0, 10.0]), zin=r["zin"], gain=r["gain"], dzin=r["dzin"],
+                        dgain=r["dgain"])
+    print(f"C4 gradient fixture: N={r['N']}, {time.perf_counter() - t0:.1f} s")
+
+
 if __name__ == "__main__":
-    if "--c3" in sys.argv:
+    if "--c4" in sys.argv:
+        c4_gradient_fixture()
+    elif "--c3" in sys.argv:
         c3_fixture()
     else:
         main()
         if "--all" in sys.argv:
             c3_fixture()
+            c4_gradient_fixture()

@@ -60,6 +60,28 @@ def test_gradients_on_a_strip_with_a_multi_edge_gap(gpu_ctx):
     assert np.max(np.abs(g["dgain"] - ref["dgain"])) <= 1e-9 * np.max(np.abs(ref["dgain"]))
 
 
+def test_bowtie_c4_gradients_match_the_oracle_fixture(gpu_ctx):
+    """The adjoint workload at full size (bowtie arm 0.25 lambda, flare 60 degrees, N = 1983,
+    degree 5) against the oracle's FP64 gradients stored in tests/golden/bowtie_c4_gradients.npz
+    (tests/golden/make_golden.py --c4): Zin and gain to 1e-10, every vertex gradient entry to 1e-9
+    of the largest."""
+    import os
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "bowtie_c4_gradients.npz"))
+    assert float(d["freq"]) == F0 and int(d["degree"]) == 5
+    sm = meshes.workload_bowtie(F0)
+    gpu_ctx.load_mesh(sm, 5)
+    assert gpu_ctx.N == int(d["N"]) == 1983
+    g = gpu_ctx.antenna_gradients(F0, sm.feed, 1.0, tuple(d["obs"]))
+    assert g["dzin"].shape == d["dzin"].shape == (sm.nv, 3)
+    zin, gain = complex(d["zin"]), float(d["gain"])
+    assert abs(g["zin"] - zin) <= 1e-10 * abs(zin)
+    assert abs(g["gain"] - gain) <= 1e-10 * abs(gain)
+    ez = np.max(np.abs(g["dzin"] - d["dzin"])) / np.max(np.abs(d["dzin"]))
+    eg = np.max(np.abs(g["dgain"] - d["dgain"])) / np.max(np.abs(d["dgain"]))
+    print(f"C4: dZin rel err {ez:.2e}, dG rel err {eg:.2e}")
+    assert ez <= 1e-9 and eg <= 1e-9
+
+
 def test_bowtie_c4_gradients_match_central_differences(gpu_ctx):
     """Bowtie with arm 0.25 lambda and 60 degree flare, N = 1983: adjoint vs central differences
     with a step of 1e-6 wavelengths on a spread of vertex coordinates."""
