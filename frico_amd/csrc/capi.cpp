@@ -172,6 +172,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->augment = std::atoi(au) != 0;
     if (const char* sr = std::getenv("FRICO_STREAM_RESIDUAL"))
         c->stream_residual = std::atoi(sr) != 0;
+    if (const char* rp = std::getenv("FRICO_RESID_PAIR"))
+        c->resid_pair = std::atoi(rp) != 0;
     if (const char* gf = std::getenv("FRICO_GEMM_F16"))
         c->gemm_f16 = std::atoi(gf) != 0;
     if (const char* ds = std::getenv("FRICO_LU_DIRECT_SOLVES"))
@@ -1041,6 +1043,8 @@ int refine_prepare(frico_ctx* c)
     if ((rc = c->R.ensure((size_t)(N * nr))) || (rc = c->D32.ensure((size_t)(N * nr))) ||
         (rc = c->norms.ensure((size_t)nr)))
         return rc;
+    if (c->stream_residual && c->resid_pair && (rc = c->R2.ensure((size_t)(N * nr))))
+        return rc;
     if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c64))))
         return rc;
     return 0;
@@ -1068,6 +1072,7 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
         rs.R = c->R.p;
         rs.ldr = N;
         rs.exact = c->resid_exact;
+        rs.R2 = c->resid_pair && c->R2.p ? c->R2.p : nullptr;
     }
     phase_begin(c, tt);
     if (first && (rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)))
@@ -1081,8 +1086,10 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
         rc = launch_convert_c128_c64(c->R.p, N, c->D32.p, N, N, nr, c->st);
     if (rc)
         return rc;
-    // the residual of this sweep starts from B
+    // the residual of this sweep starts from B (and its second accumulator from zero)
     if (stream && (rc = launch_copy_bytes(c->B.p, c->R.p, (size_t)(N * nr) * sizeof(c128), c->st)))
+        return rc;
+    if (stream && rs.R2 && (rc = launch_zero_bytes(rs.R2, (size_t)(N * nr) * sizeof(c128), c->st)))
         return rc;
     if (carried)
         rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st, stream ? &rs : nullptr);

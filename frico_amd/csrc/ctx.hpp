@@ -220,6 +220,7 @@ struct frico_ctx {
     // Refinement residuals updated beside the backward substitution on the second stream
     // (FRICO_STREAM_RESIDUAL=0: one full residual product after each solve).
     bool stream_residual = true;
+    bool resid_pair = true; // two residual blocks a launch, the second into R2 (FRICO_RESID_PAIR=0: one)
     // Residual products of the current solve in the exact four-product form: taken when the
     // tolerance is within 1e3 of the three-product form's rounding floor (~1e-13 relative, where a
     // refinement to 1e-13 stalled), i.e. below 1e-10.
@@ -230,7 +231,7 @@ struct frico_ctx {
     // rhs / solution
     int64_t nrhs = 0;
     bool have_x = false;
-    DevBuf<c128> B, X, R;
+    DevBuf<c128> B, X, R, R2;
     DevBuf<c64> D32;
     DevBuf<double> norms;
     // fields

@@ -295,6 +295,25 @@ __global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub(int64_t M, int64_t
                                           ldb, C, ldc, sm);
 }
 
+// Two products of the same M x N shape in one launch, C_s -= A_s B_s for s = blockIdx.z, so that
+// the tiles of both share the CUs' last wave (a K sum split over two accumulators).
+template <typename T> struct GemmPair {
+    const cx<T>* A[2];
+    const cx<T>* B[2];
+    cx<T>* C[2];
+    int64_t K[2];
+    int64_t lda, ldb, ldc;
+};
+template <typename T, bool M3>
+__global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub2(int64_t M, int64_t N, GemmPair<T> p)
+{
+    using TL = typename TileBig<T>::type;
+    __shared__ GemmSmem<T, TL> sm;
+    const int s = blockIdx.z;
+    gemm_sub_tile<T, T, false, M3, TL>((int64_t)blockIdx.x * TL::BM, (int64_t)blockIdx.y * TL::BN, M, N, p.K[s], p.A[s],
+                                        p.lda, p.B[s], p.ldb, p.C[s], p.ldc, sm);
+}
+
 // The same product on the tiles (i, j) with i >= j only (row tile i, column tile j; square tiles,
 // so with C's first entry on the diagonal of a symmetric matrix these are the tiles that hold its
 // lower triangle).  One workgroup per such tile, numbered down the columns: column j holds the
@@ -1293,6 +1312,18 @@ __global__ __launch_bounds__(256) void add_correction(c128* __restrict__ X, int6
 }
 
 // out[c] = sum_r |X[r,c]|^2, one workgroup per column.
+// R += R2 (rows x cols, leading dimension ld).
+__global__ __launch_bounds__(256) void add_into(c128* __restrict__ R, const c128* __restrict__ R2, int64_t ld,
+                                                int64_t rows)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x, c = blockIdx.y;
+    if (r < rows) {
+        const c128 v = R2[r + c * ld];
+        R[r + c * ld].re += v.re;
+        R[r + c * ld].im += v.im;
+    }
+}
+
 __global__ __launch_bounds__(256) void col_norm2(const c128* __restrict__ X, int64_t ldx, int64_t rows,
                                                  double* __restrict__ out)
 {
@@ -1381,6 +1412,24 @@ static int gemm_sub_lower(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64
     const double cols = (double)N, full = (double)M * cols - cols * (cols - TL::BN) / 2.0;
     if (kt)
         kt->end(st, 8.0 * full * (double)K);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+// C_s -= A_s B_s, s = 0, 1 (M x K_s times K_s x N), in one launch.
+static int gemm_sub_pair(int64_t M, int64_t N, const GemmPair<double>& p, hipStream_t st, bool exact)
+{
+    if (M <= 0 || N <= 0 || p.K[0] <= 0 || p.K[1] <= 0)
+        return set_error("gemm pair: empty product"), -1;
+    using TL = typename TileBig<double>::type;
+    const int64_t gx = (M + TL::BM - 1) / TL::BM, gy = (N + TL::BN - 1) / TL::BN;
+    if (gy > 65535)
+        return set_error("gemm: too many column tiles"), -1;
+    const dim3 grid((unsigned)gx, (unsigned)gy, 2);
+    if (!exact && gemm_three_products() && p.K[0] >= 128 && p.K[1] >= 128)
+        launch_kernel(&cgemm_sub2<double, true>, grid, dim3(256), 0, st, M, N, p);
+    else
+        launch_kernel(&cgemm_sub2<double, false>, grid, dim3(256), 0, st, M, N, p);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2302,6 +2351,273 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
 
 constexpr int RESID_ROWS = 512; // rows of the solution per residual product beside the substitution
 
+// ----------------------------------------------------------------------------------------------
+// Blocked triangular solves (complex64 factors without interchanges to apply).  The rows are taken
+// in blocks of BSOLVE_ROWS = RESID_ROWS.  One launch of block_solve solves a block's diagonal
+// block against the right-hand sides, and one product carries the block's solution to the rows
+// beyond it, so a block costs two launches instead of two per 64-row chunk.
+//
+// block_solve: workgroup = BSOLVE_COLS right-hand sides, whose rows of the block it keeps in LDS.
+// Per 64-row chunk of the block, in solve order:
+//   * every thread of the update below issues its loads of the factor first (they do not depend
+//     on the chunk's solution), then the chunk's diagonal block, loaded during the previous chunk,
+//     goes to LDS;
+//   * substitution, one wavefront per right-hand side, one row per lane: the rows' finished values
+//     are broadcast by readlane, the coefficients come from LDS a few steps ahead (U_jj is applied
+//     by substitution, not through an explicit inverse: an inverse loses accuracy in proportion to
+//     cond(U_jj) on EFIE matrices);
+//   * the rows of the block not yet solved take the chunk off, y_r -= D(r, chunk) x_chunk: thread
+//     (row slot, half of the chunk's 64 columns), the two halves summed through LDS.
+// UPPER: U (non-unit diagonal), chunks from the last, rows above; else L (unit diagonal), chunks
+// from the first, rows below.
+// ----------------------------------------------------------------------------------------------
+constexpr int BSOLVE_ROWS = RESID_ROWS;
+constexpr int BSOLVE_COLS = 4;
+constexpr int BSOLVE_PF = 4; // coefficient loads in flight ahead of the substitution step
+
+// Substitution of one chunk (w rows, w == 64 when FULL) for one right-hand side held as b (lane =
+// row); sd[c][r] = D(r, c); rd = 1 / D(lane, lane) (UPPER).  Returns x (lane = row).
+template <bool UPPER, bool FULL>
+__device__ inline c64 chunk_substitute(c64 b, c64 rd, int w, const c64 (*sd)[65], int lane)
+{
+    if constexpr (FULL) {
+        constexpr int W = 64;
+        c64 ring[BSOLVE_PF];
+#pragma unroll
+        for (int p = 0; p < BSOLVE_PF; p++) {
+            const int i = UPPER ? W - 1 - p : p;
+            const bool dep = UPPER ? lane < i : lane > i;
+            ring[p] = dep ? sd[i][lane] : c64{0.0f, 0.0f};
+        }
+#pragma unroll
+        for (int t = 0; t < W; t++) {
+            const int i = UPPER ? W - 1 - t : t;
+            const c64 e = ring[t % BSOLVE_PF];
+            if (t + BSOLVE_PF < W) {
+                const int i2 = UPPER ? W - 1 - (t + BSOLVE_PF) : t + BSOLVE_PF;
+                const bool dep2 = UPPER ? lane < i2 : lane > i2;
+                ring[t % BSOLVE_PF] = dep2 ? sd[i2][lane] : c64{0.0f, 0.0f};
+            }
+            const c64 xi = lane_value(UPPER ? b * rd : b, i);
+            b.re -= e.re * xi.re - e.im * xi.im;
+            b.im -= e.re * xi.im + e.im * xi.re;
+        }
+    } else {
+        // a short chunk (the last rows of the matrix): one coefficient read a step
+        for (int t = 0; t < w; t++) {
+            const int i = UPPER ? w - 1 - t : t;
+            const bool dep = UPPER ? lane < i : lane > i;
+            const c64 e = dep ? sd[i][lane] : c64{0.0f, 0.0f};
+            const c64 xi = lane_value(UPPER ? b * rd : b, i);
+            b.re -= e.re * xi.re - e.im * xi.im;
+            b.im -= e.re * xi.im + e.im * xi.re;
+        }
+    }
+    return UPPER ? b * rd : b;
+}
+
+template <bool UPPER>
+__global__ __launch_bounds__(1024) void block_solve(const c64* __restrict__ A, int64_t lda, int64_t kb, int bw,
+                                                    const c64* Y, int64_t ldy, c64* X, int64_t ldx, int64_t nrhs)
+{
+    __shared__ c64 sy[BSOLVE_ROWS][BSOLVE_COLS];
+    __shared__ c64 sp[BSOLVE_ROWS][BSOLVE_COLS]; // the second half's partial sums
+    __shared__ c64 sd[64][65];                   // sd[c][r] = D(r, c) of the chunk's diagonal block
+    __shared__ c64 sx[64][BSOLVE_COLS];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int rs = tid & (BSOLVE_ROWS - 1), kh = tid / BSOLVE_ROWS; // update: row slot, column half
+    const int64_t q0 = (int64_t)blockIdx.x * BSOLVE_COLS;
+    for (int e = tid; e < BSOLVE_ROWS * BSOLVE_COLS; e += 1024) {
+        const int r = e % BSOLVE_ROWS, q = e / BSOLVE_ROWS;
+        sy[r][q] = (r < bw && q0 + q < nrhs) ? Y[(kb + r) + (q0 + q) * ldy] : c64{0.0f, 0.0f};
+    }
+    const int nch = (bw + 63) / 64;
+    // the diagonal block of chunk j, four entries a thread
+    auto dload = [&](int j, c64* dv) {
+        const int c0 = 64 * j, w = (bw - c0) < 64 ? (bw - c0) : 64;
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int e = tid + 1024 * t, r = e & 63, c = e >> 6;
+            dv[t] = (r < w && c < w) ? A[(kb + c0 + r) + (kb + c0 + c) * lda] : c64{0.0f, 0.0f};
+        }
+    };
+    c64 dpre[4];
+    dload(UPPER ? nch - 1 : 0, dpre);
+    for (int s = 0; s < nch; s++) {
+        const int j = UPPER ? nch - 1 - s : s;
+        const int c0 = 64 * j, w = (bw - c0) < 64 ? (bw - c0) : 64;
+        // this thread's row of the update and its 32 factor entries
+        const int row = UPPER ? rs : c0 + w + rs;
+        const bool act = UPPER ? row < c0 : row < bw;
+        c64 dv[32];
+        {
+            const c64* dcol = A + (kb + (act ? row : 0)) + (kb + c0 + 32 * kh) * lda;
+#pragma unroll
+            for (int t = 0; t < 32; t++)
+                dv[t] = (act && 32 * kh + t < w) ? dcol[(int64_t)t * lda] : c64{0.0f, 0.0f};
+        }
+        __syncthreads(); // the previous chunk is done with sd, sx and sp
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int e = tid + 1024 * t;
+            sd[e >> 6][e & 63] = dpre[t];
+        }
+        if (s + 1 < nch)
+            dload(UPPER ? j - 1 : j + 1, dpre);
+        __syncthreads();
+        if (wv < BSOLVE_COLS) {
+            const c64 b = lane < w ? sy[c0 + lane][wv] : c64{0.0f, 0.0f};
+            c64 rd{1.0f, 0.0f};
+            if (UPPER && lane < w)
+                rd = cdiv(c64{1.0f, 0.0f}, sd[lane][lane]);
+            const c64 x = w == 64 ? chunk_substitute<UPPER, true>(b, rd, w, sd, lane)
+                                  : chunk_substitute<UPPER, false>(b, rd, w, sd, lane);
+            // every lane: rows past a short chunk hold x = 0, which the update multiplies by zeros
+            sx[lane][wv] = x;
+            if (lane < w && q0 + wv < nrhs)
+                X[(kb + c0 + lane) + (q0 + wv) * ldx] = x;
+        }
+        __syncthreads();
+        c64 acc[BSOLVE_COLS];
+        if (act) {
+#pragma unroll
+            for (int q = 0; q < BSOLVE_COLS; q++)
+                acc[q] = c64{0.0f, 0.0f};
+#pragma unroll
+            for (int t = 0; t < 32; t++) {
+#pragma unroll
+                for (int q = 0; q < BSOLVE_COLS; q++) {
+                    const c64 x = sx[32 * kh + t][q];
+                    acc[q].re += dv[t].re * x.re - dv[t].im * x.im;
+                    acc[q].im += dv[t].re * x.im + dv[t].im * x.re;
+                }
+            }
+            if (kh == 1)
+#pragma unroll
+                for (int q = 0; q < BSOLVE_COLS; q++)
+                    sp[row][q] = acc[q];
+        }
+        __syncthreads();
+        if (act && kh == 0)
+#pragma unroll
+            for (int q = 0; q < BSOLVE_COLS; q++) {
+                const c64 o = sp[row][q];
+                sy[row][q].re -= acc[q].re + o.re;
+                sy[row][q].im -= acc[q].im + o.im;
+            }
+    }
+}
+
+template <bool UPPER>
+static void launch_block_solve(const c64* A, int64_t lda, int64_t kb, int bw, const c64* Y, int64_t ldy, c64* X,
+                               int64_t ldx, int64_t nrhs, hipStream_t st)
+{
+    launch_kernel(&block_solve<UPPER>, dim3((unsigned)((nrhs + BSOLVE_COLS - 1) / BSOLVE_COLS)), dim3(1024), 0, st, A,
+                  lda, kb, bw, Y, ldy, X, ldx, nrhs);
+}
+
+// FRICO_SOLVE_BLOCKED=0 returns the complex64 solves to one substitution and one product per chunk.
+static bool solve_blocked()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FRICO_SOLVE_BLOCKED");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
+// Residual product of one finished block of the solution beside the substitution (see lu_backward).
+static int resid_block(const ResidStream* rs, int64_t n, int64_t kb, int64_t bw, const c64* X, int64_t ldx,
+                       int64_t nrhs, hipStream_t st)
+{
+    const int64_t b = kb / RESID_ROWS;
+    FRICO_HIP_CHECK(hipEventRecord(rs->ev[b], st));
+    FRICO_HIP_CHECK(hipStreamWaitEvent(rs->side, rs->ev[b], 0));
+    int rc = launch_add_correction(rs->S + kb, rs->lds, X + kb, ldx, bw, nrhs, rs->side);
+    if (!rc)
+        rc = gemm_sub<double>(n, nrhs, bw, rs->Z + kb * rs->ldz, rs->ldz, rs->S + kb, rs->lds, rs->R, rs->ldr, rs->side,
+                              nullptr, rs->exact);
+    return rc;
+}
+
+// Blocked backward half (complex64): per block from the last, X(block) = U(block)^{-1} Y(block),
+// then Y(above) -= U(above, block) X(block).
+static int lu_backward_blocked(const c64* A, int64_t n, int64_t lda, c64* Y, int64_t ldy, c64* X, int64_t ldx,
+                               int64_t nrhs, hipStream_t st, const ResidStream* rs)
+{
+    const int64_t nblk = (n + BSOLVE_ROWS - 1) / BSOLVE_ROWS;
+    const bool pair = rs && rs->R2;
+    int64_t pend = -1, pend_bw = 0; // with R2: a finished block whose residual product waits for the next
+    for (int64_t b = nblk - 1; b >= 0; b--) {
+        const int64_t kb = b * BSOLVE_ROWS, bw = (n - kb) < BSOLVE_ROWS ? (n - kb) : BSOLVE_ROWS;
+        launch_block_solve<true>(A, lda, kb, (int)bw, Y, ldy, X, ldx, nrhs, st);
+        FRICO_HIP_CHECK(hipGetLastError());
+        int rc = 0;
+        if (rs && !pair) {
+            rc = resid_block(rs, n, kb, bw, X, ldx, nrhs, st);
+        } else if (pair) {
+            FRICO_HIP_CHECK(hipEventRecord(rs->ev[b], st));
+            FRICO_HIP_CHECK(hipStreamWaitEvent(rs->side, rs->ev[b], 0));
+            rc = launch_add_correction(rs->S + kb, rs->lds, X + kb, ldx, bw, nrhs, rs->side);
+            if (!rc && pend < 0) {
+                pend = kb;
+                pend_bw = bw;
+            } else if (!rc) {
+                GemmPair<double> gp;
+                gp.A[0] = rs->Z + pend * rs->ldz;
+                gp.B[0] = rs->S + pend;
+                gp.C[0] = rs->R;
+                gp.K[0] = pend_bw;
+                gp.A[1] = rs->Z + kb * rs->ldz;
+                gp.B[1] = rs->S + kb;
+                gp.C[1] = rs->R2;
+                gp.K[1] = bw;
+                gp.lda = rs->ldz;
+                gp.ldb = rs->lds;
+                gp.ldc = rs->ldr;
+                rc = gemm_sub_pair(n, nrhs, gp, rs->side, rs->exact);
+                pend = -1;
+            }
+        }
+        if (!rc && kb > 0)
+            rc = gemm_sub<float>(kb, nrhs, bw, A + kb * lda, lda, X + kb, ldx, Y, ldy, st, nullptr);
+        if (rc)
+            return rc;
+    }
+    if (pair) {
+        int rc = 0;
+        if (pend >= 0)
+            rc = gemm_sub<double>(n, nrhs, pend_bw, rs->Z + pend * rs->ldz, rs->ldz, rs->S + pend, rs->lds, rs->R,
+                                  rs->ldr, rs->side, nullptr, rs->exact);
+        if (rc)
+            return rc;
+        launch_kernel(&add_into, dim3((unsigned)((n + 255) / 256), (unsigned)nrhs), dim3(256), 0, rs->side, rs->R,
+                      (const c128*)rs->R2, rs->ldr, n);
+        FRICO_HIP_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+// Blocked forward half (complex64, unit L, no interchanges), in place: per block from the first,
+// B(block) = L(block)^{-1} B(block), then B(below) -= L(below, block) B(block).
+static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int64_t ldb, int64_t nrhs, hipStream_t st)
+{
+    const int64_t nblk = (n + BSOLVE_ROWS - 1) / BSOLVE_ROWS;
+    for (int64_t b = 0; b < nblk; b++) {
+        const int64_t kb = b * BSOLVE_ROWS, bw = (n - kb) < BSOLVE_ROWS ? (n - kb) : BSOLVE_ROWS;
+        launch_block_solve<false>(A, lda, kb, (int)bw, B, ldb, B, ldb, nrhs, st);
+        FRICO_HIP_CHECK(hipGetLastError());
+        const int64_t below = n - kb - bw;
+        if (below > 0) {
+            int rc = gemm_sub<float>(below, nrhs, bw, A + (kb + bw) + kb * lda, lda, B + kb, ldb, B + kb + bw, ldb, st,
+                                     nullptr);
+            if (rc)
+                return rc;
+        }
+    }
+    return 0;
+}
+
 // Backward half of the solve: per 64-row chunk j from the last, x_j = U_jj^{-1} y_j by substitution
 // (from Y into X, or in place when Y == X), then y_above -= U(above, j) x_j.  With rs (complex64
 // solves of a refinement), each RESID_ROWS-row block of X, once final, is added to the FP64 solution and
@@ -2315,6 +2631,19 @@ static int lu_backward(const cx<T>* A, int64_t n, int64_t lda, cx<T>* Y, int64_t
     const int64_t nblk = (n + RESID_ROWS - 1) / RESID_ROWS;
     if (rs && (!std::is_same<T, float>::value || rs->nev < nblk + 1))
         return set_error("solve: residual stream needs complex64 factors and %lld events", (long long)(nblk + 1)), -1;
+    if constexpr (std::is_same<T, float>::value) {
+        if (solve_blocked()) {
+            int rc = lu_backward_blocked(A, n, lda, Y, ldy, X, ldx, nrhs, st, rs);
+            if (rc)
+                return rc;
+            if (rs) {
+                FRICO_HIP_CHECK(hipEventRecord(rs->ev[nblk], rs->side));
+                FRICO_HIP_CHECK(hipStreamWaitEvent(st, rs->ev[nblk], 0));
+            }
+            FRICO_HIP_CHECK(hipGetLastError());
+            return 0;
+        }
+    }
     for (int64_t ch = nch - 1; ch >= 0; ch--) {
         const int64_t kc = ch * 64, ws = (n - kc) < 64 ? (n - kc) : 64;
         launch_trsm<T, true>((int)ws, A + kc + kc * lda, lda, Y + kc, ldy, X + kc, ldx, nrhs, st);
@@ -2394,6 +2723,13 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx
 {
     const int64_t npan = (n + nb - 1) / nb;
     const cx<T>* inv = (const cx<T>*)dinv;
+    if constexpr (std::is_same<T, float>::value) {
+        // no interchanges: both halves blocked, in place
+        if (!perm && solve_blocked()) {
+            int rc = lu_forward_blocked(A, n, lda, B, ldb, nrhs, st);
+            return rc ? rc : lu_backward<T>(A, n, lda, B, ldb, B, ldb, nrhs, st, rs);
+        }
+    }
     if (inv && wv && ybuf && nb % 64 == 0 && n > 64)
         return lu_solve_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf, rs);
     if (rs)

@@ -103,10 +103,11 @@ size_t lu_exchange_words(int gmax);
 // in-LDS triangular solves.  wv: the W/V products of lu_factor (LuWork::wv), or null; with wv the
 // solve runs one product per 64-row chunk and needs ybuf, an n x nrhs scratch (leading dimension n).
 // A second stream on which the backward half of a complex64 refinement solve adds each finished
-// 128-row block of the correction X to the FP64 solution S and takes it off the residual:
+// 512-row block of the correction X to the FP64 solution S and takes it off the residual:
 // S(block) += X(block, :), R -= Z(:, block) S(block, :).  With R = B beforehand, R ends as
 // B - Z S, computed as the full product would (the K sum in blocks).  ev: at least
-// ceil(n / 128) + 1 events.
+// ceil(n / 128) + 1 events.  With R2 (zero beforehand, same layout as R), the blocked solve takes
+// two blocks a launch, the second into R2, and adds R2 to R at the end.
 struct ResidStream {
     hipStream_t side = nullptr;
     hipEvent_t* ev = nullptr;
@@ -117,6 +118,7 @@ struct ResidStream {
     int64_t lds = 0;
     c128* R = nullptr;
     int64_t ldr = 0;
+    c128* R2 = nullptr;
     bool exact = true; // four real products per complex multiply-add (see gemm_sub)
 };
 // perm: the factorisation's interchange lists, or null for factors without interchanges.
