@@ -178,6 +178,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->gemm_f16 = std::atoi(gf) != 0;
     if (const char* ds = std::getenv("FRICO_LU_DIRECT_SOLVES"))
         c->direct_solves = std::atoi(ds) != 0;
+    if (const char* fs = std::getenv("FRICO_LU_FUSED_STEP"))
+        c->fused_step = std::atoi(fs) != 0;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
         c->max_groups = prop.multiProcessorCount; // one panel workgroup per CU (LDS-limited)
@@ -899,6 +901,7 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
         wk.tbuf = c->tbuf.p;
         wk.ntbuf = (int64_t)c->tbuf.n;
         wk.direct_solves = c->direct_solves;
+        wk.fused_step = c->fused_step;
         if (c->gemm_f16) {
             // L21 of two steps and U12 of two streams' products, padded to 128-row tiles
             const int64_t acap = (N + 127) / 128 * 128, bcap = (N + c->aug_cols + 127) / 128 * 128;

@@ -211,6 +211,7 @@ struct frico_ctx {
     DevBuf<c64> tbuf;             // transposed block column of the unpivoted LU
     DevBuf<unsigned char> hsplit; // fp16 split operands of the unpivoted LU's trailing products
     bool gemm_f16 = true;         // FRICO_GEMM_F16=0: fp32 trailing products in the unpivoted LU
+    bool fused_step = true;       // FRICO_LU_FUSED_STEP=0: the unpivoted LU's block solves and fp16 splits as separate launches
     bool direct_solves = false;   // FRICO_LU_DIRECT_SOLVES=1: lane-per-row block solves in the unpivoted LU
     bool wv_ok = false;           // wv holds the products of the current factors
     bool fused_solve = true;

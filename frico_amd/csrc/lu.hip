@@ -2180,6 +2180,16 @@ static int gemm_sub_ta(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t 
     return 0;
 }
 
+// Widest block solved by tri_solve_split (FRICO_LU_FUSED_MAX overrides).
+static int64_t fused_step_max_cols()
+{
+    static const int64_t v = [] {
+        const char* e = std::getenv("FRICO_LU_FUSED_MAX");
+        return e ? (int64_t)std::atoll(e) : (int64_t)4096;
+    }();
+    return v;
+}
+
 template <typename T>
 int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
                     KernelTimer* gemm_timer, int64_t naug)
@@ -2205,10 +2215,26 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
     FRICO_HIP_CHECK(hipGetLastError());
     // diagonal block and block column of step k
     const bool direct = c64f && wk.direct_solves;
+    // The fused block solves (one launch with the split) pay where a step is short and the host's
+    // launches set its pace; wide blocks keep the throughput-oriented solves and a separate split.
+    const bool fused_any = f16 && wk.fused_step && !direct;
+    const int64_t fused_max = fused_step_max_cols();
+    auto fused = [&](int64_t cols) { return fused_any && cols <= fused_max; };
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
         launch_kernel(&lu_diag<T>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
         if constexpr (c64f) {
+            if (fused(m)) {
+                // L21 = A21 U11^{-1} and its fp16 split in one launch
+                const int par = (int)((k / nb) & 1);
+                int rc = m > 0 ? launch_tri_solve_split(reinterpret_cast<c64*>(A), lda, k, w, 0, m, true, wk.ha[par],
+                                                        wk.hea[par], wk.hacap, st)
+                               : 0;
+                if (rc)
+                    return rc;
+                FRICO_HIP_CHECK(hipGetLastError());
+                return 0;
+            }
             if (direct) {
                 // L21 = A21 U11^{-1} a row per lane, then its fp16 split straight from A
                 int rc = launch_solve_l21(reinterpret_cast<c64*>(A), lda, k, w, m, st);
@@ -2257,6 +2283,20 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         if (nc <= 0)
             return 0;
         int rc = 0;
+        if constexpr (c64f) {
+            if (fused(nc)) {
+                // U12 = L11^{-1} A12 with its split (per column, on this stream's buffer), then the product
+                const int64_t m = n - k - w;
+                const int par = (int)((k / nb) & 1), sb = s == st ? 0 : 1;
+                if (parts & 1)
+                    rc = launch_tri_solve_split(reinterpret_cast<c64*>(A), lda, k, w, c0, nc, false, wk.hb[sb], wk.heb[sb],
+                                                wk.hbcap, s);
+                if (rc || !(parts & 2) || m <= 0)
+                    return rc;
+                return gemm_sub_f16(m, nc, w, wk.ha[par], wk.hea[par], wk.hacap, wk.hb[sb], wk.heb[sb], wk.hbcap,
+                                    reinterpret_cast<c64*>(A + (k + w) + c0 * lda), lda, s, gemm_timer);
+            }
+        }
         if (parts & 1) {
             if constexpr (c64f) {
                 rc = direct ? launch_solve_u12(reinterpret_cast<c64*>(A), lda, k, w, c0, nc, s)

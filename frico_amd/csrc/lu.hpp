@@ -42,6 +42,7 @@ struct LuWork {
     int64_t hbcap = 0;
     // lu_nopiv_factor (complex64): block-column and block-row solves by the lane-per-row kernels
     // (false: transposed block column through tbuf and the 64-row triangular-solve kernels)
+    bool fused_step = true; // complex64 unpivoted LU with fp16 products: tri_solve_split for the block solves
     bool direct_solves = true;
 };
 
@@ -58,6 +59,11 @@ int launch_split_f16_cm(const c64* S, int64_t ld, int64_t R, int K, _Float16* pl
                         hipStream_t st);
 // Block solves of the unpivoted complex64 LU after A11 = L11 U11 (w <= 128 columns from k):
 // L21 = A21 U11^{-1} for the m rows below the block, and U12 = L11^{-1} A12 for columns c0 .. c0 + nc.
+// The step's block solve with the fp16 split of its result (blocksolve.hip): trans false: U12 =
+// L11^{-1} A(k .. k+w, c0 .. c0+nc) in place, split per column; trans true: L21 = A21 U11^{-1} for
+// rows k+w+c0 .. k+w+c0+nc of A21, in place, split per row.
+int launch_tri_solve_split(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, bool trans, _Float16* planes,
+                           int* e, int64_t cap, hipStream_t st);
 int launch_solve_l21(c64* A, int64_t lda, int64_t k, int w, int64_t m, hipStream_t st);
 int launch_solve_u12(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, hipStream_t st);
 // C[M x N] -= L U, L as split rows (La, ea; acap rows), U as split columns (Ub, eb; bcap columns).
