@@ -903,13 +903,14 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
         wk.direct_solves = c->direct_solves;
         wk.fused_step = c->fused_step;
         if (c->gemm_f16) {
-            // L21 of two steps and U12 of two streams' products, padded to 128-row tiles
+            // L21 of four steps and U12 of two products a stream, padded to 128-row tiles
             const int64_t acap = (N + 127) / 128 * 128, bcap = (N + c->aug_cols + 127) / 128 * 128;
             const size_t abytes = split_f16_bytes(acap), bbytes = split_f16_bytes(bcap);
-            if ((rc = c->hsplit.ensure(2 * (abytes + bbytes))))
+            if ((rc = c->hsplit.ensure(4 * (abytes + bbytes))))
                 return rc;
             unsigned char* p = c->hsplit.p;
-            for (int i = 0; i < 2; i++) {
+            wk.nsplit = 4;
+            for (int i = 0; i < 4; i++) {
                 wk.ha[i] = reinterpret_cast<_Float16*>(p);
                 wk.hea[i] = reinterpret_cast<int*>(p + (size_t)acap * 4 * HSPLIT_K * 2);
                 p += abytes;

@@ -120,12 +120,24 @@ __device__ inline bool tile_of(int64_t b, int64_t gx, int64_t gy, int64_t& tm, i
     return true;
 }
 
-template <int RB, int CB, int OCC>
+// A second k half (TWO): La1/ea1 and Ub1/eb1 with kc1 more k, each half split with its own
+// exponents.  After the first half the accumulators are rescaled to the second half's exponents
+// (a power of two per entry: D = 2^-(e0) D0 + 2^-(e1) D1 = 2^-(e1) (2^(e1-e0) D0 + D1)), so that one
+// read-modify-write of C takes both: two steps' trailing updates at the C traffic of one.
+struct F16Half {
+    const _Float16* La;
+    const int* ea;
+    const _Float16* Ub;
+    const int* eb;
+    int kc;
+};
+
+template <int RB, int CB, int OCC, bool TWO = false>
 __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, int kc, const _Float16* __restrict__ La,
                                                          const int* __restrict__ ea, int64_t acap,
                                                          const _Float16* __restrict__ Ub, const int* __restrict__ eb,
                                                          int64_t bcap, c64* __restrict__ C, int64_t ldc, int64_t gx,
-                                                         int64_t gy)
+                                                         int64_t gy, F16Half h1 = F16Half{})
 {
     constexpr int BM = 32 * RB, BN = 32 * CB;
     constexpr bool PREC = RB * CB <= 4;
@@ -165,6 +177,7 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
         }
 
     constexpr int SA = BM / 16, SB = BN / 16; // 16-byte segments per thread per chunk
+    auto kloop = [&](const _Float16* __restrict__ La, const _Float16* __restrict__ Ub, int kc) {
     for (int k0 = 0; k0 < kc; k0 += HKC) {
         // global -> registers: thread e takes segment e % 4 of row (e / 4) % BM of plane e / (4 BM)
         half8 ra[SA], rb_[SB];
@@ -242,6 +255,31 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
         }
     }
 
+    };
+    kloop(La, Ub, kc);
+    if constexpr (TWO) {
+        // rescale the first half's sums to the second half's exponents, then the second half
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int64_t row = rbase + rb * 16;
+            const int64_t rr = row < M ? row : 0;
+            const int dr = h1.ea[rr] - ea[rr];
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int64_t col = cbase + cb * 16 + r;
+                    const int64_t cc = col < N ? col : 0;
+                    int d = dr + h1.eb[cc] - eb[cc];
+                    d = d < -200 ? -200 : (d > 100 ? 100 : d);
+                    are[rb][cb][r] = ldexpf(are[rb][cb][r], d);
+                    aim[rb][cb][r] = ldexpf(aim[rb][cb][r], d);
+                }
+        }
+        kloop(h1.La, h1.Ub, h1.kc);
+        ea = h1.ea;
+        eb = h1.eb;
+    }
     // C -= 2^-(ea + eb) D^T: lane holds row (lane & 15) of each row block and columns
     // 4 (lane >> 4) + r of each column block
     int ecol[CB][4];
@@ -334,14 +372,50 @@ int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea,
         kt->begin(st);
     if (tile == 128)
         launch_kernel(&cgemm_sub_f16<4, 4, 2>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap,
-                      C, ldc, gx, gy);
+                      C, ldc, gx, gy, F16Half{});
     else
         launch_kernel(&cgemm_sub_f16<4, 2, 3>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap,
-                      C, ldc, gx, gy);
+                      C, ldc, gx, gy, F16Half{});
     // algorithmic bytes: C read and written once (complex64), L and U read once as complex64
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K,
                 16.0 * (double)M * (double)N + 8.0 * ((double)M + (double)N) * (double)K);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int gemm_sub_f16_pair(int64_t M, int64_t N, int K0, const _Float16* La0, const int* ea0, const _Float16* Ub0,
+                      const int* eb0, int K1, const _Float16* La1, const int* ea1, const _Float16* Ub1, const int* eb1,
+                      int64_t acap, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt)
+{
+    if (M <= 0 || N <= 0)
+        return 0;
+    if (K0 <= 0 || K0 > HSPLIT_K || K1 <= 0 || K1 > HSPLIT_K)
+        return set_error("fp16 product: halves of K = %d + %d outside 1..%d", K0, K1, HSPLIT_K), -1;
+    if (K0 % HKC != 0)
+        return set_error("fp16 product: first half K = %d not a multiple of %d", K0, HKC), -1;
+    int tile = f16_tile();
+    if (tile == 128 && ((M + 127) / 128) * ((N + 127) / 128) < 1024)
+        tile = 12864;
+    const int BM = 128, BN = tile == 128 ? 128 : 64;
+    const int64_t gx = (M + BM - 1) / BM, gy = (N + BN - 1) / BN;
+    if ((M + 127) / 128 * 128 > acap || (N + 127) / 128 * 128 > bcap)
+        return set_error("fp16 product: operands not padded to whole tiles"), -1;
+    const int64_t nblk = (gx * gy + 7) / 8 * 8;
+    if (nblk > 0x7fffffff)
+        return set_error("fp16 product: too many tiles"), -1;
+    const F16Half h1{La1, ea1, Ub1, eb1, (K1 + HKC - 1) / HKC * HKC};
+    if (kt)
+        kt->begin(st);
+    if (tile == 128)
+        launch_kernel(&cgemm_sub_f16<4, 4, 2, true>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, K0, La0, ea0, acap,
+                      Ub0, eb0, bcap, C, ldc, gx, gy, h1);
+    else
+        launch_kernel(&cgemm_sub_f16<4, 2, 3, true>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, K0, La0, ea0, acap,
+                      Ub0, eb0, bcap, C, ldc, gx, gy, h1);
+    const double K = (double)K0 + (double)K1;
+    if (kt)
+        kt->end(st, 8.0 * (double)M * (double)N * K, 16.0 * (double)M * (double)N + 8.0 * ((double)M + (double)N) * K);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }

@@ -2190,6 +2190,156 @@ static int64_t fused_step_max_cols()
     return v;
 }
 
+// ----------------------------------------------------------------------------------------------
+// Paired steps of the unpivoted complex64 LU (fp16 products, look-ahead).  Steps a (even) and b
+// = a + 1 share their trailing product: after both block columns are factored, the rows of both
+// blocks are solved for the columns right of the pair (U12(a) = L11(a)^{-1} A(a, :);
+// A(b, :) -= L(b, a) U12(a); U12(b) = L11(b)^{-1} A(b, :)) and the rows below take
+// L21(a) U12(a) + L21(b) U12(b) in one K = 256 pass over C.  The next pair's 256 columns are
+// updated first, on the main stream, so that its first diagonal block can be factored while the
+// side stream updates the remaining columns:
+//   main: narrow(a -> b), column(b), [wait W(prev)] next-pair block rows + product, column(a + 2)
+//   side: [wait column(b)] block rows of the remaining columns, [wait next-pair product] product
+// L21 of step s is split into ha[s mod 4]; U12 of the main stream's products into hb[0..1], of
+// the side stream's into hb[2..3].
+// ----------------------------------------------------------------------------------------------
+static bool pairs_on(const LuWork& wk, bool f16, bool direct, int nb)
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FRICO_LU_PAIRS");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on && f16 && !direct && wk.nsplit >= 4 && nb == 128 && wk.fused_step;
+}
+
+static int lu_nopiv_pairs(LuWork& wk, c64* A, int64_t n, int64_t lda, int nb, hipStream_t st, bool lookahead,
+                          KernelTimer* gemm_timer, int64_t naug)
+{
+    const int64_t npan = (n + nb - 1) / nb;
+    // without the look-ahead the side stream's work runs on the main stream, in the same order and
+    // with the same arithmetic, and no event is used
+    hipStream_t side = lookahead ? wk.side : st;
+    auto record = [&](hipEvent_t e, hipStream_t s) -> hipError_t { return lookahead ? hipEventRecord(e, s) : hipSuccess; };
+    auto wait = [&](hipStream_t s, hipEvent_t e) -> hipError_t { return lookahead ? hipStreamWaitEvent(s, e, 0) : hipSuccess; };
+    c64* Tb = (c64*)wk.tbuf;
+    const int64_t fused_max = fused_step_max_cols();
+    auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
+    // diagonal block and L21 of the step at k, L21 split into ha[ai]
+    auto col_step = [&](int64_t k, int w, int ai) -> int {
+        const int64_t m = n - k - w;
+        launch_kernel(&lu_diag<float>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
+        if (m <= 0)
+            return 0;
+        if (m <= fused_max)
+            return launch_tri_solve_split(A, lda, k, w, 0, m, true, wk.ha[ai], wk.hea[ai], wk.hacap, st);
+        const dim3 g((unsigned)((m + kSymCols<float> - 1) / kSymCols<float>));
+        launch_kernel(&col_transpose<float, false>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
+        for (int s = 0; s < w; s += 64) {
+            const int ws = (w - s) < 64 ? (w - s) : 64;
+            launch_trsm<float, false, true, false>(ws, A + (k + s) + (k + s) * lda, lda, Tb + s, w, Tb + s, w, m, st);
+            if (s + ws < w) {
+                int rc = gemm_sub_ta<float>(w - s - ws, m, ws, A + (k + s) + (k + s + ws) * lda, lda, Tb + s, w,
+                                            Tb + s + ws, w, st);
+                if (rc)
+                    return rc;
+            }
+        }
+        launch_kernel(&col_transpose<float, true>, g, dim3(256), 0, st, A, lda, k, w, m, Tb);
+        return launch_split_f16(Tb, w, m, w, wk.ha[ai], wk.hea[ai], wk.hacap, st);
+    };
+    // U12 = L11^{-1} A(k .. k+w, c0 .. c0+nc) of the step at k, split into hb[bi]
+    auto usolve = [&](int64_t k, int w, int64_t c0, int64_t nc, hipStream_t s, int bi) -> int {
+        if (nc <= 0)
+            return 0;
+        if (nc <= fused_max)
+            return launch_tri_solve_split(A, lda, k, w, c0, nc, false, wk.hb[bi], wk.heb[bi], wk.hbcap, s);
+        int rc = row_block_solve<float>(A, lda, k, w, c0, nc, s);
+        return rc ? rc : launch_split_f16(A + k + c0 * lda, lda, nc, w, wk.hb[bi], wk.heb[bi], wk.hbcap, s);
+    };
+    // C (M x nc at A(r0, c0)) -= L21 rows [off, off + M) of step ai (K = w) times U12 in hb[bi]
+    auto prod1 = [&](int64_t M, int64_t nc, int w, int ai, int64_t off, int bi, int64_t r0, int64_t c0,
+                     hipStream_t s) -> int {
+        if (M <= 0 || nc <= 0)
+            return 0;
+        return gemm_sub_f16(M, nc, w, wk.ha[ai] + off * HSPLIT_K, wk.hea[ai] + off, wk.hacap, wk.hb[bi], wk.heb[bi],
+                            wk.hbcap, A + r0 + c0 * lda, lda, s, gemm_timer);
+    };
+    // the pair's product on rows [r0, n) x columns [c0, c0 + nc): L21(a) from its row off_a, L21(b)
+    auto prod2 = [&](int64_t nc, int wa, int ia, int64_t off_a, int b0, int wb, int ib, int b1, int64_t r0, int64_t c0,
+                     hipStream_t s) -> int {
+        const int64_t M = n - r0;
+        if (M <= 0 || nc <= 0)
+            return 0;
+        return gemm_sub_f16_pair(M, nc, wa, wk.ha[ia] + off_a * HSPLIT_K, wk.hea[ia] + off_a, wk.hb[b0], wk.heb[b0], wb,
+                                 wk.ha[ib], wk.hea[ib], wk.hb[b1], wk.heb[b1], wk.hacap, wk.hbcap, A + r0 + c0 * lda, lda,
+                                 s, gemm_timer);
+    };
+    hipEvent_t* evP = lookahead ? wk.events : nullptr;
+    hipEvent_t* evW = lookahead ? wk.events + npan : nullptr;
+    hipEvent_t ev0 = lookahead ? wk.events[2 * npan] : nullptr;
+    FRICO_HIP_CHECK(record(ev0, st));
+    FRICO_HIP_CHECK(wait(side, ev0));
+    int rc;
+    if ((rc = col_step(0, width(0), 0)))
+        return rc;
+    int64_t prev = -1; // pair whose side-stream product the next narrow update waits for
+    for (int64_t p = 0; p < npan; p += 2) {
+        const int64_t ka = p * nb;
+        const int wa = width(ka), ia = (int)(p % 4);
+        if (p + 1 >= npan) {
+            // a last single block: its rows of the appended columns
+            if (naug > 0) {
+                if (prev >= 0)
+                    FRICO_HIP_CHECK(wait(st, (lookahead ? evW[prev] : nullptr)));
+                if ((rc = usolve(ka, wa, n, naug, st, 0)))
+                    return rc;
+            }
+            break;
+        }
+        const int64_t kb = ka + nb;
+        const int wb = width(kb), ib = (int)((p + 1) % 4);
+        // step a's update of block b, then block b's factorisation
+        if ((rc = usolve(ka, wa, kb, wb, st, 0)) || (rc = prod1(n - kb, wb, wa, ia, 0, 0, kb, kb, st)) ||
+            (rc = col_step(kb, wb, ib)))
+            return rc;
+        FRICO_HIP_CHECK(record((lookahead ? evP[p] : nullptr), st));
+        const int64_t c2 = kb + wb;                             // first column right of the pair
+        const int64_t nnext = (n - c2) < 2 * nb ? (n - c2) : 2 * nb; // the next pair's matrix columns
+        const int64_t cw = c2 + nnext, nw = n + naug - cw;       // the rest, with the appended columns
+        // side: the pair's block rows of the remaining columns
+        if (nw > 0) {
+            FRICO_HIP_CHECK(wait(side, (lookahead ? evP[p] : nullptr)));
+            if ((rc = usolve(ka, wa, cw, nw, side, 2)) || (rc = prod1(wb, nw, wa, ia, 0, 2, kb, cw, side)) ||
+                (rc = usolve(kb, wb, cw, nw, side, 3)))
+                return rc;
+        }
+        // main: the next pair's columns, which the previous pair's side product updated
+        if (nnext > 0) {
+            if (prev >= 0)
+                FRICO_HIP_CHECK(wait(st, (lookahead ? evW[prev] : nullptr)));
+            if ((rc = usolve(ka, wa, c2, nnext, st, 0)) || (rc = prod1(wb, nnext, wa, ia, 0, 0, kb, c2, st)) ||
+                (rc = usolve(kb, wb, c2, nnext, st, 1)) || (rc = prod2(nnext, wa, ia, wb, 0, wb, ib, 1, c2, c2, st)))
+                return rc;
+        }
+        // side: the pair's product on the remaining columns, after the next pair's (so that the
+        // next diagonal block is dispatched before this product fills the CUs)
+        if (nw > 0) {
+            FRICO_HIP_CHECK(record(ev0, st));
+            FRICO_HIP_CHECK(wait(side, ev0));
+            if ((rc = prod2(nw, wa, ia, wb, 2, wb, ib, 3, c2, cw, side)))
+                return rc;
+            FRICO_HIP_CHECK(record((lookahead ? evW[p] : nullptr), side));
+            prev = p;
+        }
+        if (c2 < n && (rc = col_step(c2, width(c2), (int)((p + 2) % 4))))
+            return rc;
+    }
+    FRICO_HIP_CHECK(record(ev0, side));
+    FRICO_HIP_CHECK(wait(st, ev0));
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
                     KernelTimer* gemm_timer, int64_t naug)
@@ -2326,6 +2476,13 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
     };
     auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
     int rc;
+    if constexpr (c64f) {
+        if (pairs_on(wk, f16, direct, nb)) {
+            if ((rc = lu_nopiv_pairs(wk, reinterpret_cast<c64*>(A), n, lda, nb, st, lookahead, gemm_timer, naug)))
+                return rc;
+            return solve_tables<T>(wk, A, n, lda, nb, st);
+        }
+    }
     if (!lookahead) {
         for (int64_t p = 0; p < npan; p++) {
             const int64_t k = p * nb;

@@ -34,12 +34,15 @@ struct LuWork {
     // or null for the fp32 products.  ha[parity]: L21 of the steps of that parity (hacap rows);
     // hb[0] / hb[1]: U12 of the products on the main / side stream (hbcap columns).  Each holds
     // four planes of cap x HSPLIT_K halves; hea / heb their row / column exponents.
-    _Float16* ha[2] = {nullptr, nullptr};
-    int* hea[2] = {nullptr, nullptr};
+    // With four of each (nsplit = 4), the paired steps keep L21 of four steps (step mod 4) and two
+    // U12 operands a stream (hb[0..1] main, hb[2..3] side).
+    _Float16* ha[4] = {nullptr, nullptr, nullptr, nullptr};
+    int* hea[4] = {nullptr, nullptr, nullptr, nullptr};
     int64_t hacap = 0;
-    _Float16* hb[2] = {nullptr, nullptr};
-    int* heb[2] = {nullptr, nullptr};
+    _Float16* hb[4] = {nullptr, nullptr, nullptr, nullptr};
+    int* heb[4] = {nullptr, nullptr, nullptr, nullptr};
     int64_t hbcap = 0;
+    int nsplit = 0;
     // lu_nopiv_factor (complex64): block-column and block-row solves by the lane-per-row kernels
     // (false: transposed block column through tbuf and the 64-row triangular-solve kernels)
     bool fused_step = true; // complex64 unpivoted LU with fp16 products: tri_solve_split for the block solves
@@ -67,6 +70,11 @@ int launch_tri_solve_split(c64* A, int64_t lda, int64_t k, int w, int64_t c0, in
 int launch_solve_l21(c64* A, int64_t lda, int64_t k, int w, int64_t m, hipStream_t st);
 int launch_solve_u12(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, hipStream_t st);
 // C[M x N] -= L U, L as split rows (La, ea; acap rows), U as split columns (Ub, eb; bcap columns).
+// Two steps' products in one pass over C: C -= L0 U0 + L1 U1 (K0 = 128 and K1 <= 128, each pair of
+// operands split with its own exponents; L0, L1 with acap rows, U0, U1 with bcap columns).
+int gemm_sub_f16_pair(int64_t M, int64_t N, int K0, const _Float16* La0, const int* ea0, const _Float16* Ub0,
+                      const int* eb0, int K1, const _Float16* La1, const int* ea1, const _Float16* Ub1, const int* eb1,
+                      int64_t acap, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);
 int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
                  const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);
 
