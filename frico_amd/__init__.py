@@ -169,6 +169,39 @@ def write_radiation_diagram(path, freq, gain):
     _write("frico_write_radiation_diagram", str(path).encode(), float(freq), _p(g))
 
 
+def write_antenna_h5(path, freqs, port, gain, rdiag_dist=10.0, Z0=50.0, voltage=1.0 + 0.0j):
+    """The antenna analysis in HDF5 (write_hdf5, src/emw_ana_antenna.cpp:207-234): port = nsteps x 10
+    and gain = nsteps x 1080 as antenna_sweep returns them.  Read back with frico_amd.h5.H5File."""
+    f = np.ascontiguousarray(freqs, np.float64).ravel()
+    po = np.ascontiguousarray(port, np.float64).reshape(f.size, 10)
+    g = np.ascontiguousarray(gain, np.float64).reshape(f.size, 1080)
+    v = complex(voltage)
+    _write("frico_write_antenna_h5", str(path).encode(), float(rdiag_dist), float(Z0), f.size, _p(f), v.real, v.imag,
+           _p(po), _p(g))
+
+
+def write_antenna_h5_results(path, freqs, results, rdiag_dist=10.0, Z0=50.0, voltage=1.0 + 0.0j):
+    """write_antenna_h5 from Context.antenna_sweep's list of per-frequency results."""
+    port = np.array([[r["current"].real, r["current"].imag, r["power"].real, r["power"].imag, r["impedance"].real,
+                      r["impedance"].imag, r["gamma"].real, r["gamma"].imag, r["swr"], r["residual"]]
+                     for r in results])
+    gain = np.array([np.asarray(r["gain"]).reshape(1080) for r in results])
+    write_antenna_h5(path, freqs, port, gain, rdiag_dist, Z0, voltage)
+
+
+def write_linear_system_h5(path, Z, V, I=None):
+    """The -d dump (src/emw_ana_antenna.cpp:260-268): /frico/Z ([i][j] = Z[i, j]), /frico/V and, if
+    given, /frico/I (n x nrhs; a vector is one column)."""
+    Zf = np.asfortranarray(Z, np.complex128)
+    n = Zf.shape[0]
+    Vf = np.asfortranarray(np.asarray(V, np.complex128).reshape(n, -1))
+    If = None if I is None else np.asfortranarray(np.asarray(I, np.complex128).reshape(n, -1))
+    if If is not None and If.shape != Vf.shape:
+        raise ValueError("I must have V's shape")
+    _write("frico_write_linear_system_h5", str(path).encode(), n, _p(Zf), n, Vf.shape[1], _p(Vf), n,
+           None if If is None else _p(If), n)
+
+
 def write_bistatic_rcs(path, table):
     """bistatic_rcs_<n>.txt (src/emw_ana_bistatic_rcs.cpp:97-117); table 359 x 4."""
     t = np.ascontiguousarray(table, np.float64).reshape(359, 4)
@@ -339,6 +372,11 @@ class Context:
         info = _lib.frico_solve_info()
         self._check(self._L.frico_solve(self._h, float(tol), int(maxit), C.byref(info)))
         return SolveInfo(info.iterations, bool(info.converged), info.max_rel_residual)
+
+    def dump_linear_system(self, path):
+        """The -d dump of this context (frico_dump_linear_system): its complex128 matrix as it stands,
+        its right-hand sides and, if solved, the solution."""
+        self._check(self._L.frico_dump_linear_system(self._h, str(path).encode()))
 
     def solution(self) -> np.ndarray:
         n = int(self._L.frico_num_rhs(self._h))

@@ -103,6 +103,11 @@ SIGNATURES = {
     "frico_write_radiation_diagram": (C.c_int, [C.c_char_p, C.c_double, _d]),
     "frico_write_bistatic_rcs": (C.c_int, [C.c_char_p, _d]),
     "frico_format_double": (C.c_int, [C.c_double, C.c_char_p, C.c_int64]),
+    "frico_write_antenna_h5": (C.c_int, [C.c_char_p, C.c_double, C.c_double, C.c_int64, _d, C.c_double, C.c_double,
+                                         _d, _d]),
+    "frico_write_linear_system_h5": (C.c_int, [C.c_char_p, C.c_int64, _d, C.c_int64, C.c_int64, _d, C.c_int64, _d,
+                                               C.c_int64]),
+    "frico_dump_linear_system": (C.c_int, [C.c_void_p, C.c_char_p]),
     "frico_msh_read": (C.c_int, [C.c_char_p, C.POINTER(_vp)]),
     "frico_msh_free": (C.c_int, [_vp]),
     "frico_msh_sizes": (C.c_int, [_vp, _i64, _i64, _i64, _i32]),

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfrico_hip.so")
-SOURCES = ["fill.hip", "lu.hip", "adjoint.hip", "capi.cpp", "adjoint_host.cpp", "maxwell.cpp", "batch.cpp", "planes.cpp", "outputs.cpp", "msh.cpp", "noalloc.hip", "gemm_f16.hip", "blocksolve.hip"]
+SOURCES = ["fill.hip", "lu.hip", "adjoint.hip", "capi.cpp", "adjoint_host.cpp", "maxwell.cpp", "batch.cpp", "planes.cpp", "outputs.cpp", "msh.cpp", "noalloc.hip", "gemm_f16.hip", "blocksolve.hip", "h5write.cpp"]
 ARCH = "gfx950"
 
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"

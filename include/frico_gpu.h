@@ -244,6 +244,30 @@ int frico_write_bistatic_rcs(const char* path, const double* table);
 int frico_format_double(double v, char* buf, int64_t size);
 
 /*
+ * HDF5 outputs (no HDF5 library needed; the file structures of the reference's own output,
+ * share/hdf5/antenna_analysis/biquad.h5: superblock 0, version-1 object headers, symbol-table groups,
+ * contiguous datasets, std::complex<double> as the compound {r, i}):
+ * frico_write_antenna_h5: write_hdf5 (src/emw_ana_antenna.cpp:207-234): /frico/antenna_analysis/
+ *   rdiag_dist, sweep_steps (uint64), Z0 and per step <n>/frequency, gain_XY, gain_YZ, gain_XZ (360
+ *   each), port/voltage, current, power, impedance, gamma (complex) and port/swr.  port = nsteps x 10
+ *   and gain = nsteps x 1080 as frico_antenna_sweep's port_out and gain_out; (v_re, v_im) the gap
+ *   voltage.
+ * frico_write_linear_system_h5: the -d dump (src/emw_ana_antenna.cpp:260-268): /frico/Z (n x n,
+ *   entry [i][j] = Z(i, j) of the column-major Z), /frico/V and /frico/I (n x nrhs; I may be NULL,
+ *   as in the bistatic sweep's dump, src/emw_solver.cpp:494-500).
+ * frico_dump_linear_system: the same for a context: its complex128 matrix as it stands (the filled
+ *   Z, or after frico_factor(FRICO_SOLVE_C128) the LU factors, as the reference's dump holds
+ *   zgetrf's in-place factors), its right-hand sides and, if solved, its solution.
+ */
+int frico_write_antenna_h5(const char* path, double rdiag_dist, double Z0, int64_t nsteps,
+                           const double* freqs, double v_re, double v_im, const double* port,
+                           const double* gain);
+int frico_write_linear_system_h5(const char* path, int64_t n, const double* Z, int64_t ldz,
+                                 int64_t nrhs, const double* V, int64_t ldv, const double* I,
+                                 int64_t ldi);
+int frico_dump_linear_system(frico_ctx* ctx, const char* path);
+
+/*
  * Frequency points of a sweep as init_sweep builds them (src/emw_solver.cpp:460-471): start, then
  * freq += step while freq <= end (accumulated, so rounding can drop the end point).  *count
  * receives the number of points; freqs (max_freqs entries, may be NULL) receives them.  Returns -5

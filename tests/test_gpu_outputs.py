@@ -36,3 +36,55 @@ def test_bistatic_text_output_round_trip(gpu_ctx, tmp_path):
     back = np.loadtxt(tmp_path / "bistatic_rcs_0.txt")
     np.testing.assert_array_equal(back[:, 0], np.arange(359))
     np.testing.assert_array_equal(back[:, 1:], tab)
+
+
+def test_antenna_sweep_hdf5_output_round_trip(gpu_ctx, tmp_path):
+    """write_hdf5's layout (src/emw_ana_antenna.cpp:207-234) from a GPU antenna sweep, read back."""
+    from frico_amd.h5 import H5File
+    sm = meshes.workload_dipole_c1()
+    gpu_ctx.load_mesh(sm, 5)
+    freqs = [2.5e8, 3e8, 3.5e8]
+    res = gpu_ctx.antenna_sweep(freqs, sm.feed, mode="approx")
+    path = tmp_path / "antenna.h5"
+    fa.write_antenna_h5_results(path, freqs, res, rdiag_dist=10.0, Z0=50.0)
+    port = np.array([[r["current"].real, r["current"].imag, r["power"].real, r["power"].imag, r["impedance"].real,
+                      r["impedance"].imag] for r in res])
+    gain = np.array([r["gain"].reshape(1080) for r in res])
+    f = H5File(str(path))
+    base = "/frico/antenna_analysis/"
+    assert int(f[base + "sweep_steps"]) == 3
+    for s in range(3):
+        p = f"{base}{s}/"
+        assert float(f[p + "frequency"]) == freqs[s]
+        np.testing.assert_array_equal(f[p + "gain_XY"], gain[s, :360])
+        np.testing.assert_array_equal(f[p + "gain_XZ"], gain[s, 720:])
+        z = complex(f[p + "port/impedance"])
+        assert z == complex(port[s, 4], port[s, 5])
+        g = complex(f[p + "port/gamma"])
+        assert abs(g - (z - 50.0) / (z + 50.0)) <= 1e-12
+        assert abs(complex(f[p + "port/current"]) * z - 1.0) <= 1e-12
+
+
+def test_linear_system_dump_of_a_context(gpu_ctx, tmp_path):
+    """The -d dump (src/emw_ana_antenna.cpp:260-268) after a complex128 solve: as in the reference,
+    where zgetrf has overwritten Z in place, /frico/Z holds the factors; V and I are the system's."""
+    from frico_amd.h5 import H5File
+    sm = meshes.workload_dipole_c1()
+    gpu_ctx.load_mesh(sm, 5)
+    gpu_ctx.fill(3e8, mode="quad")
+    Z = gpu_ctx.matrix()
+    gpu_ctx.rhs_deltagap(3e8, sm.feed, 1.0)
+    gpu_ctx.dump_linear_system(tmp_path / "frico_linear_system_0.h5")
+    f = H5File(str(tmp_path / "frico_linear_system_0.h5"))
+    np.testing.assert_array_equal(f["/frico/Z"], Z)
+    V = f["/frico/V"]
+    assert V.shape == (Z.shape[0], 1)
+    assert "I" not in f.keys("/frico")
+    gpu_ctx.factor("c128")
+    gpu_ctx.solve()
+    gpu_ctx.dump_linear_system(tmp_path / "frico_linear_system_1.h5")
+    f = H5File(str(tmp_path / "frico_linear_system_1.h5"))
+    I = f["/frico/I"][:, 0]
+    assert np.max(np.abs(Z @ I - V[:, 0])) <= 1e-9 * np.max(np.abs(V))
+    LU = f["/frico/Z"]
+    assert not np.array_equal(LU, Z)  # the factors, as the reference's dump after zgetrf
