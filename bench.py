@@ -130,9 +130,9 @@ def batch_bench(args):
     with frico_amd.Batch(0, workers=args.workers) as b:
         for sm in ms:
             b.add_mesh(sm)
-        b.antenna(fr, fill="c64", form="c64_ir", tol=1e-10)  # warm-up: allocation, tuning table
+        b.antenna(fr, fill="c64", form=args.batch_form, tol=1e-10)  # warm-up: allocation, tuning table
         t0 = time.perf_counter()
-        z, info = b.antenna(fr, fill="c64", form="c64_ir", tol=1e-10)
+        z, info = b.antenna(fr, fill="c64", form=args.batch_form, tol=1e-10)
         t = time.perf_counter() - t0
         nb, ngraph = b.block_sizes()
         used = b.workers_used()
@@ -144,12 +144,17 @@ def batch_bench(args):
     F = sum(8.0 * n ** 3 / 3.0 + 8.0 * n ** 2 for n in N) * len(fr)
     nsys = len(ms) * len(fr)
     ok = all(i.converged and i.max_rel_residual <= 1e-10 for row in info for i in row)
+    tf = F / t / 1e12
     out = {"metric": "uneven batch systems/s (fill + LU + refinement + Zin)", "value": nsys / t,
            "unit": "systems/s", "n_gpus": 1, "higher_is_better": True, "seconds": t,
-           "tflops": F / t / 1e12, "dtype": "c64 fill + c64 LU + c128 residual", "data": "synthetic",
+           "tflops": tf, "dtype": "c64 fill + c64 LU + c128 residual", "data": "synthetic",
+           # the whole batch's LU flops (8N^3/3 + 8N^2 a system) over its wall time, against the
+           # dense FP32 MFMA peak (the complex64 LU's arithmetic class)
+           "roofline": {"bound": "mfma", "achieved": tf, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
+                        "frac": tf / PEAK_FP32_MFMA, "traffic": None},
            "config": {"workload": f"{len(ms)} strip dipoles/loops, N {min(N)}..{max(N)}, {len(fr)} frequencies "
                                   f"100..500 MHz", "workers": used, "hw_queues": os.environ["GPU_MAX_HW_QUEUES"],
-                      "graphs": True,
+                      "graphs": True, "form": args.batch_form,
                       "block_widths": sorted(set(nb.tolist())), "graph_meshes": ngraph},
            "all_converged": ok}
     print(json.dumps(out))
@@ -272,6 +277,8 @@ def main():
                          "approx: the same plate in approximate mode with the symmetric solve; "
                          "nearfield: E and H on a 90601-point sampling plane over that plate")
     ap.add_argument("--workers", type=int, default=16, help="batch workload: concurrent contexts")
+    ap.add_argument("--batch-form", default="c64_ir", choices=["c64_ir", "c64_np_ir"],
+                    help="batch workload: factorisation form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported by this build "

@@ -8,9 +8,10 @@
 //   - per unit the worker loads the mesh, orients the delta gap once, and picks the LU block width
 //     from a table filled by timing the candidates on the first system of each size class;
 //   - per frequency: impedance fill and delta-gap RHS are enqueued eagerly; the complex64
-//     conversion, the LU and the first refinement sweep are one HIP graph, built on the second
-//     frequency from the recorded kernel launches and replayed for the rest, and every further
-//     sweep is a second graph.  The graphs are assembled node by node on the worker's thread
+//     conversion, the LU (pivoted, or without pivoting for FRICO_SOLVE_C64_NP_IR, which returns
+//     to the pivoted LU eagerly on a zero pivot or a stalled refinement) and the first refinement
+//     sweep are one HIP graph, built on the second frequency from the recorded kernel launches and
+//     replayed for the rest, and every further sweep is a second graph.  The graphs are assembled node by node on the worker's thread
 //     (no stream capture), so the other workers' launches and allocations cannot disturb them.
 // Co-residency: a panel launch needs ceil(N / R) workgroups resident at once (one per CU), so the
 // worker count is capped at CUs / ceil(Nmax / R); it is also capped at the process's hardware
@@ -160,7 +161,7 @@ int solve_graphs(frico_ctx* c, const RunOpts& o, Graph& g1, Graph& g2, frico_sol
         return rc;
     FRICO_HIP_CHECK(hipGraphLaunch(g1.x, c->st));
     if ((rc = factor_finish(c, o.form, wk)))
-        return rc;
+        return rc; // > 0: a zero pivot (the unpivoted form); the caller takes the pivoted LU
     frico_solve_info inf{0, 0, -1.0};
     for (int it = 1; it <= o.maxit; it++) {
         if (it > 1)
@@ -199,7 +200,10 @@ int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, double freq, int*
             return 0;
         }
     }
-    const int cand[2] = {64, 32};
+    // the unpivoted form's wider steps (nb = 128: paired trailing products) pay from a few
+    // thousand unknowns; the pivoted panel's per-column cost favours narrower blocks
+    const bool np = o.form == FRICO_SOLVE_C64_NP_IR && c->N >= 1536;
+    const int cand[2] = {np ? 128 : 64, np ? 64 : 32};
     double best = 1e300;
     int best_nb = 64;
     int rc = 0;
@@ -262,8 +266,17 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
                 return rc;
             b->mesh_nb[mi] = nb;
             rc = solve_eager(c, o, &inf);
-        } else if (o.graphs && o.form == FRICO_SOLVE_C64_IR) {
+        } else if (o.graphs && (o.form == FRICO_SOLVE_C64_IR || o.form == FRICO_SOLVE_C64_NP_IR)) {
             rc = solve_graphs(c, o, g1, g2, &inf);
+            if (o.form == FRICO_SOLVE_C64_NP_IR && (rc > 0 || rc == FRICO_ERR_NOCONV)) {
+                // a zero pivot or a stalled refinement of the unpivoted factors: the pivoted LU, as
+                // frico_factor and frico_solve do; the graphs are recorded again next time
+                g1.reset();
+                g2.reset();
+                RunOpts op = o;
+                op.form = FRICO_SOLVE_C64_IR;
+                rc = solve_eager(c, op, &inf);
+            }
         } else {
             rc = solve_eager(c, o, &inf);
         }

@@ -154,14 +154,18 @@ def test_symmetric_form_is_rejected_in_a_batch():
 
 def test_uneven_workload_32_meshes_16_frequencies():
     """32 strip dipoles and loops with N ~ U[500, 4000] at 16 frequencies from 100 to 500 MHz,
-    complex64 fill, complex64 LU with refinement to 1e-10, graphs and autotuning on.  Every system
-    converges.  Two of the meshes, swept again in a complex128-fill batch, match single-context
-    complex128 sweeps to 1e-8.  The complex64 fill itself moves Zin by about cond(Z) times its
-    rounding (6e-8): up to ~3e-3 relative for the electrically small dipoles at 100 MHz, whose
-    radiation resistance is a tiny part of |Zin|, so the complex64 batch is held to 1e-2."""
+    complex64 fill, the pivoted complex64 LU with refinement to 1e-10, graphs and autotuning on
+    (the unpivoted form returns to the pivoted LU on about one system in seven of these small,
+    low-frequency antennas, which costs more than it saves).  Every system converges.  Three of the meshes, swept again in a
+    complex128-fill batch, match the oracle (FP64 fill + LAPACK, oracle.antenna_analysis) to 1e-8.
+    The complex64 fill is held to 1e-2 against the same oracle values: rounding Z to complex64 is a
+    relative perturbation of ~6e-8 per entry, which moves Zin by up to cond(Z) times that; for the
+    electrically small dipoles at 100 MHz, whose radiation resistance is a tiny part of |Zin|, that
+    reaches ~3e-3, and the refinement (against the complex64 Z it was given) cannot remove it."""
     import time
 
     import frico_amd
+    import oracle
     ms = meshes.batch_workload()
     fr = meshes.batch_frequencies()
     with frico_amd.Batch(0, workers=16) as b:
@@ -175,17 +179,16 @@ def test_uneven_workload_32_meshes_16_frequencies():
           f"{sorted(set(nb.tolist()))}, {ngraph} meshes replayed graphs")
     assert all(i.converged and i.max_rel_residual <= 1e-10 for row in info for i in row)
     assert np.all(np.isfinite(z))
-    pick, fsel = (0, 17), [fr[0], fr[9]]
-    with frico_amd.Batch(0, workers=2) as b:
+    assert ngraph == len(ms)
+    pick, fsel = (7, 17, 14), (0, 9)
+    with frico_amd.Batch(0, workers=3) as b:
         for m in pick:
             b.add_mesh(ms[m])
-        z128, _ = b.antenna(fsel, fill="c128", form="c64_ir", tol=1e-12)
-    with frico_amd.Context(0) as ctx:
-        for i, m in enumerate(pick):
-            ctx.load_mesh(ms[m], 5)
-            ref = ctx.antenna_sweep(fsel, ms[m].feed, form="c128")
-            for j, fi in enumerate((0, 9)):
-                zr = ref[j]["impedance"]
-                assert abs(z128[i, j] - zr) <= 1e-8 * abs(zr), (m, fi, z128[i, j], zr)
-                print(f"mesh {m} f{fi}: complex64-fill Zin error {abs(z[m, fi] - zr) / abs(zr):.2e}")
-                assert abs(z[m, fi] - zr) <= 1e-2 * abs(zr), (m, fi, z[m, fi], zr)
+        z128, _ = b.antenna([fr[i] for i in fsel], fill="c128", form="c64_ir", tol=1e-12)
+    for i, m in enumerate(pick):
+        for j, fi in enumerate(fsel):
+            zr = oracle.antenna_analysis(ms[m], fr[fi], 5)["port"]["impedance"]
+            e128, e64 = abs(z128[i, j] - zr) / abs(zr), abs(z[m, fi] - zr) / abs(zr)
+            print(f"mesh {m} f{fi}: complex128-fill Zin error {e128:.2e}, complex64-fill {e64:.2e}")
+            assert e128 <= 1e-8, (m, fi, z128[i, j], zr)
+            assert e64 <= 1e-2, (m, fi, z[m, fi], zr)
