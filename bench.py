@@ -39,9 +39,12 @@ PEAK_HBM = 8000.0       # GB/s
 # default form, 699.7e6; plate_N12160_c64ir_hbm_traffic.md for the pivoted one, 700.1e6);
 # algorithmic bytes per launch: 419.3e6.
 GEMM_F32_TRAFFIC = {"c64_np_ir": 699.7e6, "c64_ir": 700.1e6}
-# HBM bytes per timed launch of cgemm_sub_f16 at N = 12160, nb = 128 (rocprofv3 FETCH_SIZE / WRITE_SIZE,
-# scaled by a copy kernel of known size: profiles/plate_N12160_c64np_hbm_traffic.md).
-GEMM_F16_TRAFFIC = 482.5e6
+# HBM bytes per timed launch of cgemm_sub_f16 at N = 12160, nb = 128, averaged over the 235 launches of
+# one factorisation in paired steps (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of
+# known size: profiles/plate_N12160_c64np_hbm_traffic.md).
+GEMM_F16_TRAFFIC = 275.4e6
+# HBM bytes of one quadrature-mode fill at N = 12160 (tripair_f64 + assemble + near_f64; same profile).
+FILL_TRAFFIC = 31.0e9
 PEAK_FP16_MFMA = 2500.0  # TFLOP/s, MI355X dense FP16 matrix
 
 
@@ -358,13 +361,18 @@ def main():
     gemm_avg_ms = prof_ir["gemm_ms"] / max(prof_ir["gemm_launches"], 1)
     f16 = args.form == "c64_np_ir" and os.environ.get("FRICO_GEMM_F16", "1") != "0"
     if f16:
-        # the trailing products on fp16 MFMA are bound by their C read-modify-write in HBM
+        # The trailing products on fp16 MFMA: a K = 128 update reads and writes C once per 3072 fp16
+        # flops an entry (192 flop/B, under the MI355X ridge of 312: HBM-bound); the paired steps'
+        # K = 256 updates do twice the flops per C byte (384 flop/B, just over the ridge).  The line
+        # gives the HBM view over all launches and, beside it, the same launches' fp16 MFMA rate
+        # (three split-term products of four real MFMAs per complex multiply-add: 3 x 8 MNK).
         roofline = {"bound": "hbm", "kernel": "cgemm_sub_f16 (LU trailing update, fp16 MFMA on split operands)",
                     "achieved": gemm_gbs, "peak": PEAK_HBM, "unit": "GB/s", "frac": gemm_gbs / PEAK_HBM,
                     "traffic": GEMM_F16_TRAFFIC if (args.n, args.nb) == (64, 128) else None,
                     "algorithmic_bytes_per_launch": prof_ir["gemm_bytes"] / max(prof_ir["gemm_launches"], 1),
                     "launches": prof_ir["gemm_launches"], "avg_launch_ms": gemm_avg_ms,
-                    "complex_tflops": gemm_tf, "fp16_mfma_peak_tflops": PEAK_FP16_MFMA}
+                    "complex_tflops": gemm_tf, "fp16_mfma_tflops": 3.0 * gemm_tf,
+                    "fp16_mfma_peak_tflops": PEAK_FP16_MFMA, "fp16_mfma_frac": 3.0 * gemm_tf / PEAK_FP16_MFMA}
     else:
         roofline = {"bound": "mfma", "kernel": "cgemm_sub<float> (LU trailing update)",
                     "achieved": gemm_tf, "peak": PEAK_FP32_MFMA, "unit": "TFLOP/s",
@@ -400,7 +408,11 @@ def main():
         "phases_ms": {k: prof_ir[k] / args.steps for k in ("fill_ms", "rhs_ms", "factor_ms", "solve_ms", "refine_ms")},
         "fill": {"ms": fill_ms, "z_write_GBps": 16.0 * N * N / (fill_ms * 1e-3) / 1e9,
                  "tripair_ms": prof_ir["tripair_ms"] / args.steps,
-                 "geval_per_s": 49.0 * T * T / max(prof_ir["tripair_ms"] / args.steps * 1e-3, 1e-12)},
+                 "geval_per_s": 49.0 * T * T / max(prof_ir["tripair_ms"] / args.steps * 1e-3, 1e-12),
+                 # measured HBM bytes of the fill (the triangle-pair moments' round trip through HBM)
+                 "hbm_traffic_GB": FILL_TRAFFIC / 1e9 if (args.n, args.nb) == (64, 128) else None,
+                 "hbm_frac": (FILL_TRAFFIC / (fill_ms * 1e-3) / 1e9 / PEAK_HBM) if (args.n, args.nb) == (64, 128)
+                 else None},
         "fp64_form": fp64,
         "speedup_vs_fp64": (fp64["ms_per_step"] / (t_ir * 1e3)) if fp64 else None,
     }

@@ -141,7 +141,15 @@ int solve_graphs(frico_ctx* c, const RunOpts& o, Graph& g1, Graph& g2, frico_sol
 {
     LuWork wk;
     int rc = factor_prepare(c, o.form, wk, false);
-    if (rc || (rc = refine_prepare(c)))
+    if (rc)
+        return rc;
+    // the state factor_finish leaves, set before recording: the recorded sweeps choose their path
+    // by it (the factors' form, the right-hand sides carried through the factorisation)
+    c->factored = o.form;
+    c->aug_valid = c->aug_cols > 0;
+    c->aug_version = c->rhs_version;
+    c->wv_ok = wk.wv != nullptr;
+    if ((rc = refine_prepare(c)))
         return rc;
     std::vector<double> bn;
     if ((rc = col_norms(c, c->B.p, c->nrhs, bn)))

@@ -930,7 +930,8 @@ int factor_prepare(frico_ctx* c, int form, LuWork& wk, bool lookahead)
     // W products for the fused solves: one more N x N matrix of the factor's type, taken only
     // when it fits with room to spare
     c->wv_ok = false;
-    if (c->fused_solve && c->nb_factored % 64 == 0 && N > 64) {
+    const bool blocked = form == FRICO_SOLVE_C64_NP_IR && lu_blocked_solves();
+    if (c->fused_solve && c->nb_factored % 64 == 0 && N > 64 && !blocked) {
         const size_t bytes = (size_t)(N * N) * (form == FRICO_SOLVE_C128 ? sizeof(c128) : sizeof(c64));
         size_t fr = 0, tot = 0;
         if (c->wv.n >= bytes || (hipMemGetInfo(&fr, &tot) == hipSuccess && fr > bytes + c->wv.n + ((size_t)2 << 30))) {
@@ -1064,7 +1065,8 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
     // Z, the second stream, its events and the carried or fused solve
     ResidStream rs;
     const bool stream = overlap && c->stream_residual && c->side && c->zprec == FRICO_C128 && !c->z64_factored &&
-                        (carried || c->wv_ok) && (int64_t)c->lu_events.size() >= (N + 127) / 128 + 1;
+                        (carried || c->wv_ok || (c->factored == FRICO_SOLVE_C64_NP_IR && lu_blocked_solves())) &&
+                        (int64_t)c->lu_events.size() >= (N + 127) / 128 + 1;
     if (stream) {
         rs.side = c->side;
         rs.ev = c->lu_events.data();
@@ -1199,7 +1201,8 @@ int frico_solve(frico_ctx* c, double tol, int maxit, frico_solve_info* info)
         }
         if (inf.converged || !sym)
             break;
-        if ((rc = frico_factor(c, FRICO_SOLVE_C64_IR)))
+        // the pivoted LU's fused solves need buffers the unpivoted factors' blocked solves did not
+        if ((rc = frico_factor(c, FRICO_SOLVE_C64_IR)) || (rc = refine_prepare(c)))
             return rc;
     }
     c->have_x = true;

@@ -2190,6 +2190,8 @@ static int64_t fused_step_max_cols()
     return v;
 }
 
+static bool solve_blocked();
+
 // ----------------------------------------------------------------------------------------------
 // Paired steps of the unpivoted complex64 LU (fp16 products, look-ahead).  Steps a (even) and b
 // = a + 1 share their trailing product: after both block columns are factored, the rows of both
@@ -2480,7 +2482,8 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         if (pairs_on(wk, f16, direct, nb)) {
             if ((rc = lu_nopiv_pairs(wk, reinterpret_cast<c64*>(A), n, lda, nb, st, lookahead, gemm_timer, naug)))
                 return rc;
-            return solve_tables<T>(wk, A, n, lda, nb, st);
+            // the blocked solves apply the factors directly: no inverse or W tables
+            return solve_blocked() ? 0 : solve_tables<T>(wk, A, n, lda, nb, st);
         }
     }
     if (!lookahead) {
@@ -2542,6 +2545,10 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
         }
         FRICO_HIP_CHECK(hipEventRecord(ev0, wk.side));
         FRICO_HIP_CHECK(hipStreamWaitEvent(st, ev0, 0));
+    }
+    if constexpr (c64f) {
+        if (solve_blocked())
+            return 0; // the blocked solves apply the factors directly: no inverse or W tables
     }
     return solve_tables<T>(wk, A, n, lda, nb, st);
 }
@@ -2722,6 +2729,8 @@ static bool solve_blocked()
     }();
     return on;
 }
+
+bool lu_blocked_solves() { return solve_blocked(); }
 
 // Residual product of one finished block of the solution beside the substitution (see lu_backward).
 static int resid_block(const ResidStream* rs, int64_t n, int64_t kb, int64_t bw, const c64* X, int64_t ldx,

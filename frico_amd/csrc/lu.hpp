@@ -135,6 +135,9 @@ struct ResidStream {
     c128* R2 = nullptr;
     bool exact = true; // four real products per complex multiply-add (see gemm_sub)
 };
+// Whether complex64 solves without interchanges take the blocked path (which needs neither the
+// diagonal-block inverses nor the W products).
+bool lu_blocked_solves();
 // perm: the factorisation's interchange lists, or null for factors without interchanges.
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
