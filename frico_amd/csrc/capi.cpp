@@ -174,6 +174,8 @@ int frico_ctx_create(int device, frico_ctx** out)
         c->stream_residual = std::atoi(sr) != 0;
     if (const char* rp = std::getenv("FRICO_RESID_PAIR"))
         c->resid_pair = std::atoi(rp) != 0;
+    if (const char* sf = std::getenv("FRICO_SOLVE_F16"))
+        c->solve_f16 = std::atoi(sf) != 0;
     if (const char* gf = std::getenv("FRICO_GEMM_F16"))
         c->gemm_f16 = std::atoi(gf) != 0;
     if (const char* ds = std::getenv("FRICO_LU_DIRECT_SOLVES"))
@@ -948,6 +950,7 @@ int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
 {
     const int64_t N = c->N;
     int rc;
+    c->sexp_ok = false;
     phase_begin(c, c->t_factor);
     if (form == FRICO_SOLVE_C128) {
         rc = lu_factor<double>(wk, c->Z64.p, N, N, c->ipiv.p, c->nb_factored, c->st, c->profiling ? &c->t_gemm : nullptr);
@@ -965,6 +968,16 @@ int factor_enqueue(frico_ctx* c, int form, LuWork& wk)
             rc = lu_nopiv_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
         else if (!rc)
             rc = lu_factor<float>(wk, c->LU32.p, N, N, c->ipiv.p, c->nb_factored, c->st, kt, c->aug_cols);
+        // factors without interchanges are solved blockwise: their row exponents per block of the
+        // solves' columns, for the solves' split products
+        if (!rc && c->solve_f16 && lu_blocked_solves() &&
+            (form == FRICO_SOLVE_C64_NP_IR || form == FRICO_SOLVE_C64_SYM_IR)) {
+            const int W = lu_solve_block_rows();
+            const int64_t nblk = (N + W - 1) / W;
+            if (!(rc = c->sexp.ensure((size_t)(nblk * N))) &&
+                !(rc = launch_rowexp_blocks(c->LU32.p, N, N, W, c->sexp.p, N, c->st)))
+                c->sexp_ok = true;
+        }
     }
     phase_end(c, c->t_factor);
     return rc;
@@ -1050,6 +1063,9 @@ int refine_prepare(frico_ctx* c)
         return rc;
     if (c->stream_residual && c->resid_pair && (rc = c->R2.ensure((size_t)(N * nr))))
         return rc;
+    // (also before a factorisation recorded into a graph, which then sets sexp_ok)
+    if (c->solve_f16 && (rc = c->bexp.ensure((size_t)(2 * nr))))
+        return rc;
     if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c64))))
         return rc;
     return 0;
@@ -1079,6 +1095,12 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
         rs.ldr = N;
         rs.exact = c->resid_exact;
         rs.R2 = c->resid_pair && c->R2.p ? c->R2.p : nullptr;
+        if (c->sexp_ok && c->bexp.p && (c->factored == FRICO_SOLVE_C64_NP_IR || c->factored == FRICO_SOLVE_C64_SYM_IR)) {
+            rs.aexp = c->sexp.p;
+            rs.ldae = N;
+            rs.bexp = c->bexp.p;
+            rs.bexp2 = c->bexp.p + nr;
+        }
     }
     phase_begin(c, tt);
     if (first && (rc = launch_zero_bytes(c->X.p, (size_t)(N * nr) * sizeof(c128), c->st)))

@@ -210,6 +210,12 @@ struct frico_ctx {
     DevBuf<unsigned char> ysolve; // n x nrhs scratch of the fused solves
     DevBuf<c64> tbuf;             // transposed block column of the unpivoted LU
     DevBuf<unsigned char> hsplit; // fp16 split operands of the unpivoted LU's trailing products
+    // Split products of the complex64 solves (FRICO_SOLVE_F16=0: fp32 products): the factors' row
+    // exponents over each block of lu_solve_block_rows() columns (valid after an unpivoted or
+    // symmetric complex64 factorisation), and two scratch vectors of the solution's column exponents
+    DevBuf<int> sexp, bexp;
+    bool sexp_ok = false;
+    bool solve_f16 = true;
     bool gemm_f16 = true;         // FRICO_GEMM_F16=0: fp32 trailing products in the unpivoted LU
     bool fused_step = true;       // FRICO_LU_FUSED_STEP=0: the unpivoted LU's block solves and fp16 splits as separate launches
     bool direct_solves = false;   // FRICO_LU_DIRECT_SOLVES=1: lane-per-row block solves in the unpivoted LU

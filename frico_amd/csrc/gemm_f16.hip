@@ -323,6 +323,257 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
     }
 }
 
+// ----------------------------------------------------------------------------------------------
+// Products of the triangular solves on fp16 MFMA, split as the operands are staged.
+//   C[M x N] -= A[M x K] B[K x N], all complex64 and column-major (A a block of the LU factors, B
+//   a block of the solution), A's row r scaled by 2^ea[r] and B's column j by 2^eb[j] before the
+//   fp16 hi/lo split, C -= 2^-(ea + eb) D in the epilogue.  The exponents put the largest |re| or
+//   |im| of the row (column) over the product's K in [2^14, 2^15), as split_f16 does, so the
+//   arithmetic is that of the factorisation's split products.
+// Exponents: rowexp_blocks gives, for every block of W columns of a matrix, each row's exponent
+// over the block (one table for the solves' blocks of the factors); colexp each column's over K
+// rows of a block of the solution.
+// ----------------------------------------------------------------------------------------------
+__device__ inline int split_exponent(float mx)
+{
+    if (!(mx > 0.0f))
+        return 0;
+    const int e = 14 - ilogbf(mx);
+    return e < -100 ? -100 : (e > 200 ? 200 : e);
+}
+
+__global__ __launch_bounds__(256) void rowexp_blocks(const c64* __restrict__ A, int64_t lda, int64_t n, int W,
+                                                     int* __restrict__ E, int64_t ldE)
+{
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = blockIdx.y, c0 = b * W, c1 = (c0 + W) < n ? (c0 + W) : n;
+    if (r >= n)
+        return;
+    float mx = 0.0f;
+    for (int64_t j = c0; j < c1; j++) {
+        const c64 v = A[r + j * lda];
+        mx = fmaxf(mx, fmaxf(fabsf(v.re), fabsf(v.im)));
+    }
+    E[b * ldE + r] = split_exponent(mx);
+}
+
+__global__ __launch_bounds__(256) void colexp(const c64* __restrict__ B, int64_t ldb, int64_t K, int* __restrict__ E)
+{
+    __shared__ float red[4];
+    const int64_t j = blockIdx.x;
+    float mx = 0.0f;
+    for (int64_t k = threadIdx.x; k < K; k += 256) {
+        const c64 v = B[k + j * ldb];
+        mx = fmaxf(mx, fmaxf(fabsf(v.re), fabsf(v.im)));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1)
+        mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if ((threadIdx.x & 63) == 0)
+        red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        E[j] = split_exponent(fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3])));
+}
+
+// x 2^e = hi + lo for eight values (one 16-byte segment of each of the four planes)
+__device__ inline void split8(const c64 (&v)[8], int e, half8& rh, half8& rl, half8& ih, half8& il)
+{
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const float xr = ldexpf(v[q].re, e), xi = ldexpf(v[q].im, e);
+        const _Float16 hr = (_Float16)xr, hi = (_Float16)xi;
+        rh[q] = hr;
+        rl[q] = (_Float16)(xr - (float)hr);
+        ih[q] = hi;
+        il[q] = (_Float16)(xi - (float)hi);
+    }
+}
+
+// Tile 128 x 64 (2 x 2 waves of 64 x 32), 32-deep k chunks.  A: thread t stages rows t % 128,
+// k segments 2 (t / 128) and 2 (t / 128) + 1 (eight k each; a load instruction reads 64
+// consecutive rows of one column); B: column t / 4, k segment t % 4 (64 contiguous bytes).
+__global__ __launch_bounds__(256, 2) void cgemm_sub_f16s(int64_t M, int64_t N, int64_t K, const c64* __restrict__ A,
+                                                         int64_t lda, const int* __restrict__ ea,
+                                                         const c64* __restrict__ B, int64_t ldb,
+                                                         const int* __restrict__ eb, c64* __restrict__ C, int64_t ldc,
+                                                         int64_t gx, int64_t gy)
+{
+    constexpr int RB = 4, CB = 2, BM = 128, BN = 64;
+    __shared__ half8 sA[4][BM][4];
+    __shared__ half8 sB[4][BN][4];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int wm = wv & 1, wn = wv >> 1;
+    int64_t tm, tn;
+    if (!tile_of(blockIdx.x, gx, gy, tm, tn))
+        return;
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    // staging: A row ar, segments as0, as0 + 1; B column bc, segment bs
+    const int ar = tid % BM, as0 = 2 * (tid / BM), bc = tid / 4, bs = tid % 4;
+    const int64_t arow = m0 + ar, bcol = n0 + bc;
+    const bool aok = arow < M, bok = bcol < N;
+    const int eA = aok ? ea[arow] : 0, eB = bok ? eb[bcol] : 0;
+    const c64* __restrict__ Ar = A + (aok ? arow : 0);
+    const c64* __restrict__ Bc = B + (bok ? bcol : 0) * ldb;
+
+    f32x4 are[RB][CB], aim[RB][CB];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+        for (int b = 0; b < CB; b++) {
+            are[a][b] = f32x4{0, 0, 0, 0};
+            aim[a][b] = f32x4{0, 0, 0, 0};
+        }
+    for (int64_t k0 = 0; k0 < K; k0 += HKC) {
+        // branch-free loads (out-of-range ones read element 0 and are zeroed after), all in flight
+        // together
+        c64 va[2][8], vb[8];
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int64_t k = k0 + 8 * (as0 + h) + q;
+                va[h][q] = Ar[(k < K ? k : 0) * lda];
+            }
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int64_t k = k0 + 8 * bs + q;
+            vb[q] = Bc[k < K ? k : 0];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (!(aok && k0 + 8 * (as0 + h) + q < K))
+                    va[h][q] = c64{0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (!(bok && k0 + 8 * bs + q < K))
+                vb[q] = c64{0.0f, 0.0f};
+        __syncthreads(); // the previous chunk's fragments have been read
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            half8 rh, rl, ih, il;
+            split8(va[h], eA, rh, rl, ih, il);
+            const int sg = hseg(ar, as0 + h);
+            sA[0][ar][sg] = rh;
+            sA[1][ar][sg] = rl;
+            sA[2][ar][sg] = ih;
+            sA[3][ar][sg] = il;
+        }
+        {
+            half8 rh, rl, ih, il;
+            split8(vb, eB, rh, rl, ih, il);
+            const int sg = hseg(bc, bs);
+            sB[0][bc][sg] = rh;
+            sB[1][bc][sg] = rl;
+            sB[2][bc][sg] = ih;
+            sB[3][bc][sg] = il;
+        }
+        __syncthreads();
+        const int sgl = lane >> 4;
+        half8 bf[CB][4];
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++) {
+            const int col = wn * 16 * CB + cb * 16 + (lane & 15);
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+                bf[cb][p] = sB[p][col][hseg(col, sgl)];
+        }
+#pragma unroll
+        for (int rb = 0; rb < RB; rb++) {
+            const int row = wm * 16 * RB + rb * 16 + (lane & 15);
+            half8 af[4];
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+                af[p] = sA[p][row][hseg(row, sgl)];
+            const half8 nih = -af[2], nil = -af[3];
+#pragma unroll
+            for (int cb = 0; cb < CB; cb++) {
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[0], are[rb][cb], 0, 0, 0);
+                aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[2], aim[rb][cb], 0, 0, 0);
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][1], af[0], are[rb][cb], 0, 0, 0);
+                aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][1], af[2], aim[rb][cb], 0, 0, 0);
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[1], are[rb][cb], 0, 0, 0);
+                aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][0], af[3], aim[rb][cb], 0, 0, 0);
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], nih, are[rb][cb], 0, 0, 0);
+                aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], af[0], aim[rb][cb], 0, 0, 0);
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][3], nih, are[rb][cb], 0, 0, 0);
+                aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][3], af[0], aim[rb][cb], 0, 0, 0);
+                are[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], nil, are[rb][cb], 0, 0, 0);
+                aim[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(bf[cb][2], af[1], aim[rb][cb], 0, 0, 0);
+            }
+        }
+    }
+    // C -= 2^-(ea + eb) D^T: lane holds row (lane & 15) of each row block and columns 4 (lane >> 4) + r
+    const int64_t rbase = m0 + wm * 16 * RB + (lane & 15), cbase = n0 + wn * 16 * CB + 4 * (lane >> 4);
+    int ecol[CB][4];
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t col = cbase + cb * 16 + r;
+            ecol[cb][r] = eb[col < N ? col : 0];
+        }
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++) {
+        const int64_t row = rbase + rb * 16;
+        if (row >= M)
+            continue;
+        const int erow = ea[row];
+        c64 cr[CB][4];
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t col = cbase + cb * 16 + r;
+                const int64_t ok = col < N;
+                cr[cb][r] = C[ok * (row + col * ldc)];
+            }
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int64_t col = cbase + cb * 16 + r;
+                if (col < N) {
+                    const int sc = -(erow + ecol[cb][r]);
+                    c64 v = cr[cb][r];
+                    v.re -= ldexpf(are[rb][cb][r], sc);
+                    v.im -= ldexpf(aim[rb][cb][r], sc);
+                    C[row + col * ldc] = v;
+                }
+            }
+    }
+}
+
+int launch_rowexp_blocks(const c64* A, int64_t lda, int64_t n, int W, int* E, int64_t ldE, hipStream_t st)
+{
+    if (n <= 0)
+        return 0;
+    const int64_t nb = (n + W - 1) / W;
+    if (nb > 65535 || ldE < n)
+        return set_error("row exponents: %lld blocks, table stride %lld", (long long)nb, (long long)ldE), -1;
+    launch_kernel(&rowexp_blocks, dim3((unsigned)((n + 255) / 256), (unsigned)nb), dim3(256), 0, st, A, lda, n, W, E, ldE);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
+int gemm_sub_f16s(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const int* ea, const c64* B, int64_t ldb,
+                  int* eb, c64* C, int64_t ldc, hipStream_t st)
+{
+    if (M <= 0 || N <= 0 || K <= 0)
+        return 0;
+    launch_kernel(&colexp, dim3((unsigned)N), dim3(256), 0, st, B, ldb, K, eb);
+    const int64_t gx = (M + 127) / 128, gy = (N + 63) / 64;
+    const int64_t nblk = (gx * gy + 7) / 8 * 8;
+    if (nblk > 0x7fffffff)
+        return set_error("fp16 product: too many tiles"), -1;
+    launch_kernel(&cgemm_sub_f16s, dim3((unsigned)nblk), dim3(256), 0, st, M, N, K, A, lda, ea, B, ldb, (const int*)eb, C,
+                  ldc, gx, gy);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 int launch_split_f16(const c64* S, int64_t ld, int64_t R, int K, _Float16* planes, int* e, int64_t cap,
                      hipStream_t st)
 {

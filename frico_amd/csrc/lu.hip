@@ -2731,6 +2731,19 @@ static bool solve_blocked()
 }
 
 bool lu_blocked_solves() { return solve_blocked(); }
+int lu_solve_block_rows() { return BSOLVE_ROWS; }
+
+// A product of a blocked solve, C -= A(factors) X(block of the solution, K rows): on fp16 MFMA
+// with the factors' exponents of block kblk when the refinement provides them, else fp32.
+static int solve_product(const ResidStream* rs, bool second, int64_t kblk, int64_t row0, int64_t M, int64_t N,
+                         int64_t K, const c64* A, int64_t lda, const c64* X, int64_t ldx, c64* C, int64_t ldc,
+                         hipStream_t st)
+{
+    if (rs && rs->aexp)
+        return gemm_sub_f16s(M, N, K, A, lda, rs->aexp + kblk * rs->ldae + row0, X, ldx, second ? rs->bexp2 : rs->bexp,
+                             C, ldc, st);
+    return gemm_sub<float>(M, N, K, A, lda, X, ldx, C, ldc, st, nullptr);
+}
 
 // Residual product of one finished block of the solution beside the substitution (see lu_backward).
 static int resid_block(const ResidStream* rs, int64_t n, int64_t kb, int64_t bw, const c64* X, int64_t ldx,
@@ -2786,7 +2799,7 @@ static int lu_backward_blocked(const c64* A, int64_t n, int64_t lda, c64* Y, int
             }
         }
         if (!rc && kb > 0)
-            rc = gemm_sub<float>(kb, nrhs, bw, A + kb * lda, lda, X + kb, ldx, Y, ldy, st, nullptr);
+            rc = solve_product(rs, false, b, 0, kb, nrhs, bw, A + kb * lda, lda, X + kb, ldx, Y, ldy, st);
         if (rc)
             return rc;
     }
@@ -2805,21 +2818,55 @@ static int lu_backward_blocked(const c64* A, int64_t n, int64_t lda, c64* Y, int
 }
 
 // Blocked forward half (complex64, unit L, no interchanges), in place: per block from the first,
-// B(block) = L(block)^{-1} B(block), then B(below) -= L(below, block) B(block).
-static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int64_t ldb, int64_t nrhs, hipStream_t st)
+// B(block) = L(block)^{-1} B(block), then B(below) -= L(below, block) B(block).  With a second
+// stream (rs), the update of the next block's rows stays on the main stream, so that the next
+// block's substitution starts after it, while the rows below the next block are updated on the
+// second stream; the main stream's update of a block's rows waits for the previous block's update
+// of the rows below (both write those rows).
+static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int64_t ldb, int64_t nrhs, hipStream_t st,
+                              const ResidStream* rs)
 {
     const int64_t nblk = (n + BSOLVE_ROWS - 1) / BSOLVE_ROWS;
+    const bool ahead = rs && rs->side && rs->nev >= nblk + 1;
+    bool pending = false; // a rows-below update of the previous block on the second stream
     for (int64_t b = 0; b < nblk; b++) {
         const int64_t kb = b * BSOLVE_ROWS, bw = (n - kb) < BSOLVE_ROWS ? (n - kb) : BSOLVE_ROWS;
         launch_block_solve<false>(A, lda, kb, (int)bw, B, ldb, B, ldb, nrhs, st);
         FRICO_HIP_CHECK(hipGetLastError());
         const int64_t below = n - kb - bw;
-        if (below > 0) {
-            int rc = gemm_sub<float>(below, nrhs, bw, A + (kb + bw) + kb * lda, lda, B + kb, ldb, B + kb + bw, ldb, st,
-                                     nullptr);
+        if (below <= 0)
+            break;
+        if (!ahead) {
+            int rc = solve_product(rs, false, b, kb + bw, below, nrhs, bw, A + (kb + bw) + kb * lda, lda, B + kb, ldb,
+                                   B + kb + bw, ldb, st);
             if (rc)
                 return rc;
+            continue;
         }
+        const int64_t nx = below < BSOLVE_ROWS ? below : BSOLVE_ROWS, rest = below - nx;
+        if (pending) {
+            FRICO_HIP_CHECK(hipEventRecord(rs->ev[b], rs->side));
+            FRICO_HIP_CHECK(hipStreamWaitEvent(st, rs->ev[b], 0));
+            pending = false;
+        }
+        int rc = solve_product(rs, false, b, kb + bw, nx, nrhs, bw, A + (kb + bw) + kb * lda, lda, B + kb, ldb,
+                               B + kb + bw, ldb, st);
+        if (rc)
+            return rc;
+        if (rest > 0) {
+            // the second stream needs this block's solution, which the main stream has at this point
+            FRICO_HIP_CHECK(hipEventRecord(rs->ev[nblk], st));
+            FRICO_HIP_CHECK(hipStreamWaitEvent(rs->side, rs->ev[nblk], 0));
+            rc = solve_product(rs, true, b, kb + bw + nx, rest, nrhs, bw, A + (kb + bw + nx) + kb * lda, lda, B + kb, ldb,
+                               B + kb + bw + nx, ldb, rs->side);
+            if (rc)
+                return rc;
+            pending = true;
+        }
+    }
+    if (pending) {
+        FRICO_HIP_CHECK(hipEventRecord(rs->ev[0], rs->side));
+        FRICO_HIP_CHECK(hipStreamWaitEvent(st, rs->ev[0], 0));
     }
     return 0;
 }
@@ -2932,7 +2979,7 @@ int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx
     if constexpr (std::is_same<T, float>::value) {
         // no interchanges: both halves blocked, in place
         if (!perm && solve_blocked()) {
-            int rc = lu_forward_blocked(A, n, lda, B, ldb, nrhs, st);
+            int rc = lu_forward_blocked(A, n, lda, B, ldb, nrhs, st, rs);
             return rc ? rc : lu_backward<T>(A, n, lda, B, ldb, B, ldb, nrhs, st, rs);
         }
     }

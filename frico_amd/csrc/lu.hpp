@@ -78,6 +78,14 @@ int gemm_sub_f16_pair(int64_t M, int64_t N, int K0, const _Float16* La0, const i
 int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
                  const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);
 
+// Products of the triangular solves on fp16 MFMA (gemm_f16.hip): C -= A B, complex64 and
+// column-major, A's rows scaled by 2^ea and B's columns by 2^eb (computed here into eb[N], over B's
+// K rows) before the hi/lo split.  launch_rowexp_blocks fills E[b ldE + r] with row r's exponent
+// over columns [b W, (b + 1) W) of A.
+int gemm_sub_f16s(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, const int* ea, const c64* B, int64_t ldb,
+                  int* eb, c64* C, int64_t ldc, hipStream_t st);
+int launch_rowexp_blocks(const c64* A, int64_t lda, int64_t n, int W, int* E, int64_t ldE, hipStream_t st);
+
 // C -= A B.  exact: four real products per complex multiply-add even where three would be used
 // (residuals, whose rounding floor bounds the refinement).
 template <typename T>
@@ -134,7 +142,16 @@ struct ResidStream {
     int64_t ldr = 0;
     c128* R2 = nullptr;
     bool exact = true; // four real products per complex multiply-add (see gemm_sub)
+    // The solves' own products on fp16 MFMA (gemm_sub_f16s) when aexp is set: row r's exponent
+    // over the factors' block b of lu_solve_block_rows() columns at aexp[b * ldae + r]; bexp, bexp2:
+    // nrhs-long scratch for the column exponents of the main and the second stream's products.
+    const int* aexp = nullptr;
+    int64_t ldae = 0;
+    int* bexp = nullptr;
+    int* bexp2 = nullptr;
 };
+// Rows of a block of the blocked complex64 solves (their substitutions and products).
+int lu_solve_block_rows();
 // Whether complex64 solves without interchanges take the blocked path (which needs neither the
 // diagonal-block inverses nor the W products).
 bool lu_blocked_solves();
