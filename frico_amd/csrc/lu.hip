@@ -1949,7 +1949,9 @@ __global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t l
     __shared__ __align__(16) cx<T> Pc[2][SB][LU_NB_MAX];      // panel column: Pc[b][t][r - s0], r >= s0
     __shared__ __align__(16) cx<T> Pr[2][SB][LU_NB_MAX - SB]; // panel row: Pr[b][i][c - e], c >= e
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
-    const int bi = 4 * (wv >> 1) + (lane >> 4), bj = 16 * (wv & 1) + (lane & 15);
+    // thread (bi, bj) holds rows 4 bi .. 4 bi + 3 of columns 4 bj .. 4 bj + 3; sixteen consecutive
+    // lanes take consecutive row blocks, so that the block's loads run down its columns
+    const int bi = 16 * (wv & 1) + (lane & 15), bj = 4 * (wv >> 1) + (lane >> 4);
     const int r0 = 4 * bi, j0 = 4 * bj;
     cx<T> a[4][4];
 #pragma unroll
