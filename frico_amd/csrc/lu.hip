@@ -304,14 +304,35 @@ template <typename T> struct GemmPair {
     int64_t K[2];
     int64_t lda, ldb, ldc;
 };
+//
+// Tile order: workgroup b runs on XCD b % 8 (each XCD with its own L2) and takes tile
+// s = (b % 8) ceil(T / 8) + b / 8 of the row-major tile sequence (all column tiles of a row tile
+// together).  A residual product has few column tiles (361 right-hand sides: six), so each XCD
+// works through a contiguous stretch of row tiles and reads each row block of Z from HBM once
+// instead of once per column tile.
+__device__ inline bool row_major_tile(int64_t b, int64_t gx, int64_t gy, int64_t& tm, int64_t& tn)
+{
+    const int64_t T = gx * gy, per = (T + 7) / 8;
+    const int64_t s = (b % 8) * per + b / 8;
+    if (s >= T)
+        return false;
+    tm = s / gy;
+    tn = s % gy;
+    return true;
+}
+
 template <typename T, bool M3>
-__global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub2(int64_t M, int64_t N, GemmPair<T> p)
+__global__ __launch_bounds__(256, kGemmOcc<T>) void cgemm_sub2(int64_t M, int64_t N, int64_t gx, int64_t gy,
+                                                                GemmPair<T> p)
 {
     using TL = typename TileBig<T>::type;
     __shared__ GemmSmem<T, TL> sm;
     const int s = blockIdx.z;
-    gemm_sub_tile<T, T, false, M3, TL>((int64_t)blockIdx.x * TL::BM, (int64_t)blockIdx.y * TL::BN, M, N, p.K[s], p.A[s],
-                                        p.lda, p.B[s], p.ldb, p.C[s], p.ldc, sm);
+    int64_t tm, tn;
+    if (!row_major_tile(blockIdx.x, gx, gy, tm, tn))
+        return;
+    gemm_sub_tile<T, T, false, M3, TL>(tm * TL::BM, tn * TL::BN, M, N, p.K[s], p.A[s], p.lda, p.B[s], p.ldb, p.C[s],
+                                        p.ldc, sm);
 }
 
 // The same product on the tiles (i, j) with i >= j only (row tile i, column tile j; square tiles,
@@ -1425,11 +1446,14 @@ static int gemm_sub_pair(int64_t M, int64_t N, const GemmPair<double>& p, hipStr
     const int64_t gx = (M + TL::BM - 1) / TL::BM, gy = (N + TL::BN - 1) / TL::BN;
     if (gy > 65535)
         return set_error("gemm: too many column tiles"), -1;
-    const dim3 grid((unsigned)gx, (unsigned)gy, 2);
+    const int64_t nwg = 8 * ((gx * gy + 7) / 8);
+    if (nwg > INT32_MAX)
+        return set_error("gemm: too many tiles"), -1;
+    const dim3 grid((unsigned)nwg, 1, 2);
     if (!exact && gemm_three_products() && p.K[0] >= 128 && p.K[1] >= 128)
-        launch_kernel(&cgemm_sub2<double, true>, grid, dim3(256), 0, st, M, N, p);
+        launch_kernel(&cgemm_sub2<double, true>, grid, dim3(256), 0, st, M, N, gx, gy, p);
     else
-        launch_kernel(&cgemm_sub2<double, false>, grid, dim3(256), 0, st, M, N, p);
+        launch_kernel(&cgemm_sub2<double, false>, grid, dim3(256), 0, st, M, N, gx, gy, p);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
 }
@@ -2563,17 +2587,21 @@ constexpr int RESID_ROWS = 512; // rows of the solution per residual product bes
 // block against the right-hand sides, and one product carries the block's solution to the rows
 // beyond it, so a block costs two launches instead of two per 64-row chunk.
 //
-// block_solve: workgroup = BSOLVE_COLS right-hand sides, whose rows of the block it keeps in LDS.
-// Per 64-row chunk of the block, in solve order:
-//   * every thread of the update below issues its loads of the factor first (they do not depend
+// block_solve: workgroup = BSOLVE_COLS right-hand sides, whose rows of the block it keeps in LDS,
+// 256 threads (one wavefront a SIMD) under 168 registers a lane.  The backward solve runs beside
+// the FP64 residual products (two workgroups a CU at 196 registers); at this size a block_solve
+// workgroup fits on a CU as soon as one of them finishes, where a 1024-thread form (128
+// registers, all of a CU's register file) waited for a CU with no other work: 530 us against 79
+// alone, every second block.  Per 64-row chunk of the block, in solve order:
+//   * every thread of the update issues the loads of its first factor columns (they do not depend
 //     on the chunk's solution), then the chunk's diagonal block, loaded during the previous chunk,
 //     goes to LDS;
 //   * substitution, one wavefront per right-hand side, one row per lane: the rows' finished values
 //     are broadcast by readlane, the coefficients come from LDS a few steps ahead (U_jj is applied
 //     by substitution, not through an explicit inverse: an inverse loses accuracy in proportion to
 //     cond(U_jj) on EFIE matrices);
-//   * the rows of the block not yet solved take the chunk off, y_r -= D(r, chunk) x_chunk: thread
-//     (row slot, half of the chunk's 64 columns), the two halves summed through LDS.
+//   * the rows of the block not yet solved take the chunk off, y_r -= D(r, chunk) x_chunk: each
+//     thread owns two rows and streams their factor entries BSOLVE_G columns at a time.
 // UPPER: U (non-unit diagonal), chunks from the last, rows above; else L (unit diagonal), chunks
 // from the first, rows below.
 // ----------------------------------------------------------------------------------------------
@@ -2622,95 +2650,133 @@ __device__ inline c64 chunk_substitute(c64 b, c64 rd, int w, const c64 (*sd)[65]
     return UPPER ? b * rd : b;
 }
 
+// Loads are branch-free: inactive rows read row kb and columns past a short chunk read its last
+// column; the result of inactive rows is dropped, and columns past a short chunk meet x = 0.
+constexpr int BSOLVE_NT = 256;
+constexpr int BSOLVE_G = 4;    // factor columns of the update a load group
+constexpr int BSOLVE_RING = 2; // a group's loads are issued this many groups before its use
+
 template <bool UPPER>
-__global__ __launch_bounds__(1024) void block_solve(const c64* __restrict__ A, int64_t lda, int64_t kb, int bw,
-                                                    const c64* Y, int64_t ldy, c64* X, int64_t ldx, int64_t nrhs)
+__global__ __launch_bounds__(BSOLVE_NT, 3) void block_solve(const c64* __restrict__ A, int64_t lda, int64_t kb, int bw,
+                                                            const c64* Y, int64_t ldy, c64* X, int64_t ldx,
+                                                            int64_t nrhs)
 {
-    __shared__ c64 sy[BSOLVE_ROWS][BSOLVE_COLS];
-    __shared__ c64 sp[BSOLVE_ROWS][BSOLVE_COLS]; // the second half's partial sums
-    __shared__ c64 sd[64][65];                   // sd[c][r] = D(r, c) of the chunk's diagonal block
+    constexpr int RPT = BSOLVE_ROWS / BSOLVE_NT; // update rows a thread
+    constexpr int NG = 64 / BSOLVE_G;
+    constexpr int DPT = 64 * 64 / BSOLVE_NT; // diagonal-block entries a thread
+    __shared__ c64 sy[BSOLVE_ROWS + 1][BSOLVE_COLS]; // + a spare row for inactive threads' stores
+    __shared__ c64 sd[64][65]; // sd[c][r] = D(r, c) of the chunk's diagonal block
     __shared__ c64 sx[64][BSOLVE_COLS];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int rs = tid & (BSOLVE_ROWS - 1), kh = tid / BSOLVE_ROWS; // update: row slot, column half
     const int64_t q0 = (int64_t)blockIdx.x * BSOLVE_COLS;
-    for (int e = tid; e < BSOLVE_ROWS * BSOLVE_COLS; e += 1024) {
+    for (int e = tid; e < BSOLVE_ROWS * BSOLVE_COLS; e += BSOLVE_NT) {
         const int r = e % BSOLVE_ROWS, q = e / BSOLVE_ROWS;
         sy[r][q] = (r < bw && q0 + q < nrhs) ? Y[(kb + r) + (q0 + q) * ldy] : c64{0.0f, 0.0f};
     }
     const int nch = (bw + 63) / 64;
-    // the diagonal block of chunk j, four entries a thread
+    // Loads address a column through a wave-uniform pointer and the row through a 32-bit byte
+    // offset, so that a load costs one register of address, not a 64-bit pointer a load.
+    const int wvu = __builtin_amdgcn_readfirstlane(wv);
+    auto at = [&](int64_t col, unsigned boff) {
+        return *reinterpret_cast<const c64*>(reinterpret_cast<const char*>(A + col * lda) + boff);
+    };
     auto dload = [&](int j, c64* dv) {
         const int c0 = 64 * j, w = (bw - c0) < 64 ? (bw - c0) : 64;
+        const unsigned boff = (unsigned)(kb + c0 + (lane < w ? lane : w - 1)) * (unsigned)sizeof(c64);
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int e = tid + 1024 * t, r = e & 63, c = e >> 6;
-            dv[t] = (r < w && c < w) ? A[(kb + c0 + r) + (kb + c0 + c) * lda] : c64{0.0f, 0.0f};
+        for (int t = 0; t < DPT; t++) {
+            const int c = wvu + (BSOLVE_NT / 64) * t, cc = c < w ? c : w - 1; // e = tid + 256 t: row lane
+            dv[t] = at(kb + c0 + cc, boff);
         }
     };
-    c64 dpre[4];
+    c64 dpre[DPT];
     dload(UPPER ? nch - 1 : 0, dpre);
     for (int s = 0; s < nch; s++) {
         const int j = UPPER ? nch - 1 - s : s;
         const int c0 = 64 * j, w = (bw - c0) < 64 ? (bw - c0) : 64;
-        // this thread's row of the update and its 32 factor entries
-        const int row = UPPER ? rs : c0 + w + rs;
-        const bool act = UPPER ? row < c0 : row < bw;
-        c64 dv[32];
-        {
-            const c64* dcol = A + (kb + (act ? row : 0)) + (kb + c0 + 32 * kh) * lda;
+        const int rb = UPPER ? 0 : c0 + w, re = UPPER ? c0 : bw; // rows of the update
+        unsigned roff[RPT];
+        bool act[RPT];
 #pragma unroll
-            for (int t = 0; t < 32; t++)
-                dv[t] = (act && 32 * kh + t < w) ? dcol[(int64_t)t * lda] : c64{0.0f, 0.0f};
+        for (int h = 0; h < RPT; h++) {
+            const int row = rb + tid + BSOLVE_NT * h;
+            act[h] = row < re;
+            roff[h] = (unsigned)(kb + (act[h] ? row : 0)) * (unsigned)sizeof(c64);
         }
-        __syncthreads(); // the previous chunk is done with sd, sx and sp
+        c64 ring[BSOLVE_RING][BSOLVE_G][RPT];
+        auto gload = [&](int g, c64 (*buf)[RPT]) {
 #pragma unroll
-        for (int t = 0; t < 4; t++) {
-            const int e = tid + 1024 * t;
-            sd[e >> 6][e & 63] = dpre[t];
+            for (int t = 0; t < BSOLVE_G; t++) {
+                const int c = BSOLVE_G * g + t, cc = c < w ? c : w - 1;
+#pragma unroll
+                for (int h = 0; h < RPT; h++)
+                    buf[t][h] = at(kb + c0 + cc, roff[h]);
+            }
+        };
+#pragma unroll
+        for (int g = 0; g < BSOLVE_RING; g++)
+            gload(g, ring[g]);
+        __syncthreads(); // the previous chunk is done with sd and sx
+        {
+            const int c0n = 64 * j, wn = (bw - c0n) < 64 ? (bw - c0n) : 64;
+#pragma unroll
+            for (int t = 0; t < DPT; t++) {
+                const int e = tid + BSOLVE_NT * t, r = e & 63, c = e >> 6;
+                sd[c][r] = (r < wn && c < wn) ? dpre[t] : c64{0.0f, 0.0f};
+            }
         }
         if (s + 1 < nch)
             dload(UPPER ? j - 1 : j + 1, dpre);
         __syncthreads();
-        if (wv < BSOLVE_COLS) {
+        {
             const c64 b = lane < w ? sy[c0 + lane][wv] : c64{0.0f, 0.0f};
             c64 rd{1.0f, 0.0f};
             if (UPPER && lane < w)
                 rd = cdiv(c64{1.0f, 0.0f}, sd[lane][lane]);
             const c64 x = w == 64 ? chunk_substitute<UPPER, true>(b, rd, w, sd, lane)
                                   : chunk_substitute<UPPER, false>(b, rd, w, sd, lane);
-            // every lane: rows past a short chunk hold x = 0, which the update multiplies by zeros
             sx[lane][wv] = x;
             if (lane < w && q0 + wv < nrhs)
                 X[(kb + c0 + lane) + (q0 + wv) * ldx] = x;
         }
         __syncthreads();
-        c64 acc[BSOLVE_COLS];
-        if (act) {
+        c64 acc[RPT][BSOLVE_COLS];
+#pragma unroll
+        for (int h = 0; h < RPT; h++)
 #pragma unroll
             for (int q = 0; q < BSOLVE_COLS; q++)
-                acc[q] = c64{0.0f, 0.0f};
+                acc[h][q] = c64{0.0f, 0.0f};
 #pragma unroll
-            for (int t = 0; t < 32; t++) {
+        for (int g = 0; g < NG; g++) {
+            c64(*cur)[RPT] = ring[g % BSOLVE_RING];
 #pragma unroll
-                for (int q = 0; q < BSOLVE_COLS; q++) {
-                    const c64 x = sx[32 * kh + t][q];
-                    acc[q].re += dv[t].re * x.re - dv[t].im * x.im;
-                    acc[q].im += dv[t].re * x.im + dv[t].im * x.re;
-                }
-            }
-            if (kh == 1)
+            for (int t = 0; t < BSOLVE_G; t++) {
+                c64 x[BSOLVE_COLS];
 #pragma unroll
                 for (int q = 0; q < BSOLVE_COLS; q++)
-                    sp[row][q] = acc[q];
+                    x[q] = sx[BSOLVE_G * g + t][q];
+#pragma unroll
+                for (int h = 0; h < RPT; h++)
+#pragma unroll
+                    for (int q = 0; q < BSOLVE_COLS; q++) {
+                        acc[h][q].re += cur[t][h].re * x[q].re - cur[t][h].im * x[q].im;
+                        acc[h][q].im += cur[t][h].re * x[q].im + cur[t][h].im * x[q].re;
+                    }
+            }
+            if (g + BSOLVE_RING < NG)
+                gload(g + BSOLVE_RING, cur);
         }
-        __syncthreads();
-        if (act && kh == 0)
+        // unconditional stores (inactive rows into the spare row): a store under act[h] lets the
+        // compiler sink the whole accumulation into the branch and keep every operand live to it
+#pragma unroll
+        for (int h = 0; h < RPT; h++) {
+            const int row = act[h] ? rb + tid + BSOLVE_NT * h : BSOLVE_ROWS;
 #pragma unroll
             for (int q = 0; q < BSOLVE_COLS; q++) {
-                const c64 o = sp[row][q];
-                sy[row][q].re -= acc[q].re + o.re;
-                sy[row][q].im -= acc[q].im + o.im;
+                sy[row][q].re -= acc[h][q].re;
+                sy[row][q].im -= acc[h][q].im;
             }
+        }
     }
 }
 
@@ -2718,8 +2784,8 @@ template <bool UPPER>
 static void launch_block_solve(const c64* A, int64_t lda, int64_t kb, int bw, const c64* Y, int64_t ldy, c64* X,
                                int64_t ldx, int64_t nrhs, hipStream_t st)
 {
-    launch_kernel(&block_solve<UPPER>, dim3((unsigned)((nrhs + BSOLVE_COLS - 1) / BSOLVE_COLS)), dim3(1024), 0, st, A,
-                  lda, kb, bw, Y, ldy, X, ldx, nrhs);
+    launch_kernel(&block_solve<UPPER>, dim3((unsigned)((nrhs + BSOLVE_COLS - 1) / BSOLVE_COLS)), dim3(BSOLVE_NT), 0, st,
+                  A, lda, kb, bw, Y, ldy, X, ldx, nrhs);
 }
 
 // FRICO_SOLVE_BLOCKED=0 returns the complex64 solves to one substitution and one product per chunk.
