@@ -262,6 +262,11 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
     std::vector<cd> I((size_t)N);
     Graph g1, g2;
     int nb = 64;
+    // the form of this unit's solves: once the unpivoted factors of a frequency have fallen back to
+    // the pivoted LU (a zero pivot, or a refinement that stalls: factors that grow without
+    // interchanges), the mesh's other frequencies go to the pivoted LU at once rather than each
+    // paying for the unpivoted factorisation and sweeps first
+    RunOpts ou = o;
     for (int f = 0; f < o.nfreq; f++) {
         const double freq = o.freqs[f];
         if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, o.fill_prec)) ||
@@ -270,26 +275,29 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
             return rc;
         frico_solve_info& inf = infos[(size_t)mi * o.nfreq + f];
         if (f == 0) {
-            if ((rc = pick_block(b, c, o, freq, &nb)) || (rc = frico_ctx_set_block(c, nb)))
+            if ((rc = pick_block(b, c, ou, freq, &nb)) || (rc = frico_ctx_set_block(c, nb)))
                 return rc;
             b->mesh_nb[mi] = nb;
-            rc = solve_eager(c, o, &inf);
-        } else if (o.graphs && (o.form == FRICO_SOLVE_C64_IR || o.form == FRICO_SOLVE_C64_NP_IR)) {
-            rc = solve_graphs(c, o, g1, g2, &inf);
-            if (o.form == FRICO_SOLVE_C64_NP_IR && (rc > 0 || rc == FRICO_ERR_NOCONV)) {
+            rc = solve_eager(c, ou, &inf);
+        } else if (ou.graphs && (ou.form == FRICO_SOLVE_C64_IR || ou.form == FRICO_SOLVE_C64_NP_IR)) {
+            rc = solve_graphs(c, ou, g1, g2, &inf);
+            if (ou.form == FRICO_SOLVE_C64_NP_IR && (rc > 0 || rc == FRICO_ERR_NOCONV)) {
                 // a zero pivot or a stalled refinement of the unpivoted factors: the pivoted LU, as
-                // frico_factor and frico_solve do; the graphs are recorded again next time
-                g1.reset();
-                g2.reset();
-                RunOpts op = o;
+                // frico_factor and frico_solve do
+                RunOpts op = ou;
                 op.form = FRICO_SOLVE_C64_IR;
                 rc = solve_eager(c, op, &inf);
             }
         } else {
-            rc = solve_eager(c, o, &inf);
+            rc = solve_eager(c, ou, &inf);
         }
         if (rc)
             return rc;
+        if (ou.form == FRICO_SOLVE_C64_NP_IR && c->factored == FRICO_SOLVE_C64_IR) {
+            ou.form = FRICO_SOLVE_C64_IR;
+            g1.reset(); // recorded for the unpivoted form: recorded again for the pivoted one
+            g2.reset();
+        }
         b->unit_form[(size_t)mi * o.nfreq + f] = c->factored;
         if (f == o.nfreq - 1 && g1.x)
             b->graph_units++;

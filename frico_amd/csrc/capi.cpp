@@ -1068,6 +1068,8 @@ int refine_prepare(frico_ctx* c)
         return rc;
     if (c->wv_ok && (rc = c->ysolve.ensure((size_t)(N * nr) * sizeof(c64))))
         return rc;
+    if (nr <= 4 && (rc = c->chain.ensure(lu_chain_words(N, nr))))
+        return rc;
     return 0;
 }
 
@@ -1119,13 +1121,16 @@ int refine_enqueue(frico_ctx* c, bool first, bool overlap)
         return rc;
     if (stream && rs.R2 && (rc = launch_zero_bytes(rs.R2, (size_t)(N * nr) * sizeof(c128), c->st)))
         return rc;
+    ChainSolve cs;
+    cs.words = c->chain.p;
+    cs.nwords = c->chain.n;
     if (carried)
-        rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st, stream ? &rs : nullptr);
+        rc = lu_solve_upper<float>(c->LU32.p, N, N, c->D32.p, N, nr, c->st, stream ? &rs : nullptr, &cs);
     else
         // the factors without pivoting have no interchanges to apply
         rc = lu_solve<float>(c->LU32.p, N, N, c->factored == FRICO_SOLVE_C64_NP_IR ? nullptr : c->perm.p,
                              c->nb_factored, c->D32.p, N, nr, c->st, c->dinv.p, c->wv_ok ? c->wv.p : nullptr,
-                             c->ysolve.p, stream ? &rs : nullptr);
+                             c->ysolve.p, stream ? &rs : nullptr, &cs);
     if (rc)
         return rc;
     if (!stream && (rc = launch_add_correction(c->X.p, N, c->D32.p, N, N, nr, c->st)))

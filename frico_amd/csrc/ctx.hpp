@@ -214,6 +214,8 @@ struct frico_ctx {
     // exponents over each block of lu_solve_block_rows() columns (valid after an unpivoted or
     // symmetric complex64 factorisation), and two scratch vectors of the solution's column exponents
     DevBuf<int> sexp, bexp;
+    // exchange area of the chained complex64 solves (one to four right-hand sides)
+    DevBuf<unsigned long long> chain;
     bool sexp_ok = false;
     bool solve_f16 = true;
     bool gemm_f16 = true;         // FRICO_GEMM_F16=0: fp32 trailing products in the unpivoted LU

@@ -1385,6 +1385,86 @@ static bool gemm_three_products()
     return on;
 }
 
+// C[:, q] -= A B[:, q] for q < ncol <= GEMV_MAXN (A: M x K, column-major), the products of solves
+// and residuals with one to four right-hand sides.  The tile product would run a 64 x 64 tile per 64
+// rows with all but one to four of its columns empty, a K step at a time (~380 us for a 3500 x
+// 3500 residual); here the product is a read of A at HBM speed.  Workgroup: 64 rows (one a lane),
+// its sixteen waves a contiguous sixteenth of K each, sixteen columns of A in flight a lane; the
+// waves' partial sums are added through LDS in a fixed order, so the result does not depend on
+// scheduling.  The four real products of each complex one are kept (no three-product form).
+constexpr int GEMV_MAXN = 4;
+constexpr int GEMV_WAVES = 16;
+
+template <typename TA, typename T>
+__global__ __launch_bounds__(64 * GEMV_WAVES) void gemv_sub(int64_t M, int64_t K, int ncol, const cx<TA>* __restrict__ A,
+                                                           int64_t lda, const cx<T>* __restrict__ B, int64_t ldb,
+                                                           cx<T>* __restrict__ C, int64_t ldc)
+{
+    __shared__ cx<T> sp[GEMV_WAVES][64][GEMV_MAXN];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t row = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t arow = row < M ? row : M - 1;
+    const int64_t kq = (K + GEMV_WAVES - 1) / GEMV_WAVES, k0 = wv * kq;
+    const int64_t k1 = (k0 + kq) < K ? (k0 + kq) : K;
+    cx<T> acc[GEMV_MAXN];
+#pragma unroll
+    for (int q = 0; q < GEMV_MAXN; q++)
+        acc[q] = cx<T>{0, 0};
+    constexpr int U = 16;
+    for (int64_t k = k0; k < k1; k += U) {
+        cx<TA> a[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int64_t kk = (k + u) < k1 ? (k + u) : k1 - 1;
+            a[u] = A[arow + kk * lda];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (k + u >= k1)
+                break;
+#pragma unroll
+            for (int q = 0; q < GEMV_MAXN; q++)
+                if (q < ncol) {
+                    const cx<T> b = B[(k + u) + q * ldb];
+                    const T ar = (T)a[u].re, ai = (T)a[u].im;
+                    acc[q].re += ar * b.re - ai * b.im;
+                    acc[q].im += ar * b.im + ai * b.re;
+                }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < GEMV_MAXN; q++)
+        sp[wv][lane][q] = acc[q];
+    __syncthreads();
+    if (wv == 0 && row < M) {
+        for (int q = 0; q < ncol; q++) {
+            cx<T> t = sp[0][lane][q];
+#pragma unroll
+            for (int w = 1; w < GEMV_WAVES; w++) {
+                t.re += sp[w][lane][q].re;
+                t.im += sp[w][lane][q].im;
+            }
+            cx<T> c = C[row + q * ldc];
+            c.re -= t.re;
+            c.im -= t.im;
+            C[row + q * ldc] = c;
+        }
+    }
+}
+
+template <typename TA, typename T>
+static int launch_gemv_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* A, int64_t lda, const cx<T>* B, int64_t ldb,
+                           cx<T>* C, int64_t ldc, hipStream_t st)
+{
+    const int64_t g = (M + 63) / 64;
+    if (g > INT32_MAX)
+        return set_error("gemv: too many rows"), -1;
+    launch_kernel(&gemv_sub<TA, T>, dim3((unsigned)g), dim3(64 * GEMV_WAVES), 0, st, M, K, (int)N, A, lda, B, ldb, C, ldc);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const cx<T>* B, int64_t ldb, cx<T>* C,
              int64_t ldc, hipStream_t st, KernelTimer* kt, bool exact)
@@ -1397,6 +1477,13 @@ int gemm_sub(int64_t M, int64_t N, int64_t K, const cx<T>* A, int64_t lda, const
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
         kt->begin(st);
+    if (N <= GEMV_MAXN) {
+        const int rc = launch_gemv_sub<T, T>(M, N, K, A, lda, B, ldb, C, ldc, st);
+        if (kt)
+            kt->end(st, 8.0 * (double)M * (double)N * (double)K,
+                    (double)sizeof(cx<T>) * (2.0 * (double)M * (double)N + ((double)M + (double)N) * (double)K));
+        return rc;
+    }
     const dim3 grid((unsigned)gx, (unsigned)gy);
     if (!exact && gemm_three_products() && (sizeof(T) == 4 || K >= 128))
         launch_kernel(&cgemm_sub<T, T, false, true>, grid, dim3(256), 0, st, M, N, K, A, lda, B, ldb, C, ldc);
@@ -1469,8 +1556,13 @@ int gemm_sub_mixed(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, c
         return set_error("gemm: too many column tiles"), -1;
     if (kt)
         kt->begin(st);
-    launch_kernel(&cgemm_sub<double, float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B,
-                       ldb, C, ldc);
+    if (N <= GEMV_MAXN) {
+        if (int rc = launch_gemv_sub<float, double>(M, N, K, A, lda, B, ldb, C, ldc, st))
+            return rc;
+    } else {
+        launch_kernel(&cgemm_sub<double, float>, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, st, M, N, K, A, lda, B,
+                      ldb, C, ldc);
+    }
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K);
     FRICO_HIP_CHECK(hipGetLastError());
@@ -2944,6 +3036,195 @@ static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int6
 // solves of a refinement), each RESID_ROWS-row block of X, once final, is added to the FP64 solution and
 // taken off the residual on a second stream while the substitution goes on:
 // S(block) += X(block), R -= Z(:, block) S(block); with R = B beforehand, R ends as B - Z S.
+// ----------------------------------------------------------------------------------------------
+// Chained triangular solves for a few right-hand sides (complex64 factors without interchanges).
+// With one to four right-hand sides a blocked solve is a chain of launches, each a single
+// workgroup's substitution followed by a product with one useful column, so a solve of a few
+// thousand rows costs about a millisecond of launch and wait latency.  Here one launch solves the
+// whole triangle: workgroup w owns the 64-row chunk j (j = w for L, forward; j = nch - 1 - w for U,
+// backward), so a workgroup depends only on chunks owned by lower workgroup ids, which the
+// dispatcher starts first: the grid needs no co-residency.  Per dependency chunk i, in the order
+// they finish, wave 0 waits for x_i and copies it to LDS, and every thread takes
+// D(j, i) x_i off its partial sums (row = lane, sixteen columns of the chunk = wave); the 64 x 16
+// entries of the next chunk's block are loaded while this one waits.  Then the four waves' partial
+// sums are reduced, wave q substitutes right-hand side q with the diagonal block (from LDS, as in
+// block_solve) and publishes x_j.
+// The exchange area holds, per chunk, row and right-hand side, two 64-bit words {sequence, re} and
+// {sequence, im} written and read with relaxed agent-scope atomics (as the panel factorisation's
+// exchange): a reader takes a word only when it carries the launch's sequence number, so the area
+// is cleared (by a kernel, which a recorded graph replays) before each solve and the forward and
+// backward halves use sequence numbers 1 and 2.  Word 0 is the abort flag of the bounded waits; a
+// workgroup whose wait gives up writes NaN into its rows, so the refinement reports the solve.
+// ----------------------------------------------------------------------------------------------
+constexpr int CHAIN_MAXR = 4;
+
+size_t lu_chain_words(int64_t n, int64_t nrhs) { return 1 + (size_t)((n + 63) / 64) * 64 * (size_t)nrhs * 2; }
+
+template <bool UPPER>
+__global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, int64_t lda, int64_t n, c64* B,
+                                                   int64_t ldb, int nr, unsigned long long* words, unsigned seq)
+{
+    __shared__ c64 sd[64][65];             // sd[c][r] = D(r, c), the chunk's diagonal block
+    __shared__ c64 sx[2][64][CHAIN_MAXR];  // x of the dependency chunk being taken off (two steps)
+    __shared__ c64 sp[4][64][CHAIN_MAXR];  // the four waves' partial sums
+    __shared__ int s_ok;
+    int* abort_flag = reinterpret_cast<int*>(words);
+    unsigned long long* xw = words + 1;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int nch = (int)((n + 63) / 64);
+    const int j = UPPER ? nch - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    const int64_t r0 = 64 * (int64_t)j;
+    const int wj = (n - r0) < 64 ? (int)(n - r0) : 64;
+    for (int e = tid; e < 64 * 64; e += 256) {
+        const int r = e & 63, c = e >> 6;
+        sd[c][r] = (r < wj && c < wj) ? A[(r0 + r) + (r0 + c) * lda] : c64{0.0f, 0.0f};
+    }
+    if (tid == 0)
+        s_ok = 1;
+    // this thread's row of D(j, i) (inactive rows read row r0) and its sixteen columns
+    const int64_t arow = r0 + (lane < wj ? lane : 0);
+    const int ndep = UPPER ? nch - 1 - j : j;
+    auto lload = [&](int t, c64* dst) {
+        const int i = UPPER ? nch - 1 - t : t;
+        const int64_t c0 = 64 * (int64_t)i + 16 * wv;
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            const int64_t cc = (c0 + c) < n ? (c0 + c) : n - 1; // past a short chunk: x = 0 there
+            dst[c] = A[arow + cc * lda];
+        }
+    };
+    c64 acc[CHAIN_MAXR];
+#pragma unroll
+    for (int q = 0; q < CHAIN_MAXR; q++)
+        acc[q] = c64{0.0f, 0.0f};
+    bool ok = true;
+    // step t: wait for x_i (wave 0) and take D(j, i) x_i off the partial sums
+    auto step = [&](int t, const c64* cur, c64* nxt) {
+        if (t + 1 < ndep)
+            lload(t + 1, nxt);
+        const int i = UPPER ? nch - 1 - t : t;
+        c64(*bx)[CHAIN_MAXR] = sx[t & 1];
+        if (wv == 0) {
+            const unsigned long long* src = xw + ((size_t)i * 64 + lane) * (size_t)nr * 2;
+            unsigned long long v[2 * CHAIN_MAXR];
+            long spins = 0;
+            for (;;) {
+                bool ready = true;
+#pragma unroll
+                for (int q = 0; q < 2 * CHAIN_MAXR; q++)
+                    if (q < 2 * nr) {
+                        v[q] = ll_load(src + q);
+                        ready = ready && (unsigned)(v[q] >> 32) == seq;
+                    }
+                ready = __all(ready);
+                if (ready || !(ok = ll_spin(spins, abort_flag)))
+                    break;
+            }
+#pragma unroll
+            for (int q = 0; q < CHAIN_MAXR; q++)
+                bx[lane][q] = q < nr ? c64{__uint_as_float((unsigned)v[2 * q]), __uint_as_float((unsigned)v[2 * q + 1])}
+                                     : c64{0.0f, 0.0f};
+            if (!ok && lane == 0)
+                s_ok = 0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < 16; c++)
+#pragma unroll
+            for (int q = 0; q < CHAIN_MAXR; q++) {
+                const c64 x = bx[16 * wv + c][q];
+                acc[q].re += cur[c].re * x.re - cur[c].im * x.im;
+                acc[q].im += cur[c].re * x.im + cur[c].im * x.re;
+            }
+    };
+    c64 l0[16], l1[16];
+    if (ndep > 0)
+        lload(0, l0);
+    __syncthreads(); // sd and s_ok written
+    for (int t = 0; t < ndep && s_ok; t += 2) {
+        step(t, l0, l1);
+        if (t + 1 < ndep && s_ok)
+            step(t + 1, l1, l0);
+    }
+    __syncthreads(); // every wave done with sx; s_ok final
+#pragma unroll
+    for (int q = 0; q < CHAIN_MAXR; q++)
+        sp[wv][lane][q] = acc[q];
+    __syncthreads();
+    if (wv < nr) {
+        const int q = wv;
+        c64 b = c64{0.0f, 0.0f};
+        if (lane < wj) {
+            b = B[(r0 + lane) + q * ldb];
+#pragma unroll
+            for (int w = 0; w < 4; w++) {
+                b.re -= sp[w][lane][q].re;
+                b.im -= sp[w][lane][q].im;
+            }
+        }
+        c64 rd{1.0f, 0.0f};
+        if (UPPER && lane < wj)
+            rd = cdiv(c64{1.0f, 0.0f}, sd[lane][lane]);
+        c64 x = wj == 64 ? chunk_substitute<UPPER, true>(b, rd, wj, sd, lane)
+                         : chunk_substitute<UPPER, false>(b, rd, wj, sd, lane);
+        if (!s_ok)
+            x = c64{__builtin_nanf(""), __builtin_nanf("")};
+        if (lane < wj)
+            B[(r0 + lane) + q * ldb] = x;
+        if (s_ok) {
+            unsigned long long* dst = xw + ((size_t)j * 64 + lane) * (size_t)nr * 2 + 2 * q;
+            ll_put(dst, seq, __float_as_uint(lane < wj ? x.re : 0.0f));
+            ll_put(dst + 1, seq, __float_as_uint(lane < wj ? x.im : 0.0f));
+        }
+    }
+}
+
+// FRICO_SOLVE_CHAIN=0 keeps the blocked solves for a few right-hand sides too.
+static bool chain_on()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FRICO_SOLVE_CHAIN");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
+static bool use_chain(const ChainSolve* cs, int64_t n, int64_t nrhs)
+{
+    return cs && cs->words && chain_on() && nrhs >= 1 && nrhs <= CHAIN_MAXR && n > 64 &&
+           cs->nwords >= lu_chain_words(n, nrhs);
+}
+
+// Both halves (forward = true) or the backward half of a solve in place by the chained kernels; with
+// a residual stream, X += D and the residual product follow as one block on the second stream.
+static int lu_chain_solve(const c64* A, int64_t n, int64_t lda, c64* B, int64_t ldb, int64_t nrhs, hipStream_t st,
+                          const ChainSolve* cs, bool forward, const ResidStream* rs)
+{
+    const int64_t nch = (n + 63) / 64;
+    if (nch > INT32_MAX)
+        return set_error("chained solve: too many chunks"), -1;
+    int rc = launch_zero_bytes(cs->words, lu_chain_words(n, nrhs) * sizeof(unsigned long long), st);
+    if (rc)
+        return rc;
+    if (forward)
+        launch_kernel(&chain_solve<false>, dim3((unsigned)nch), dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, cs->words,
+                      1u);
+    launch_kernel(&chain_solve<true>, dim3((unsigned)nch), dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, cs->words, 2u);
+    FRICO_HIP_CHECK(hipGetLastError());
+    if (!rs)
+        return 0;
+    if (rs->nev < 2)
+        return set_error("chained solve: the residual stream needs two events"), -1;
+    FRICO_HIP_CHECK(hipEventRecord(rs->ev[0], st));
+    FRICO_HIP_CHECK(hipStreamWaitEvent(rs->side, rs->ev[0], 0));
+    if ((rc = launch_add_correction(rs->S, rs->lds, B, ldb, n, nrhs, rs->side)) ||
+        (rc = gemm_sub<double>(n, nrhs, n, rs->Z, rs->ldz, rs->S, rs->lds, rs->R, rs->ldr, rs->side, nullptr, rs->exact)))
+        return rc;
+    FRICO_HIP_CHECK(hipEventRecord(rs->ev[1], rs->side));
+    FRICO_HIP_CHECK(hipStreamWaitEvent(st, rs->ev[1], 0));
+    return 0;
+}
+
 template <typename T>
 static int lu_backward(const cx<T>* A, int64_t n, int64_t lda, cx<T>* Y, int64_t ldy, cx<T>* X, int64_t ldx,
                        int64_t nrhs, hipStream_t st, const ResidStream* rs)
@@ -3024,35 +3305,51 @@ static int lu_forward_fused(const cx<T>* A, int64_t n, int64_t lda, const int* p
 template <typename T>
 static int lu_solve_fused(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb,
                           int64_t nrhs, hipStream_t st, const cx<T>* inv, const cx<T>* WV, cx<T>* Y,
-                          const ResidStream* rs = nullptr)
+                          const ResidStream* rs = nullptr, const ChainSolve* cs = nullptr)
 {
     int rc = lu_forward_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, WV, Y);
-    return rc ? rc : lu_backward<T>(A, n, lda, Y, n, B, ldb, nrhs, st, rs);
+    if (rc)
+        return rc;
+    if constexpr (std::is_same<T, float>::value) {
+        // a few right-hand sides: the backward half by the chained kernel, in place in B
+        if (use_chain(cs, n, nrhs)) {
+            for (int64_t q = 0; q < nrhs; q++)
+                if ((rc = launch_copy_bytes(Y + q * n, B + q * ldb, (size_t)n * sizeof(cx<T>), st)))
+                    return rc;
+            return lu_chain_solve(A, n, lda, B, ldb, nrhs, st, cs, false, rs);
+        }
+    }
+    return lu_backward<T>(A, n, lda, Y, n, B, ldb, nrhs, st, rs);
 }
 
 // Backward half of the solve in place (see lu_backward).
 template <typename T>
 int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st,
-                   const ResidStream* rs)
+                   const ResidStream* rs, const ChainSolve* cs)
 {
+    if constexpr (std::is_same<T, float>::value)
+        if (use_chain(cs, n, nrhs))
+            return lu_chain_solve(A, n, lda, B, ldb, nrhs, st, cs, false, rs);
     return lu_backward<T>(A, n, lda, B, ldb, B, ldb, nrhs, st, rs);
 }
 
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
-             hipStream_t st, const void* dinv, const void* wv, void* ybuf, const ResidStream* rs)
+             hipStream_t st, const void* dinv, const void* wv, void* ybuf, const ResidStream* rs, const ChainSolve* cs)
 {
     const int64_t npan = (n + nb - 1) / nb;
     const cx<T>* inv = (const cx<T>*)dinv;
     if constexpr (std::is_same<T, float>::value) {
-        // no interchanges: both halves blocked, in place
+        // no interchanges: a few right-hand sides by the chained kernels, else both halves blocked
+        if (!perm && use_chain(cs, n, nrhs))
+            return lu_chain_solve(A, n, lda, B, ldb, nrhs, st, cs, true, rs);
         if (!perm && solve_blocked()) {
             int rc = lu_forward_blocked(A, n, lda, B, ldb, nrhs, st, rs);
             return rc ? rc : lu_backward<T>(A, n, lda, B, ldb, B, ldb, nrhs, st, rs);
         }
     }
     if (inv && wv && ybuf && nb % 64 == 0 && n > 64)
-        return lu_solve_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf, rs);
+        return lu_solve_fused<T>(A, n, lda, perm, nb, B, ldb, nrhs, st, inv, (const cx<T>*)wv, (cx<T>*)ybuf, rs, cs);
     if (rs)
         return set_error("solve: the residual stream needs the fused solve"), -1;
     const unsigned gcols = (unsigned)((nrhs + 63) / 64);
@@ -3185,12 +3482,13 @@ template int ldlt_factor<double>(LuWork&, c128*, int64_t, int64_t, int*, int, hi
 template int ldlt_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_nopiv_factor<float>(LuWork&, c64*, int64_t, int64_t, int*, int, hipStream_t, KernelTimer*, int64_t);
 template int lu_solve_upper<double>(const c128*, int64_t, int64_t, c128*, int64_t, int64_t, hipStream_t,
-                                    const ResidStream*);
-template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t, const ResidStream*);
+                                    const ResidStream*, const ChainSolve*);
+template int lu_solve_upper<float>(const c64*, int64_t, int64_t, c64*, int64_t, int64_t, hipStream_t, const ResidStream*,
+                                   const ChainSolve*);
 template int lu_solve<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t, hipStream_t,
-                              const void*, const void*, void*, const ResidStream*);
+                              const void*, const void*, void*, const ResidStream*, const ChainSolve*);
 template int lu_solve<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t, hipStream_t,
-                             const void*, const void*, void*, const ResidStream*);
+                             const void*, const void*, void*, const ResidStream*, const ChainSolve*);
 template int lu_solve_transposed<double>(const c128*, int64_t, int64_t, const int*, int, c128*, int64_t, int64_t,
                                          hipStream_t);
 template int lu_solve_transposed<float>(const c64*, int64_t, int64_t, const int*, int, c64*, int64_t, int64_t,

@@ -150,6 +150,13 @@ struct ResidStream {
     int* bexp = nullptr;
     int* bexp2 = nullptr;
 };
+// Chained complex64 solves (one launch a triangle) for one to four right-hand sides of factors
+// without interchanges: the exchange area of lu_chain_words(n, nrhs) 64-bit words.
+struct ChainSolve {
+    unsigned long long* words = nullptr;
+    size_t nwords = 0;
+};
+size_t lu_chain_words(int64_t n, int64_t nrhs);
 // Rows of a block of the blocked complex64 solves (their substitutions and products).
 int lu_solve_block_rows();
 // Whether complex64 solves without interchanges take the blocked path (which needs neither the
@@ -159,11 +166,11 @@ bool lu_blocked_solves();
 template <typename T>
 int lu_solve(const cx<T>* A, int64_t n, int64_t lda, const int* perm, int nb, cx<T>* B, int64_t ldb, int64_t nrhs,
              hipStream_t st, const void* dinv, const void* wv = nullptr, void* ybuf = nullptr,
-             const ResidStream* rs = nullptr);
+             const ResidStream* rs = nullptr, const ChainSolve* cs = nullptr);
 // B = U^{-1} B in place (the backward half of the solve), U the upper factor of lu_factor.
 template <typename T>
 int lu_solve_upper(const cx<T>* A, int64_t n, int64_t lda, cx<T>* B, int64_t ldb, int64_t nrhs, hipStream_t st,
-                   const ResidStream* rs = nullptr);
+                   const ResidStream* rs = nullptr, const ChainSolve* cs = nullptr);
 template <typename T>
 int lu_solve_transposed(const cx<T>* A, int64_t n, int64_t lda, const int* ipiv, int nb, cx<T>* B, int64_t ldb,
                         int64_t nrhs, hipStream_t st);

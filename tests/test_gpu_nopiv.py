@@ -34,11 +34,14 @@ def exact_lu_system(n, seed, per_row=2, nrhs=4):
     return A, A @ X, X
 
 
+# one to four right-hand sides take the chained solves (one launch a triangle), five the blocked ones;
+# n = 130 ends in a short chunk, n = 64 is a single chunk (blocked)
 @pytest.mark.parametrize("nb", [32, 64, 128])
 @pytest.mark.parametrize("carried", [True, False])
-def test_unpivoted_lu_is_exact_on_integer_factors(nb, carried):
+@pytest.mark.parametrize("n,nrhs", [(700, 4), (700, 1), (130, 2), (1000, 5), (64, 3)])
+def test_unpivoted_lu_is_exact_on_integer_factors(nb, carried, n, nrhs):
     import frico_amd
-    A, B, X = exact_lu_system(700, 31)
+    A, B, X = exact_lu_system(n, 31, nrhs=nrhs)
     assert np.max(np.abs(A)) < 100
     with frico_amd.Context(0) as c:
         c.set_block(nb)
@@ -55,12 +58,13 @@ def test_unpivoted_lu_is_exact_on_integer_factors(nb, carried):
 
 @pytest.mark.parametrize("n,nb", [(1, 128), (63, 64), (200, 64), (517, 128), (1300, 128), (2000, 64),
                                   (45, 32), (700, 32)])
-def test_unpivoted_lu_solves_dominant_systems(n, nb):
+@pytest.mark.parametrize("nrhs", [1, 3, 6])
+def test_unpivoted_lu_solves_dominant_systems(n, nb, nrhs):
     import frico_amd
     rng = np.random.default_rng(n + 1)
     A = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)) + 4.0 * np.sqrt(n) * np.eye(n)
-    B = rng.uniform(-1, 1, (n, 3)) + 1j * rng.uniform(-1, 1, (n, 3))
-    print(f"unpivoted n={n} nb={nb}", flush=True)
+    B = rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs))
+    print(f"unpivoted n={n} nb={nb} nrhs={nrhs}", flush=True)
     with frico_amd.Context(0) as c:
         c.set_block(nb)
         c.set_matrix(A)
