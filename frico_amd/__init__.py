@@ -460,8 +460,13 @@ class Context:
         return sig, [SolveInfo(i.iterations, bool(i.converged), i.max_rel_residual) for i in infos]
 
     # ---- adjoint (autograd-Function-shaped pair) ---------------------------------------------
-    def adjoint_forward(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0)):
-        """Quadrature-mode antenna analysis kept for a backward pass: returns (Zin, gain at obs)."""
+    def _adjoint_mode(self, mode):
+        self._check(self._L.frico_adjoint_set_fill(self._h, {"quad": FILL_QUAD, "approx": FILL_APPROX}[mode]))
+
+    def adjoint_forward(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0), mode="quad"):
+        """Antenna analysis (quadrature or approximate mode) kept for a backward pass: returns
+        (Zin, gain at obs)."""
+        self._adjoint_mode(mode)
         ifa = np.ascontiguousarray(ifaces, np.int32)
         ob = np.ascontiguousarray(obs, np.float64)
         out = np.zeros(4)
@@ -476,9 +481,11 @@ class Context:
         self._check(self._L.frico_adjoint_backward(self._h, float(g_zin_re), float(g_zin_im), float(g_gain), _p(g)))
         return g.reshape(-1, 3)
 
-    def rcs_forward(self, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, R=10.0):
-        """Quadrature-mode RCS kept for a backward pass: (sigma_VV, sigma_HH) at (R, theta_s, phi_s)
-        for the plane wave from (theta_i, phi_i); monostatic when theta_s, phi_s are omitted."""
+    def rcs_forward(self, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, R=10.0, mode="quad"):
+        """RCS (quadrature or approximate mode) kept for a backward pass: (sigma_VV, sigma_HH) at
+        (R, theta_s, phi_s) for the plane wave from (theta_i, phi_i); monostatic when theta_s, phi_s
+        are omitted."""
+        self._adjoint_mode(mode)
         th = complex(eth)
         ph_ = complex(eph)
         a = np.array([th.real, th.imag])
@@ -496,15 +503,15 @@ class Context:
         self._check(self._L.frico_adjoint_rcs_backward(self._h, float(g_vv), float(g_hh), _p(g)))
         return g.reshape(-1, 3)
 
-    def rcs_gradients(self, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, R=10.0):
+    def rcs_gradients(self, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, R=10.0, mode="quad"):
         """sigma_VV, sigma_HH and their vertex gradients (each (nv, 3))."""
-        s = self.rcs_forward(freq, theta_i, phi_i, eth, eph, theta_s, phi_s, R)
+        s = self.rcs_forward(freq, theta_i, phi_i, eth, eph, theta_s, phi_s, R, mode)
         return {"sigma": np.array(s), "dsigma_vv": self.rcs_backward(1.0, 0.0),
                 "dsigma_hh": self.rcs_backward(0.0, 1.0)}
 
-    def antenna_gradients(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0)):
+    def antenna_gradients(self, freq, ifaces, voltage=1.0, obs=(0.0, 0.0, 10.0), mode="quad"):
         """Zin, G and their gradients: dZin (complex, (nv,3)) and dG (real, (nv,3))."""
-        zin, gain = self.adjoint_forward(freq, ifaces, voltage, obs)
+        zin, gain = self.adjoint_forward(freq, ifaces, voltage, obs, mode)
         dre = self.adjoint_backward(1.0, 0.0, 0.0)
         dim = self.adjoint_backward(0.0, 1.0, 0.0)
         dg = self.adjoint_backward(0.0, 0.0, 1.0)

@@ -88,6 +88,7 @@ SIGNATURES = {
                                       C.c_double, C.c_int, _d, _d]),
     "frico_bistatic_sweep": (C.c_int, [_vp, _d, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                        C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, _d]),
+    "frico_adjoint_set_fill": (C.c_int, [_vp, C.c_int]),
     "frico_adjoint_forward": (C.c_int, [_vp, C.c_double, _i32, C.c_int, C.c_double, C.c_double, _d, _d]),
     "frico_adjoint_backward": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _d]),
     "frico_adjoint_rcs_forward": (C.c_int, [_vp, C.c_double, C.c_double, C.c_double, _d, _d, C.c_double, C.c_double,

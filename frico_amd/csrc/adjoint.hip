@@ -1,4 +1,4 @@
-// Adjoint of the quadrature-mode impedance fill: the contraction
+// Adjoint of the quadrature-mode (and approximate-mode) impedance fill: the contraction
 //     g(x) = sum_{i,j} W_j I_i dZ(j,i)/dx          (x = vertex coordinates, lengths held fixed)
 // for a forward current I and an adjoint vector W.  It has no counterpart in the reference; it
 // differentiates compute_matrix (src/emw_solver.cpp:45-142).
@@ -364,6 +364,120 @@ __global__ __launch_bounds__(256) void adjoint_weights(DevMesh m, const c128* __
     const double s = m.tri_sgn[x] * m.rwg_len[n];
     aw[x] = c128{s * I[n].re, s * I[n].im};
     bw[x] = c128{s * W[n].re * 0.25 * INV_PI, s * W[n].im * 0.25 * INV_PI};
+}
+
+// ----------------------------------------------------------------------------------------------
+// Approximate mode (compute_matrix_approx, src/emw_solver.cpp:144-220).  Its entry for a triangle
+// pair P != Q is the quadrature form's with the one-point rule at the barycenter (weight 1, so
+// w_P w_Q / (A_P A_Q) = 1): tripair_adjoint<1> on the centroid rule below.  A pair of equal
+// triangles takes G = 0.25 (sqrt(4 pi) / sqrt(A) - jk) / pi instead of the singular 1 / R, whose
+// pair tripair_adjoint skips (R = 0): approx_self_adjoint adds it.  With a_m, b_m as above (b
+// carrying 1 / (4 pi)), S_PP = h g, h = 0.25 U.V - A0 B0 / k^2, U = sum a_m (c - f_m), V = sum
+// b_m (c - f_m), g = 4 pi G = sqrt(4 pi / A) - j k; dS = g dh + h (dg/dA) dA, dh/dc =
+// 0.25 (A0 V + B0 U) (a third to each vertex), dh/df_m = -0.25 (a_m V + b_m U), dg/dA =
+// -0.5 sqrt(4 pi) A^(-3/2), dA/dv the cross-product area gradient.
+// ----------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void centroid_rule(DevMesh m, double4* __restrict__ out)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t < m.T)
+        out[t] = double4{0.0, 0.0, 0.0, m.area[t]};
+}
+
+__global__ __launch_bounds__(256) void approx_self_adjoint(DevMesh m, AdjMesh am, double k, double inv_ksq,
+                                                           const c128* __restrict__ aw, const c128* __restrict__ bw,
+                                                           double* __restrict__ gvert)
+{
+    const int P = blockIdx.x * 256 + threadIdx.x;
+    if (P >= m.T)
+        return;
+    double a0r = 0, a0i = 0, b0r = 0, b0i = 0, U[6] = {0, 0, 0, 0, 0, 0}, V[6] = {0, 0, 0, 0, 0, 0};
+    double ar[3], ai[3], br[3], bi[3];
+    for (int mm = 0; mm < 3; mm++) {
+        const c128 a = aw[3 * P + mm], b = bw[3 * P + mm];
+        ar[mm] = a.re;
+        ai[mm] = a.im;
+        br[mm] = b.re;
+        bi[mm] = b.im;
+        a0r += a.re;
+        a0i += a.im;
+        b0r += b.re;
+        b0i += b.im;
+        const double* al = m.alpha + 9 * (size_t)P + 3 * mm; // c - f_m
+        cadd3(U, a.re, a.im, al[0], al[1], al[2]);
+        cadd3(V, b.re, b.im, al[0], al[1], al[2]);
+    }
+    const double A = m.area[P];
+    const double s4pi = sqrt(4.0 * PI);
+    const double gre = s4pi / sqrt(A), gim = -k;
+    double hr = 0, hi = 0;
+    for (int d = 0; d < 3; d++) {
+        hr += U[2 * d] * V[2 * d] - U[2 * d + 1] * V[2 * d + 1];
+        hi += U[2 * d] * V[2 * d + 1] + U[2 * d + 1] * V[2 * d];
+    }
+    hr = 0.25 * hr - (a0r * b0r - a0i * b0i) * inv_ksq;
+    hi = 0.25 * hi - (a0r * b0i + a0i * b0r) * inv_ksq;
+    // g dh/dc, a third to each vertex
+    double dc[6] = {0, 0, 0, 0, 0, 0}, gc[6] = {0, 0, 0, 0, 0, 0};
+    caddc3(dc, 0.25 * a0r, 0.25 * a0i, V);
+    caddc3(dc, 0.25 * b0r, 0.25 * b0i, U);
+    caddc3(gc, gre, gim, dc);
+    // vertex k = c - alpha of local edge minv[k] (vertex k is the free vertex of that edge)
+    const int minv[3] = {1, 2, 0};
+    const double* c = m.cen + 3 * (size_t)P;
+    double v[3][3];
+    for (int kk = 0; kk < 3; kk++)
+        for (int d = 0; d < 3; d++)
+            v[kk][d] = c[d] - m.alpha[9 * (size_t)P + 3 * minv[kk] + d];
+    // cross-product area gradient: d/dv0 = 0.5 n x (v2 - v1), cyclic
+    const double e1[3] = {v[1][0] - v[0][0], v[1][1] - v[0][1], v[1][2] - v[0][2]};
+    const double e2[3] = {v[2][0] - v[0][0], v[2][1] - v[0][1], v[2][2] - v[0][2]};
+    double n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+    const double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    const double dgA = -0.5 * s4pi / (A * sqrt(A));
+    for (int kk = 0; kk < 3; kk++) {
+        double g[6];
+        for (int x = 0; x < 6; x++)
+            g[x] = gc[x] / 3.0;
+        // free vertex of local edge minv[kk]: g dh/df = -0.25 g (a_m V + b_m U)
+        const int mm = minv[kk];
+        double df[6] = {0, 0, 0, 0, 0, 0};
+        caddc3(df, -0.25 * ar[mm], -0.25 * ai[mm], V);
+        caddc3(df, -0.25 * br[mm], -0.25 * bi[mm], U);
+        caddc3(g, gre, gim, df);
+        if (nn > 0) {
+            const double* p1 = v[(kk + 1) % 3];
+            const double* p2 = v[(kk + 2) % 3];
+            const double e[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+            const double ga[3] = {0.5 * (n[1] * e[2] - n[2] * e[1]) / nn, 0.5 * (n[2] * e[0] - n[0] * e[2]) / nn,
+                                  0.5 * (n[0] * e[1] - n[1] * e[0]) / nn};
+            cadd3(g, hr * dgA, hi * dgA, ga[0], ga[1], ga[2]);
+        }
+        const int vtx = am.tri_v[3 * P + kk];
+        for (int x = 0; x < 6; x++)
+            atomicAdd(gvert + 6 * (size_t)vtx + x, g[x]);
+    }
+}
+
+int launch_adjoint_contract_approx(const DevMesh& m, const AdjMesh& am, double k, double inv_ksq, const c128* I,
+                                   const c128* W, c128* aw, c128* bw, double4* cent_rule, const double* third,
+                                   double* gvert, int64_t nv, hipStream_t st)
+{
+    FRICO_HIP_CHECK(hipMemsetAsync(gvert, 0, (size_t)nv * 6 * sizeof(double), st));
+    const unsigned gt = (unsigned)((m.T + 255) / 256);
+    hipLaunchKernelGGL(adjoint_weights, dim3((unsigned)((3 * (int64_t)m.T + 255) / 256)), dim3(256), 0, st, m, I, W, aw, bw);
+    hipLaunchKernelGGL(centroid_rule, dim3(gt), dim3(256), 0, st, m, cent_rule);
+    DevMesh mc = m;
+    mc.q = 1;
+    mc.qreg = cent_rule;
+    AdjMesh amc = am;
+    amc.qbeta = third;
+    dim3 g((m.T + 15) / 16, (m.T + 15) / 16);
+    hipLaunchKernelGGL((tripair_adjoint<1>), g, dim3(256), 2 * 16 * sizeof(double4), st, mc, amc, k, inv_ksq, aw, bw,
+                       gvert);
+    hipLaunchKernelGGL(approx_self_adjoint, dim3(gt), dim3(256), 0, st, m, am, k, inv_ksq, aw, bw, gvert);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
 }
 
 int launch_adjoint_contract(const DevMesh& m, const AdjMesh& am, double k, double inv_ksq, const int2* near_pairs,

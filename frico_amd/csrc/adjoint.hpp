@@ -1,4 +1,4 @@
-// Adjoint (vertex-gradient) kernels of the quadrature-mode fill.
+// Adjoint (vertex-gradient) kernels of the quadrature-mode and approximate-mode fills.
 #pragma once
 
 #include "common.hpp"
@@ -21,5 +21,11 @@ struct AdjMesh {
 int launch_adjoint_contract(const DevMesh& m, const AdjMesh& am, double k, double inv_ksq, const int2* near_pairs,
                             int64_t nnear, const c128* I, const c128* W, c128* aw, c128* bw, double* gvert, int64_t nv,
                             hipStream_t st);
+// The same contraction for the approximate-mode Z (compute_matrix_approx, src/emw_solver.cpp:144-220):
+// cent_rule (T double4, written here) and third ({1/3, 1/3, 1/3} on the device) are workspaces of
+// its one-point rule.
+int launch_adjoint_contract_approx(const DevMesh& m, const AdjMesh& am, double k, double inv_ksq, const c128* I,
+                                   const c128* W, c128* aw, c128* bw, double4* cent_rule, const double* third,
+                                   double* gvert, int64_t nv, hipStream_t st);
 
 } // namespace frico

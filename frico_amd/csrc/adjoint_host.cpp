@@ -204,6 +204,25 @@ void planewave_vjp(const HostMesh& hm, double freq, double theta, double phi, cd
     }
 }
 
+// The device contraction sum W_j I_i dZ(j,i)/dx at fixed edge lengths for the forward fill's mode.
+int contract(frico_ctx* c, const c128* I, const c128* W, double k, double inv_ksq)
+{
+    const HostMesh& hm = c->hm;
+    if (!c->adj.approx)
+        return launch_adjoint_contract(c->dm, c->am, k, inv_ksq, c->d_near.p, (int64_t)hm.near.size(), I, W, c->aw.p,
+                                       c->bw.p, c->d_gvert.p, hm.nv, c->st);
+    int rc;
+    if ((rc = c->adj_cent.ensure((size_t)hm.nt)))
+        return rc;
+    if (!c->adj_third.p) {
+        const double third[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};
+        if ((rc = c->adj_third.upload(third, 3, c->st)))
+            return rc;
+    }
+    return launch_adjoint_contract_approx(c->dm, c->am, k, inv_ksq, I, W, c->aw.p, c->bw.p, c->adj_cent.p,
+                                          c->adj_third.p, c->d_gvert.p, hm.nv, c->st);
+}
+
 int compute_gradients(frico_ctx* c)
 {
     auto& A = c->adj;
@@ -245,8 +264,7 @@ int compute_gradients(frico_ctx* c)
     const double k = omega * std::sqrt(MU0 * EPS0), inv_ksq = 1. / (omega * omega * MU0 * EPS0);
     std::vector<cd> C[2];
     for (int w = 0; w < 2; w++) {
-        if ((rc = launch_adjoint_contract(c->dm, c->am, k, inv_ksq, c->d_near.p, (int64_t)hm.near.size(), c->X.p,
-                                          c->adjW.p + w * N, c->aw.p, c->bw.p, c->d_gvert.p, nv, st)))
+        if ((rc = contract(c, c->X.p, c->adjW.p + w * N, k, inv_ksq)))
             return rc;
         C[w].assign(3 * nv, cd(0));
         FRICO_HIP_CHECK(hipMemcpyAsync(C[w].data(), c->d_gvert.p, (size_t)(6 * nv) * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -294,6 +312,16 @@ int compute_gradients(frico_ctx* c)
 
 extern "C" {
 
+int frico_adjoint_set_fill(frico_ctx* c, int mode)
+{
+    if (!c)
+        return set_error("null context"), -1;
+    if (mode != FRICO_FILL_QUAD && mode != FRICO_FILL_APPROX)
+        return set_error("fill mode must be FRICO_FILL_QUAD or FRICO_FILL_APPROX"), -2;
+    c->adj_fill = mode;
+    return 0;
+}
+
 int frico_adjoint_forward(frico_ctx* c, double freq, const int32_t* ifaces, int nif, double v_re, double v_im,
                           const double* obs, double* out)
 {
@@ -306,10 +334,12 @@ int frico_adjoint_forward(frico_ctx* c, double freq, const int32_t* ifaces, int 
     if (!obs || !out)
         return set_error("null observation point or output"), -7;
     const int q = c->hm.q;
-    if (q != 1 && q != 3 && q != 6 && q != 7)
-        return set_error("adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
+    const bool approx = c->adj_fill == FRICO_FILL_APPROX;
+    if (!approx && q != 1 && q != 3 && q != 6 && q != 7)
+        return set_error("quadrature-mode adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
     auto& A = c->adj;
     A = frico_ctx::AdjointState{};
+    A.approx = approx;
     const int64_t N = c->N;
     std::vector<double> signs((size_t)N);
     int applied = 0;
@@ -318,7 +348,7 @@ int frico_adjoint_forward(frico_ctx* c, double freq, const int32_t* ifaces, int 
         return rc;
     if (applied)
         A.signs = signs;
-    if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, FRICO_C128)) ||
+    if ((rc = frico_fill(c, freq, c->adj_fill, FRICO_C128)) ||
         (rc = frico_rhs_deltagap(c, freq, ifaces, nif, v_re, v_im, applied ? signs.data() : nullptr)))
         return rc;
     A.V.resize((size_t)N);
@@ -379,13 +409,15 @@ int frico_adjoint_rcs_forward(frico_ctx* c, double freq, double theta_i, double 
     if (!out)
         return set_error("null output"), -10;
     const int q = c->hm.q;
-    if (q != 1 && q != 3 && q != 6 && q != 7)
-        return set_error("adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
+    const bool approx = c->adj_fill == FRICO_FILL_APPROX;
+    if (!approx && q != 1 && q != 3 && q != 6 && q != 7)
+        return set_error("quadrature-mode adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
     auto& A = c->adj;
     A = frico_ctx::AdjointState{};
+    A.approx = approx;
     const int64_t N = c->N;
     int rc;
-    if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, FRICO_C128)) ||
+    if ((rc = frico_fill(c, freq, c->adj_fill, FRICO_C128)) ||
         (rc = frico_rhs_planewave(c, freq, 1, &theta_i, &phi_i, eth, eph)))
         return rc;
     A.V.resize((size_t)N);
@@ -462,8 +494,7 @@ int frico_adjoint_rcs_backward(frico_ctx* c, double g_vv, double g_hh, double* g
     // mu^T dZ I: the device contraction at fixed edge lengths, then the l_i l_j prefactor
     const double omega = 2.0 * PI * A.freq;
     const double k = omega * std::sqrt(MU0 * EPS0), inv_ksq = 1. / (omega * omega * MU0 * EPS0);
-    if ((rc = launch_adjoint_contract(c->dm, c->am, k, inv_ksq, c->d_near.p, (int64_t)hm.near.size(), c->X.p, c->adjW.p,
-                                      c->aw.p, c->bw.p, c->d_gvert.p, nv, st)))
+    if ((rc = contract(c, c->X.p, c->adjW.p, k, inv_ksq)))
         return rc;
     std::vector<cd> C((size_t)(3 * nv));
     FRICO_HIP_CHECK(hipMemcpyAsync(C.data(), c->d_gvert.p, (size_t)(6 * nv) * sizeof(double), hipMemcpyDeviceToHost, st));

@@ -253,10 +253,16 @@ struct frico_ctx {
     // adjoint
     DevBuf<int> d_tri_v;
     DevBuf<double> d_qbeta, d_qsub_beta, d_gvert;
+    // fill mode of the adjoint pairs (FRICO_FILL_QUAD or FRICO_FILL_APPROX) and the approximate
+    // mode's one-point rule workspaces
+    int adj_fill = FRICO_FILL_QUAD;
+    DevBuf<double4> adj_cent;
+    DevBuf<double> adj_third;
     DevBuf<c128> adjW, aw, bw;
     AdjMesh am{};
     struct AdjointState {
         bool forward_done = false, grads_done = false;
+        bool approx = false; // the forward fill was the approximate mode's
         double freq = 0;
         std::vector<int32_t> ifaces;
         std::complex<double> Vg = 0, Iport = 0, zin = 0;

@@ -3043,12 +3043,13 @@ static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int6
 // thousand rows costs about a millisecond of launch and wait latency.  Here one launch solves the
 // whole triangle: workgroup w owns the 64-row chunk j (j = w for L, forward; j = nch - 1 - w for U,
 // backward), so a workgroup depends only on chunks owned by lower workgroup ids, which the
-// dispatcher starts first: the grid needs no co-residency.  Per dependency chunk i, in the order
-// they finish, wave 0 waits for x_i and copies it to LDS, and every thread takes
-// D(j, i) x_i off its partial sums (row = lane, sixteen columns of the chunk = wave); the 64 x 16
-// entries of the next chunk's block are loaded while this one waits.  Then the four waves' partial
-// sums are reduced, wave q substitutes right-hand side q with the diagonal block (from LDS, as in
-// block_solve) and publishes x_j.
+// dispatcher starts first: the grid needs no co-residency.  The dependency chunks i are taken in
+// the order they finish, up to CHAIN_WIN a step: wave 0 waits for the first one's x_i and takes the
+// following ones already published in the same poll, and every thread takes D(j, i) x_i off its
+// partial sums (row = lane, sixteen columns of the chunk = wave), the blocks of D loaded before
+// the wait.  A workgroup far behind the front so takes four chunks per round trip of the exchange.
+// Then the four waves' partial sums are reduced, wave q substitutes right-hand side q with the
+// diagonal block (from LDS, as in block_solve) and publishes x_j.
 // The exchange area holds, per chunk, row and right-hand side, two 64-bit words {sequence, re} and
 // {sequence, im} written and read with relaxed agent-scope atomics (as the panel factorisation's
 // exchange): a reader takes a word only when it carries the launch's sequence number, so the area
@@ -3060,14 +3061,16 @@ constexpr int CHAIN_MAXR = 4;
 
 size_t lu_chain_words(int64_t n, int64_t nrhs) { return 1 + (size_t)((n + 63) / 64) * 64 * (size_t)nrhs * 2; }
 
+constexpr int CHAIN_WIN = 4; // dependency chunks taken per wait
+
 template <bool UPPER>
 __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, int64_t lda, int64_t n, c64* B,
                                                    int64_t ldb, int nr, unsigned long long* words, unsigned seq)
 {
-    __shared__ c64 sd[64][65];             // sd[c][r] = D(r, c), the chunk's diagonal block
-    __shared__ c64 sx[2][64][CHAIN_MAXR];  // x of the dependency chunk being taken off (two steps)
-    __shared__ c64 sp[4][64][CHAIN_MAXR];  // the four waves' partial sums
-    __shared__ int s_ok;
+    __shared__ c64 sd[64][65];                         // sd[c][r] = D(r, c), the chunk's diagonal block
+    __shared__ c64 sx[2][CHAIN_WIN][64][CHAIN_MAXR];   // x of the chunks of a step (two steps)
+    __shared__ c64 sp[4][64][CHAIN_MAXR];              // the four waves' partial sums
+    __shared__ int s_ok, s_nav[2];
     int* abort_flag = reinterpret_cast<int*>(words);
     unsigned long long* xw = words + 1;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -3085,7 +3088,8 @@ __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, in
     const int64_t arow = r0 + (lane < wj ? lane : 0);
     const int ndep = UPPER ? nch - 1 - j : j;
     auto lload = [&](int t, c64* dst) {
-        const int i = UPPER ? nch - 1 - t : t;
+        const int tt = t < ndep ? t : ndep - 1; // past the last dependency: a valid block, unused
+        const int i = UPPER ? nch - 1 - tt : tt;
         const int64_t c0 = 64 * (int64_t)i + 16 * wv;
 #pragma unroll
         for (int c = 0; c < 16; c++) {
@@ -3097,54 +3101,75 @@ __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, in
 #pragma unroll
     for (int q = 0; q < CHAIN_MAXR; q++)
         acc[q] = c64{0.0f, 0.0f};
-    bool ok = true;
-    // step t: wait for x_i (wave 0) and take D(j, i) x_i off the partial sums
-    auto step = [&](int t, const c64* cur, c64* nxt) {
-        if (t + 1 < ndep)
-            lload(t + 1, nxt);
-        const int i = UPPER ? nch - 1 - t : t;
-        c64(*bx)[CHAIN_MAXR] = sx[t & 1];
+    // the chunk's right-hand side (wave q: column q), read now rather than on the chain's path
+    const c64 bq = (wv < nr && lane < wj) ? B[(r0 + lane) + wv * ldb] : c64{0.0f, 0.0f};
+    __syncthreads(); // sd and s_ok written
+    // Per step: the blocks of the next CHAIN_WIN dependency chunks are loaded, wave 0 waits for the
+    // first one's x and takes every following one already published in the same poll (up to
+    // CHAIN_WIN), and every thread takes the chunks found off its partial sums.
+    int t = 0, par = 0;
+    while (t < ndep && s_ok) {
+        c64 lv[CHAIN_WIN][16];
+#pragma unroll
+        for (int u = 0; u < CHAIN_WIN; u++)
+            lload(t + u, lv[u]);
         if (wv == 0) {
-            const unsigned long long* src = xw + ((size_t)i * 64 + lane) * (size_t)nr * 2;
-            unsigned long long v[2 * CHAIN_MAXR];
+            bool ok = true;
+            int nav = 0;
+            unsigned long long v[CHAIN_WIN][2 * CHAIN_MAXR];
             long spins = 0;
             for (;;) {
-                bool ready = true;
+                nav = 0;
+                bool run = true;
 #pragma unroll
-                for (int q = 0; q < 2 * CHAIN_MAXR; q++)
-                    if (q < 2 * nr) {
-                        v[q] = ll_load(src + q);
-                        ready = ready && (unsigned)(v[q] >> 32) == seq;
+                for (int u = 0; u < CHAIN_WIN; u++) {
+                    if (t + u < ndep) {
+                        const int i = UPPER ? nch - 1 - (t + u) : t + u;
+                        const unsigned long long* src = xw + ((size_t)i * 64 + lane) * (size_t)nr * 2;
+                        bool ready = true;
+#pragma unroll
+                        for (int q = 0; q < 2 * CHAIN_MAXR; q++)
+                            if (q < 2 * nr) {
+                                v[u][q] = ll_load(src + q);
+                                ready = ready && (unsigned)(v[u][q] >> 32) == seq;
+                            }
+                        ready = __all(ready);
+                        run = run && ready;
+                        nav += run ? 1 : 0;
                     }
-                ready = __all(ready);
-                if (ready || !(ok = ll_spin(spins, abort_flag)))
+                }
+                if (nav > 0 || !(ok = ll_spin(spins, abort_flag)))
                     break;
             }
 #pragma unroll
-            for (int q = 0; q < CHAIN_MAXR; q++)
-                bx[lane][q] = q < nr ? c64{__uint_as_float((unsigned)v[2 * q]), __uint_as_float((unsigned)v[2 * q + 1])}
-                                     : c64{0.0f, 0.0f};
-            if (!ok && lane == 0)
-                s_ok = 0;
+            for (int u = 0; u < CHAIN_WIN; u++)
+                if (u < nav)
+#pragma unroll
+                    for (int q = 0; q < CHAIN_MAXR; q++)
+                        sx[par][u][lane][q] = q < nr ? c64{__uint_as_float((unsigned)v[u][2 * q]),
+                                                           __uint_as_float((unsigned)v[u][2 * q + 1])}
+                                                     : c64{0.0f, 0.0f};
+            if (lane == 0) {
+                s_nav[par] = nav;
+                if (!ok)
+                    s_ok = 0;
+            }
         }
         __syncthreads();
+        const int nav = s_nav[par];
 #pragma unroll
-        for (int c = 0; c < 16; c++)
+        for (int u = 0; u < CHAIN_WIN; u++)
+            if (u < nav)
 #pragma unroll
-            for (int q = 0; q < CHAIN_MAXR; q++) {
-                const c64 x = bx[16 * wv + c][q];
-                acc[q].re += cur[c].re * x.re - cur[c].im * x.im;
-                acc[q].im += cur[c].re * x.im + cur[c].im * x.re;
-            }
-    };
-    c64 l0[16], l1[16];
-    if (ndep > 0)
-        lload(0, l0);
-    __syncthreads(); // sd and s_ok written
-    for (int t = 0; t < ndep && s_ok; t += 2) {
-        step(t, l0, l1);
-        if (t + 1 < ndep && s_ok)
-            step(t + 1, l1, l0);
+                for (int c = 0; c < 16; c++)
+#pragma unroll
+                    for (int q = 0; q < CHAIN_MAXR; q++) {
+                        const c64 x = sx[par][u][16 * wv + c][q];
+                        acc[q].re += lv[u][c].re * x.re - lv[u][c].im * x.im;
+                        acc[q].im += lv[u][c].re * x.im + lv[u][c].im * x.re;
+                    }
+        t += nav;
+        par ^= 1;
     }
     __syncthreads(); // every wave done with sx; s_ok final
 #pragma unroll
@@ -3155,7 +3180,7 @@ __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, in
         const int q = wv;
         c64 b = c64{0.0f, 0.0f};
         if (lane < wj) {
-            b = B[(r0 + lane) + q * ldb];
+            b = bq;
 #pragma unroll
             for (int w = 0; w < 4; w++) {
                 b.re -= sp[w][lane][q].re;

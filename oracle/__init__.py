@@ -310,6 +310,7 @@ def bistatic_analysis(sm, freq, theta, phi, eth=1.0, eph=1.0, degree=5, approx=F
 def _adj_sigs():
     L = lib()
     L.orc_adjoint_contract.argtypes = [C.c_void_p, C.c_double, C.c_int, _d, _d, _d]
+    L.orc_adjoint_contract_approx.argtypes = [C.c_void_p, C.c_double, _d, _d, _d]
     L.orc_rhs_planewave_vjp.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
                                         C.c_double, C.c_double, C.c_double, _d, _d]
     L.orc_fields_vjp.argtypes = [C.c_void_p, C.c_double, _d, _d, _d, _d, _d, _d]
@@ -317,30 +318,35 @@ def _adj_sigs():
     return L
 
 
-def adjoint_contract(om, freq, degree, I, W):
-    """g[v, c] = sum_ij W_j I_i dZ(j,i)/dx_{v,c} (complex), quadrature-mode Z."""
+def adjoint_contract(om, freq, degree, I, W, approx=False):
+    """g[v, c] = sum_ij W_j I_i dZ(j,i)/dx_{v,c} (complex), quadrature-mode Z, or the
+    approximate-mode Z of compute_matrix_approx (degree unused) when approx."""
     L = _adj_sigs()
     I = np.ascontiguousarray(I, np.complex128)
     W = np.ascontiguousarray(W, np.complex128)
     g = np.zeros(om.sm.nv * 3, np.complex128)
-    rc = L.orc_adjoint_contract(om.h, freq, degree, _p(I), _p(W), _p(g))
+    if approx:
+        rc = L.orc_adjoint_contract_approx(om.h, freq, _p(I), _p(W), _p(g))
+    else:
+        rc = L.orc_adjoint_contract(om.h, freq, degree, _p(I), _p(W), _p(g))
     if rc:
         raise ValueError("bad degree")
     return g.reshape(-1, 3)
 
 
-def antenna_gradients(sm, freq, degree=5, obs=(0.0, 0.0, 10.0), voltage=1.0):
+def antenna_gradients(sm, freq, degree=5, obs=(0.0, 0.0, 10.0), voltage=1.0, approx=False):
     """FP64 CPU reference of the adjoint: input impedance Zin and gain G at the observation point
     `obs`, with dZin/dx (complex) and dG/dx (real) for every vertex coordinate.
 
     Forward: Z I = V (delta gap), Zin = V / sum_gap l s I, G = 4 pi |obs|^2 |S| / P with
     S = E x conj(H) / 2 (src/emw_ana_antenna.cpp:33-105).  Adjoint: Z^T lam = c (c = l s on the
-    gap), Z^T mu = d|S|/dI (Wirtinger); dI_port = (dc) I + lam^T (dV - dZ I)."""
+    gap), Z^T mu = d|S|/dI (Wirtinger); dI_port = (dc) I + lam^T (dV - dZ I).  approx: the
+    approximate-mode Z (the delta-gap moments do not depend on it)."""
     L = _adj_sigs()
     m = OracleMesh(sm)
     signs = m.reorient(sm.feed)
     s = np.ones(m.N) if signs is None else signs
-    Z = m.fill(freq, degree)
+    Z = m.fill(freq, degree, approx)
     V = m.rhs_deltagap(freq, sm.feed, voltage, signs)
     I, info = lu_solve(Z, V)
     gap = np.isin(m.iface, sm.feed) & (m.has_iface != 0)
@@ -368,8 +374,8 @@ def antenna_gradients(sm, freq, degree=5, obs=(0.0, 0.0, 10.0), voltage=1.0):
                      _p(np.ascontiguousarray(eH)), _p(gx_f), _p(gI))
     lam = np.linalg.solve(Z.T, c)
     mu = np.linalg.solve(Z.T, gI)
-    Clam = adjoint_contract(m, freq, degree, I, lam).reshape(-1)
-    Cmu = adjoint_contract(m, freq, degree, I, mu).reshape(-1)
+    Clam = adjoint_contract(m, freq, degree, I, lam, approx).reshape(-1)
+    Cmu = adjoint_contract(m, freq, degree, I, mu, approx).reshape(-1)
     # edge-length terms of V and of the port weights
     dc_I = np.zeros(sm.nv * 3, np.complex128)
     lam_dV = np.zeros(sm.nv * 3, np.complex128)
@@ -402,7 +408,8 @@ def rhs_planewave_vjp(om, freq, degree, theta, phi, eth, eph, W):
     return g.reshape(-1, 3)
 
 
-def rcs_gradients(sm, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, degree=5, R=10.0):
+def rcs_gradients(sm, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_s=None, degree=5, R=10.0,
+                  approx=False):
     """FP64 CPU reference of the RCS adjoint: sigma_VV and sigma_HH at (R, theta_s, phi_s) for the
     plane wave from (theta_i, phi_i) (bistatic_RCS_unit_Ei, src/emw_ana_bistatic_rcs.cpp:62-82, of
     the currents of update_rhs(plane_wave), src/emw_solver.cpp:380-416, and compute_matrix), with
@@ -411,12 +418,13 @@ def rcs_gradients(sm, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_
 
     sigma = 4 pi R^2 |a|^2 with a the theta (or phi) component of E_s; with Z^T mu = dE-cotangent
     pulled back to the currents, d sigma = 2 Re(g_x + mu^T dV - mu^T dZ I), g_x the explicit
-    vertex dependence of the field sums."""
+    vertex dependence of the field sums.  approx: the approximate-mode Z (the plane-wave moments
+    keep the degree's rule, as update_rhs does)."""
     L = _adj_sigs()
     theta_s = theta_i if theta_s is None else theta_s
     phi_s = phi_i if phi_s is None else phi_s
     m = OracleMesh(sm)
-    Z = m.fill(freq, degree)
+    Z = m.fill(freq, degree, approx)
     V = m.rhs_planewave(freq, theta_i, phi_i, eth, eph, degree)
     I, info = lu_solve(Z, V)
     AJ, AdivJ, _ = m.scatter(I)
@@ -436,6 +444,6 @@ def rcs_gradients(sm, freq, theta_i, phi_i, eth=1.0, eph=0.0, theta_s=None, phi_
                          _p(np.ascontiguousarray(eE)), _p(np.zeros(3, np.complex128)), _p(gx), _p(gI))
         mu = np.linalg.solve(Z.T, gI)
         dV = rhs_planewave_vjp(m, freq, degree, theta_i, phi_i, eth, eph, mu).reshape(-1)
-        Cmu = adjoint_contract(m, freq, degree, I, mu).reshape(-1)
+        Cmu = adjoint_contract(m, freq, degree, I, mu, approx).reshape(-1)
         out[key] = (2 * (gx + dV - Cmu).real).reshape(-1, 3)
     return out

@@ -1226,6 +1226,121 @@ int orc_adjoint_contract(orc_mesh* m, double freq, int degree, const double* Iin
 
 
 /*
+ * grad[3*v + c] (complex) = sum_{i,j} W_j I_i dZ(j,i)/dx_{v,c} for the approximate-mode Z of
+ * compute_matrix_approx (src/emw_solver.cpp:144-220), differentiated by hand through its per-RWG-
+ * pair loop: each of the four terms s l_i l_j (0.25 rho_c G - G / k^2) depends on the two
+ * barycenters (mean of the vertices), the two free vertices (rho_c = (c_i - p_i).(c_j - p_j)),
+ * their distance R through G = exp(-jkR) / (4 pi R), or, for a pair of equal triangles, on the
+ * area through G = 0.25 (sqrt(4 pi) / sqrt(A) - jk) / pi; the lengths are hypot(dx, dy).
+ */
+int orc_adjoint_contract_approx(orc_mesh* m, double freq, const double* Iin, const double* Win, double* gout)
+{
+    const int64_t N = (int64_t)m->bf.size();
+    const cd* I = reinterpret_cast<const cd*>(Iin);
+    const cd* W = reinterpret_cast<const cd*>(Win);
+    const size_t nv = m->V.size();
+    const double omega = 2.0 * M_PI * freq;
+    const double k = omega * std::sqrt(MU0 * EPS0);
+    const double inv_ksq = 1. / (omega * omega * MU0 * EPS0);
+    const double sqrt_4pi = std::sqrt(4 * M_PI);
+    const cd jk{0., k};
+    const int evmap[3] = {2, 0, 1};
+    std::vector<cd> total(3 * nv, cd(0));
+#pragma omp parallel
+    {
+        std::vector<cd> G(3 * nv, cd(0));
+        auto addv = [&](int64_t v, const vec3& d, cd s) {
+            G[3 * v] += d.x * s;
+            G[3 * v + 1] += d.y * s;
+            G[3 * v + 2] += d.z * s;
+        };
+        auto add_bar = [&](int64_t t, const vec3& d, cd s) { // d/d(barycenter) -> a third to each vertex
+            for (int kk = 0; kk < 3; kk++)
+                addv(m->T[t][kk], d, s / 3.0);
+        };
+        auto edge_grad = [&](const rwg& b, cd s) { // s dl/dx, l = hypot(dx, dy)
+            const edge_t& e = m->edges[b.edge_index];
+            vec3 d = m->V[e.iv1] - m->V[e.iv0];
+            if (b.length == 0)
+                return;
+            const vec3 g{d.x / b.length, d.y / b.length, 0.0};
+            addv(e.iv1, g, s);
+            addv(e.iv0, g, -s);
+        };
+        auto area_grad_add = [&](int64_t t, cd s) {
+            const auto& tr = m->T[t];
+            vec3 g[3];
+            area_grad(m->V[tr[0]], m->V[tr[1]], m->V[tr[2]], g);
+            for (int kk = 0; kk < 3; kk++)
+                addv(tr[kk], g[kk], s);
+        };
+#pragma omp for schedule(dynamic, 4)
+        for (int64_t ii = 0; ii < N; ii++) {
+            const rwg& ibf = m->bf[ii];
+            for (int64_t jj = 0; jj < N; jj++) {
+                const rwg& jbf = m->bf[jj];
+                const cd om = W[jbf.matrix_index] * I[ibf.matrix_index];
+                if (om == cd(0))
+                    continue;
+                const double ll = ibf.length * jbf.length;
+                cd entry_over_ll = 0;
+                for (int a = 0; a < 2; a++) {
+                    for (int b = 0; b < 2; b++) {
+                        const int64_t P = a == 0 ? ibf.itminus : ibf.itplus;
+                        const int64_t Q = b == 0 ? jbf.itminus : jbf.itplus;
+                        const double sg = a == b ? 1.0 : -1.0;
+                        const vec3 ci = tri_bar(*m, P), cj = tri_bar(*m, Q);
+                        const vec3 pi = a == 0 ? ibf.pminus : ibf.pplus, pj = b == 0 ? jbf.pminus : jbf.pplus;
+                        const int64_t piv = m->T[P][evmap[a == 0 ? ibf.loc_eminus : ibf.loc_eplus]];
+                        const int64_t pjv = m->T[Q][evmap[b == 0 ? jbf.loc_eminus : jbf.loc_eplus]];
+                        const vec3 ui = ci - pi, uj = cj - pj;
+                        const double rho_c = dot(ui, uj);
+                        const bool self = P == Q;
+                        cd Gv, dGdR = 0;
+                        double A = 0;
+                        vec3 dR{0, 0, 0};
+                        if (self) {
+                            A = tri_measure(*m, P);
+                            Gv = 0.25 * (sqrt_4pi / std::sqrt(A) - jk) * M_1_PI;
+                        } else {
+                            const vec3 d = ci - cj;
+                            const double R = norm(d);
+                            Gv = std::exp(cd{0, -k * R}) / (4.0 * M_PI * R);
+                            dGdR = -(jk + 1.0 / R) * Gv;
+                            dR = d / R; // dR/dci = d / R, dR/dcj = -d / R
+                        }
+                        const cd h = 0.25 * rho_c - inv_ksq;
+                        entry_over_ll += sg * h * Gv;
+                        const cd c0 = om * sg * ll;
+                        // rho_c: d/dci = uj, d/dpi = -uj, d/dcj = ui, d/dpj = -ui (times 0.25 G)
+                        add_bar(P, uj, c0 * 0.25 * Gv);
+                        addv(piv, uj, -c0 * 0.25 * Gv);
+                        add_bar(Q, ui, c0 * 0.25 * Gv);
+                        addv(pjv, ui, -c0 * 0.25 * Gv);
+                        if (self) {
+                            // dG/dA = -0.125 sqrt(4 pi) A^(-3/2) / pi
+                            const double dGdA = -0.125 * sqrt_4pi / (A * std::sqrt(A)) * M_1_PI;
+                            area_grad_add(P, c0 * h * dGdA);
+                        } else {
+                            add_bar(P, dR, c0 * h * dGdR);
+                            add_bar(Q, dR, -c0 * h * dGdR);
+                        }
+                    }
+                }
+                // l_i l_j: d/dl_i = l_j (entry / ll), d/dl_j = l_i (entry / ll)
+                edge_grad(ibf, om * entry_over_ll * jbf.length);
+                edge_grad(jbf, om * entry_over_ll * ibf.length);
+            }
+        }
+#pragma omp critical
+        for (size_t x = 0; x < 3 * nv; x++)
+            total[x] += G[x];
+    }
+    std::memcpy(gout, total.data(), total.size() * sizeof(cd));
+    return 0;
+}
+
+/*
  * Vector-Jacobian product of the field sums at one point (src/emw_postpro_common.cpp:31-66 with
  * the scatter of src/emw_solver.h:137-164) for the current vector I:
  *   gx[3v+c] = sum_d eE_d dE_d/dx_{v,c} + eH_d dH_d/dx_{v,c}   (I held fixed)

@@ -181,3 +181,66 @@ def test_monostatic_rcs_gradient_matches_central_differences(gpu_ctx):
         print(f"vertex {v} coord {c}: adjoint {grad[v, c]:.6e} central difference {fd:.6e}")
         assert abs(fd - grad[v, c]) <= 1e-6 * scale
     assert s0[0] > 0
+
+
+# ---- approximate mode (the reference's -A: compute_matrix_approx, src/emw_solver.cpp:144-220) ---
+
+@pytest.mark.parametrize("degree", [5, 3, 8])
+def test_approx_gradients_match_the_cpu_adjoint(gpu_ctx, degree):
+    """Zin and gain gradients with the approximate-mode Z against the oracle's FP64 approximate
+    adjoint, at degrees the quadrature-mode adjoint has no kernel for (the approximate Z does not
+    use the rule)."""
+    lam = meshes.wavelength(F0)
+    sm = meshes.bowtie(0.25 * lam, 60.0, 4, 3)
+    ref = oracle.antenna_gradients(sm, F0, degree, obs=OBS, approx=True)
+    gpu_ctx.load_mesh(sm, degree)
+    g = gpu_ctx.antenna_gradients(F0, sm.feed, 1.0, OBS, mode="approx")
+    assert abs(g["zin"] - ref["zin"]) <= 1e-10 * abs(ref["zin"])
+    assert abs(g["gain"] - ref["gain"]) <= 1e-10 * abs(ref["gain"])
+    ez = np.max(np.abs(g["dzin"] - ref["dzin"])) / np.max(np.abs(ref["dzin"]))
+    eg = np.max(np.abs(g["dgain"] - ref["dgain"])) / np.max(np.abs(ref["dgain"]))
+    print(f"approx degree {degree}: dZin rel err {ez:.2e}, dG rel err {eg:.2e}")
+    assert ez <= 1e-9 and eg <= 1e-9
+    # the quadrature-mode gradients of the same mesh differ (the mode is not ignored)
+    if degree == 5:
+        q = gpu_ctx.antenna_gradients(F0, sm.feed, 1.0, OBS)
+        assert np.max(np.abs(q["dzin"] - g["dzin"])) > 1e-3 * np.max(np.abs(g["dzin"]))
+
+
+@pytest.mark.parametrize("bistatic", [False, True])
+def test_approx_rcs_gradients_match_the_cpu_adjoint(gpu_ctx, bistatic):
+    lam = meshes.wavelength(F0)
+    sm = meshes.plate(0.6 * lam, 3)
+    thi, phi_ = 0.5, 0.3
+    ths, phs = (1.1, -0.7) if bistatic else (thi, phi_)
+    ref = oracle.rcs_gradients(sm, F0, thi, phi_, 1.0, 0.5j, ths, phs, degree=5, R=10.0, approx=True)
+    gpu_ctx.load_mesh(sm, 5)
+    g = gpu_ctx.rcs_gradients(F0, thi, phi_, 1.0, 0.5j, ths, phs, R=10.0, mode="approx")
+    assert np.allclose(g["sigma"], ref["sigma"], rtol=1e-10)
+    for key in ("dsigma_vv", "dsigma_hh"):
+        e = np.max(np.abs(g[key] - ref[key])) / np.max(np.abs(ref[key]))
+        print(f"approx rcs bistatic={bistatic} {key} rel err {e:.2e}")
+        assert e <= 1e-9
+
+
+def test_approx_gradient_matches_central_differences():
+    """The GPU's approximate-mode dZin against central differences (1e-6 wavelengths) of its own
+    approximate-mode forward, on a strip with a 2-edge gap."""
+    import frico_amd
+    lam = meshes.wavelength(F0)
+    sm = meshes.strip_dipole(0.5 * lam, 0.02 * lam, 12, 2)
+    h = 1e-6 * lam
+    with frico_amd.Context(0) as c:
+        c.load_mesh(sm, 1)
+        g = c.antenna_gradients(F0, sm.feed, 1.0, OBS, mode="approx")
+        for v in range(0, sm.nv, 5):
+            for d in range(3):
+                Vp, Vm = sm.vertices.copy(), sm.vertices.copy()
+                Vp[v, d] += h
+                Vm[v, d] -= h
+                c.load_mesh(_perturbed(sm, Vp), 1)
+                zp, _ = c.adjoint_forward(F0, sm.feed, 1.0, OBS, mode="approx")
+                c.load_mesh(_perturbed(sm, Vm), 1)
+                zm, _ = c.adjoint_forward(F0, sm.feed, 1.0, OBS, mode="approx")
+                fd = (zp - zm) / (2 * h)
+                assert abs(fd - g["dzin"][v, d]) <= 1e-6 * np.max(np.abs(g["dzin"]))

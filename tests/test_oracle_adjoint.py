@@ -120,3 +120,63 @@ def test_rcs_gradients_match_central_differences(bistatic):
             d = (fwd(Vp) - fwd(Vm)) / (2 * h)
             assert abs(d[0] - r["dsigma_vv"][v, c]) <= 1e-6 * np.max(np.abs(r["dsigma_vv"]))
             assert abs(d[1] - r["dsigma_hh"][v, c]) <= 1e-6 * np.max(np.abs(r["dsigma_hh"]))
+
+
+# ---- approximate mode (compute_matrix_approx, src/emw_solver.cpp:144-220: barycenters, free
+# vertices, the self terms' sqrt(area), hypot(dx, dy) lengths) -------------------------------------
+
+@pytest.mark.parametrize("sm", [meshes.strip_dipole(0.3, 0.04, 4, 2), meshes.icosphere(0.4, 1),
+                                meshes.bowtie(0.25, 60.0, 3, 2)], ids=lambda s: s.name)
+def test_approx_contraction_matches_central_differences(sm):
+    om = oracle.OracleMesh(sm)
+    rng = np.random.default_rng(2)
+    I = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    W = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    g = oracle.adjoint_contract(om, F0, 5, I, W, approx=True)
+    h = 1e-6
+    for v in range(0, sm.nv, max(1, sm.nv // 6)):
+        for c in range(3):
+            Vp, Vm = sm.vertices.copy(), sm.vertices.copy()
+            Vp[v, c] += h
+            Vm[v, c] -= h
+            Zp = oracle.OracleMesh(_perturbed(sm, Vp)).fill(F0, 5, approx=True)
+            Zm = oracle.OracleMesh(_perturbed(sm, Vm)).fill(F0, 5, approx=True)
+            fd = W @ ((Zp - Zm) / (2 * h)) @ I
+            assert abs(fd - g[v, c]) <= 1e-7 * np.max(np.abs(g))
+
+
+def test_approx_contraction_is_not_the_quadrature_one():
+    """The two modes' contractions differ (a mode switch that is ignored fails here)."""
+    sm = meshes.strip_dipole(0.3, 0.04, 4, 2)
+    om = oracle.OracleMesh(sm)
+    rng = np.random.default_rng(4)
+    I = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    W = rng.normal(size=om.N) + 1j * rng.normal(size=om.N)
+    ga = oracle.adjoint_contract(om, F0, 5, I, W, approx=True)
+    gq = oracle.adjoint_contract(om, F0, 5, I, W)
+    assert np.max(np.abs(ga - gq)) > 1e-3 * np.max(np.abs(gq))
+
+
+def test_approx_impedance_gradient_matches_central_differences():
+    lam = meshes.wavelength(F0)
+    sm = meshes.bowtie(0.25 * lam, 60.0, 4, 3)
+    r = oracle.antenna_gradients(sm, F0, 5, obs=(0.0, 0.0, 10.0), approx=True)
+    h = 1e-6 * lam
+
+    def zin(V):
+        s2 = _perturbed(sm, V)
+        m = oracle.OracleMesh(s2)
+        signs = m.reorient(s2.feed)
+        I, _ = oracle.lu_solve(m.fill(F0, 5, approx=True), m.rhs_deltagap(F0, s2.feed, 1.0, signs))
+        s = np.ones(m.N) if signs is None else signs
+        gap = np.isin(m.iface, s2.feed) & (m.has_iface != 0)
+        return 1.0 / (np.where(gap, m.length * s, 0.0) @ I)
+
+    assert abs(zin(sm.vertices) - r["zin"]) <= 1e-12 * abs(r["zin"])
+    for v in range(0, sm.nv, 3):
+        for c in range(3):
+            Vp, Vm = sm.vertices.copy(), sm.vertices.copy()
+            Vp[v, c] += h
+            Vm[v, c] -= h
+            d = (zin(Vp) - zin(Vm)) / (2 * h)
+            assert abs(d - r["dzin"][v, c]) <= 1e-6 * np.max(np.abs(r["dzin"]))
