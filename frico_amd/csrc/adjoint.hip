@@ -495,6 +495,15 @@ int launch_adjoint_contract(const DevMesh& m, const AdjMesh& am, double k, doubl
     case 3:
         hipLaunchKernelGGL((tripair_adjoint<3>), g, dim3(256), sh, st, m, am, k, inv_ksq, aw, bw, gvert);
         break;
+    case 4:
+        hipLaunchKernelGGL((tripair_adjoint<4>), g, dim3(256), sh, st, m, am, k, inv_ksq, aw, bw, gvert);
+        break;
+    case 12:
+        hipLaunchKernelGGL((tripair_adjoint<12>), g, dim3(256), sh, st, m, am, k, inv_ksq, aw, bw, gvert);
+        break;
+    case 13:
+        hipLaunchKernelGGL((tripair_adjoint<13>), g, dim3(256), sh, st, m, am, k, inv_ksq, aw, bw, gvert);
+        break;
     case 6:
         hipLaunchKernelGGL((tripair_adjoint<6>), g, dim3(256), sh, st, m, am, k, inv_ksq, aw, bw, gvert);
         break;
@@ -502,7 +511,7 @@ int launch_adjoint_contract(const DevMesh& m, const AdjMesh& am, double k, doubl
         hipLaunchKernelGGL((tripair_adjoint<7>), g, dim3(256), sh, st, m, am, k, inv_ksq, aw, bw, gvert);
         break;
     default:
-        return set_error("adjoint: Dunavant degree with %d points is not instantiated (use degree 1, 2, 4 or 5)", m.q), -1;
+        return set_error("adjoint: Dunavant degree with %d points is not instantiated (degrees 0 to 7 are)", m.q), -1;
     }
     FRICO_HIP_CHECK(hipGetLastError());
     if (nnear > 0)

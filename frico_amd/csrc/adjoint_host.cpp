@@ -333,10 +333,9 @@ int frico_adjoint_forward(frico_ctx* c, double freq, const int32_t* ifaces, int 
         return set_error("no delta-gap interfaces"), -3;
     if (!obs || !out)
         return set_error("null observation point or output"), -7;
-    const int q = c->hm.q;
     const bool approx = c->adj_fill == FRICO_FILL_APPROX;
-    if (!approx && q != 1 && q != 3 && q != 6 && q != 7)
-        return set_error("quadrature-mode adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
+    if (!approx && c->hm.degree > 7)
+        return set_error("quadrature-mode adjoint supports Dunavant degrees 0 to 7"), FRICO_ERR_LIMIT;
     auto& A = c->adj;
     A = frico_ctx::AdjointState{};
     A.approx = approx;
@@ -408,10 +407,9 @@ int frico_adjoint_rcs_forward(frico_ctx* c, double freq, double theta_i, double 
         return set_error("observation distance must be positive"), -9;
     if (!out)
         return set_error("null output"), -10;
-    const int q = c->hm.q;
     const bool approx = c->adj_fill == FRICO_FILL_APPROX;
-    if (!approx && q != 1 && q != 3 && q != 6 && q != 7)
-        return set_error("quadrature-mode adjoint supports Dunavant degrees 1, 2, 4 and 5"), FRICO_ERR_LIMIT;
+    if (!approx && c->hm.degree > 7)
+        return set_error("quadrature-mode adjoint supports Dunavant degrees 0 to 7"), FRICO_ERR_LIMIT;
     auto& A = c->adj;
     A = frico_ctx::AdjointState{};
     A.approx = approx;

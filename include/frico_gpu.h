@@ -311,7 +311,7 @@ int frico_monostatic_sweep(frico_ctx* ctx, const double* freqs, int nfreq, int m
  * frico_adjoint_set_fill: the fill of the forward entry points below, FRICO_FILL_QUAD (the default,
  *   compute_matrix) or FRICO_FILL_APPROX (compute_matrix_approx, src/emw_solver.cpp:144-220, the
  *   reference's -A); the backward pass differentiates the fill its forward pass used.
- * frico_adjoint_forward: fill at `freq` (quadrature mode: Dunavant degree 1, 2, 4 or 5 of the loaded
+ * frico_adjoint_forward: fill at `freq` (quadrature mode: Dunavant degree 0 to 7 of the loaded
  *   mesh), delta-gap excitation on ifaces, complex128 LU and solve; out[0..2] = Re Zin, Im Zin and
  *   the gain G = 4 pi |obs|^2 |S| / Re(V conj(I_port)/2) at the point obs[3].
  * frico_adjoint_backward: grad[3*v + c] = g_zin_re dRe(Zin)/dx + g_zin_im dIm(Zin)/dx + g_gain dG/dx
@@ -329,7 +329,7 @@ int frico_adjoint_backward(frico_ctx* ctx, double g_zin_re, double g_zin_im, dou
  * src/emw_ana_bistatic_rcs.cpp:62-82, of the currents of update_rhs(plane_wave),
  * src/emw_solver.cpp:380-416, through compute_matrix, src/emw_solver.cpp:45-142).
  * frico_adjoint_rcs_forward: fill at `freq` in the mode of frico_adjoint_set_fill (quadrature mode:
- *   Dunavant degree 1, 2, 4 or 5; the plane-wave moments keep the loaded degree's rule), one plane
+ *   Dunavant degree 0 to 7; the plane-wave moments keep the loaded degree's rule), one plane
  *   wave from (theta_i, phi_i) with complex amplitudes eth[2], eph[2], complex128 LU and solve;
  *   out[0] = sigma_VV, out[1] = sigma_HH at sph2rect(R, theta_s, phi_s) (monostatic: theta_s =
  *   theta_i, phi_s = phi_i).

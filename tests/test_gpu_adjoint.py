@@ -19,7 +19,7 @@ def _perturbed(sm, V):
     return meshes.SurfaceMesh(V, sm.triangles, sm.tags, sm.bedges, sm.btags, feed=sm.feed)
 
 
-@pytest.mark.parametrize("degree", [5, 2, 1])
+@pytest.mark.parametrize("degree", [5, 2, 1, 3, 7])
 def test_gradients_match_the_cpu_adjoint(gpu_ctx, degree):
     lam = meshes.wavelength(F0)
     sm = meshes.bowtie(0.25 * lam, 60.0, 4, 3)
