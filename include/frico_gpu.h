@@ -355,9 +355,11 @@ int frico_adjoint_rcs_backward(frico_ctx* ctx, double g_vv, double g_hh, double*
  *   zin[2*(m*nfreq + f) ..] = V / I_port (compute_port_values, src/emw_ana_antenna.cpp:76-105);
  *   info[m*nfreq + f] is the solve status.  Meshes run largest first on up to `workers` concurrent
  *   contexts (fewer when ceil(Nmax / R) panel workgroups per context would exceed the CUs).
- *   FRICO_BATCH_GRAPHS replays a captured HIP graph per mesh for the complex64 LU
- *   (FRICO_SOLVE_C64_IR) and each refinement sweep; FRICO_SOLVE_C64_NP_IR runs eagerly, with its
- *   fall-back to the pivoted LU (the worker count is sized for that fall-back's panel).  The
+ *   FRICO_BATCH_GRAPHS replays a HIP graph per mesh, recorded from the kernel launches, for the
+ *   complex64 LU (FRICO_SOLVE_C64_IR or FRICO_SOLVE_C64_NP_IR) and each refinement sweep; the
+ *   unpivoted form falls back to the pivoted LU on a zero pivot or a stalled refinement, and the
+ *   mesh's later frequencies then take the pivoted LU directly (the worker count is sized for
+ *   that fall-back's panel).  The
  *   symmetric form is rejected (-5): the batch's quadrature-mode Z is not symmetric, and
  *   run_context solves symmetrically only approximate-mode or symmetrised matrices
  *   (src/emw_solver.h:124-128).
