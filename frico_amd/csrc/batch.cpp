@@ -242,7 +242,24 @@ int pick_block(frico_batch* b, frico_ctx* c, const RunOpts& o, double freq, int*
     return 0;
 }
 
-int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin, frico_solve_info* infos)
+// compute_port_values (src/emw_ana_antenna.cpp:76-105): Zin = V / I_port, I_port = sum l I sign over the feed
+cd port_zin(const frico_batch::Mesh& m, const std::vector<frico_rwg>& bf, const std::vector<double>& signs, bool applied,
+            const cd* I, int64_t N, cd voltage)
+{
+    cd ip = 0.0;
+    for (int64_t n = 0; n < N; n++) {
+        if (!bf[n].has_interface)
+            continue;
+        for (int32_t itf : m.feed)
+            if (bf[n].interface_tag == itf)
+                ip += bf[n].length * I[n] * (applied ? signs[n] : 1.0);
+    }
+    return voltage / ip;
+}
+
+// only: the frequencies to solve (null: all)
+int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin, frico_solve_info* infos,
+             const std::vector<char>* only = nullptr)
 {
     const frico_batch::Mesh& m = b->meshes[mi];
     int rc = frico_mesh_load(c, m.V.data(), (int64_t)(m.V.size() / 3), m.T.data(), m.tags.data(),
@@ -267,14 +284,18 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
     // interchanges), the mesh's other frequencies go to the pivoted LU at once rather than each
     // paying for the unpivoted factorisation and sweeps first
     RunOpts ou = o;
+    bool first = true;
     for (int f = 0; f < o.nfreq; f++) {
+        if (only && !(*only)[f])
+            continue;
         const double freq = o.freqs[f];
         if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, o.fill_prec)) ||
             (rc = frico_rhs_deltagap(c, freq, m.feed.data(), (int)m.feed.size(), o.voltage.real(), o.voltage.imag(),
                                      applied ? signs.data() : nullptr)))
             return rc;
         frico_solve_info& inf = infos[(size_t)mi * o.nfreq + f];
-        if (f == 0) {
+        if (first) {
+            first = false;
             if ((rc = pick_block(b, c, ou, freq, &nb)) || (rc = frico_ctx_set_block(c, nb)))
                 return rc;
             b->mesh_nb[mi] = nb;
@@ -301,22 +322,280 @@ int run_unit(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin
         b->unit_form[(size_t)mi * o.nfreq + f] = c->factored;
         if (f == o.nfreq - 1 && g1.x)
             b->graph_units++;
-        // compute_port_values (src/emw_ana_antenna.cpp:76-105): I_port = sum l I sign over the feed
         if ((rc = frico_solution_get(c, reinterpret_cast<double*>(I.data()), N)))
             return rc;
-        cd ip = 0.0;
-        for (int64_t n = 0; n < N; n++) {
-            if (!bf[n].has_interface)
-                continue;
-            for (int32_t itf : m.feed)
-                if (bf[n].interface_tag == itf)
-                    ip += bf[n].length * I[n] * (applied ? signs[n] : 1.0);
-        }
-        const cd z = o.voltage / ip;
+        const cd z = port_zin(m, bf, signs, applied != 0, I.data(), N, o.voltage);
         zin[2 * ((size_t)mi * o.nfreq + f)] = z.real();
         zin[2 * ((size_t)mi * o.nfreq + f) + 1] = z.imag();
     }
     return 0;
+}
+
+// Buffers of a worker's lockstep units (system z of a unit at z times the per-system size).
+struct LockBufs {
+    DevBuf<c64> Z, LU, D;
+    DevBuf<c128> B, X, R;
+    DevBuf<double> norms;
+    DevBuf<unsigned long long> words;
+    DevBuf<int> info, perm;
+    DevBuf<unsigned char> split;
+};
+
+// FRICO_BATCH_LOCKSTEP=0 keeps one system per launch for the unpivoted form.
+bool lockstep_on()
+{
+    static const bool on = [] {
+        const char* e = std::getenv("FRICO_BATCH_LOCKSTEP");
+        return !(e && std::atoi(e) == 0);
+    }();
+    return on;
+}
+
+bool lockstep_fits(const frico_ctx* c, const RunOpts& o, int64_t N)
+{
+    return lockstep_on() && o.form == FRICO_SOLVE_C64_NP_IR && o.fill_prec == FRICO_C64 && o.nfreq >= 2 &&
+           o.nfreq <= 65535 && N >= 256 && N + 1 <= 4096 && c->gemm_f16;
+}
+
+// Row gathers for a static row order of the unpivoted factors: dst(i, j) = src(perm[i], j).
+__global__ __launch_bounds__(256) void gather_rows_c64(const c64* __restrict__ src, int64_t lds, c64* __restrict__ dst,
+                                                       int64_t ldd, int64_t rows, const int* __restrict__ perm)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i < rows)
+        dst[i + j * ldd] = src[perm[i] + j * lds];
+}
+// dst(i, j) = complex64(src(perm[i], j)), or without perm the conversion alone; column j of dst
+// at dst + j sd (systems as columns)
+__global__ __launch_bounds__(256) void gather_c128_c64(const c128* __restrict__ src, int64_t lds, c64* __restrict__ dst,
+                                                       int64_t sd, int64_t rows, const int* __restrict__ perm)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x, j = blockIdx.y;
+    if (i < rows) {
+        const c128 v = src[(perm ? perm[i] : i) + j * lds];
+        dst[i + j * sd] = c64{(float)v.re, (float)v.im};
+    }
+}
+
+struct UnitMesh {
+    const frico_batch::Mesh* m;
+    std::vector<double> signs;
+    int applied = 0;
+    std::vector<frico_rwg> bf;
+    int64_t N = 0;
+};
+
+// One lockstep pass over the frequencies sel of a loaded mesh: the systems are filled one after
+// another into the unit's buffers, factored together (lu_nopiv_batched, the right-hand side carried
+// as column N) and refined together (chained solves and matrix-vector residuals of all systems in
+// one launch each) until every system has met the tolerance or stalled.  perm (device, or null): a
+// static row order, the factors being those of P Z (the refinement's residual is of Z itself, and
+// each sweep's right-hand side is taken through P).  The frequencies that do not converge are
+// returned in failed.
+int lockstep_pass(frico_batch* b, frico_ctx* c, int mi, const UnitMesh& um, const RunOpts& o, double* zin,
+                  frico_solve_info* infos, LockBufs& lb, const std::vector<int>& sel, const int* perm,
+                  std::vector<int>& failed)
+{
+    const frico_batch::Mesh& m = *um.m;
+    const int64_t N = um.N;
+    const int nz = (int)sel.size();
+    const int64_t sa = N * (N + 1), sz = N * N;
+    const int64_t acap = (N + 127) / 128 * 128, bcap = (N + 1 + 127) / 128 * 128;
+    const size_t pa = (size_t)acap * 4 * HSPLIT_K, pb = (size_t)bcap * 4 * HSPLIT_K; // halves a system
+    const size_t abytes = (pa * 2 + (size_t)acap * sizeof(int)) * nz, bbytes = (pb * 2 + (size_t)bcap * sizeof(int)) * nz;
+    const size_t nw = lu_chain_words(N, 1);
+    int rc;
+    if ((rc = lb.Z.ensure((size_t)(sz * nz))) || (rc = lb.LU.ensure((size_t)(sa * nz))) ||
+        (rc = lb.D.ensure((size_t)(N * nz))) || (rc = lb.B.ensure((size_t)(N * nz))) ||
+        (rc = lb.X.ensure((size_t)(N * nz))) || (rc = lb.R.ensure((size_t)(N * nz))) ||
+        (rc = lb.norms.ensure((size_t)nz)) || (rc = lb.words.ensure(nw * nz)) || (rc = lb.info.ensure((size_t)nz)) ||
+        (rc = lb.split.ensure(4 * (abytes + bbytes))))
+        return rc;
+    BatchLu w;
+    {
+        unsigned char* p = lb.split.p;
+        for (int i = 0; i < 4; i++) {
+            w.ha[i] = reinterpret_cast<_Float16*>(p);
+            w.hea[i] = reinterpret_cast<int*>(p + pa * 2 * nz);
+            p += abytes;
+            w.hb[i] = reinterpret_cast<_Float16*>(p);
+            w.heb[i] = reinterpret_cast<int*>(p + pb * 2 * nz);
+            p += bbytes;
+        }
+        w.hacap = acap;
+        w.hbcap = bcap;
+        w.spa = (int64_t)pa;
+        w.sea = acap;
+        w.spb = (int64_t)pb;
+        w.seb = bcap;
+        w.info = lb.info.p;
+    }
+    hipStream_t st = c->st;
+    const unsigned gr = (unsigned)((N + 255) / 256);
+    for (int z = 0; z < nz; z++) {
+        const double freq = o.freqs[sel[z]];
+        if ((rc = frico_fill(c, freq, FRICO_FILL_QUAD, FRICO_C64)) ||
+            (rc = frico_rhs_deltagap(c, freq, m.feed.data(), (int)m.feed.size(), o.voltage.real(), o.voltage.imag(),
+                                     um.applied ? um.signs.data() : nullptr)))
+            return rc;
+        if ((rc = launch_copy_bytes(c->Z32.p, lb.Z.p + z * sz, (size_t)sz * sizeof(c64), st)) ||
+            (rc = launch_copy_bytes(c->B.p, lb.B.p + z * N, (size_t)N * sizeof(c128), st)))
+            return rc;
+        if (perm)
+            launch_kernel(&gather_rows_c64, dim3(gr, (unsigned)N), dim3(256), 0, st, (const c64*)c->Z32.p, N,
+                          lb.LU.p + z * sa, N, N, perm);
+        else if ((rc = launch_copy_bytes(c->Z32.p, lb.LU.p + z * sa, (size_t)sz * sizeof(c64), st)))
+            return rc;
+        launch_kernel(&gather_c128_c64, dim3(gr, 1), dim3(256), 0, st, (const c128*)c->B.p, N, lb.LU.p + z * sa + sz, N,
+                      N, perm);
+        FRICO_HIP_CHECK(hipGetLastError());
+    }
+    if ((rc = lu_nopiv_batched(w, lb.LU.p, sa, N, N, 1, nz, st)))
+        return rc;
+    std::vector<int> zp((size_t)nz);
+    std::vector<double> bn((size_t)nz), rn((size_t)nz), prev((size_t)nz, HUGE_VAL);
+    if ((rc = launch_col_norm2(lb.B.p, N, N, nz, lb.norms.p, st)))
+        return rc;
+    FRICO_HIP_CHECK(hipMemcpyAsync(rn.data(), lb.norms.p, (size_t)nz * sizeof(double), hipMemcpyDeviceToHost, st));
+    FRICO_HIP_CHECK(hipMemcpyAsync(zp.data(), lb.info.p, (size_t)nz * sizeof(int), hipMemcpyDeviceToHost, st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(st));
+    for (int z = 0; z < nz; z++)
+        bn[z] = std::sqrt(rn[z]);
+    // 0: refining, 1: converged, 2: not converging
+    std::vector<int> state((size_t)nz, 0);
+    for (int z = 0; z < nz; z++)
+        if (zp[z] > 0)
+            state[z] = 2;
+    const int64_t N16 = N * (int64_t)sizeof(c128);
+    for (int it = 1; it <= o.maxit; it++) {
+        if (it == 1) {
+            // the carried column (L^{-1} P B): the first sweep is the backward half
+            for (int z = 0; z < nz; z++)
+                if ((rc = launch_copy_bytes(lb.LU.p + z * sa + sz, lb.D.p + z * N, (size_t)N * sizeof(c64), st)))
+                    return rc;
+            if ((rc = lu_chain_solve_batched(lb.LU.p, sa, N, N, lb.D.p, N, N, 1, nz, lb.words.p, false, st)) ||
+                (rc = launch_zero_bytes(lb.X.p, (size_t)(N16 * nz), st)))
+                return rc;
+        } else {
+            launch_kernel(&gather_c128_c64, dim3(gr, (unsigned)nz), dim3(256), 0, st, (const c128*)lb.R.p, N, lb.D.p, N, N,
+                          perm);
+            FRICO_HIP_CHECK(hipGetLastError());
+            if ((rc = lu_chain_solve_batched(lb.LU.p, sa, N, N, lb.D.p, N, N, 1, nz, lb.words.p, true, st)))
+                return rc;
+        }
+        if ((rc = launch_add_correction(lb.X.p, N, lb.D.p, N, N, nz, st)) ||
+            (rc = launch_copy_bytes(lb.B.p, lb.R.p, (size_t)(N16 * nz), st)) ||
+            (rc = gemv_sub_mixed_batched(N, 1, N, lb.Z.p, N, sz, lb.X.p, N, N, lb.R.p, N, N, nz, st)) ||
+            (rc = launch_col_norm2(lb.R.p, N, N, nz, lb.norms.p, st)))
+            return rc;
+        FRICO_HIP_CHECK(hipMemcpyAsync(rn.data(), lb.norms.p, (size_t)nz * sizeof(double), hipMemcpyDeviceToHost, st));
+        FRICO_HIP_CHECK(hipStreamSynchronize(st));
+        bool active = false;
+        for (int z = 0; z < nz; z++) {
+            if (state[z] != 0)
+                continue;
+            const double r = std::sqrt(rn[z]), rel = bn[z] > 0 ? r / bn[z] : r;
+            frico_solve_info& inf = infos[(size_t)mi * o.nfreq + sel[z]];
+            inf.iterations = it;
+            inf.max_rel_residual = std::isfinite(rel) ? rel : HUGE_VAL;
+            if (rel <= o.tol) {
+                state[z] = 1;
+                inf.converged = 1;
+            } else if (!(rel < 0.5 * prev[z])) {
+                state[z] = 2; // stalled: factors that grew without interchanges
+            } else {
+                prev[z] = rel;
+                active = true;
+            }
+        }
+        if (!active)
+            break;
+    }
+    std::vector<cd> X((size_t)(N * nz));
+    FRICO_HIP_CHECK(hipMemcpyAsync(X.data(), lb.X.p, (size_t)(N16 * nz), hipMemcpyDeviceToHost, st));
+    FRICO_HIP_CHECK(hipStreamSynchronize(st));
+    failed.clear();
+    for (int z = 0; z < nz; z++) {
+        const int f = sel[z];
+        if (state[z] == 1) {
+            const cd zz = port_zin(m, um.bf, um.signs, um.applied != 0, X.data() + z * N, N, o.voltage);
+            zin[2 * ((size_t)mi * o.nfreq + f)] = zz.real();
+            zin[2 * ((size_t)mi * o.nfreq + f) + 1] = zz.imag();
+            b->unit_form[(size_t)mi * o.nfreq + f] = FRICO_SOLVE_C64_NP_IR;
+        } else {
+            infos[(size_t)mi * o.nfreq + f] = frico_solve_info{0, 0, -1.0};
+            failed.push_back(f);
+        }
+    }
+    return 0;
+}
+
+// One mesh at all of its frequencies in lockstep (lockstep_pass).  Frequencies whose unpivoted
+// factors fail (a zero pivot, or a refinement that stalls: factors that grow without interchanges)
+// are solved once more in lockstep with the row order of the pivoted LU of the first of them (the
+// partial-pivoting order of one frequency keeps the factors of its neighbours bounded as well); what
+// still fails goes to the pivoted LU one system at a time (run_unit).
+int run_unit_lockstep(frico_batch* b, frico_ctx* c, int mi, const RunOpts& o, double* zin, frico_solve_info* infos,
+                      LockBufs& lb)
+{
+    const frico_batch::Mesh& m = b->meshes[mi];
+    int rc = frico_mesh_load(c, m.V.data(), (int64_t)(m.V.size() / 3), m.T.data(), m.tags.data(),
+                             (int64_t)m.tags.size(), m.iface.empty() ? nullptr : m.iface.data(),
+                             m.itags.empty() ? nullptr : m.itags.data(), (int64_t)m.itags.size(), m.degree);
+    if (rc)
+        return rc;
+    UnitMesh um;
+    um.m = &m;
+    um.N = c->N;
+    const int64_t N = um.N;
+    if (!lockstep_fits(c, o, N))
+        return run_unit(b, c, mi, o, zin, infos);
+    um.signs.assign((size_t)N, 0.0);
+    if ((rc = frico_reorient_deltagap(c, m.feed.data(), (int)m.feed.size(), 0, um.signs.data(), &um.applied)))
+        return rc;
+    um.bf.resize((size_t)N);
+    if ((rc = frico_rwg_table(c, um.bf.data())))
+        return rc;
+    b->mesh_nb[mi] = 128;
+    std::vector<int> all((size_t)o.nfreq), failed;
+    for (int f = 0; f < o.nfreq; f++)
+        all[f] = f;
+    if ((rc = lockstep_pass(b, c, mi, um, o, zin, infos, lb, all, nullptr, failed)))
+        return rc;
+    if (failed.empty())
+        return 0;
+    // the pivoted LU of the first failed frequency, and its row order for the others
+    const int f0 = failed[0];
+    std::vector<char> one((size_t)o.nfreq, 0);
+    one[f0] = 1;
+    RunOpts op = o;
+    op.form = FRICO_SOLVE_C64_IR;
+    op.graphs = false;
+    if ((rc = run_unit(b, c, mi, op, zin, infos, &one)))
+        return rc;
+    std::vector<int> rest(failed.begin() + 1, failed.end()), still;
+    if (!rest.empty() && c->factored == FRICO_SOLVE_C64_IR) {
+        std::vector<int> ipiv((size_t)N), perm((size_t)N);
+        FRICO_HIP_CHECK(hipMemcpy(ipiv.data(), c->ipiv.p, (size_t)N * sizeof(int), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < N; i++)
+            perm[i] = (int)i;
+        for (int64_t i = 0; i < N; i++) {
+            const int p = ipiv[i];
+            if (p < 0 || p >= N)
+                return set_error("batch: pivot %d of row %lld outside 0..%lld", p, (long long)i, (long long)N - 1), -1;
+            std::swap(perm[i], perm[p]);
+        }
+        if ((rc = lb.perm.upload(perm.data(), (size_t)N, c->st)) ||
+            (rc = lockstep_pass(b, c, mi, um, o, zin, infos, lb, rest, lb.perm.p, still)))
+            return rc;
+        rest.swap(still);
+    }
+    if (rest.empty())
+        return 0;
+    std::vector<char> mask((size_t)o.nfreq, 0);
+    for (int f : rest)
+        mask[f] = 1;
+    return run_unit(b, c, mi, op, zin, infos, &mask);
 }
 
 } // namespace
@@ -444,6 +723,7 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
     std::string first_msg;
     auto worker = [&](int w) {
         frico_ctx* c = b->ctx[w];
+        LockBufs lb;
         if (hipSetDevice(b->device) != hipSuccess) {
             failed = 1;
             return;
@@ -454,7 +734,9 @@ int frico_batch_antenna(frico_batch* b, const double* freqs, int nfreq, int fill
             const int k = next.fetch_add(1);
             if (k >= nm)
                 return;
-            const int rc = run_unit(b, c, order[k], o, zin, info);
+            const int rc = lockstep_on() && o.form == FRICO_SOLVE_C64_NP_IR
+                               ? run_unit_lockstep(b, c, order[k], o, zin, info, lb)
+                               : run_unit(b, c, order[k], o, zin, info);
             if (rc) {
                 std::lock_guard<std::mutex> lock(emu);
                 if (!first_rc) {

@@ -203,8 +203,12 @@ constexpr int TS_STRIP = 16; // columns of M a lane holds at a time
 template <bool TRANS, int NC>
 __global__ __launch_bounds__(256) void tri_solve_split(c64* __restrict__ A, int64_t lda, int64_t k, int w, int64_t c0,
                                                        int64_t nc, _Float16* __restrict__ P, int* __restrict__ E,
-                                                       int64_t cap)
+                                                       int64_t cap, Batch bt)
 {
+    // batched: A += z s0, P += z s1, E += z s2
+    A += blockIdx.z * bt.s0;
+    P += blockIdx.z * bt.s1;
+    E += blockIdx.z * bt.s2;
     constexpr int COLS = 4 * NC;
     __shared__ c64 sT[TRANS ? COLS : 1][BSW + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -370,8 +374,12 @@ __device__ inline int tri_off(int c) { return (BSW - 1) * c - c * (c - 1) / 2; }
 template <bool TRANS, int NC>
 __global__ __launch_bounds__(256) void tri_solve_split_lds(c64* __restrict__ A, int64_t lda, int64_t k, int w,
                                                            int64_t c0, int64_t nc, _Float16* __restrict__ P,
-                                                           int* __restrict__ E, int64_t cap)
+                                                           int* __restrict__ E, int64_t cap, Batch bt)
 {
+    // batched: A += z s0, P += z s1, E += z s2
+    A += blockIdx.z * bt.s0;
+    P += blockIdx.z * bt.s1;
+    E += blockIdx.z * bt.s2;
     constexpr int COLS = 4 * NC;
     constexpr int PER = (TS_TRI + 255) / 256; // 32
     constexpr int PF = 4;
@@ -428,13 +436,13 @@ __global__ __launch_bounds__(256) void tri_solve_split_lds(c64* __restrict__ A, 
             entry(e, r, c, a, b);
         v[t] = (e < TS_TRI && r < w) ? D[a + (int64_t)b * lda] : c64{0.0f, 0.0f};
     }
-    c64 bt[TRANS ? COLS / 2 : 1];
+    c64 bcol[TRANS ? COLS / 2 : 1];
     if constexpr (TRANS) {
 #pragma unroll
         for (int t = 0; t < COLS / 2; t++) {
             const int e = tid + 256 * t, jj = e % COLS, kk = e / COLS;
             const int64_t j = j0 + jj;
-            bt[t] = (j < nc && kk < w) ? A[(k + w + c0 + j) + (k + kk) * lda] : c64{0.0f, 0.0f};
+            bcol[t] = (j < nc && kk < w) ? A[(k + w + c0 + j) + (k + kk) * lda] : c64{0.0f, 0.0f};
         }
     }
     __syncthreads(); // sR
@@ -451,7 +459,7 @@ __global__ __launch_bounds__(256) void tri_solve_split_lds(c64* __restrict__ A, 
 #pragma unroll
         for (int t = 0; t < COLS / 2; t++) {
             const int e = tid + 256 * t;
-            sT[e % COLS][e / COLS] = bt[t];
+            sT[e % COLS][e / COLS] = bcol[t];
         }
     }
     __syncthreads();
@@ -557,18 +565,18 @@ __global__ __launch_bounds__(256) void tri_solve_split_lds(c64* __restrict__ A, 
 
 template <bool TRANS, int NC>
 static void launch_tss_lds(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, int64_t Rp, _Float16* planes,
-                           int* e, int64_t cap, hipStream_t st)
+                           int* e, int64_t cap, hipStream_t st, const Batch& bt, int nz)
 {
-    launch_kernel(&tri_solve_split_lds<TRANS, NC>, dim3((unsigned)(Rp / (4 * NC))), dim3(256), 0, st, A, lda, k, w, c0,
-                  nc, planes, e, cap);
+    launch_kernel(&tri_solve_split_lds<TRANS, NC>, dim3((unsigned)(Rp / (4 * NC)), 1, (unsigned)nz), dim3(256), 0, st, A,
+                  lda, k, w, c0, nc, planes, e, cap, bt);
 }
 
 template <bool TRANS, int NC>
 static void launch_tss(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, int64_t Rp, _Float16* planes,
-                       int* e, int64_t cap, hipStream_t st)
+                       int* e, int64_t cap, hipStream_t st, const Batch& bt, int nz)
 {
-    launch_kernel(&tri_solve_split<TRANS, NC>, dim3((unsigned)(Rp / (4 * NC))), dim3(256), 0, st, A, lda, k, w, c0, nc,
-                  planes, e, cap);
+    launch_kernel(&tri_solve_split<TRANS, NC>, dim3((unsigned)(Rp / (4 * NC)), 1, (unsigned)nz), dim3(256), 0, st, A, lda,
+                  k, w, c0, nc, planes, e, cap, bt);
 }
 
 // FRICO_TSS_LDS: 0 never stages M in LDS, 1 (default) as chosen below, 2 always
@@ -584,8 +592,10 @@ static bool tss_lds(bool trans, int64_t nc)
 }
 
 int launch_tri_solve_split(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, bool trans, _Float16* planes,
-                           int* e, int64_t cap, hipStream_t st)
+                           int* e, int64_t cap, hipStream_t st, const Batch& bt, int nz)
 {
+    if (nz < 1 || nz > 65535)
+        return set_error("block solve: batch of %d systems", nz), -1;
     if (nc <= 0)
         return 0;
     if (w < 1 || w > BSW)
@@ -600,34 +610,34 @@ int launch_tri_solve_split(c64* A, int64_t lda, int64_t k, int w, int64_t c0, in
     if (tss_lds(trans, nc)) {
         if (trans) {
             if (ncw == 1)
-                launch_tss_lds<true, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+                launch_tss_lds<true, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
             else
-                launch_tss_lds<true, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+                launch_tss_lds<true, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
         } else {
             if (ncw == 1)
-                launch_tss_lds<false, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+                launch_tss_lds<false, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
             else if (ncw == 2)
-                launch_tss_lds<false, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+                launch_tss_lds<false, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
             else
-                launch_tss_lds<false, 4>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+                launch_tss_lds<false, 4>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
         }
         FRICO_HIP_CHECK(hipGetLastError());
         return 0;
     }
     if (trans) {
         if (ncw == 1)
-            launch_tss<true, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+            launch_tss<true, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
         else if (ncw == 2)
-            launch_tss<true, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+            launch_tss<true, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
         else
-            launch_tss<true, 4>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+            launch_tss<true, 4>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
     } else {
         if (ncw == 1)
-            launch_tss<false, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+            launch_tss<false, 1>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
         else if (ncw == 2)
-            launch_tss<false, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+            launch_tss<false, 2>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
         else
-            launch_tss<false, 4>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st);
+            launch_tss<false, 4>(A, lda, k, w, c0, nc, Rp, planes, e, cap, st, bt, nz);
     }
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;

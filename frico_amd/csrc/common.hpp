@@ -64,6 +64,13 @@ template <typename T> __host__ __device__ inline T cabs1(cx<T> a)
     } while (0)
 
 namespace frico {
+// Element strides between the systems of a batched launch, the system being blockIdx.z: a kernel
+// that takes a Batch offsets its pointers by z times the stride it assigns to each (all zero for
+// one system).  Each such kernel states which pointer takes which stride.
+struct Batch {
+    int64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0;
+};
+
 int hip_fail(hipError_t e, const char* what, const char* file, int line);
 void set_error(const char* fmt, ...);
 // Raises a kernel's dynamic-LDS limit to `bytes` (gfx950 CUs have 160 KB, shared with static LDS).

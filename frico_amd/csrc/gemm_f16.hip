@@ -137,8 +137,24 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
                                                          const int* __restrict__ ea, int64_t acap,
                                                          const _Float16* __restrict__ Ub, const int* __restrict__ eb,
                                                          int64_t bcap, c64* __restrict__ C, int64_t ldc, int64_t gx,
-                                                         int64_t gy, F16Half h1 = F16Half{})
+                                                         int64_t gy, F16Half h1, Batch bt)
 {
+    // batched: the L planes (both halves) += z s0, their exponents += z s1, the U planes += z s2,
+    // their exponents += z s3, C += z s4
+    {
+        const int64_t z = blockIdx.z;
+        La += z * bt.s0;
+        ea += z * bt.s1;
+        Ub += z * bt.s2;
+        eb += z * bt.s3;
+        C += z * bt.s4;
+        if constexpr (TWO) {
+            h1.La += z * bt.s0;
+            h1.ea += z * bt.s1;
+            h1.Ub += z * bt.s2;
+            h1.eb += z * bt.s3;
+        }
+    }
     constexpr int BM = 32 * RB, BN = 32 * CB;
     constexpr bool PREC = RB * CB <= 4;
     __shared__ half8 sA[4][BM][4]; // [plane][row][segment]
@@ -600,8 +616,11 @@ static int f16_tile()
 }
 
 int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
-                 const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt)
+                 const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt, const Batch& bt,
+                 int nz)
 {
+    if (nz < 1 || nz > 65535)
+        return set_error("fp16 product: batch of %d systems", nz), -1;
     if (M <= 0 || N <= 0 || K <= 0)
         return 0;
     if (K > HSPLIT_K)
@@ -621,12 +640,13 @@ int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea,
     const int kc = (K + HKC - 1) / HKC * HKC;
     if (kt)
         kt->begin(st);
+    const dim3 grid((unsigned)nblk, 1, (unsigned)nz);
     if (tile == 128)
-        launch_kernel(&cgemm_sub_f16<4, 4, 2>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap,
-                      C, ldc, gx, gy, F16Half{});
+        launch_kernel(&cgemm_sub_f16<4, 4, 2>, grid, dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap, C, ldc, gx, gy,
+                      F16Half{}, bt);
     else
-        launch_kernel(&cgemm_sub_f16<4, 2, 3>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap,
-                      C, ldc, gx, gy, F16Half{});
+        launch_kernel(&cgemm_sub_f16<4, 2, 3>, grid, dim3(256), 0, st, M, N, kc, La, ea, acap, Ub, eb, bcap, C, ldc, gx, gy,
+                      F16Half{}, bt);
     // algorithmic bytes: C read and written once (complex64), L and U read once as complex64
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * (double)K,
@@ -637,10 +657,13 @@ int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea,
 
 int gemm_sub_f16_pair(int64_t M, int64_t N, int K0, const _Float16* La0, const int* ea0, const _Float16* Ub0,
                       const int* eb0, int K1, const _Float16* La1, const int* ea1, const _Float16* Ub1, const int* eb1,
-                      int64_t acap, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt)
+                      int64_t acap, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt,
+                      const Batch& bt, int nz)
 {
     if (M <= 0 || N <= 0)
         return 0;
+    if (nz < 1 || nz > 65535)
+        return set_error("fp16 product: batch of %d systems", nz), -1;
     if (K0 <= 0 || K0 > HSPLIT_K || K1 <= 0 || K1 > HSPLIT_K)
         return set_error("fp16 product: halves of K = %d + %d outside 1..%d", K0, K1, HSPLIT_K), -1;
     if (K0 % HKC != 0)
@@ -658,12 +681,13 @@ int gemm_sub_f16_pair(int64_t M, int64_t N, int K0, const _Float16* La0, const i
     const F16Half h1{La1, ea1, Ub1, eb1, (K1 + HKC - 1) / HKC * HKC};
     if (kt)
         kt->begin(st);
+    const dim3 grid((unsigned)nblk, 1, (unsigned)nz);
     if (tile == 128)
-        launch_kernel(&cgemm_sub_f16<4, 4, 2, true>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, K0, La0, ea0, acap,
-                      Ub0, eb0, bcap, C, ldc, gx, gy, h1);
+        launch_kernel(&cgemm_sub_f16<4, 4, 2, true>, grid, dim3(256), 0, st, M, N, K0, La0, ea0, acap, Ub0, eb0, bcap, C, ldc,
+                      gx, gy, h1, bt);
     else
-        launch_kernel(&cgemm_sub_f16<4, 2, 3, true>, dim3((unsigned)nblk), dim3(256), 0, st, M, N, K0, La0, ea0, acap,
-                      Ub0, eb0, bcap, C, ldc, gx, gy, h1);
+        launch_kernel(&cgemm_sub_f16<4, 2, 3, true>, grid, dim3(256), 0, st, M, N, K0, La0, ea0, acap, Ub0, eb0, bcap, C, ldc,
+                      gx, gy, h1, bt);
     const double K = (double)K0 + (double)K1;
     if (kt)
         kt->end(st, 8.0 * (double)M * (double)N * K, 16.0 * (double)M * (double)N + 8.0 * ((double)M + (double)N) * K);

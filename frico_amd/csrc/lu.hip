@@ -1398,8 +1398,12 @@ constexpr int GEMV_WAVES = 16;
 template <typename TA, typename T>
 __global__ __launch_bounds__(64 * GEMV_WAVES) void gemv_sub(int64_t M, int64_t K, int ncol, const cx<TA>* __restrict__ A,
                                                            int64_t lda, const cx<T>* __restrict__ B, int64_t ldb,
-                                                           cx<T>* __restrict__ C, int64_t ldc)
+                                                           cx<T>* __restrict__ C, int64_t ldc, Batch bt)
 {
+    // batched: A += z s0, B += z s1, C += z s2
+    A += blockIdx.z * bt.s0;
+    B += blockIdx.z * bt.s1;
+    C += blockIdx.z * bt.s2;
     __shared__ cx<T> sp[GEMV_WAVES][64][GEMV_MAXN];
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -1455,14 +1459,29 @@ __global__ __launch_bounds__(64 * GEMV_WAVES) void gemv_sub(int64_t M, int64_t K
 
 template <typename TA, typename T>
 static int launch_gemv_sub(int64_t M, int64_t N, int64_t K, const cx<TA>* A, int64_t lda, const cx<T>* B, int64_t ldb,
-                           cx<T>* C, int64_t ldc, hipStream_t st)
+                           cx<T>* C, int64_t ldc, hipStream_t st, const Batch& bt = Batch{}, int nz = 1)
 {
     const int64_t g = (M + 63) / 64;
-    if (g > INT32_MAX)
-        return set_error("gemv: too many rows"), -1;
-    launch_kernel(&gemv_sub<TA, T>, dim3((unsigned)g), dim3(64 * GEMV_WAVES), 0, st, M, K, (int)N, A, lda, B, ldb, C, ldc);
+    if (g > INT32_MAX || nz < 1 || nz > 65535)
+        return set_error("gemv: %lld row blocks, %d systems", (long long)g, nz), -1;
+    launch_kernel(&gemv_sub<TA, T>, dim3((unsigned)g, 1, (unsigned)nz), dim3(64 * GEMV_WAVES), 0, st, M, K, (int)N, A, lda,
+                  B, ldb, C, ldc, bt);
     FRICO_HIP_CHECK(hipGetLastError());
     return 0;
+}
+
+int gemv_sub_mixed_batched(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, int64_t sa, const c128* B,
+                           int64_t ldb, int64_t sb, c128* C, int64_t ldc, int64_t sc, int nz, hipStream_t st)
+{
+    if (M <= 0 || N <= 0 || K <= 0)
+        return 0;
+    if (N > GEMV_MAXN)
+        return set_error("gemv: %lld columns", (long long)N), -1;
+    Batch bt;
+    bt.s0 = sa;
+    bt.s1 = sb;
+    bt.s2 = sc;
+    return launch_gemv_sub<float, double>(M, N, K, A, lda, B, ldb, C, ldc, st, bt, nz);
 }
 
 template <typename T>
@@ -1833,7 +1852,7 @@ static int solve_tables(LuWork& wk, const cx<T>* A, int64_t n, int64_t lda, int 
 // pivoted LU when the refinement stalls).
 // ----------------------------------------------------------------------------------------------
 template <typename T, bool SYM>
-__global__ void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info);
+__global__ void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info, Batch bt);
 // block-row columns per transpose workgroup
 template <typename T> constexpr int kSymCols = sizeof(T) == 4 ? 32 : 16;
 
@@ -1930,7 +1949,7 @@ int ldlt_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb,
     // diagonal block, block row and block column of step k
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
-        launch_kernel(&lu_diag<T, true>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
+        launch_kernel(&lu_diag<T, true>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1, Batch{});
         if (m > 0) {
             const dim3 g((unsigned)((m + kSymCols<T> - 1) / kSymCols<T>));
             launch_kernel(&sym_transpose<T, false>, g, dim3(256), 0, st, A, lda, k, w, m);
@@ -2059,8 +2078,11 @@ __device__ inline cx<float> pivot_inverse(cx<float> d)
 __device__ inline cx<double> pivot_inverse(cx<double> d) { return cdiv(cx<double>{1, 0}, d); }
 
 template <typename T, bool SYM = false>
-__global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info)
+__global__ __launch_bounds__(1024) void lu_diag(cx<T>* __restrict__ A, int64_t lda, int64_t k, int w, int* info, Batch bt)
 {
+    // batched: A += z s0, info += z s1
+    A += blockIdx.z * bt.s0;
+    info += blockIdx.z * bt.s1;
     constexpr int SB = LUD_SUB, CR = LUD_CHAIN;
     __shared__ __align__(16) cx<T> Pc[2][SB][LU_NB_MAX];      // panel column: Pc[b][t][r - s0], r >= s0
     __shared__ __align__(16) cx<T> Pr[2][SB][LU_NB_MAX - SB]; // panel row: Pr[b][i][c - e], c >= e
@@ -2347,7 +2369,7 @@ static int lu_nopiv_pairs(LuWork& wk, c64* A, int64_t n, int64_t lda, int nb, hi
     // diagonal block and L21 of the step at k, L21 split into ha[ai]
     auto col_step = [&](int64_t k, int w, int ai) -> int {
         const int64_t m = n - k - w;
-        launch_kernel(&lu_diag<float>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
+        launch_kernel(&lu_diag<float>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1, Batch{});
         if (m <= 0)
             return 0;
         if (m <= fused_max)
@@ -2460,6 +2482,107 @@ static int lu_nopiv_pairs(LuWork& wk, c64* A, int64_t n, int64_t lda, int nb, hi
     return 0;
 }
 
+// ----------------------------------------------------------------------------------------------
+// Lockstep batch of the paired-step factorisation: nz systems of one size, each launch of
+// lu_nopiv_pairs' sequence (without the look-ahead stream) taking all of them through blockIdx.z,
+// so that a diagonal block, a block solve or a product is one launch for the batch instead of one
+// per system.  System z: A + z sa; its split operands at w.ha[i] + z w.spa (planes) and
+// w.hea[i] + z w.sea (exponents), the same for hb; its first zero pivot in w.info[z].  Blocks are
+// solved by tri_solve_split only, so n + naug <= the fused width (4096).
+// ----------------------------------------------------------------------------------------------
+int lu_nopiv_batched(const BatchLu& w, c64* A, int64_t sa, int64_t n, int64_t lda, int64_t naug, int nz,
+                     hipStream_t st)
+{
+    constexpr int nb = 128;
+    if (nz < 1 || nz > 65535)
+        return set_error("batched lu: %d systems", nz), -1;
+    if (n < 1 || n + naug > fused_step_max_cols())
+        return set_error("batched lu: %lld + %lld columns outside 1..%lld", (long long)n, (long long)naug,
+                         (long long)fused_step_max_cols()), -1;
+    if ((n + 127) / 128 * 128 > w.hacap || (n + naug + 127) / 128 * 128 > w.hbcap)
+        return set_error("batched lu: split buffers too small"), -1;
+    const int64_t npan = (n + nb - 1) / nb;
+    launch_kernel(&zero_u32, dim3((unsigned)((nz + 255) / 256)), dim3(256), 0, st, (unsigned*)w.info, (int64_t)nz);
+    Batch bd;
+    bd.s0 = sa;
+    bd.s1 = 1;
+    Batch ba, bb, bp;
+    ba.s0 = sa;
+    ba.s1 = w.spa;
+    ba.s2 = w.sea;
+    bb.s0 = sa;
+    bb.s1 = w.spb;
+    bb.s2 = w.seb;
+    bp.s0 = w.spa;
+    bp.s1 = w.sea;
+    bp.s2 = w.spb;
+    bp.s3 = w.seb;
+    bp.s4 = sa;
+    auto width = [&](int64_t k) { return (int)((n - k) < nb ? (n - k) : nb); };
+    auto col_step = [&](int64_t k, int wd, int ai) -> int {
+        const int64_t m = n - k - wd;
+        launch_kernel(&lu_diag<float>, dim3(1, 1, (unsigned)nz), dim3(1024), 0, st, A, lda, k, wd, w.info, bd);
+        if (m <= 0)
+            return 0;
+        return launch_tri_solve_split(A, lda, k, wd, 0, m, true, w.ha[ai], w.hea[ai], w.hacap, st, ba, nz);
+    };
+    auto usolve = [&](int64_t k, int wd, int64_t c0, int64_t nc, int bi) -> int {
+        if (nc <= 0)
+            return 0;
+        return launch_tri_solve_split(A, lda, k, wd, c0, nc, false, w.hb[bi], w.heb[bi], w.hbcap, st, bb, nz);
+    };
+    auto prod1 = [&](int64_t M, int64_t nc, int wd, int ai, int64_t off, int bi, int64_t r0, int64_t c0) -> int {
+        if (M <= 0 || nc <= 0)
+            return 0;
+        return gemm_sub_f16(M, nc, wd, w.ha[ai] + off * HSPLIT_K, w.hea[ai] + off, w.hacap, w.hb[bi], w.heb[bi], w.hbcap,
+                            A + r0 + c0 * lda, lda, st, nullptr, bp, nz);
+    };
+    // the pair's product on rows [r0, n): L21(a) from its row off_a (L21(a) starts at block b's first
+    // row), L21(b) from its first
+    auto prod2 = [&](int64_t nc, int wa, int ia, int64_t off_a, int b0, int wb, int ib, int b1, int64_t r0,
+                     int64_t c0) -> int {
+        const int64_t M = n - r0;
+        if (M <= 0 || nc <= 0)
+            return 0;
+        return gemm_sub_f16_pair(M, nc, wa, w.ha[ia] + off_a * HSPLIT_K, w.hea[ia] + off_a, w.hb[b0], w.heb[b0], wb,
+                                 w.ha[ib], w.hea[ib], w.hb[b1], w.heb[b1], w.hacap, w.hbcap, A + r0 + c0 * lda, lda, st,
+                                 nullptr, bp, nz);
+    };
+    int rc;
+    if ((rc = col_step(0, width(0), 0)))
+        return rc;
+    for (int64_t p = 0; p < npan; p += 2) {
+        const int64_t ka = p * nb;
+        const int wa = width(ka), ia = (int)(p % 4);
+        if (p + 1 >= npan) {
+            if (naug > 0 && (rc = usolve(ka, wa, n, naug, 0)))
+                return rc;
+            break;
+        }
+        const int64_t kb = ka + nb;
+        const int wb = width(kb), ib = (int)((p + 1) % 4);
+        if ((rc = usolve(ka, wa, kb, wb, 0)) || (rc = prod1(n - kb, wb, wa, ia, 0, 0, kb, kb)) ||
+            (rc = col_step(kb, wb, ib)))
+            return rc;
+        const int64_t c2 = kb + wb;
+        const int64_t nnext = (n - c2) < 2 * nb ? (n - c2) : 2 * nb;
+        const int64_t cw = c2 + nnext, nw = n + naug - cw;
+        if (nw > 0 && ((rc = usolve(ka, wa, cw, nw, 2)) || (rc = prod1(wb, nw, wa, ia, 0, 2, kb, cw)) ||
+                       (rc = usolve(kb, wb, cw, nw, 3))))
+            return rc;
+        if (nnext > 0 && ((rc = usolve(ka, wa, c2, nnext, 0)) || (rc = prod1(wb, nnext, wa, ia, 0, 0, kb, c2)) ||
+                          (rc = usolve(kb, wb, c2, nnext, 1)) ||
+                          (rc = prod2(nnext, wa, ia, wb, 0, wb, ib, 1, c2, c2))))
+            return rc;
+        if (nw > 0 && (rc = prod2(nw, wa, ia, wb, 2, wb, ib, 3, c2, cw)))
+            return rc;
+        if (c2 < n && (rc = col_step(c2, width(c2), (int)((p + 2) % 4))))
+            return rc;
+    }
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 template <typename T>
 int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int nb, hipStream_t st,
                     KernelTimer* gemm_timer, int64_t naug)
@@ -2492,7 +2615,7 @@ int lu_nopiv_factor(LuWork& wk, cx<T>* A, int64_t n, int64_t lda, int* ipiv, int
     auto fused = [&](int64_t cols) { return fused_any && cols <= fused_max; };
     auto column = [&](int64_t k, int w) -> int {
         const int64_t m = n - k - w;
-        launch_kernel(&lu_diag<T>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1);
+        launch_kernel(&lu_diag<T>, dim3(1), dim3(1024), 0, st, A, lda, k, w, wk.flags + 1, Batch{});
         if constexpr (c64f) {
             if (fused(m)) {
                 // L21 = A21 U11^{-1} and its fp16 split in one launch
@@ -3064,13 +3187,20 @@ size_t lu_chain_words(int64_t n, int64_t nrhs) { return 1 + (size_t)((n + 63) / 
 constexpr int CHAIN_WIN = 4; // dependency chunks taken per wait
 
 template <bool UPPER>
-__global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, int64_t lda, int64_t n, c64* B,
-                                                   int64_t ldb, int nr, unsigned long long* words, unsigned seq)
+__global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A0, int64_t lda, int64_t n, c64* B0,
+                                                   int64_t ldb, int nr, unsigned long long* words0, unsigned seq,
+                                                   Batch bt, int nz)
 {
     __shared__ c64 sd[64][65];                         // sd[c][r] = D(r, c), the chunk's diagonal block
     __shared__ c64 sx[2][CHAIN_WIN][64][CHAIN_MAXR];   // x of the chunks of a step (two steps)
     __shared__ c64 sp[4][64][CHAIN_MAXR];              // the four waves' partial sums
     __shared__ int s_ok, s_nav[2];
+    // batched: the workgroup solves its chunk of every system in turn, system z at A0 + z s0,
+    // B0 + z s1 with its exchange area at words0 + z s2
+    for (int zs = 0; zs < nz; zs++) {
+    const c64* __restrict__ A = A0 + zs * bt.s0;
+    c64* B = B0 + zs * bt.s1;
+    unsigned long long* words = words0 + zs * bt.s2;
     int* abort_flag = reinterpret_cast<int*>(words);
     unsigned long long* xw = words + 1;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
@@ -3202,6 +3332,8 @@ __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A, in
             ll_put(dst + 1, seq, __float_as_uint(lane < wj ? x.im : 0.0f));
         }
     }
+    __syncthreads(); // LDS reused by the next system
+    }
 }
 
 // FRICO_SOLVE_CHAIN=0 keeps the blocked solves for a few right-hand sides too.
@@ -3222,20 +3354,40 @@ static bool use_chain(const ChainSolve* cs, int64_t n, int64_t nrhs)
 
 // Both halves (forward = true) or the backward half of a solve in place by the chained kernels; with
 // a residual stream, X += D and the residual product follow as one block on the second stream.
+// Both halves (forward) or the backward half of nz systems' solves by the chained kernels, system z
+// at A + z sa, B + z sb, its exchange area at words + z lu_chain_words(n, nrhs).
+int lu_chain_solve_batched(const c64* A, int64_t sa, int64_t n, int64_t lda, c64* B, int64_t sb, int64_t ldb,
+                           int64_t nrhs, int nz, unsigned long long* words, bool forward, hipStream_t st)
+{
+    const int64_t nch = (n + 63) / 64;
+    if (nch > INT32_MAX || nz < 1 || nz > 65535)
+        return set_error("chained solve: %lld chunks, %d systems", (long long)nch, nz), -1;
+    if (nrhs < 1 || nrhs > CHAIN_MAXR)
+        return set_error("chained solve: %lld right-hand sides", (long long)nrhs), -1;
+    const int64_t nw = (int64_t)lu_chain_words(n, nrhs);
+    int rc = launch_zero_bytes(words, (size_t)(nw * nz) * sizeof(unsigned long long), st);
+    if (rc)
+        return rc;
+    Batch bt;
+    bt.s0 = sa;
+    bt.s1 = sb;
+    bt.s2 = nw;
+    // one workgroup a chunk for all systems (each in turn): nch workgroups whatever nz, so that the
+    // batch's waits need no more resident workgroups than one system's
+    const dim3 grid((unsigned)nch);
+    if (forward)
+        launch_kernel(&chain_solve<false>, grid, dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, words, 1u, bt, nz);
+    launch_kernel(&chain_solve<true>, grid, dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, words, 2u, bt, nz);
+    FRICO_HIP_CHECK(hipGetLastError());
+    return 0;
+}
+
 static int lu_chain_solve(const c64* A, int64_t n, int64_t lda, c64* B, int64_t ldb, int64_t nrhs, hipStream_t st,
                           const ChainSolve* cs, bool forward, const ResidStream* rs)
 {
-    const int64_t nch = (n + 63) / 64;
-    if (nch > INT32_MAX)
-        return set_error("chained solve: too many chunks"), -1;
-    int rc = launch_zero_bytes(cs->words, lu_chain_words(n, nrhs) * sizeof(unsigned long long), st);
+    int rc = lu_chain_solve_batched(A, 0, n, lda, B, 0, ldb, nrhs, 1, cs->words, forward, st);
     if (rc)
         return rc;
-    if (forward)
-        launch_kernel(&chain_solve<false>, dim3((unsigned)nch), dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, cs->words,
-                      1u);
-    launch_kernel(&chain_solve<true>, dim3((unsigned)nch), dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, cs->words, 2u);
-    FRICO_HIP_CHECK(hipGetLastError());
     if (!rs)
         return 0;
     if (rs->nev < 2)

@@ -66,7 +66,7 @@ int launch_split_f16_cm(const c64* S, int64_t ld, int64_t R, int K, _Float16* pl
 // L11^{-1} A(k .. k+w, c0 .. c0+nc) in place, split per column; trans true: L21 = A21 U11^{-1} for
 // rows k+w+c0 .. k+w+c0+nc of A21, in place, split per row.
 int launch_tri_solve_split(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, bool trans, _Float16* planes,
-                           int* e, int64_t cap, hipStream_t st);
+                           int* e, int64_t cap, hipStream_t st, const Batch& bt = Batch{}, int nz = 1);
 int launch_solve_l21(c64* A, int64_t lda, int64_t k, int w, int64_t m, hipStream_t st);
 int launch_solve_u12(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t nc, hipStream_t st);
 // C[M x N] -= L U, L as split rows (La, ea; acap rows), U as split columns (Ub, eb; bcap columns).
@@ -74,9 +74,11 @@ int launch_solve_u12(c64* A, int64_t lda, int64_t k, int w, int64_t c0, int64_t 
 // operands split with its own exponents; L0, L1 with acap rows, U0, U1 with bcap columns).
 int gemm_sub_f16_pair(int64_t M, int64_t N, int K0, const _Float16* La0, const int* ea0, const _Float16* Ub0,
                       const int* eb0, int K1, const _Float16* La1, const int* ea1, const _Float16* Ub1, const int* eb1,
-                      int64_t acap, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);
+                      int64_t acap, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt,
+                      const Batch& bt = Batch{}, int nz = 1);
 int gemm_sub_f16(int64_t M, int64_t N, int K, const _Float16* La, const int* ea, int64_t acap, const _Float16* Ub,
-                 const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt);
+                 const int* eb, int64_t bcap, c64* C, int64_t ldc, hipStream_t st, KernelTimer* kt,
+                 const Batch& bt = Batch{}, int nz = 1);
 
 // Products of the triangular solves on fp16 MFMA (gemm_f16.hip): C -= A B, complex64 and
 // column-major, A's rows scaled by 2^ea and B's columns by 2^eb (computed here into eb[N], over B's
@@ -150,6 +152,21 @@ struct ResidStream {
     int* bexp = nullptr;
     int* bexp2 = nullptr;
 };
+// Split operands and zero-pivot words of a lockstep batch of unpivoted complex64 factorisations
+// (lu_nopiv_batched): ha[i] / hea[i] hold system z's planes at + z spa and exponents at + z sea
+// (hacap rows a system), hb likewise; info[z] the first zero pivot (1-based) of system z, 0 if none.
+struct BatchLu {
+    _Float16* ha[4] = {nullptr, nullptr, nullptr, nullptr};
+    int* hea[4] = {nullptr, nullptr, nullptr, nullptr};
+    _Float16* hb[4] = {nullptr, nullptr, nullptr, nullptr};
+    int* heb[4] = {nullptr, nullptr, nullptr, nullptr};
+    int64_t hacap = 0, hbcap = 0, spa = 0, sea = 0, spb = 0, seb = 0;
+    int* info = nullptr;
+};
+// The paired-step unpivoted complex64 LU (fp16 products) of nz systems of size n in lockstep, system
+// z at A + z sa with naug carried columns; n + naug <= 4096.
+int lu_nopiv_batched(const BatchLu& w, c64* A, int64_t sa, int64_t n, int64_t lda, int64_t naug, int nz,
+                     hipStream_t st);
 // Chained complex64 solves (one launch a triangle) for one to four right-hand sides of factors
 // without interchanges: the exchange area of lu_chain_words(n, nrhs) 64-bit words.
 struct ChainSolve {
@@ -157,6 +174,15 @@ struct ChainSolve {
     size_t nwords = 0;
 };
 size_t lu_chain_words(int64_t n, int64_t nrhs);
+// Both halves (forward) or the backward half of nz systems' solves with factors without interchanges,
+// in place: system z's factors at A + z sa, its right-hand sides at B + z sb, its exchange area at
+// words + z lu_chain_words(n, nrhs) (cleared here).  1 <= nrhs <= 4.
+int lu_chain_solve_batched(const c64* A, int64_t sa, int64_t n, int64_t lda, c64* B, int64_t sb, int64_t ldb,
+                           int64_t nrhs, int nz, unsigned long long* words, bool forward, hipStream_t st);
+// C_z -= A_z B_z for nz systems (A complex64, B and C complex128; N <= 4 columns), system z's
+// operands at A + z sa, B + z sb, C + z sc.
+int gemv_sub_mixed_batched(int64_t M, int64_t N, int64_t K, const c64* A, int64_t lda, int64_t sa, const c128* B,
+                           int64_t ldb, int64_t sb, c128* C, int64_t ldc, int64_t sc, int nz, hipStream_t st);
 // Rows of a block of the blocked complex64 solves (their substitutions and products).
 int lu_solve_block_rows();
 // Whether complex64 solves without interchanges take the blocked path (which needs neither the

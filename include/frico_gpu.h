@@ -355,6 +355,9 @@ int frico_adjoint_rcs_backward(frico_ctx* ctx, double g_vv, double g_hh, double*
  *   zin[2*(m*nfreq + f) ..] = V / I_port (compute_port_values, src/emw_ana_antenna.cpp:76-105);
  *   info[m*nfreq + f] is the solve status.  Meshes run largest first on up to `workers` concurrent
  *   contexts (fewer when ceil(Nmax / R) panel workgroups per context would exceed the CUs).
+ *   The unpivoted form with a complex64 fill (256 <= N < 4096) runs each mesh's frequencies in
+ *   lockstep: one factorisation launch sequence and one refinement sequence for all of them;
+ *   systems that stall are retried in lockstep with the row order of one frequency's pivoted LU.
  *   FRICO_BATCH_GRAPHS replays a HIP graph per mesh, recorded from the kernel launches, for the
  *   complex64 LU (FRICO_SOLVE_C64_IR or FRICO_SOLVE_C64_NP_IR) and each refinement sweep; the
  *   unpivoted form falls back to the pivoted LU on a zero pivot or a stalled refinement, and the

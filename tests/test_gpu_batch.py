@@ -192,3 +192,60 @@ def test_uneven_workload_32_meshes_16_frequencies():
             print(f"mesh {m} f{fi}: complex128-fill Zin error {e128:.2e}, complex64-fill {e64:.2e}")
             assert e128 <= 1e-8, (m, fi, z128[i, j], zr)
             assert e64 <= 1e-2, (m, fi, z[m, fi], zr)
+
+
+def test_lockstep_unpivoted_batch_matches_the_pivoted_batch():
+    """The unpivoted form with a complex64 fill runs each mesh's frequencies in lockstep (one launch
+    per step for all of them: lu_nopiv_batched, batched chained solves and residuals).  Each system
+    converges to the solution of its complex64 Z, as the pivoted form's does, so the two batches'
+    input impedances agree to the refinement tolerance.  The mesh below 256 unknowns runs one system
+    at a time; systems whose unpivoted factors stall go to the pivoted LU."""
+    import frico_amd
+    ms = meshes.batch_workload(count=4, seed0=11, nmin=180, nmax=1500)
+    fr = [150e6, 280e6, 430e6, 500e6]
+    with frico_amd.Batch(0, workers=2) as b:
+        for sm in ms:
+            b.add_mesh(sm)
+        z_np, info_np = b.antenna(fr, fill="c64", form="c64_np_ir", tol=1e-12, autotune=False)
+        forms = b.factored_forms(len(fr))
+    with frico_amd.Batch(0, workers=2) as b:
+        for sm in ms:
+            b.add_mesh(sm)
+        z_p, info_p = b.antenna(fr, fill="c64", form="c64_ir", tol=1e-12, autotune=False)
+    Ns = []
+    with frico_amd.Context(0) as c:
+        for sm in ms:
+            c.load_mesh(sm, 5)
+            Ns.append(c.N)
+    print("N", Ns, "forms", forms.tolist())
+    assert all(i.converged and i.max_rel_residual <= 1e-12 for row in info_np for i in row)
+    assert all(i.converged for row in info_p for i in row)
+    assert any(256 <= n <= 4095 for n in Ns)
+    err = np.max(np.abs(z_np - z_p) / np.abs(z_p))
+    print(f"lockstep vs pivoted batch: max rel Zin difference {err:.2e}")
+    assert err <= 1e-9
+
+
+def test_lockstep_static_row_order_for_growing_factors():
+    """A long strip dipole (N = 3563) whose unpivoted complex64 factors grow too much at every
+    frequency: the lockstep pass stalls, the first frequency is solved by the pivoted LU, and the
+    others are solved again in lockstep with that LU's row order (the factors of P Z without further
+    interchanges).  Every system converges to the pivoted batch's answer."""
+    import frico_amd
+    sm = meshes.batch_workload()[4]
+    fr = [150e6, 250e6, 350e6, 450e6]
+    with frico_amd.Batch(0, workers=1) as b:
+        b.add_mesh(sm)
+        z_np, info_np = b.antenna(fr, fill="c64", form="c64_np_ir", tol=1e-10, autotune=False)
+        forms = b.factored_forms(len(fr))
+    with frico_amd.Batch(0, workers=1) as b:
+        b.add_mesh(sm)
+        z_p, info_p = b.antenna(fr, fill="c64", form="c64_ir", tol=1e-10, autotune=False)
+    print("forms", forms.tolist(), "iterations", [i.iterations for i in info_np[0]])
+    assert all(i.converged and i.max_rel_residual <= 1e-10 for row in info_np for i in row)
+    # the first frequency by the pivoted LU, the others by the unpivoted factors of P Z
+    assert forms[0, 0] == frico_amd.SOLVE_C64_IR
+    assert np.all(forms[0, 1:] == frico_amd.SOLVE_C64_NP_IR), forms
+    err = np.max(np.abs(z_np - z_p) / np.abs(z_p))
+    print(f"static row order vs pivoted batch: max rel Zin difference {err:.2e}")
+    assert err <= 1e-8
