@@ -280,8 +280,8 @@ def main():
                     help="plate: the N=12160 headline; batch: the uneven 32-mesh x 16-frequency sweep; "
                          "approx: the same plate in approximate mode with the symmetric solve; "
                          "nearfield: E and H on a 90601-point sampling plane over that plate")
-    ap.add_argument("--workers", type=int, default=8,
-                    help="batch workload: concurrent contexts (lockstep units: 583-602 systems/s at 8, 595 at 4)")
+    ap.add_argument("--workers", type=int, default=4,
+                    help="batch workload: concurrent contexts (lockstep units: 725 systems/s at 4, 694 at 8)")
     ap.add_argument("--batch-form", default="c64_np_ir", choices=["c64_ir", "c64_np_ir"],
                     help="batch workload: factorisation form")
     args = ap.parse_args()

@@ -3164,9 +3164,10 @@ static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int6
 // With one to four right-hand sides a blocked solve is a chain of launches, each a single
 // workgroup's substitution followed by a product with one useful column, so a solve of a few
 // thousand rows costs about a millisecond of launch and wait latency.  Here one launch solves the
-// whole triangle: workgroup w owns the 64-row chunk j (j = w for L, forward; j = nch - 1 - w for U,
-// backward), so a workgroup depends only on chunks owned by lower workgroup ids, which the
-// dispatcher starts first: the grid needs no co-residency.  The dependency chunks i are taken in
+// whole triangle (of each of nz systems): a workgroup takes 64-row chunks from a counter in solve
+// order (chunk 0 first for L, forward; the last for U, backward; the systems interleaved), so it
+// depends only on chunks taken before its own by running workgroups: neither co-residency nor the
+// dispatch order is assumed, and any grid size is safe.  The dependency chunks i are taken in
 // the order they finish, up to CHAIN_WIN a step: wave 0 waits for the first one's x_i and takes the
 // following ones already published in the same poll, and every thread takes D(j, i) x_i off its
 // partial sums (row = lane, sixteen columns of the chunk = wave), the blocks of D loaded before
@@ -3182,7 +3183,9 @@ static int lu_forward_blocked(const c64* A, int64_t n, int64_t lda, c64* B, int6
 // ----------------------------------------------------------------------------------------------
 constexpr int CHAIN_MAXR = 4;
 
-size_t lu_chain_words(int64_t n, int64_t nrhs) { return 1 + (size_t)((n + 63) / 64) * 64 * (size_t)nrhs * 2; }
+// per system: word 0 the abort flag (low half), word 1 the forward and backward launches' item
+// counters (only system 0's are used), then the x words
+size_t lu_chain_words(int64_t n, int64_t nrhs) { return 2 + (size_t)((n + 63) / 64) * 64 * (size_t)nrhs * 2; }
 
 constexpr int CHAIN_WIN = 4; // dependency chunks taken per wait
 
@@ -3194,18 +3197,30 @@ __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A0, i
     __shared__ c64 sd[64][65];                         // sd[c][r] = D(r, c), the chunk's diagonal block
     __shared__ c64 sx[2][CHAIN_WIN][64][CHAIN_MAXR];   // x of the chunks of a step (two steps)
     __shared__ c64 sp[4][64][CHAIN_MAXR];              // the four waves' partial sums
-    __shared__ int s_ok, s_nav[2];
-    // batched: the workgroup solves its chunk of every system in turn, system z at A0 + z s0,
-    // B0 + z s1 with its exchange area at words0 + z s2
-    for (int zs = 0; zs < nz; zs++) {
+    __shared__ int s_ok, s_nav[2], s_item;
+    // Items (system z, chunk number t) are taken from a counter in the order t nz + z, so that every
+    // item's dependencies (the same system's earlier chunks) are taken before it, by workgroups that
+    // are running: a workgroup waits only on items held by running workgroups, whatever the grid
+    // size, the dispatch order or the other kernels on the GPU.  System z at A0 + z s0, B0 + z s1,
+    // its exchange area at words0 + z s2.
+    const int nch = (int)((n + 63) / 64);
+    int* counter = reinterpret_cast<int*>(words0 + 1) + (UPPER ? 1 : 0);
+    const int64_t nitems = (int64_t)nch * nz;
+    for (;;) {
+    if (threadIdx.x == 0)
+        s_item = atomicAdd(counter, 1);
+    __syncthreads();
+    const int64_t item = s_item;
+    if (item >= nitems)
+        break;
+    const int zs = (int)(item % nz), jj = (int)(item / nz);
     const c64* __restrict__ A = A0 + zs * bt.s0;
     c64* B = B0 + zs * bt.s1;
     unsigned long long* words = words0 + zs * bt.s2;
     int* abort_flag = reinterpret_cast<int*>(words);
-    unsigned long long* xw = words + 1;
+    unsigned long long* xw = words + 2;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int nch = (int)((n + 63) / 64);
-    const int j = UPPER ? nch - 1 - (int)blockIdx.x : (int)blockIdx.x;
+    const int j = UPPER ? nch - 1 - jj : jj;
     const int64_t r0 = 64 * (int64_t)j;
     const int wj = (n - r0) < 64 ? (int)(n - r0) : 64;
     for (int e = tid; e < 64 * 64; e += 256) {
@@ -3332,7 +3347,7 @@ __global__ __launch_bounds__(256) void chain_solve(const c64* __restrict__ A0, i
             ll_put(dst + 1, seq, __float_as_uint(lane < wj ? x.im : 0.0f));
         }
     }
-    __syncthreads(); // LDS reused by the next system
+    __syncthreads(); // LDS (and s_item) reused by the next item
     }
 }
 
@@ -3372,9 +3387,10 @@ int lu_chain_solve_batched(const c64* A, int64_t sa, int64_t n, int64_t lda, c64
     bt.s0 = sa;
     bt.s1 = sb;
     bt.s2 = nw;
-    // one workgroup a chunk for all systems (each in turn): nch workgroups whatever nz, so that the
-    // batch's waits need no more resident workgroups than one system's
-    const dim3 grid((unsigned)nch);
+    // the items are taken from a counter: any grid size is safe; one workgroup an item up to about
+    // one a CU
+    const int64_t items = nch * nz;
+    const dim3 grid((unsigned)(items < 240 ? items : 240));
     if (forward)
         launch_kernel(&chain_solve<false>, grid, dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, words, 1u, bt, nz);
     launch_kernel(&chain_solve<true>, grid, dim3(256), 0, st, A, lda, n, B, ldb, (int)nrhs, words, 2u, bt, nz);
