@@ -130,7 +130,7 @@ __global__ __launch_bounds__(64) void solve_u12(c64* __restrict__ A, int64_t lda
 }
 
 // The fp16 split of S[r + k ld] (r < R, k < K <= 128; rows contiguous, as L21 lies in A) into the
-// planes P[p][r][k] and exponents of split_f16 (gemm_f16.hip).  1024 threads: 64 rows x 16 groups
+// planes (k slabs, hsplit_at) and exponents of split_f16 (gemm_f16.hip).  1024 threads: 64 rows x 16 groups
 // of eight k; the row's largest magnitude is reduced over its 16 groups through LDS.
 __global__ __launch_bounds__(1024) void split_f16_cm(const c64* __restrict__ S, int64_t ld, int64_t R, int K,
                                                      _Float16* __restrict__ P, int64_t cap, int* __restrict__ E)
@@ -172,7 +172,7 @@ __global__ __launch_bounds__(1024) void split_f16_cm(const c64* __restrict__ S, 
         ih[q] = hi;
         il[q] = (_Float16)(xi - (float)hi);
     }
-    const int64_t off = r * HSPLIT_K + 8 * g, ps = cap * HSPLIT_K;
+    const int64_t off = hsplit_at(r, 8 * g, cap), ps = cap * HSPLIT_K;
     *reinterpret_cast<half8*>(P + off) = rh;
     *reinterpret_cast<half8*>(P + ps + off) = rlo;
     *reinterpret_cast<half8*>(P + 2 * ps + off) = ih;
@@ -323,17 +323,16 @@ __global__ __launch_bounds__(256) void tri_solve_split(c64* __restrict__ A, int6
             e = 14 - ilogbf(mx); // mx 2^e in [2^14, 2^15)
             e = e < -100 ? -100 : (e > 200 ? 200 : e);
         }
-        _Float16* pr = P + j * HSPLIT_K;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const c64 v = h ? hi[q] : lo[q];
-            const int kk = lane + 64 * h;
+            _Float16* pr = P + hsplit_at(j, lane + 64 * h, cap);
             const float xr = ldexpf(v.re, e), xi = ldexpf(v.im, e);
             const _Float16 hr = (_Float16)xr, hm = (_Float16)xi;
-            pr[kk] = hr;
-            pr[ps + kk] = (_Float16)(xr - (float)hr);
-            pr[2 * ps + kk] = hm;
-            pr[3 * ps + kk] = (_Float16)(xi - (float)hm);
+            pr[0] = hr;
+            pr[ps] = (_Float16)(xr - (float)hr);
+            pr[2 * ps] = hm;
+            pr[3 * ps] = (_Float16)(xi - (float)hm);
         }
         if (lane == 0)
             E[j] = e;
@@ -523,17 +522,16 @@ __global__ __launch_bounds__(256) void tri_solve_split_lds(c64* __restrict__ A, 
             e = 14 - ilogbf(mx); // mx 2^e in [2^14, 2^15)
             e = e < -100 ? -100 : (e > 200 ? 200 : e);
         }
-        _Float16* pr = P + j * HSPLIT_K;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const c64 vv = h ? hi[q] : lo[q];
-            const int kk = lane + 64 * h;
+            _Float16* pr = P + hsplit_at(j, lane + 64 * h, cap);
             const float xr = ldexpf(vv.re, e), xi = ldexpf(vv.im, e);
             const _Float16 hr = (_Float16)xr, hm = (_Float16)xi;
-            pr[kk] = hr;
-            pr[ps + kk] = (_Float16)(xr - (float)hr);
-            pr[2 * ps + kk] = hm;
-            pr[3 * ps + kk] = (_Float16)(xi - (float)hm);
+            pr[0] = hr;
+            pr[ps] = (_Float16)(xr - (float)hr);
+            pr[2 * ps] = hm;
+            pr[3 * ps] = (_Float16)(xi - (float)hm);
         }
         if (lane == 0)
             E[j] = e;

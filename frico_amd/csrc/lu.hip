@@ -2403,7 +2403,7 @@ static int lu_nopiv_pairs(LuWork& wk, c64* A, int64_t n, int64_t lda, int nb, hi
                      hipStream_t s) -> int {
         if (M <= 0 || nc <= 0)
             return 0;
-        return gemm_sub_f16(M, nc, w, wk.ha[ai] + off * HSPLIT_K, wk.hea[ai] + off, wk.hacap, wk.hb[bi], wk.heb[bi],
+        return gemm_sub_f16(M, nc, w, wk.ha[ai] + off * HSLAB, wk.hea[ai] + off, wk.hacap, wk.hb[bi], wk.heb[bi],
                             wk.hbcap, A + r0 + c0 * lda, lda, s, gemm_timer);
     };
     // the pair's product on rows [r0, n) x columns [c0, c0 + nc): L21(a) from its row off_a, L21(b)
@@ -2412,7 +2412,7 @@ static int lu_nopiv_pairs(LuWork& wk, c64* A, int64_t n, int64_t lda, int nb, hi
         const int64_t M = n - r0;
         if (M <= 0 || nc <= 0)
             return 0;
-        return gemm_sub_f16_pair(M, nc, wa, wk.ha[ia] + off_a * HSPLIT_K, wk.hea[ia] + off_a, wk.hb[b0], wk.heb[b0], wb,
+        return gemm_sub_f16_pair(M, nc, wa, wk.ha[ia] + off_a * HSLAB, wk.hea[ia] + off_a, wk.hb[b0], wk.heb[b0], wb,
                                  wk.ha[ib], wk.hea[ib], wk.hb[b1], wk.heb[b1], wk.hacap, wk.hbcap, A + r0 + c0 * lda, lda,
                                  s, gemm_timer);
     };
@@ -2534,7 +2534,7 @@ int lu_nopiv_batched(const BatchLu& w, c64* A, int64_t sa, int64_t n, int64_t ld
     auto prod1 = [&](int64_t M, int64_t nc, int wd, int ai, int64_t off, int bi, int64_t r0, int64_t c0) -> int {
         if (M <= 0 || nc <= 0)
             return 0;
-        return gemm_sub_f16(M, nc, wd, w.ha[ai] + off * HSPLIT_K, w.hea[ai] + off, w.hacap, w.hb[bi], w.heb[bi], w.hbcap,
+        return gemm_sub_f16(M, nc, wd, w.ha[ai] + off * HSLAB, w.hea[ai] + off, w.hacap, w.hb[bi], w.heb[bi], w.hbcap,
                             A + r0 + c0 * lda, lda, st, nullptr, bp, nz);
     };
     // the pair's product on rows [r0, n): L21(a) from its row off_a (L21(a) starts at block b's first
@@ -2544,7 +2544,7 @@ int lu_nopiv_batched(const BatchLu& w, c64* A, int64_t sa, int64_t n, int64_t ld
         const int64_t M = n - r0;
         if (M <= 0 || nc <= 0)
             return 0;
-        return gemm_sub_f16_pair(M, nc, wa, w.ha[ia] + off_a * HSPLIT_K, w.hea[ia] + off_a, w.hb[b0], w.heb[b0], wb,
+        return gemm_sub_f16_pair(M, nc, wa, w.ha[ia] + off_a * HSLAB, w.hea[ia] + off_a, w.hb[b0], w.heb[b0], wb,
                                  w.ha[ib], w.hea[ib], w.hb[b1], w.heb[b1], w.hacap, w.hbcap, A + r0 + c0 * lda, lda, st,
                                  nullptr, bp, nz);
     };

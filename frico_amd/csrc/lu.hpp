@@ -49,8 +49,16 @@ struct LuWork {
     bool direct_solves = true;
 };
 
-// fp16 split products (gemm_f16.hip).  Rows of a split operand: HSPLIT_K halves each.
+// fp16 split products (gemm_f16.hip).  Rows of a split operand: HSPLIT_K halves each, stored in
+// slabs of HSLAB consecutive k: element (r, k) of a plane of cap rows sits at hsplit_at(r, k, cap),
+// so that the k slab of a run of rows is one contiguous stretch (32 bytes a row).  A view that starts
+// at row off of a plane begins at element off * HSLAB.
 constexpr int HSPLIT_K = 128;
+constexpr int HSLAB = 16;
+__host__ __device__ inline int64_t hsplit_at(int64_t r, int k, int64_t cap)
+{
+    return (int64_t)(k / HSLAB) * cap * HSLAB + r * HSLAB + (k % HSLAB);
+}
 // Device bytes of a split operand of cap rows (cap a multiple of 128): planes and exponents.
 inline size_t split_f16_bytes(int64_t cap) { return (size_t)cap * (4 * HSPLIT_K * 2 + sizeof(int)); }
 // S[k + r ld], r < R, k < K <= HSPLIT_K -> planes (cap rows each) and exponents of the R rows, padded
