@@ -42,7 +42,7 @@ GEMM_F32_TRAFFIC = {"c64_np_ir": 699.7e6, "c64_ir": 700.1e6}
 # HBM bytes per timed launch of cgemm_sub_f16 at N = 12160, nb = 128, averaged over the 235 launches of
 # one factorisation in paired steps (rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled by a copy kernel of
 # known size: profiles/plate_N12160_c64np_hbm_traffic.md); algorithmic bytes per launch 176.7e6.
-GEMM_F16_TRAFFIC = 257.9e6
+GEMM_F16_TRAFFIC = 237.6e6
 # HBM bytes of one quadrature-mode fill at N = 12160 (tripair_f64 + assemble + near_f64; same profile).
 FILL_TRAFFIC = 31.1e9
 PEAK_FP16_MFMA = 2500.0  # TFLOP/s, MI355X dense FP16 matrix
