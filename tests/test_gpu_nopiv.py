@@ -75,6 +75,28 @@ def test_unpivoted_lu_solves_dominant_systems(n, nb, nrhs):
         assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
 
 
+@pytest.mark.parametrize("diag", [4.0, 1.0])
+def test_unpivoted_lu_with_long_block_columns(diag):
+    """n = 4700 at nb = 128: the first block columns have more than 4096 rows below the diagonal
+    block, so their L21 takes the transposed path (col_transpose, trsm_block and the joining
+    product, split_f16) instead of the fused block solve; the weaker diagonal gives the diagonal
+    blocks larger condition numbers."""
+    import frico_amd
+    n, nrhs = 4700, 3
+    rng = np.random.default_rng(4700)
+    A = rng.uniform(-1, 1, (n, n)) + 1j * rng.uniform(-1, 1, (n, n)) + diag * np.sqrt(n) * np.eye(n)
+    B = rng.uniform(-1, 1, (n, nrhs)) + 1j * rng.uniform(-1, 1, (n, nrhs))
+    print(f"unpivoted long block columns n={n} diag={diag}", flush=True)
+    with frico_amd.Context(0) as c:
+        c.set_block(128)
+        c.set_matrix(A)
+        c.set_rhs(B)
+        c.factor("c64_np_ir")
+        info = c.solve(tol=1e-12, maxit=10)
+        assert info.converged and c.factored_form == frico_amd.SOLVE_C64_NP_IR and info.iterations <= 4
+        assert relmax(c.solution(), np.linalg.solve(A, B)) <= 1e-10
+
+
 def test_unpivoted_factors_do_not_depend_on_the_lookahead(monkeypatch):
     import frico_amd
     rng = np.random.default_rng(3)
