@@ -502,10 +502,10 @@ __global__ __launch_bounds__(256, 2) void cgemm_sub_f16s(int64_t M, int64_t N, i
             are[a][b] = f32x4{0, 0, 0, 0};
             aim[a][b] = f32x4{0, 0, 0, 0};
         }
-    for (int64_t k0 = 0; k0 < K; k0 += HKC) {
-        // branch-free loads (out-of-range ones read element 0 and are zeroed after), all in flight
-        // together
-        c64 va[2][8], vb[8];
+    // branch-free loads of a chunk (out-of-range ones read element 0 and are zeroed when the chunk
+    // is split), all in flight together; the next chunk's are issued before this chunk's MFMAs
+    c64 va[2][8], vb[8];
+    auto gload = [&](int64_t k0) {
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
@@ -518,6 +518,9 @@ __global__ __launch_bounds__(256, 2) void cgemm_sub_f16s(int64_t M, int64_t N, i
             const int64_t k = k0 + 8 * bs + q;
             vb[q] = Bc[k < K ? k : 0];
         }
+    };
+    gload(0);
+    for (int64_t k0 = 0; k0 < K; k0 += HKC) {
 #pragma unroll
         for (int h = 0; h < 2; h++)
 #pragma unroll
@@ -549,6 +552,8 @@ __global__ __launch_bounds__(256, 2) void cgemm_sub_f16s(int64_t M, int64_t N, i
             sB[3][bc][sg] = il;
         }
         __syncthreads();
+        if (k0 + HKC < K)
+            gload(k0 + HKC);
         const int sgl = lane >> 4;
         half8 bf[CB][4];
 #pragma unroll
