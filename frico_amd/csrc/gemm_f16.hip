@@ -152,14 +152,11 @@ __device__ inline void load_lds16(const _Float16* g, char* lds_wave)
                                      (__attribute__((address_space(3))) void*)lds_wave, 16, 0, 0);
 }
 
-// s_waitcnt vmcnt(n) lgkmcnt(0): all but this wave's last n vector memory loads have landed
-template <int n> __device__ inline void wait_vm()
-{
-    static_assert(n >= 0 && n < 64, "vmcnt range");
-    __builtin_amdgcn_s_waitcnt((n & 15) | ((n >> 4) << 14) | 0x70);
-}
+// s_waitcnt vmcnt(0) lgkmcnt(0): this wave's vector memory loads (the LDS DMA included) and LDS
+// reads have landed
+__device__ inline void wait_loads() { __builtin_amdgcn_s_waitcnt(0x70); }
 
-template <int CBW, int OCC, int RING, bool TWO = false>
+template <int CBW, int OCC, bool TWO = false>
 __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, int kc, const _Float16* __restrict__ La,
                                                          const int* __restrict__ ea, int64_t acap,
                                                          const _Float16* __restrict__ Ub, const int* __restrict__ eb,
@@ -186,8 +183,7 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
     constexpr int ASL = 4 * BM * 2, BSL = 4 * BN * 2; // 16-byte slots of a stage: L, U (4 planes, 2 a row)
     constexpr int NA = ASL / 256, NB = BSL / 256;      // loads a thread a stage
     constexpr int STAGE = (ASL + BSL) * 16;            // bytes of a stage
-    constexpr int NL = NA + NB;                        // loads a thread a stage
-    extern __shared__ __attribute__((aligned(16))) char lds[]; // RING stages
+    extern __shared__ __attribute__((aligned(16))) char lds[]; // two stages
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int wm = wv & 1, wn = wv >> 1;
     int64_t tm, tn;
@@ -242,22 +238,11 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
     const int64_t rbase = m0 + wm * 64 + (lane & 31), cbase = n0 + wn * 32 * CBW + 4 * (lane >> 5);
     const int sgl = lane >> 5;
 
-    // wait until stage st + 1 has landed, with stages st + 2 .. st + RING - 1 (those below S) in flight
-    auto wait_next = [&](int st) {
-        const int behind = S - 2 - st < RING - 2 ? S - 2 - st : RING - 2;
-        if (RING >= 4 && behind >= 2)
-            wait_vm<(RING >= 4 ? 2 * NL : 0)>();
-        else if (RING >= 3 && behind >= 1)
-            wait_vm<(RING >= 3 ? NL : 0)>();
-        else
-            wait_vm<0>();
-    };
-    for (int st = 0; st < RING - 1 && st < S; st++)
-        issue(st, st);
-    wait_next(-1);
+    issue(0, 0);
+    wait_loads();
     __syncthreads();
     for (int st = 0; st < S; st++) {
-        const char* buf = lds + (st % RING) * STAGE;
+        const char* buf = lds + (st & 1) * STAGE;
         if (TWO && st == S0) {
             // rescale the first half's sums to the second half's exponents (the exponent tables
             // cover the padded rows and columns of whole tiles, so their reads need no bounds)
@@ -308,12 +293,9 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
         rd_l(3);
         rd_u(2);
         rd_u(3);
-        // the loads of stage st + RING - 1; past the last stage the last one again into the free
-        // buffer (no branch, so that the scheduler can interleave all of this stage's work)
-        {
-            const int nx = st + RING - 1 < S ? st + RING - 1 : S - 1;
-            issue(nx, (st + RING - 1) % RING);
-        }
+        // the loads of stage st + 1; past the last stage the last one again into the free buffer
+        // (no branch, so that the scheduler can interleave all of this stage's work)
+        issue(st + 1 < S ? st + 1 : S - 1, (st + 1) & 1);
         half8 nh[2], nl[2];
 #pragma unroll
         for (int rb = 0; rb < 2; rb++) {
@@ -345,13 +327,13 @@ __global__ __launch_bounds__(256, OCC) void cgemm_sub_f16(int64_t M, int64_t N, 
             __builtin_amdgcn_sched_group_barrier(0x008, MB / 2, 0);            // MFMA
         }
 #pragma unroll
-        for (int g = 0; g < NL; g++) {
+        for (int g = 0; g < NA + NB; g++) {
             __builtin_amdgcn_sched_group_barrier(0x020, 1, 0); // VMEM (LDS DMA)
             __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
         }
 #undef FRICO_MF
         __builtin_amdgcn_sched_barrier(0); // the wait below stays after the MFMAs
-        wait_next(st);                     // this wave's loads of the next stage have landed
+        wait_loads();                      // this wave's loads of the next stage have landed
         __syncthreads();                   // everyone's have, and this stage's buffer is free
     }
     if constexpr (TWO) {
@@ -679,7 +661,7 @@ static hipError_t launch_f16_prod(dim3 grid, hipStream_t st, int64_t M, int64_t 
 {
     // two 16-deep stages of the four planes of 128 rows of L and 64 CBW columns of U
     constexpr size_t lds = 2 * (size_t)(4 * 128 * 2 + 4 * 64 * CBW * 2) * 16;
-    auto kern = &cgemm_sub_f16<CBW, CBW == 2 ? 2 : 3, 2, TWO>;
+    auto kern = &cgemm_sub_f16<CBW, CBW == 2 ? 2 : 3, TWO>;
     const hipError_t e = allow_lds(reinterpret_cast<const void*>(kern), lds);
     if (e != hipSuccess)
         return e;
